@@ -1,0 +1,204 @@
+/*
+ * tmps.h -- C ABI of libtmps_hip.so, the MI355X (gfx950) implementation of
+ * the taichimps per-step DEM path.
+ *
+ * Every entry point returns an int status (TMPS_OK == 0, negative on error;
+ * tmps_last_error() gives the message). Host arrays are borrowed for the
+ * duration of the call; device memory is owned by the library. One caller
+ * thread per system; all work runs on the system's own HIP stream.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to src/taichimps/ of the reference repository).
+ */
+#ifndef TMPS_H
+#define TMPS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define TMPS_OK 0
+#define TMPS_ERR_ARG -1          /* ValueError in the reference */
+#define TMPS_ERR_HIP -2          /* a HIP runtime call failed */
+#define TMPS_ERR_OVERFLOW -3     /* neighbor.py:426-431 RuntimeError */
+#define TMPS_ERR_BOX -4          /* deform_pressure.py:1167-1171 RuntimeError */
+#define TMPS_ERR_CAPACITY -5     /* atom.py:74-77 ValueError */
+#define TMPS_ERR_UNSUPPORTED -6  /* a setting this build does not implement */
+
+/* ---- precision of the per-atom state --------------------------------- */
+#define TMPS_FP64 0        /* all arithmetic in FP64 (the reference's f64 path) */
+#define TMPS_FP32_SPLIT 1  /* FP64 cell origin + FP32 offset; FP32 contact and
+                              integration; FP64 virial, pressure and servo */
+
+/* ---- per-atom fields (AtomSystem, atom.py:36-53; NeighborList.x0) ---- */
+enum tmps_field {
+    TMPS_X = 0,      /* double[n][3] */
+    TMPS_V = 1,      /* double[n][3] */
+    TMPS_OMEGA = 2,  /* double[n][3] */
+    TMPS_F = 3,      /* double[n][3] */
+    TMPS_TORQUE = 4, /* double[n][3] */
+    TMPS_RADIUS = 5, /* double[n]    */
+    TMPS_RMASS = 6,  /* double[n]    */
+    TMPS_TAG = 7,    /* int[n]       */
+    TMPS_VIRIAL = 8, /* double[n][6] */
+    TMPS_X0 = 9      /* double[n][3], read-only */
+};
+
+/* Pair-style coefficients after the reference's host-side derivations:
+ * gran/hooke/history (GRANULAR/hooke_history.py:48-68), gran/hertz/history
+ * (GRANULAR/hertz_history.py:36-55) and pair granular
+ * (GRANULAR/granular.py:122-269). */
+#define TMPS_PAIR_HOOKE_HISTORY 0
+#define TMPS_PAIR_HERTZ_HISTORY 1
+#define TMPS_PAIR_GRANULAR 2
+typedef struct {
+    int style;
+    int limit_damping;
+    double kn, kt, gamman, gammat, xmu; /* legacy styles; gammat 0 if dampflag 0 */
+    int normal_model;     /* granular.py:50-52  hooke, hertz, hertz/material */
+    int damping_model;    /* granular.py:54-59  none .. coeff_restitution */
+    int tangential_model; /* granular.py:61-64  none .. mindlin */
+    double g_kn, g_damp, g_kt, g_xt, g_mu;
+} tmps_pair_params;
+
+/* fix deform/pressure settings for the x/y/z entries
+ * (EXTRA_FIX/deform_pressure.py:151-238, AxisSet :90-132). Style codes are
+ * DEVICE_STYLE_CODE (:76-82), couple codes COUPLE_CODE (:83). */
+typedef struct {
+    int enabled;
+    int nevery;
+    int couple;    /* 0 none, 1 xyz, 2 xy, 3 yz, 4 xz */
+    int remap;     /* 0 x, 1 v, 2 none */
+    int normalize; /* normalize/pressure */
+    double max_rate;
+    int style[3];
+    double rate[3], vel[3], scale[3], flo[3], fhi[3], dlo[3], dhi[3];
+    double amp[3], tper[3], ptarget[3], pgain[3];
+} tmps_servo_params;
+
+/* Bookkeeping shared with the Python NeighborList / Simulation objects. */
+typedef struct {
+    int timestep;        /* Simulation.timestep */
+    int has_built;       /* NeighborList.has_built */
+    int build_count;     /* NeighborList.build_count */
+    int last_build_step; /* NeighborList.last_build_step */
+    int current_step;    /* NeighborList.current_step */
+    int setup_done;      /* Simulation.setup_done */
+    int npairs;          /* half-list pairs (NeighborList.npairs) */
+    int nslots;          /* full-list slots (2 * npairs) */
+    int ntouch;          /* touching full-list slots after the last force pass */
+    int rebuilds;        /* rebuilds performed by the last tmps_run */
+    int servo_step_count;
+    int servo_nsteps;
+} tmps_counters;
+
+typedef struct tmps_system tmps_system;
+
+/* ---- library --------------------------------------------------------- */
+const char* tmps_version(void);
+const char* tmps_last_error(void);
+int tmps_device_name(char* buf, int len);
+
+/* ---- system (AtomSystem.__init__ atom.py:30-53 together with the field
+ *      allocations of NeighborList neighbor.py:23-120 and ContactHistory
+ *      contact_history.py:37-80) ------------------------------------------ */
+int tmps_create(int max_atoms, int max_neighbors_per_atom, int precision, tmps_system** out);
+int tmps_destroy(tmps_system* s);
+int tmps_set_nlocal(tmps_system* s, int nlocal);          /* AtomSystem.nlocal */
+int tmps_set_field(tmps_system* s, int field, const void* host, int count);  /* Field.from_numpy */
+int tmps_get_field(tmps_system* s, int field, void* host, int count);        /* Field.to_numpy */
+int tmps_sync(tmps_system* s);                                               /* ti.sync */
+
+/* ---- Domain (domain.py) ---------------------------------------------- */
+/* Domain._sync / set_box_and_h_rate, domain.py:235-307 */
+int tmps_set_box(tmps_system* s, const double lo[3], const double hi[3], const int periodic[3],
+                 const double h_rate[6], int vremap);
+/* Domain.pull, domain.py:104-128 */
+int tmps_get_box(tmps_system* s, double lo[3], double hi[3], double h_rate[6], int* vremap);
+/* Domain.pbc, domain.py:573-584 */
+int tmps_pbc(tmps_system* s);
+
+/* ---- AtomSystem.clear_forces, atom.py:208-220 --------------------------- */
+int tmps_clear_forces(tmps_system* s);
+
+/* ---- NeighborList (neighbor.py) ---------------------------------------- */
+/* skin and neigh_modify delay/every/check (neighbor.py:88-94), max_grid_cells
+ * (:115) and max_neighbors_per_atom (:35-38, the half-list overflow bound) */
+int tmps_neigh_config(tmps_system* s, double skin, int delay, int every, int check,
+                      int max_grid_cells, int max_neighbors_per_atom);
+/* check_displacement, neighbor.py:214-245: *moved = 1 if any |x-x0|^2 > skin^2/4 */
+int tmps_neigh_check(tmps_system* s, int* moved);
+/* NeighborList.decide, neighbor.py:371-403 */
+int tmps_neigh_decide(tmps_system* s, int timestep, int have_timestep, int* rebuild);
+/* NeighborList.build, neighbor.py:414-435 (grid, bins, list, x0, counters) */
+int tmps_neigh_build(tmps_system* s);
+/* grid_dim / cell_size, neighbor.py:185-189; npairs (half list) */
+int tmps_neigh_info(tmps_system* s, int gdim[3], double csize[3], int* npairs);
+/* pair_i / pair_j / ContactHistory.partner / shear in the reference's flat
+ * half-list layout (neighbor.py:284-303, contact_history.py:53-59) */
+int tmps_neigh_get_pairs(tmps_system* s, int* pair_i, int* pair_j, int* partner, double* shear,
+                         int cap);
+/* num_neighbors (half list) per atom, neighbor.py:45 */
+int tmps_neigh_get_counts(tmps_system* s, int* num_neighbors, int n);
+/* cell_of / cell_start / cell_particles, neighbor.py:116-120 */
+int tmps_neigh_get_bins(tmps_system* s, int* cell_of, int* cell_start, int* cell_particles);
+
+/* ---- ContactHistory (contact_history.py) ------------------------------- */
+int tmps_history_reset(tmps_system* s);    /* reset, :82-89 */
+int tmps_history_save(tmps_system* s);     /* save_state, :141-145 */
+int tmps_history_restore(tmps_system* s);  /* restore_state, :147-154 */
+/* seed the current list's history in the flat half-list layout
+ * (partner tag or -1, shear), as ContactHistory.partner/shear.from_numpy */
+int tmps_history_set_pairs(tmps_system* s, const int* partner, const double* shear, int npairs);
+
+/* ---- pair styles (GRANULAR/base.py:25-38) ------------------------------- */
+int tmps_pair_set(tmps_system* s, const tmps_pair_params* p);
+/* GranularPair.compute: forces, torques and per-atom virial of every contact,
+ * history update when shearupdate != 0. accumulate != 0 adds to the current
+ * f/torque/virial (the reference's atomic_add after clear_forces). */
+int tmps_pair_compute(tmps_system* s, double dt, int shearupdate, int accumulate);
+
+/* ---- fixes ---------------------------------------------------------------- */
+int tmps_nve_initial(tmps_system* s, double dt); /* FixNVESphere, nve_sphere.py:98-105 */
+int tmps_nve_final(tmps_system* s, double dt);   /* nve_sphere.py:107-113 */
+/* FixDampingCundall.post_force, damping_cundall.py:55-63 */
+int tmps_cundall(tmps_system* s, double gamma_lin, double gamma_ang);
+/* FixDeformPressure: settings, setup (:392-445), end_of_step (:1113-1219).
+ * tensor_out (nullable) receives the pressure tensor the servo used. */
+int tmps_servo_set(tmps_system* s, const tmps_servo_params* p);
+int tmps_servo_setup(tmps_system* s, int nsteps_total, double dt);
+int tmps_servo_end_of_step(tmps_system* s, double dt, double tensor_out[6]);
+
+/* ---- computes (computes/thermo.py) --------------------------------------- */
+/* compute_virial_kernel, thermo.py:62-109: sum of the per-atom virial
+ * (+ m v_a v_b when keflag) as a pairwise tree over atoms */
+int tmps_virial_reduce(tmps_system* s, int keflag, double out6[6]);
+/* compute_ke_kernel, thermo.py:46-60 */
+int tmps_ke(tmps_system* s, double* ke_trans, double* ke_rot);
+
+/* ---- Simulation (simulation.py) ------------------------------------------- */
+/* Simulation.run(steps), simulation.py:320-332, with an nve/sphere
+ * integrator, the configured pair style, optional damping/cundall
+ * (cundall != 0) and optional deform/pressure servo (tmps_servo_set).
+ * Runs device-resident: steps are replayed from captured HIP graphs and the
+ * host synchronises once per segment (or on a neighbor rebuild). */
+int tmps_run(tmps_system* s, int nsteps, double dt, int run_pre, int cundall, double gamma_lin,
+             double gamma_ang);
+int tmps_get_counters(tmps_system* s, tmps_counters* out);
+int tmps_set_counters(tmps_system* s, const tmps_counters* in);
+/* pressure tensor of the last servo update (virial+kinetic over volume) */
+int tmps_last_pressure(tmps_system* s, double out6[6]);
+
+/* ---- measurement ----------------------------------------------------------- */
+/* Average duration (ms) of each kernel class over the last tmps_run, timed
+ * with HIP events on the system's stream when profiling is enabled.
+ * names: "integrate", "force", "reduce"; out_ms[3]. */
+int tmps_set_profiling(tmps_system* s, int enabled);
+int tmps_kernel_times(tmps_system* s, double out_ms[3], int counts[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMPS_H */

@@ -1,0 +1,397 @@
+"""ctypes wrapper of the CPU FP64 parity oracle -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker. The product
+package (``taichimps_amd``) never imports it.
+
+The oracle restates the reference's per-step DEM path (``dem_oracle.cpp``).
+This module adds the two host-side derivations the reference performs in
+Python before any kernel runs, restated independently of the product code so
+that the product's own derivations are checked against them:
+
+* the pair-style coefficients (hooke_history.py:48-68, hertz_history.py:36-55,
+  granular.py:94-269, input.py:444-472 for the NULL shorthands);
+* per-atom mass from radius and density (atom.py:110-111).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class PairParams(C.Structure):
+    _fields_ = [
+        ("style", C.c_int), ("limit_damping", C.c_int),
+        ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+        ("gammat", C.c_double), ("xmu", C.c_double),
+        ("normal_model", C.c_int), ("damping_model", C.c_int), ("tangential_model", C.c_int),
+        ("g_kn", C.c_double), ("g_damp", C.c_double), ("g_kt", C.c_double),
+        ("g_xt", C.c_double), ("g_mu", C.c_double),
+    ]
+
+
+class ServoParams(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int), ("nevery", C.c_int), ("couple", C.c_int), ("remap", C.c_int),
+        ("normalize", C.c_int), ("max_rate", C.c_double),
+        ("style", C.c_int * 3),
+        ("rate", C.c_double * 3), ("vel", C.c_double * 3), ("scale", C.c_double * 3),
+        ("flo", C.c_double * 3), ("fhi", C.c_double * 3), ("dlo", C.c_double * 3),
+        ("dhi", C.c_double * 3), ("amp", C.c_double * 3), ("tper", C.c_double * 3),
+        ("ptarget", C.c_double * 3), ("pgain", C.c_double * 3),
+    ]
+
+
+STYLE_CODE = {"none": 0, "erate": 1, "trate": 2, "vel": 3, "wiggle": 4, "final": 5,
+              "delta": 6, "scale": 7, "pressure": 8, "pressure/mean": 9}
+COUPLE_CODE = {"none": 0, "xyz": 1, "xy": 2, "yz": 3, "xz": 4}
+REMAP_CODE = {"x": 0, "v": 1, "none": 2}
+
+
+def build() -> str:
+    """Compile liboracle.so with its Makefile (idempotent)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        vp = C.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_atoms.argtypes = [vp, dp, dp, dp, dp, dp, ip]
+        L.orc_get_atoms.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_set_forces.argtypes = [vp, dp, dp]
+        L.orc_set_box.argtypes = [vp, dp, dp, ip]
+        L.orc_set_h_rate.argtypes = [vp, dp, C.c_int]
+        L.orc_get_box.argtypes = [vp, dp, dp, dp]
+        L.orc_set_neigh.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_set_pair.argtypes = [vp, C.POINTER(PairParams)]
+        L.orc_set_servo.argtypes = [vp, C.POINTER(ServoParams)]
+        L.orc_set_cundall.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.orc_run.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        L.orc_init_simulation.argtypes = [vp, C.c_double]
+        L.orc_step.argtypes = [vp, C.c_double]
+        L.orc_servo_setup.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_nlist_build.argtypes = [vp]
+        L.orc_nlist_decide.argtypes = [vp, C.c_int]
+        L.orc_reneighbor.argtypes = [vp, C.c_int]
+        L.orc_pbc.argtypes = [vp]
+        L.orc_clear_forces.argtypes = [vp]
+        L.orc_pair_compute.argtypes = [vp, C.c_double, C.c_int]
+        L.orc_nve_initial.argtypes = [vp, C.c_double]
+        L.orc_nve_final.argtypes = [vp, C.c_double]
+        L.orc_cundall.argtypes = [vp]
+        L.orc_virial.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.orc_servo_end_of_step.argtypes = [vp, C.c_double]
+        L.orc_npairs.argtypes = [vp]
+        L.orc_get_pairs.argtypes = [vp, ip, ip, ip, dp]
+        L.orc_set_history.argtypes = [vp, ip, dp]
+        L.orc_get_grid.argtypes = [vp, ip, dp, ip]
+        L.orc_get_counters.argtypes = [vp, ip]
+        L.orc_get_tensor.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+# ------------------------------------------------------------ derivations
+
+def mass_from_radius(radius, density):
+    """atom.py:110-111: rmass = (4/3) pi r^3 rho, in the reference's numpy order."""
+    rad = np.asarray(radius, dtype=np.float64)
+    vol = (4.0 / 3.0) * np.pi * (rad**3)
+    return vol * np.asarray(density, dtype=np.float64)
+
+
+def legacy_pair(style, kn, gamman, kt=None, gammat=None, xmu=0.0, dampflag=1,
+                limit_damping=False):
+    """gran/hooke/history or gran/hertz/history, constructor order kn, gamman, kt, gammat.
+
+    NULL (None) shorthands follow input.py:444-472: kt = 2/7 kn, gammat = gamman/2;
+    dampflag 0 zeroes gammat (hooke_history.py:64-66).
+    """
+    if kt is None:
+        kt = 2.0 / 7.0 * kn
+    if gammat is None:
+        gammat = 0.5 * gamman
+    p = PairParams()
+    p.style = {"hooke/history": 0, "hertz/history": 1}[style]
+    p.limit_damping = 1 if limit_damping else 0
+    p.kn, p.kt, p.gamman, p.xmu = float(kn), float(kt), float(gamman), float(xmu)
+    p.gammat = 0.0 if int(dampflag) == 0 else float(gammat)
+    return p
+
+
+def granular_pair(normal, normal_coeffs, tangential="none", tangential_coeffs=None,
+                  damping="viscoelastic", limit_damping=False):
+    """PairGranular coefficient derivation, granular.py:94-269."""
+    normals = {"hooke": 0, "hertz": 1, "hertz/material": 2}
+    dampings = {"none": 0, "velocity": 1, "mass_velocity": 2, "viscoelastic": 3,
+                "tsuji": 4, "coeff_restitution": 5}
+    tangs = {"none": 0, "linear_nohistory": 1, "linear_history": 2, "mindlin": 3}
+    p = PairParams()
+    p.style = 2
+    p.limit_damping = 1 if limit_damping else 0
+    p.normal_model = normals[normal]
+    p.damping_model = dampings[damping]
+    p.tangential_model = tangs[tangential]
+    emod = poiss = 0.0
+    if normal == "hertz/material":
+        emod, cor, poiss = (float(c) for c in normal_coeffs)
+        mix_e = 1.0 / ((1 - poiss * poiss) / emod + (1 - poiss * poiss) / emod)
+        p.g_kn = (4.0 / 3.0) * mix_e
+    else:
+        kn, cor = (float(c) for c in normal_coeffs)
+        p.g_kn = kn
+    if damping in ("tsuji", "coeff_restitution"):
+        logcor = math.log(cor)
+        if damping == "tsuji":
+            d = (1.2728 - 4.2783 * cor + 11.087 * cor**2 - 22.348 * cor**3
+                 + 27.467 * cor**4 - 18.022 * cor**5 + 4.8218 * cor**6)
+            p.g_damp = d * math.sqrt(2.0)
+        elif normal == "hooke":
+            p.g_damp = -2 * logcor / math.sqrt(math.pi**2 + logcor**2)
+        else:
+            p.g_damp = (-1.22474487139158894067 * 1.82574185835055380345 * logcor
+                        / math.sqrt(math.pi**2 + logcor**2))
+    else:
+        p.g_damp = cor
+    tc = list(tangential_coeffs or [])
+    if tangential == "linear_nohistory":
+        p.g_xt, p.g_mu = float(tc[0]), float(tc[1])
+    elif tangential in ("linear_history", "mindlin"):
+        k_raw, xt, mu = tc
+        p.g_xt, p.g_mu = float(xt), float(mu)
+        if k_raw is None:
+            mix_g = 1.0 / (2 * (2 - poiss) * (1 + poiss) / emod + 2 * (2 - poiss) * (1 + poiss) / emod)
+            p.g_kt = 8.0 * mix_g
+        else:
+            p.g_kt = float(k_raw)
+    return p
+
+
+def servo_params(axes=None, couple="none", max_rate=0.0, normalize=False, remap="x",
+                 nevery=1, p_target=None, p_gain=None):
+    """FixDeformPressure settings for x/y/z (deform_pressure.py:151-238,332-356)."""
+    s = ServoParams()
+    s.enabled = 1
+    s.nevery = int(nevery)
+    s.remap = REMAP_CODE[remap]
+    s.normalize = 1 if normalize else 0
+    s.max_rate = float(max_rate)
+    for i in range(3):
+        s.scale[i] = 1.0
+    if axes is None:
+        target = 50000.0 if p_target is None else float(p_target)
+        gain = 1e-4 if p_gain is None else float(p_gain)
+        for i in range(3):
+            s.style[i] = STYLE_CODE["pressure"]
+            s.ptarget[i] = target
+            s.pgain[i] = gain
+        if couple == "none":
+            couple = "xyz"
+    else:
+        names = {"x": 0, "y": 1, "z": 2}
+        for name, spec in axes.items():
+            i = names[name]
+            s.style[i] = STYLE_CODE[spec["style"]]
+            for key in ("rate", "vel", "scale", "flo", "fhi", "dlo", "dhi", "ptarget", "pgain"):
+                if key in spec:
+                    getattr(s, key)[i] = float(spec[key])
+            if "amplitude" in spec:
+                s.amp[i] = float(spec["amplitude"])
+            if "tperiod" in spec:
+                s.tper[i] = float(spec["tperiod"])
+        if couple != "none":
+            idx = [i for i, c in enumerate("xyz") if c in couple]
+            ref = next(i for i in idx if s.style[i] in (8, 9))
+            for i in idx:
+                if s.style[i] == 0:
+                    s.style[i] = s.style[ref]
+                    s.ptarget[i] = s.ptarget[ref]
+                    s.pgain[i] = s.pgain[ref]
+    s.couple = COUPLE_CODE[couple]
+    return s
+
+
+# ------------------------------------------------------------------ handle
+
+class OracleSim:
+    """One simulated system held by the C++ oracle."""
+
+    def __init__(self, n: int, max_nb: int = 64):
+        self.n = int(n)
+        self.max_nb = int(max_nb)
+        self._h = lib().orc_create(self.n, self.max_nb)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_destroy(h)
+            self._h = None
+
+    # state -----------------------------------------------------------------
+    def set_atoms(self, x, v, omega, radius, rmass, tag=None):
+        n = self.n
+        arr = lambda a, s: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(s))
+        xx, vv, oo = arr(x, (n, 3)), arr(v, (n, 3)), arr(omega, (n, 3))
+        rr, mm = arr(radius, (n,)), arr(rmass, (n,))
+        tt = np.ascontiguousarray(
+            np.arange(1, n + 1, dtype=np.int32) if tag is None else np.asarray(tag, dtype=np.int32))
+        lib().orc_set_atoms(self._h, _dp(xx), _dp(vv), _dp(oo), _dp(rr), _dp(mm), _ip(tt))
+
+    def set_forces(self, f, torque):
+        ff = np.ascontiguousarray(np.asarray(f, dtype=np.float64))
+        tt = np.ascontiguousarray(np.asarray(torque, dtype=np.float64))
+        lib().orc_set_forces(self._h, _dp(ff), _dp(tt))
+
+    def atoms(self) -> dict:
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
+        out["virial"] = np.zeros((n, 6))
+        lib().orc_get_atoms(self._h, *(_dp(out[k]) for k in ("x", "v", "omega", "f", "torque", "virial")))
+        return out
+
+    def set_box(self, lo, hi, periodic=(1, 1, 1)):
+        lo_ = np.asarray(lo, dtype=np.float64)
+        hi_ = np.asarray(hi, dtype=np.float64)
+        per = np.asarray(periodic, dtype=np.int32)
+        lib().orc_set_box(self._h, _dp(lo_), _dp(hi_), _ip(per))
+
+    def set_h_rate(self, h_rate, vremap):
+        h = np.asarray(h_rate, dtype=np.float64)
+        lib().orc_set_h_rate(self._h, _dp(h), int(vremap))
+
+    def box(self):
+        lo, hi, hr = np.zeros(3), np.zeros(3), np.zeros(6)
+        lib().orc_get_box(self._h, _dp(lo), _dp(hi), _dp(hr))
+        return lo, hi, hr
+
+    def set_neigh(self, skin, delay=0, every=1, check=True, max_grid_cells=200000):
+        lib().orc_set_neigh(self._h, float(skin), int(delay), int(every), 1 if check else 0,
+                            int(max_grid_cells))
+
+    def set_pair(self, p: PairParams):
+        self._pair = p
+        lib().orc_set_pair(self._h, C.byref(p))
+
+    def set_servo(self, s: ServoParams):
+        self._servo = s
+        lib().orc_set_servo(self._h, C.byref(s))
+
+    def set_cundall(self, gamma_lin, gamma_ang, enabled=True):
+        lib().orc_set_cundall(self._h, 1 if enabled else 0, float(gamma_lin), float(gamma_ang))
+
+    # stages ----------------------------------------------------------------
+    def run(self, nsteps, dt, run_pre=True) -> int:
+        return lib().orc_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0)
+
+    def init_simulation(self, dt) -> int:
+        return lib().orc_init_simulation(self._h, float(dt))
+
+    def step(self, dt) -> int:
+        return lib().orc_step(self._h, float(dt))
+
+    def servo_setup(self, nsteps_total, dt):
+        lib().orc_servo_setup(self._h, int(nsteps_total), float(dt))
+
+    def nlist_build(self) -> int:
+        return lib().orc_nlist_build(self._h)
+
+    def decide(self, timestep) -> bool:
+        return bool(lib().orc_nlist_decide(self._h, int(timestep)))
+
+    def reneighbor(self, force=False) -> int:
+        return lib().orc_reneighbor(self._h, 1 if force else 0)
+
+    def pbc(self):
+        lib().orc_pbc(self._h)
+
+    def clear_forces(self):
+        lib().orc_clear_forces(self._h)
+
+    def pair_compute(self, dt, shearupdate=True):
+        lib().orc_pair_compute(self._h, float(dt), 1 if shearupdate else 0)
+
+    def nve_initial(self, dt):
+        lib().orc_nve_initial(self._h, float(dt))
+
+    def nve_final(self, dt):
+        lib().orc_nve_final(self._h, float(dt))
+
+    def cundall(self):
+        lib().orc_cundall(self._h)
+
+    def virial(self, keflag=True, mode=0):
+        out = np.zeros(6)
+        lib().orc_virial(self._h, 1 if keflag else 0, int(mode), _dp(out))
+        return out
+
+    def servo_end_of_step(self, dt) -> int:
+        return lib().orc_servo_end_of_step(self._h, float(dt))
+
+    # inspection ------------------------------------------------------------
+    def pairs(self):
+        m = lib().orc_npairs(self._h)
+        pi, pj, pa = (np.zeros(m, dtype=np.int32) for _ in range(3))
+        sh = np.zeros((m, 3))
+        lib().orc_get_pairs(self._h, _ip(pi), _ip(pj), _ip(pa), _dp(sh))
+        return pi, pj, pa, sh
+
+    def set_history(self, partner, shear):
+        pa = np.ascontiguousarray(np.asarray(partner, dtype=np.int32))
+        sh = np.ascontiguousarray(np.asarray(shear, dtype=np.float64))
+        lib().orc_set_history(self._h, _ip(pa), _dp(sh))
+
+    def grid(self):
+        g = np.zeros(3, dtype=np.int32)
+        cs = np.zeros(3)
+        cell_of = np.zeros(self.n, dtype=np.int32)
+        lib().orc_get_grid(self._h, _ip(g), _dp(cs), _ip(cell_of))
+        return g, cs, cell_of
+
+    def counters(self) -> dict:
+        c = np.zeros(6, dtype=np.int32)
+        lib().orc_get_counters(self._h, _ip(c))
+        keys = ("timestep", "build_count", "last_build_step", "npairs", "rebuilds_in_run", "has_built")
+        return {k: int(v) for k, v in zip(keys, c)}
+
+    def last_pressure_tensor(self):
+        out = np.zeros(6)
+        lib().orc_get_tensor(self._h, _dp(out))
+        return out
+
+
+def from_workload(w: dict, max_nb: int = 64) -> OracleSim:
+    """An OracleSim loaded with a synthetic workload dict (see taichimps_amd.workloads)."""
+    o = OracleSim(len(w["x"]), max_nb=max_nb)
+    o.set_atoms(w["x"], w["v"], w["omega"], w["radius"],
+                mass_from_radius(w["radius"], w["density"]), w["tag"])
+    o.set_box(w["boxlo"], w["boxhi"], (1, 1, 1))
+    return o

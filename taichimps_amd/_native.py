@@ -1,0 +1,395 @@
+"""ctypes binding of libtmps_hip.so (the C ABI declared in include/tmps.h).
+
+The library owns all device memory; these wrappers hand it borrowed numpy
+buffers. Status codes become the exceptions the reference raises for the same
+conditions (ValueError for configuration, RuntimeError at run time).
+
+torch is imported before the library is loaded when it is importable, so the
+process holds exactly one HIP runtime (torch ships its own copy under the same
+soname); the library itself does not use torch.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtmps_hip.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_OVERFLOW = -3
+ERR_BOX = -4
+ERR_CAPACITY = -5
+ERR_UNSUPPORTED = -6
+
+FP64 = 0
+FP32_SPLIT = 1
+
+F_X, F_V, F_OMEGA, F_F, F_TORQUE, F_RADIUS, F_RMASS, F_TAG, F_VIRIAL, F_X0 = range(10)
+FIELD_WIDTH = {F_X: 3, F_V: 3, F_OMEGA: 3, F_F: 3, F_TORQUE: 3, F_RADIUS: 1, F_RMASS: 1,
+               F_TAG: 1, F_VIRIAL: 6, F_X0: 3}
+
+PAIR_HOOKE_HISTORY = 0
+PAIR_HERTZ_HISTORY = 1
+PAIR_GRANULAR = 2
+
+# the exported entry points (include/tmps.h)
+SYMBOLS = (
+    "tmps_version", "tmps_last_error", "tmps_device_name", "tmps_create", "tmps_destroy",
+    "tmps_set_nlocal", "tmps_set_field", "tmps_get_field", "tmps_sync", "tmps_set_box",
+    "tmps_get_box", "tmps_pbc", "tmps_clear_forces", "tmps_neigh_config", "tmps_neigh_check",
+    "tmps_neigh_decide", "tmps_neigh_build", "tmps_neigh_info", "tmps_neigh_get_pairs",
+    "tmps_neigh_get_counts", "tmps_neigh_get_bins", "tmps_history_reset", "tmps_history_save",
+    "tmps_history_restore", "tmps_history_set_pairs", "tmps_pair_set", "tmps_pair_compute",
+    "tmps_nve_initial", "tmps_nve_final", "tmps_cundall", "tmps_servo_set", "tmps_servo_setup",
+    "tmps_servo_end_of_step", "tmps_virial_reduce", "tmps_ke", "tmps_run", "tmps_get_counters",
+    "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
+)
+
+
+class PairParams(C.Structure):
+    """tmps_pair_params"""
+
+    _fields_ = [
+        ("style", C.c_int), ("limit_damping", C.c_int),
+        ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+        ("gammat", C.c_double), ("xmu", C.c_double),
+        ("normal_model", C.c_int), ("damping_model", C.c_int), ("tangential_model", C.c_int),
+        ("g_kn", C.c_double), ("g_damp", C.c_double), ("g_kt", C.c_double),
+        ("g_xt", C.c_double), ("g_mu", C.c_double),
+    ]
+
+
+class ServoParams(C.Structure):
+    """tmps_servo_params"""
+
+    _fields_ = [
+        ("enabled", C.c_int), ("nevery", C.c_int), ("couple", C.c_int), ("remap", C.c_int),
+        ("normalize", C.c_int), ("max_rate", C.c_double),
+        ("style", C.c_int * 3),
+        ("rate", C.c_double * 3), ("vel", C.c_double * 3), ("scale", C.c_double * 3),
+        ("flo", C.c_double * 3), ("fhi", C.c_double * 3), ("dlo", C.c_double * 3),
+        ("dhi", C.c_double * 3), ("amp", C.c_double * 3), ("tper", C.c_double * 3),
+        ("ptarget", C.c_double * 3), ("pgain", C.c_double * 3),
+    ]
+
+
+class Counters(C.Structure):
+    """tmps_counters"""
+
+    _fields_ = [(name, C.c_int) for name in (
+        "timestep", "has_built", "build_count", "last_build_step", "current_step", "setup_done",
+        "npairs", "nslots", "ntouch", "rebuilds", "servo_step_count", "servo_nsteps")]
+
+
+_lib = None
+
+
+def _preload_torch():
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
+def load(path: str | None = None):
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: the HIP library has not been built "
+            "(run __graft_entry__.build() or make -C taichimps_amd/csrc)")
+    _preload_torch()
+    L = C.CDLL(p)
+    vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    sig = {
+        "tmps_version": (C.c_char_p, []),
+        "tmps_last_error": (C.c_char_p, []),
+        "tmps_device_name": (C.c_int, [C.c_char_p, C.c_int]),
+        "tmps_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tmps_destroy": (C.c_int, [vp]),
+        "tmps_set_nlocal": (C.c_int, [vp, C.c_int]),
+        "tmps_set_field": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
+        "tmps_get_field": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
+        "tmps_sync": (C.c_int, [vp]),
+        "tmps_set_box": (C.c_int, [vp, dp, dp, ip, dp, C.c_int]),
+        "tmps_get_box": (C.c_int, [vp, dp, dp, dp, ip]),
+        "tmps_pbc": (C.c_int, [vp]),
+        "tmps_clear_forces": (C.c_int, [vp]),
+        "tmps_neigh_config": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "tmps_neigh_check": (C.c_int, [vp, ip]),
+        "tmps_neigh_decide": (C.c_int, [vp, C.c_int, C.c_int, ip]),
+        "tmps_neigh_build": (C.c_int, [vp]),
+        "tmps_neigh_info": (C.c_int, [vp, ip, dp, ip]),
+        "tmps_neigh_get_pairs": (C.c_int, [vp, ip, ip, ip, dp, C.c_int]),
+        "tmps_neigh_get_counts": (C.c_int, [vp, ip, C.c_int]),
+        "tmps_neigh_get_bins": (C.c_int, [vp, ip, ip, ip]),
+        "tmps_history_reset": (C.c_int, [vp]),
+        "tmps_history_save": (C.c_int, [vp]),
+        "tmps_history_restore": (C.c_int, [vp]),
+        "tmps_history_set_pairs": (C.c_int, [vp, ip, dp, C.c_int]),
+        "tmps_pair_set": (C.c_int, [vp, C.POINTER(PairParams)]),
+        "tmps_pair_compute": (C.c_int, [vp, C.c_double, C.c_int, C.c_int]),
+        "tmps_nve_initial": (C.c_int, [vp, C.c_double]),
+        "tmps_nve_final": (C.c_int, [vp, C.c_double]),
+        "tmps_cundall": (C.c_int, [vp, C.c_double, C.c_double]),
+        "tmps_servo_set": (C.c_int, [vp, C.POINTER(ServoParams)]),
+        "tmps_servo_setup": (C.c_int, [vp, C.c_int, C.c_double]),
+        "tmps_servo_end_of_step": (C.c_int, [vp, C.c_double, dp]),
+        "tmps_virial_reduce": (C.c_int, [vp, C.c_int, dp]),
+        "tmps_ke": (C.c_int, [vp, dp, dp]),
+        "tmps_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "tmps_get_counters": (C.c_int, [vp, C.POINTER(Counters)]),
+        "tmps_set_counters": (C.c_int, [vp, C.POINTER(Counters)]),
+        "tmps_last_pressure": (C.c_int, [vp, dp]),
+        "tmps_set_profiling": (C.c_int, [vp, C.c_int]),
+        "tmps_kernel_times": (C.c_int, [vp, dp, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = load().tmps_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int) -> None:
+    """Raise the reference's exception type for a non-zero status."""
+    if rc == OK:
+        return
+    msg = last_error()
+    if rc in (ERR_ARG, ERR_CAPACITY):
+        raise ValueError(msg)
+    if rc == ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class NativeSystem:
+    """One tmps_system: the device-resident state of one particle system."""
+
+    def __init__(self, max_atoms: int, max_neighbors_per_atom: int, precision: int):
+        L = load()
+        h = C.c_void_p()
+        check(L.tmps_create(int(max_atoms), int(max_neighbors_per_atom), int(precision), C.byref(h)))
+        self._h = h
+        self._L = L
+        self.max_atoms = int(max_atoms)
+        self.precision = int(precision)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._L.tmps_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # fields -----------------------------------------------------------------
+    def set_nlocal(self, n):
+        check(self._L.tmps_set_nlocal(self._h, int(n)))
+
+    def set_field(self, field, values, count):
+        width = FIELD_WIDTH[field]
+        dtype = np.int32 if field == F_TAG else np.float64
+        a = np.ascontiguousarray(np.asarray(values, dtype=dtype))
+        if a.size < count * width:
+            raise ValueError("field data shorter than count")
+        check(self._L.tmps_set_field(self._h, field, a.ctypes.data_as(C.c_void_p), int(count)))
+
+    def get_field(self, field, count):
+        width = FIELD_WIDTH[field]
+        dtype = np.int32 if field == F_TAG else np.float64
+        shape = (count,) if width == 1 else (count, width)
+        out = np.zeros(shape, dtype=dtype)
+        check(self._L.tmps_get_field(self._h, field, out.ctypes.data_as(C.c_void_p), int(count)))
+        return out
+
+    def sync(self):
+        check(self._L.tmps_sync(self._h))
+
+    # box --------------------------------------------------------------------
+    def set_box(self, lo, hi, periodic, h_rate=None, vremap=False):
+        lo_ = np.ascontiguousarray(lo, dtype=np.float64)
+        hi_ = np.ascontiguousarray(hi, dtype=np.float64)
+        per = np.ascontiguousarray(periodic, dtype=np.int32)
+        hr = np.ascontiguousarray(np.zeros(6) if h_rate is None else h_rate, dtype=np.float64)
+        check(self._L.tmps_set_box(self._h, _dp(lo_), _dp(hi_), _ip(per), _dp(hr),
+                                   1 if vremap else 0))
+
+    def get_box(self):
+        lo, hi, hr = np.zeros(3), np.zeros(3), np.zeros(6)
+        vr = np.zeros(1, dtype=np.int32)
+        check(self._L.tmps_get_box(self._h, _dp(lo), _dp(hi), _dp(hr), _ip(vr)))
+        return lo, hi, hr, bool(vr[0])
+
+    def pbc(self):
+        check(self._L.tmps_pbc(self._h))
+
+    def clear_forces(self):
+        check(self._L.tmps_clear_forces(self._h))
+
+    # neighbor list ----------------------------------------------------------
+    def neigh_config(self, skin, delay, every, check_, max_grid_cells, max_nb):
+        check(self._L.tmps_neigh_config(self._h, float(skin), int(delay), int(every),
+                                        1 if check_ else 0, int(max_grid_cells), int(max_nb)))
+
+    def neigh_check(self) -> bool:
+        m = np.zeros(1, dtype=np.int32)
+        check(self._L.tmps_neigh_check(self._h, _ip(m)))
+        return bool(m[0])
+
+    def neigh_decide(self, timestep) -> bool:
+        r = np.zeros(1, dtype=np.int32)
+        have = timestep is not None
+        check(self._L.tmps_neigh_decide(self._h, int(timestep or 0), 1 if have else 0, _ip(r)))
+        return bool(r[0])
+
+    def neigh_build(self):
+        check(self._L.tmps_neigh_build(self._h))
+
+    def neigh_info(self):
+        g = np.zeros(3, dtype=np.int32)
+        cs = np.zeros(3)
+        npairs = np.zeros(1, dtype=np.int32)
+        check(self._L.tmps_neigh_info(self._h, _ip(g), _dp(cs), _ip(npairs)))
+        return g, cs, int(npairs[0])
+
+    def neigh_pairs(self):
+        _, _, m = self.neigh_info()
+        pi, pj, pa = (np.zeros(m, dtype=np.int32) for _ in range(3))
+        sh = np.zeros((m, 3))
+        if m:
+            check(self._L.tmps_neigh_get_pairs(self._h, _ip(pi), _ip(pj), _ip(pa), _dp(sh), m))
+        return pi, pj, pa, sh
+
+    def neigh_counts(self, n):
+        out = np.zeros(n, dtype=np.int32)
+        if n:
+            check(self._L.tmps_neigh_get_counts(self._h, _ip(out), int(n)))
+        return out
+
+    def neigh_bins(self, n):
+        g, _, _ = self.neigh_info()
+        ncell = int(g[0] * g[1] * g[2])
+        cell_of = np.zeros(n, dtype=np.int32)
+        start = np.zeros(ncell + 1, dtype=np.int32)
+        parts = np.zeros(n, dtype=np.int32)
+        check(self._L.tmps_neigh_get_bins(self._h, _ip(cell_of), _ip(start), _ip(parts)))
+        return cell_of, start, parts
+
+    # history ----------------------------------------------------------------
+    def history_reset(self):
+        check(self._L.tmps_history_reset(self._h))
+
+    def history_save(self):
+        check(self._L.tmps_history_save(self._h))
+
+    def history_restore(self):
+        check(self._L.tmps_history_restore(self._h))
+
+    def history_set_pairs(self, partner, shear):
+        pa = np.ascontiguousarray(partner, dtype=np.int32)
+        sh = np.ascontiguousarray(shear, dtype=np.float64)
+        check(self._L.tmps_history_set_pairs(self._h, _ip(pa), _dp(sh), int(len(pa))))
+
+    # stages -----------------------------------------------------------------
+    def pair_set(self, p: PairParams):
+        check(self._L.tmps_pair_set(self._h, C.byref(p)))
+
+    def pair_compute(self, dt, shearupdate=True, accumulate=True):
+        check(self._L.tmps_pair_compute(self._h, float(dt), 1 if shearupdate else 0,
+                                        1 if accumulate else 0))
+
+    def nve_initial(self, dt):
+        check(self._L.tmps_nve_initial(self._h, float(dt)))
+
+    def nve_final(self, dt):
+        check(self._L.tmps_nve_final(self._h, float(dt)))
+
+    def cundall(self, gl, ga):
+        check(self._L.tmps_cundall(self._h, float(gl), float(ga)))
+
+    def servo_set(self, p: ServoParams):
+        check(self._L.tmps_servo_set(self._h, C.byref(p)))
+
+    def servo_setup(self, nsteps_total, dt):
+        check(self._L.tmps_servo_setup(self._h, int(nsteps_total), float(dt)))
+
+    def servo_end_of_step(self, dt):
+        t = np.zeros(6)
+        check(self._L.tmps_servo_end_of_step(self._h, float(dt), _dp(t)))
+        return t
+
+    def virial(self, keflag=True):
+        out = np.zeros(6)
+        check(self._L.tmps_virial_reduce(self._h, 1 if keflag else 0, _dp(out)))
+        return out
+
+    def ke(self):
+        a, b = np.zeros(1), np.zeros(1)
+        check(self._L.tmps_ke(self._h, _dp(a), _dp(b)))
+        return float(a[0]), float(b[0])
+
+    def run(self, nsteps, dt, run_pre=True, cundall=None):
+        gl, ga = (0.0, 0.0) if cundall is None else cundall
+        check(self._L.tmps_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0,
+                               0 if cundall is None else 1, float(gl), float(ga)))
+
+    def counters(self) -> dict:
+        c = Counters()
+        check(self._L.tmps_get_counters(self._h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in Counters._fields_}
+
+    def set_counters(self, **kw):
+        c = Counters()
+        check(self._L.tmps_get_counters(self._h, C.byref(c)))
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        check(self._L.tmps_set_counters(self._h, C.byref(c)))
+
+    def last_pressure(self):
+        out = np.zeros(6)
+        check(self._L.tmps_last_pressure(self._h, _dp(out)))
+        return out
+
+    def set_profiling(self, on: bool):
+        check(self._L.tmps_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(3)
+        cnt = np.zeros(3, dtype=np.int32)
+        check(self._L.tmps_kernel_times(self._h, _dp(ms), _ip(cnt)))
+        return ms, cnt
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(load().tmps_device_name(buf, 256))
+    return buf.value.decode()

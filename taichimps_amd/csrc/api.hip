@@ -1,0 +1,1364 @@
+// C ABI of libtmps_hip.so: system lifetime, field transfer, the stage entry
+// points of the DEM path, and the device-resident fused runner.
+//
+// Host arithmetic that feeds the kernels (the bin grid of
+// NeighborList.setup_grid, the strain-style h_rate of FixDeformPressure.setup)
+// is built with -ffp-contract=off like the kernels, so it rounds as the
+// reference's Python does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tmps_internal.h"
+
+namespace {
+thread_local std::string g_err;
+
+System* sys(tmps_system* h) { return reinterpret_cast<System*>(h); }
+}  // namespace
+
+int tmps_fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int tmps_fail_hip(hipError_t e, const char* what, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at api line %d", (int)e, hipGetErrorString(e),
+             what, line);
+    g_err = buf;
+    return TMPS_ERR_HIP;
+}
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void**)p, count * sizeof(T)));
+    HIP_TRY(hipMemset(*p, 0, count * sizeof(T)));
+    // the null-stream memset is not ordered with the system's stream
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int ctl_pull(System& s) {
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(s.hctl, s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ctl_push(System& s) {
+    HIP_TRY(hipMemcpy(s.ctl, s.hctl, sizeof(Ctl), hipMemcpyHostToDevice));
+    return 0;
+}
+
+bool split(const System& s) { return s.precision == TMPS_FP32_SPLIT; }
+
+// NeighborList.setup_grid, neighbor.py:143-189 (orthogonal perpendicular
+// widths, :122-141), evaluated in the reference's order.
+void setup_grid(System& s, const DevBox& B, double max_cutoff) {
+    double cut = max_cutoff + s.skin;
+    if (cut <= 0) cut = 1.0;
+    const double xprd = B.prd[0], yprd = B.prd[1], zprd = B.prd[2];
+    const double vol = xprd * yprd * zprd;
+    const double c0 = yprd * zprd - 0.0 * 0.0;
+    const double ca = zprd * xprd - 0.0 * 0.0;
+    const double c2 = xprd * yprd - 0.0 * 0.0;
+    const double w0 = vol / std::sqrt(c0 * c0);
+    const double w1 = vol / std::sqrt(ca * ca);
+    const double w2 = vol / std::sqrt(c2 * c2);
+    int gx = std::max(1, (int)std::floor(w0 / cut));
+    int gy = std::max(1, (int)std::floor(w1 / cut));
+    int gz = std::max(1, (int)std::floor(w2 / cut));
+    if (B.per[0] == 1 && gx == 2) gx = 1;
+    if (B.per[1] == 1 && gy == 2) gy = 1;
+    if (B.per[2] == 1 && gz == 2) gz = 1;
+    const long long total = (long long)gx * gy * gz;
+    if (total > s.max_grid_cells) {
+        const double factor = std::pow((double)s.max_grid_cells / (double)total, 1.0 / 3.0);
+        gx = std::max(1, (int)(gx * factor));
+        gy = std::max(1, (int)(gy * factor));
+        gz = std::max(1, (int)(gz * factor));
+    }
+    s.grid.g[0] = gx;
+    s.grid.g[1] = gy;
+    s.grid.g[2] = gz;
+    s.grid.csize[0] = 1.0 / gx;
+    s.grid.csize[1] = 1.0 / gy;
+    s.grid.csize[2] = 1.0 / gz;
+    s.grid.ncell = gx * gy * gz;
+}
+
+int ensure_cells(System& s, int ncell) {
+    if (ncell + 1 <= s.cell_cap) return 0;
+    dfree(s.cell_count);
+    dfree(s.cell_start);
+    dfree(s.cell_fill);
+    int cap = ncell + 1;
+    int rc = dalloc(&s.cell_count, cap);
+    if (!rc) rc = dalloc(&s.cell_start, cap);
+    if (!rc) rc = dalloc(&s.cell_fill, cap);
+    if (rc) return rc;
+    s.cell_cap = cap;
+    const int need_scan = (std::max(cap, s.max_atoms + 1) + 1023) / 1024 + 64;
+    if (need_scan > s.scan_tmp_cap) {
+        dfree(s.scan_tmp);
+        rc = dalloc(&s.scan_tmp, need_scan);
+        if (rc) return rc;
+        s.scan_tmp_cap = need_scan;
+    }
+    return 0;
+}
+
+void invalidate_graphs(System& s) { drop_graphs(s); }
+
+double max_radius(const System& s) {
+    if (s.n <= 0) return 1.0;
+    double m = s.h_radius[0];
+    for (int i = 1; i < s.n; ++i) m = std::max(m, s.h_radius[i]);
+    return m;
+}
+
+// FP32-split: bring the split encoding up to date with xr when xr holds the
+// authoritative positions (after a position upload, a PBC wrap or a box set).
+int ensure_split(System& s) {
+    if (!split(s) || !s.x64_valid) return 0;
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    if (s.grid.ncell == 0) {
+        setup_grid(s, s.hctl->box, 2.0 * max_radius(s));
+        rc = ensure_cells(s, s.grid.ncell);
+        if (rc) return rc;
+    }
+    if (s.grid.g[0] > 2047 || s.grid.g[1] > 2047 || s.grid.g[2] > 1023)
+        return tmps_fail(TMPS_ERR_UNSUPPORTED,
+                         "FP32-split encoding holds at most 2047 x 2047 x 1023 bins");
+    for (int k = 0; k < 3; ++k) s.hctl->sc.gi[k] = s.grid.g[k];
+    const bool first = !s.has_built;
+    rc = ctl_push(s);
+    if (rc) return rc;
+    if (first) {
+        rc = launch_store_x0(s);
+        if (!rc) rc = launch_split_consts(s, 1);
+        if (rc) return rc;
+    } else {
+        rc = launch_split_consts(s, 0);
+        if (rc) return rc;
+    }
+    rc = launch_x64_to_split(s);
+    if (rc) return rc;
+    s.x64_valid = false;
+    return 0;
+}
+
+// FP32-split: make xr hold the current positions (FP64) without changing
+// which representation is authoritative.
+int split_positions_to_xr(System& s) {
+    if (!split(s) || s.x64_valid) return 0;
+    return launch_split_to_x64(s);
+}
+
+// ------------------------------------------------------------ neighbor build
+
+int neigh_build(System& s) {
+    if (s.n <= 0) return 0;
+    int rc = 0;
+    if (split(s)) {
+        rc = split_positions_to_xr(s);
+        if (rc) return rc;
+        s.x64_valid = true;
+    }
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    const DevBox B = s.hctl->box;
+    setup_grid(s, B, 2.0 * max_radius(s));
+    rc = ensure_cells(s, s.grid.ncell);
+    if (rc) return rc;
+    rc = launch_bins(s, s.xr);
+    if (rc) return rc;
+    int overflow = 0, total = 0;
+    rc = launch_list(s, s.xr, &overflow, &total);
+    if (rc) return rc;
+    if (overflow) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "Neighbor list overflow: more than %d neighbors for at least one particle. "
+                 "Increase max_neighbors_per_atom, or reduce the neighbor skin.",
+                 s.max_nb);
+        return tmps_fail(TMPS_ERR_OVERFLOW, buf);
+    }
+    rc = launch_hist_clear(s);
+    if (rc) return rc;
+    rc = launch_store_x0(s);
+    if (rc) return rc;
+    s.npairs = total / 2;
+    s.has_built = 1;
+    s.build_count += 1;
+    s.last_build_step = s.current_step;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    s.hctl->last_build_step = s.last_build_step;
+    if (split(s)) {
+        if (s.grid.g[0] > 2047 || s.grid.g[1] > 2047 || s.grid.g[2] > 1023)
+            return tmps_fail(TMPS_ERR_UNSUPPORTED,
+                             "FP32-split encoding holds at most 2047 x 2047 x 1023 bins");
+        for (int k = 0; k < 3; ++k) s.hctl->sc.gi[k] = s.grid.g[k];
+    }
+    rc = ctl_push(s);
+    if (rc) return rc;
+    if (split(s)) {
+        rc = launch_split_consts(s, 1);
+        if (!rc) rc = launch_x64_to_split(s);
+        if (rc) return rc;
+        s.x64_valid = false;
+    }
+    return 0;
+}
+
+int history_save(System& s) {
+    // keep a copy of the current list: the restore merges the new list with it
+    const size_t need = (size_t)std::max(s.nslots, 0);
+    const size_t hsz = split(s) ? sizeof(float4) : sizeof(double4);
+    if (need > s.onb_cap || s.onb_j == nullptr) {
+        dfree(s.onb_j);
+        dfree(s.onb_flag);
+        if (s.onb_hist) (void)hipFree(s.onb_hist);
+        s.onb_hist = nullptr;
+        const size_t cap = need + need / 4 + 1024;
+        int rc = dalloc(&s.onb_j, cap);
+        if (!rc) rc = dalloc(&s.onb_flag, cap);
+        if (rc) return rc;
+        HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
+        s.onb_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(s.onb_start, s.nb_start, sizeof(int) * (s.n + 1), hipMemcpyDeviceToDevice,
+                           s.stream));
+    if (need > 0) {
+        HIP_TRY(hipMemcpyAsync(s.onb_j, s.nb_j, sizeof(int) * need, hipMemcpyDeviceToDevice, s.stream));
+        HIP_TRY(hipMemcpyAsync(s.onb_flag, s.nb_flag, need, hipMemcpyDeviceToDevice, s.stream));
+        HIP_TRY(hipMemcpyAsync(s.onb_hist, s.nb_hist, need * hsz, hipMemcpyDeviceToDevice, s.stream));
+    }
+    s.onslots = (int)need;
+    s.onb_n = s.n;
+    s.have_old = true;
+    return 0;
+}
+
+int history_restore(System& s) {
+    if (!s.have_old) return 0;
+    int rc = launch_hist_restore(s);
+    s.have_old = false;
+    return rc;
+}
+
+int pbc(System& s) {
+    if (s.n <= 0) return 0;
+    if (split(s)) {
+        int rc = ctl_pull(s);
+        if (rc) return rc;
+        if (s.hctl->box.vremap)
+            return tmps_fail(TMPS_ERR_UNSUPPORTED, "remap v is not supported with FP32-split");
+        rc = split_positions_to_xr(s);
+        if (rc) return rc;
+        s.x64_valid = true;
+    }
+    return launch_pbc(s);
+}
+
+int neigh_check(System& s, int* moved) {
+    if (s.n <= 0) {
+        *moved = 0;
+        return 0;
+    }
+    int rc = ensure_split(s);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    s.hctl->moved = 0;
+    s.hctl->trigger_sq = 0.25 * s.skin * s.skin;
+    rc = ctl_push(s);
+    if (rc) return rc;
+    rc = split(s) ? launch_check_split(s) : launch_check_f64(s);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    *moved = s.hctl->moved;
+    return 0;
+}
+
+// NeighborList.decide, neighbor.py:371-403
+int neigh_decide(System& s, int timestep, int have_timestep, int* rebuild) {
+    *rebuild = 0;
+    if (!s.has_built) {
+        *rebuild = 1;
+        return 0;
+    }
+    if (have_timestep) s.current_step = timestep;
+    else s.current_step += 1;
+    const int since = s.current_step - s.last_build_step;
+    if (since < s.delay) return 0;
+    if (since % s.every != 0) return 0;
+    if (!s.check) {
+        *rebuild = 1;
+        return 0;
+    }
+    if (s.n == 0) return 0;
+    int moved = 0;
+    int rc = neigh_check(s, &moved);
+    if (rc) return rc;
+    *rebuild = moved;
+    return 0;
+}
+
+// Simulation.reneighbor_if_needed, simulation.py:122-159
+int reneighbor(System& s, bool force, int* rebuilt) {
+    *rebuilt = 0;
+    int dec = 0;
+    int rc = neigh_decide(s, s.timestep, 1, &dec);
+    if (rc) return rc;
+    if (!dec && !force) return 0;
+    if (s.has_built) {
+        rc = history_save(s);
+        if (rc) return rc;
+    }
+    rc = pbc(s);
+    if (!rc) rc = neigh_build(s);
+    if (!rc) rc = history_restore(s);
+    if (rc) return rc;
+    *rebuilt = 1;
+    return 0;
+}
+
+int clear_forces(System& s) { return split(s) ? launch_clear_split(s) : launch_clear_f64(s); }
+
+int pair_compute(System& s, double dt, int shearupdate, int accumulate) {
+    if (!s.have_pair) return tmps_fail(TMPS_ERR_ARG, "no pair style set");
+    if (s.n <= 0) return 0;
+    int rc = ensure_split(s);
+    if (rc) return rc;
+    return split(s) ? launch_pair_split(s, dt, shearupdate, accumulate)
+                    : launch_pair_f64(s, dt, shearupdate, accumulate);
+}
+
+int cundall(System& s, double gl, double ga) {
+    return split(s) ? launch_cundall_split(s, gl, ga) : launch_cundall_f64(s, gl, ga);
+}
+
+// FixDeformPressure._strain_h_rate, deform_pressure.py:447-472 (x/y/z)
+void strain_h_rate(const tmps_servo_params& S, const DevServo& SV, double dt, double h[6]) {
+    for (int k = 0; k < 6; ++k) h[k] = 0.0;
+    const double delt = SV.nsteps_total * dt;
+    for (int i = 0; i < 3; ++i) {
+        const double length = SV.hi_start[i] - SV.lo_start[i];
+        const int st = S.style[i];
+        if (st == 1 || st == 2) h[i] = S.rate[i] * length;
+        else if (st == 3) h[i] = S.vel[i];
+        else if (st == 4) h[i] = 2.0 * M_PI / S.tper[i] * S.amp[i];
+        else if ((st == 5 || st == 6 || st == 7) && delt > 0.0) {
+            double stop;
+            if (st == 5) stop = S.fhi[i] - S.flo[i];
+            else if (st == 6) stop = length + S.dhi[i] - S.dlo[i];
+            else stop = S.scale[i] * length;
+            h[i] = (stop - length) / delt;
+        }
+    }
+}
+
+int servo_end_of_step(System& s, double dt, double* tensor_out) {
+    if (!s.have_servo || !s.servo.enabled) return 0;
+    int rc = ensure_split(s);
+    if (rc) return rc;
+    if (split(s)) rc = launch_leaves_split(s, 1);
+    else rc = launch_leaves_f64(s, 1);
+    if (rc) return rc;
+    rc = launch_step_end(s, dt, nblocks_for(std::max(s.n, 1)), 0, 1);
+    if (rc) return rc;
+    if (!split(s)) {
+        rc = launch_remap_apply(s);
+        if (rc) return rc;
+    }
+    rc = launch_remap_clear(s);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    s.servo_step_count = s.hctl->servo.step_count;
+    s.servo_nsteps = s.hctl->servo.nsteps;
+    if (tensor_out)
+        for (int k = 0; k < 6; ++k) tensor_out[k] = s.hctl->tensor[k];
+    if (s.hctl->error) {
+        const int e = s.hctl->error;
+        s.hctl->error = 0;
+        s.hctl->abort = 0;
+        rc = ctl_push(s);
+        if (rc) return rc;
+        char buf[256];
+        const DevBox& B = s.hctl->box;
+        snprintf(buf, sizeof buf,
+                 "fix deform/pressure drove a box dimension to non-positive (box [%g %g %g]); "
+                 "check the gain, max/rate or strain rate",
+                 B.prd[0], B.prd[1], B.prd[2]);
+        return tmps_fail(e, buf);
+    }
+    return 0;
+}
+
+// Verlet::setup as Simulation.init_simulation does it, simulation.py:170-203
+int init_simulation(System& s, double dt, int use_cundall, double gl, double ga) {
+    s.setup_done = 1;
+    int rc = clear_forces(s);
+    if (rc) return rc;
+    int rebuilt = 0;
+    rc = reneighbor(s, true, &rebuilt);
+    if (rc) return rc;
+    s.rebuilds += rebuilt;
+    rc = pair_compute(s, dt, 0, 1);
+    if (rc) return rc;
+    if (use_cundall) rc = cundall(s, gl, ga);
+    return rc;
+}
+
+uint64_t mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+    return h;
+}
+
+uint64_t dbits(double d) {
+    uint64_t u;
+    std::memcpy(&u, &d, sizeof u);
+    return u;
+}
+
+// One fused step on v/omega buffer `buf`: the force pass writes the other
+// buffer. Returns the buffer that holds v/omega afterwards.
+int step_kernels(System& s, int buf, int kick, int fin, double dt, int use_cundall, double gl,
+                 double ga, int nparts, int servo_on, int* out_buf) {
+    int rc = split(s) ? launch_step_begin_split(s, dt, kick, buf)
+                      : launch_step_begin_f64(s, dt, kick, buf);
+    if (rc) return rc;
+    rc = split(s) ? launch_step_force_split(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf)
+                  : launch_step_force_f64(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf);
+    if (rc) return rc;
+    rc = launch_step_end(s, dt, nparts, 1, servo_on);
+    *out_buf = 1 - buf;
+    return rc;
+}
+
+// The end of a segment: v/omega back into buffer 0, the last step's remap.
+int segment_tail(System& s, int buf) {
+    int rc = 0;
+    if (buf == 1) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
+    if (rc) return rc;
+    if (!split(s)) {
+        rc = launch_remap_apply_fused(s);
+        if (rc) return rc;
+    }
+    return launch_remap_clear(s);
+}
+
+// Enqueue the kernels of `len` fused steps (no synchronisation).
+int enqueue_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    int buf = 0;
+    for (int k = 0; k < len; ++k) {
+        int rc = step_kernels(s, buf, k == 0, k == len - 1, dt, use_cundall, gl, ga, nparts,
+                              servo_on, &buf);
+        if (rc) return rc;
+    }
+    return segment_tail(s, buf);
+}
+
+// Same as enqueue_steps, timing every kernel with HIP events.
+int profiled_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    std::vector<hipEvent_t> ev(3 * len + 1);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(ev[0], s.stream));
+    int rc = 0, buf = 0;
+    for (int k = 0; k < len && !rc; ++k) {
+        const int kick = k == 0 ? 1 : 0;
+        const int fin = k == len - 1 ? 1 : 0;
+        rc = split(s) ? launch_step_begin_split(s, dt, kick, buf)
+                      : launch_step_begin_f64(s, dt, kick, buf);
+        if (!rc) rc = hipEventRecord(ev[3 * k + 1], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+        if (!rc)
+            rc = split(s) ? launch_step_force_split(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf)
+                          : launch_step_force_f64(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf);
+        if (!rc) rc = hipEventRecord(ev[3 * k + 2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+        if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
+        if (!rc) rc = hipEventRecord(ev[3 * k + 3], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+        buf = 1 - buf;
+    }
+    if (!rc) rc = segment_tail(s, buf);
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (!rc) {
+        for (int k = 0; k < len; ++k)
+            for (int c = 0; c < 3; ++c) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, ev[3 * k + c], ev[3 * k + c + 1]));
+                s.prof_ms[c] += ms;
+                s.prof_cnt[c] += 1;
+            }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int launch_segment(System& s, int len, double dt, int use_cundall, double gl, double ga) {
+    if (s.profiling) return profiled_steps(s, len, dt, use_cundall, gl, ga);
+    uint64_t key = 1469598103934665603ULL;
+    key = mix(key, (uint64_t)len);
+    key = mix(key, dbits(dt));
+    key = mix(key, (uint64_t)use_cundall);
+    key = mix(key, dbits(gl));
+    key = mix(key, dbits(ga));
+    key = mix(key, (uint64_t)s.pair.style);
+    key = mix(key, (uint64_t)s.n);
+    key = mix(key, (uint64_t)(s.have_servo && s.servo.enabled));
+    key = mix(key, (uint64_t)(uintptr_t)s.nb_j);
+    key = mix(key, (uint64_t)(uintptr_t)s.nb_hist);
+    int slot = -1;
+    for (int k = 0; k < System::kGraphSlots; ++k)
+        if (s.seg_exec[k] && s.seg_key[k] == key) slot = k;
+    if (slot < 0) {
+        slot = s.seg_next;
+        s.seg_next = (s.seg_next + 1) % System::kGraphSlots;
+        if (s.seg_exec[slot]) {
+            // the slot's graph may still be in flight
+            HIP_TRY(hipStreamSynchronize(s.stream));
+            (void)hipGraphExecDestroy(s.seg_exec[slot]);
+            s.seg_exec[slot] = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_steps(s, len, dt, use_cundall, gl, ga);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s.stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return tmps_fail_hip(e, "hipStreamEndCapture", __LINE__);
+        e = hipGraphInstantiate(&s.seg_exec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return tmps_fail_hip(e, "hipGraphInstantiate", __LINE__);
+        s.seg_key[slot] = key;
+    }
+    HIP_TRY(hipGraphLaunch(s.seg_exec[slot], s.stream));
+    return 0;
+}
+
+int check_run_error(System& s) {
+    if (!s.hctl->error) return 0;
+    const int e = s.hctl->error;
+    s.hctl->error = 0;
+    s.hctl->abort = 0;
+    int rc = ctl_push(s);
+    if (rc) return rc;
+    return tmps_fail(e,
+                     "fix deform/pressure drove a box dimension to non-positive; check the gain, "
+                     "max/rate or strain rate");
+}
+
+// Simulation.run(steps) with nve/sphere + pair [+ damping/cundall]
+// [+ deform/pressure], device-resident.
+int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double gl, double ga) {
+    if (!s.have_pair) return tmps_fail(TMPS_ERR_ARG, "tmps_run needs a pair style");
+    if (nsteps < 0) return tmps_fail(TMPS_ERR_ARG, "negative step count");
+    s.rebuilds = 0;
+    int rc = 0;
+    if (run_pre || !s.setup_done) {
+        rc = init_simulation(s, dt, use_cundall, gl, ga);
+        if (rc) return rc;
+    }
+    if (nsteps == 0 || s.n == 0) {
+        s.timestep += nsteps;
+        return 0;
+    }
+    rc = ensure_split(s);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    Ctl& H = *s.hctl;
+    H.timestep = s.timestep;
+    H.trigger = -1;
+    H.abort = 0;
+    H.error = 0;
+    H.last_build_step = s.last_build_step;
+    H.delay = s.delay;
+    H.every = s.every;
+    H.check = s.check;
+    H.trigger_sq = 0.25 * s.skin * s.skin;
+    H.remap.step = -1;
+    rc = ctl_push(s);
+    if (rc) return rc;
+    if (split(s)) {
+        rc = launch_split_consts(s, 0);
+        if (rc) return rc;
+    }
+    const int t_end = s.timestep + nsteps;
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    std::vector<int> seg_start;
+    while (s.timestep < t_end) {
+        seg_start.clear();
+        int t = s.timestep;
+        while (t < t_end) {
+            const int len = std::min(TMPS_MAX_SEG, t_end - t);
+            rc = launch_segment(s, len, dt, use_cundall, gl, ga);
+            if (rc) return rc;
+            seg_start.push_back(t);
+            t += len;
+        }
+        rc = ctl_pull(s);
+        if (rc) return rc;
+        rc = check_run_error(s);
+        if (rc) return rc;
+        s.timestep = s.hctl->timestep;
+        if (s.has_built) s.current_step = s.timestep;
+        if (!s.hctl->abort) continue;
+        // The skin check of step st asked for a rebuild: its drift is done,
+        // its force pass is not. Steps of a segment alternate v/omega buffers
+        // starting from buffer 0; bring the current one back to buffer 0.
+        const int st = s.hctl->timestep + 1;
+        int t0 = seg_start.empty() ? st : seg_start[0];
+        for (int a : seg_start)
+            if (a <= st - 1) t0 = a;
+        if (((st - 1) - t0) % 2 == 1) {
+            rc = split(s) ? launch_copy_vw_split(s, 0) : launch_copy_vw(s, 0);
+            if (rc) return rc;
+        }
+        s.current_step = st;
+        rc = history_save(s);
+        if (!rc) rc = pbc(s);
+        if (!rc) rc = neigh_build(s);
+        if (!rc) rc = history_restore(s);
+        if (!rc) rc = ensure_split(s);
+        if (rc) return rc;
+        s.rebuilds += 1;
+        rc = launch_ctl_rebuilt(s, st);
+        if (rc) return rc;
+        if (split(s)) {
+            rc = launch_split_consts(s, 0);
+            if (rc) return rc;
+        }
+        rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
+                      : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
+        if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
+        if (!rc) rc = segment_tail(s, 1);
+        if (rc) return rc;
+        rc = ctl_pull(s);
+        if (rc) return rc;
+        rc = check_run_error(s);
+        if (rc) return rc;
+        s.timestep = s.hctl->timestep;
+        s.current_step = s.timestep;
+    }
+    s.servo_step_count = s.hctl->servo.step_count;
+    s.servo_nsteps = s.hctl->servo.nsteps;
+    return 0;
+}
+
+// --------------------------------------------------------------- field I/O
+
+int set_field(System& s, int field, const void* host, int count) {
+    if (count < 0 || count > s.max_atoms) return tmps_fail(TMPS_ERR_CAPACITY, "field count exceeds max_atoms");
+    const int n = count;
+    if (n == 0) return 0;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    const double* hd = static_cast<const double*>(host);
+    if (field == TMPS_TAG) {
+        HIP_TRY(hipMemcpy(s.tag, host, sizeof(int) * n, hipMemcpyHostToDevice));
+        return 0;
+    }
+    if (field == TMPS_VIRIAL) {
+        HIP_TRY(hipMemcpy(s.vir, host, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+        return 0;
+    }
+    if (field == TMPS_X0) return tmps_fail(TMPS_ERR_ARG, "x0 is read-only");
+    if (!split(s)) {
+        std::vector<double4> buf(n);
+        double4* dst = nullptr;
+        switch (field) {
+            case TMPS_X: dst = s.xr; break;
+            case TMPS_V: dst = s.vm; break;
+            case TMPS_OMEGA: dst = s.om; break;
+            case TMPS_F: dst = s.f; break;
+            case TMPS_TORQUE: dst = s.tq; break;
+            case TMPS_RADIUS: dst = s.xr; break;
+            case TMPS_RMASS: dst = s.vm; break;
+            default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+        }
+        HIP_TRY(hipMemcpy(buf.data(), dst, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            if (field == TMPS_RADIUS) {
+                buf[i].w = hd[i];
+                s.h_radius[i] = hd[i];
+            } else if (field == TMPS_RMASS) {
+                buf[i].w = hd[i];
+            } else {
+                buf[i].x = hd[3 * i];
+                buf[i].y = hd[3 * i + 1];
+                buf[i].z = hd[3 * i + 2];
+            }
+        }
+        HIP_TRY(hipMemcpy(dst, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        return 0;
+    }
+    // FP32-split
+    if (field == TMPS_X) {
+        std::vector<double4> buf(n);
+        for (int i = 0; i < n; ++i)
+            buf[i] = make_double4(hd[3 * i], hd[3 * i + 1], hd[3 * i + 2], s.h_radius[i]);
+        if (!s.x64_valid && s.n > n) {
+            // keep the atoms beyond `count` where they are
+            int rc = launch_split_to_x64(s);
+            if (rc) return rc;
+            HIP_TRY(hipStreamSynchronize(s.stream));
+        }
+        HIP_TRY(hipMemcpy(s.xr, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        s.x64_valid = true;
+        return 0;
+    }
+    std::vector<float4> buf(n);
+    float4* dst = nullptr;
+    switch (field) {
+        case TMPS_V: dst = s.vmf; break;
+        case TMPS_OMEGA: dst = s.omf; break;
+        case TMPS_F: dst = s.ff; break;
+        case TMPS_TORQUE: dst = s.tqf; break;
+        case TMPS_RADIUS: dst = s.up; break;
+        case TMPS_RMASS: dst = s.vmf; break;
+        default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+    }
+    HIP_TRY(hipMemcpy(buf.data(), dst, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        if (field == TMPS_RADIUS) {
+            buf[i].w = (float)hd[i];
+            s.h_radius[i] = hd[i];
+        } else if (field == TMPS_RMASS) {
+            buf[i].w = (float)hd[i];
+        } else {
+            buf[i].x = (float)hd[3 * i];
+            buf[i].y = (float)hd[3 * i + 1];
+            buf[i].z = (float)hd[3 * i + 2];
+        }
+    }
+    HIP_TRY(hipMemcpy(dst, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    if (field == TMPS_RADIUS) {
+        // the FP64 scratch positions carry the radius too
+        std::vector<double4> xb(n);
+        HIP_TRY(hipMemcpy(xb.data(), s.xr, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) xb[i].w = hd[i];
+        HIP_TRY(hipMemcpy(s.xr, xb.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    }
+    if (field == TMPS_RADIUS || field == TMPS_RMASS) {
+        // omega.w = 1 / (0.4 m r^2)
+        std::vector<float4> om(n), vm(n);
+        HIP_TRY(hipMemcpy(om.data(), s.omf, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(vm.data(), s.vmf, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            const double r = s.h_radius[i], m = vm[i].w;
+            om[i].w = (m > 0 && r > 0) ? (float)(1.0 / (0.4 * m * r * r)) : 0.f;
+        }
+        HIP_TRY(hipMemcpy(s.omf, om.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int get_field(System& s, int field, void* host, int count) {
+    if (count < 0 || count > s.max_atoms) return tmps_fail(TMPS_ERR_CAPACITY, "field count exceeds max_atoms");
+    const int n = count;
+    if (n == 0) return 0;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    double* hd = static_cast<double*>(host);
+    if (field == TMPS_TAG) {
+        HIP_TRY(hipMemcpy(host, s.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if (field == TMPS_VIRIAL) {
+        HIP_TRY(hipMemcpy(host, s.vir, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if (field == TMPS_RADIUS) {
+        for (int i = 0; i < n; ++i) hd[i] = s.h_radius[i];
+        return 0;
+    }
+    if (!split(s) || field == TMPS_X || field == TMPS_X0) {
+        double4* src = nullptr;
+        if (split(s) && field == TMPS_X) {
+            int rc = split_positions_to_xr(s);
+            if (rc) return rc;
+            HIP_TRY(hipStreamSynchronize(s.stream));
+        }
+        switch (field) {
+            case TMPS_X: src = s.xr; break;
+            case TMPS_V: src = s.vm; break;
+            case TMPS_OMEGA: src = s.om; break;
+            case TMPS_F: src = s.f; break;
+            case TMPS_TORQUE: src = s.tq; break;
+            case TMPS_RMASS: src = s.vm; break;
+            case TMPS_X0: src = s.x0; break;
+            default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+        }
+        std::vector<double4> buf(n);
+        HIP_TRY(hipMemcpy(buf.data(), src, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            if (field == TMPS_RMASS) {
+                hd[i] = buf[i].w;
+            } else {
+                hd[3 * i] = buf[i].x;
+                hd[3 * i + 1] = buf[i].y;
+                hd[3 * i + 2] = buf[i].z;
+            }
+        }
+        return 0;
+    }
+    float4* src = nullptr;
+    switch (field) {
+        case TMPS_V: src = s.vmf; break;
+        case TMPS_OMEGA: src = s.omf; break;
+        case TMPS_F: src = s.ff; break;
+        case TMPS_TORQUE: src = s.tqf; break;
+        case TMPS_RMASS: src = s.vmf; break;
+        default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+    }
+    std::vector<float4> buf(n);
+    HIP_TRY(hipMemcpy(buf.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        if (field == TMPS_RMASS) {
+            hd[i] = buf[i].w;
+        } else {
+            hd[3 * i] = buf[i].x;
+            hd[3 * i + 1] = buf[i].y;
+            hd[3 * i + 2] = buf[i].z;
+        }
+    }
+    return 0;
+}
+
+void free_all(System& s) {
+    invalidate_graphs(s);
+    dfree(s.xr); dfree(s.vm); dfree(s.om); dfree(s.f); dfree(s.tq); dfree(s.x0); dfree(s.vir);
+    dfree(s.vm2); dfree(s.om2); dfree(s.vmf2); dfree(s.omf2);
+    dfree(s.tag);
+    dfree(s.up); dfree(s.vmf); dfree(s.omf); dfree(s.ff); dfree(s.tqf); dfree(s.u0); dfree(s.cell);
+    dfree(s.nb_start); dfree(s.nb_j); dfree(s.nb_flag);
+    dfree(s.onb_start); dfree(s.onb_j); dfree(s.onb_flag);
+    if (s.nb_hist) (void)hipFree(s.nb_hist);
+    if (s.onb_hist) (void)hipFree(s.onb_hist);
+    s.nb_hist = s.onb_hist = nullptr;
+    dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
+    dfree(s.cell_particles);
+    dfree(s.full_count); dfree(s.half_count); dfree(s.scan_tmp); dfree(s.flags);
+    dfree(s.partial);
+    dfree(s.ctl);
+    if (s.hctl) (void)hipHostFree(s.hctl);
+    s.hctl = nullptr;
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+}
+
+int create(System& s, int max_atoms, int max_nb, int precision) {
+    s.max_atoms = max_atoms;
+    s.max_nb = max_nb;
+    s.precision = precision;
+    HIP_TRY(hipGetDevice(&s.device));
+    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    const size_t na = (size_t)max_atoms;
+    int rc = 0;
+    rc = rc ? rc : dalloc(&s.xr, na);
+    rc = rc ? rc : dalloc(&s.x0, na);
+    rc = rc ? rc : dalloc(&s.vir, 6 * na);
+    rc = rc ? rc : dalloc(&s.tag, na);
+    if (precision == TMPS_FP64) {
+        rc = rc ? rc : dalloc(&s.vm, na);
+        rc = rc ? rc : dalloc(&s.om, na);
+        rc = rc ? rc : dalloc(&s.vm2, na);
+        rc = rc ? rc : dalloc(&s.om2, na);
+        rc = rc ? rc : dalloc(&s.f, na);
+        rc = rc ? rc : dalloc(&s.tq, na);
+    } else {
+        rc = rc ? rc : dalloc(&s.up, na);
+        rc = rc ? rc : dalloc(&s.vmf, na);
+        rc = rc ? rc : dalloc(&s.omf, na);
+        rc = rc ? rc : dalloc(&s.vmf2, na);
+        rc = rc ? rc : dalloc(&s.omf2, na);
+        rc = rc ? rc : dalloc(&s.ff, na);
+        rc = rc ? rc : dalloc(&s.tqf, na);
+        rc = rc ? rc : dalloc(&s.u0, na);
+        rc = rc ? rc : dalloc(&s.cell, na);
+    }
+    rc = rc ? rc : dalloc(&s.nb_start, na + 1);
+    rc = rc ? rc : dalloc(&s.onb_start, na + 1);
+    rc = rc ? rc : dalloc(&s.cell_of, na);
+    rc = rc ? rc : dalloc(&s.cell_particles, na);
+    rc = rc ? rc : dalloc(&s.full_count, na + 1);
+    rc = rc ? rc : dalloc(&s.half_count, na + 1);
+    rc = rc ? rc : dalloc(&s.flags, 4);
+    const int nparts = nblocks_for(std::max(max_atoms, 1)) + 1;
+    rc = rc ? rc : dalloc(&s.partial, (size_t)nparts * 8);
+    s.partial_cap = nparts;
+    rc = rc ? rc : dalloc(&s.ctl, 1);
+    if (rc) return rc;
+    HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));
+    std::memset(s.hctl, 0, sizeof(Ctl));
+    s.hctl->remap.step = -1;
+    s.hctl->trigger = -1;
+    s.hctl->last_build_step = -1;
+    s.hctl->every = 1;
+    s.hctl->check = 1;
+    for (int k = 0; k < 3; ++k) {
+        s.hctl->box.hi[k] = 1.0;
+        s.hctl->box.prd[k] = 1.0;
+        s.hctl->box.per[k] = 1;
+        s.hctl->sc.gi[k] = 1;
+        s.hctl->prd_build[k] = 1.0;
+    }
+    s.hctl->servo.p.nevery = 1;
+    rc = ctl_push(s);
+    if (rc) return rc;
+    rc = ensure_cells(s, 1);
+    if (rc) return rc;
+    s.h_radius.assign(na, 0.0);
+    return 0;
+}
+
+}  // namespace
+
+void drop_graphs(System& s) {
+    bool any = false;
+    for (int k = 0; k < System::kGraphSlots; ++k) any = any || s.seg_exec[k] != nullptr;
+    if (!any) return;
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (int k = 0; k < System::kGraphSlots; ++k) {
+        if (s.seg_exec[k]) (void)hipGraphExecDestroy(s.seg_exec[k]);
+        s.seg_exec[k] = nullptr;
+        s.seg_key[k] = 0;
+    }
+    s.seg_next = 0;
+}
+
+// ================================================================== C ABI
+
+extern "C" {
+
+const char* tmps_version(void) { return "taichimps_amd 0.1 (gfx950)"; }
+
+const char* tmps_last_error(void) { return g_err.c_str(); }
+
+int tmps_device_name(char* buf, int len) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+int tmps_create(int max_atoms, int max_neighbors_per_atom, int precision, tmps_system** out) {
+    if (!out) return tmps_fail(TMPS_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (max_atoms <= 0 || max_neighbors_per_atom <= 0)
+        return tmps_fail(TMPS_ERR_ARG, "max_atoms and max_neighbors_per_atom must be positive");
+    if (precision != TMPS_FP64 && precision != TMPS_FP32_SPLIT)
+        return tmps_fail(TMPS_ERR_ARG, "unknown precision");
+    System* s = new System();
+    int rc = create(*s, max_atoms, max_neighbors_per_atom, precision);
+    if (rc) {
+        free_all(*s);
+        delete s;
+        return rc;
+    }
+    *out = reinterpret_cast<tmps_system*>(s);
+    return 0;
+}
+
+int tmps_destroy(tmps_system* h) {
+    if (!h) return 0;
+    System* s = sys(h);
+    (void)hipStreamSynchronize(s->stream);
+    free_all(*s);
+    delete s;
+    return 0;
+}
+
+int tmps_set_nlocal(tmps_system* h, int nlocal) {
+    System& s = *sys(h);
+    if (nlocal < 0 || nlocal > s.max_atoms)
+        return tmps_fail(TMPS_ERR_CAPACITY, "Exceeded max_atoms capacity");
+    if (nlocal != s.n) invalidate_graphs(s);
+    s.n = nlocal;
+    return 0;
+}
+
+int tmps_set_field(tmps_system* h, int field, const void* host, int count) {
+    return set_field(*sys(h), field, host, count);
+}
+
+int tmps_get_field(tmps_system* h, int field, void* host, int count) {
+    return get_field(*sys(h), field, host, count);
+}
+
+int tmps_sync(tmps_system* h) {
+    HIP_TRY(hipStreamSynchronize(sys(h)->stream));
+    return 0;
+}
+
+int tmps_set_box(tmps_system* h, const double lo[3], const double hi[3], const int periodic[3],
+                 const double h_rate[6], int vremap) {
+    System& s = *sys(h);
+    if (split(s)) {
+        // the atoms stay where they are while the box changes under them
+        int rc = split_positions_to_xr(s);
+        if (rc) return rc;
+        s.x64_valid = true;
+    }
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    DevBox& B = s.hctl->box;
+    for (int k = 0; k < 3; ++k) {
+        B.lo[k] = lo[k];
+        B.hi[k] = hi[k];
+        B.prd[k] = B.hi[k] - B.lo[k];
+        B.per[k] = periodic[k];
+    }
+    for (int k = 0; k < 6; ++k) B.h_rate[k] = h_rate ? h_rate[k] : 0.0;
+    B.vremap = vremap;
+    return ctl_push(s);
+}
+
+int tmps_get_box(tmps_system* h, double lo[3], double hi[3], double h_rate[6], int* vremap) {
+    System& s = *sys(h);
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    const DevBox& B = s.hctl->box;
+    for (int k = 0; k < 3; ++k) {
+        if (lo) lo[k] = B.lo[k];
+        if (hi) hi[k] = B.hi[k];
+    }
+    if (h_rate)
+        for (int k = 0; k < 6; ++k) h_rate[k] = B.h_rate[k];
+    if (vremap) *vremap = B.vremap;
+    return 0;
+}
+
+int tmps_pbc(tmps_system* h) { return pbc(*sys(h)); }
+
+int tmps_clear_forces(tmps_system* h) { return clear_forces(*sys(h)); }
+
+int tmps_neigh_config(tmps_system* h, double skin, int delay, int every, int check,
+                      int max_grid_cells, int max_neighbors_per_atom) {
+    System& s = *sys(h);
+    if (every < 1) return tmps_fail(TMPS_ERR_ARG, "neigh_modify every must be >= 1");
+    if (max_grid_cells < 1) return tmps_fail(TMPS_ERR_ARG, "max_grid_cells must be >= 1");
+    if (max_neighbors_per_atom < 1)
+        return tmps_fail(TMPS_ERR_ARG, "max_neighbors_per_atom must be >= 1");
+    s.max_nb = max_neighbors_per_atom;
+    s.skin = skin;
+    s.delay = delay;
+    s.every = every;
+    s.check = check ? 1 : 0;
+    s.max_grid_cells = max_grid_cells;
+    return 0;
+}
+
+int tmps_neigh_check(tmps_system* h, int* moved) { return neigh_check(*sys(h), moved); }
+
+int tmps_neigh_decide(tmps_system* h, int timestep, int have_timestep, int* rebuild) {
+    return neigh_decide(*sys(h), timestep, have_timestep, rebuild);
+}
+
+int tmps_neigh_build(tmps_system* h) { return neigh_build(*sys(h)); }
+
+int tmps_neigh_info(tmps_system* h, int gdim[3], double csize[3], int* npairs) {
+    System& s = *sys(h);
+    for (int k = 0; k < 3; ++k) {
+        if (gdim) gdim[k] = s.grid.g[k];
+        if (csize) csize[k] = s.grid.csize[k];
+    }
+    if (npairs) *npairs = s.npairs;
+    return 0;
+}
+
+int tmps_neigh_get_pairs(tmps_system* h, int* pair_i, int* pair_j, int* partner, double* shear,
+                         int cap) {
+    System& s = *sys(h);
+    if (s.n <= 0 || s.npairs == 0) return 0;
+    if (cap < s.npairs) return tmps_fail(TMPS_ERR_ARG, "pair buffer too small");
+    const int m = s.npairs;
+    int *dpi = nullptr, *dpj = nullptr, *dpa = nullptr, *dst = nullptr;
+    double* dsh = nullptr;
+    int rc = dalloc(&dpi, m);
+    if (!rc) rc = dalloc(&dpj, m);
+    if (!rc) rc = dalloc(&dpa, m);
+    if (!rc) rc = dalloc(&dsh, 3 * (size_t)m);
+    if (!rc) rc = dalloc(&dst, s.n + 1);
+    if (!rc) rc = launch_half_pairs(s, dpi, dpj, dpa, dsh, dst);
+    if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
+    if (!rc) {
+        if (pair_i) (void)hipMemcpy(pair_i, dpi, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (pair_j) (void)hipMemcpy(pair_j, dpj, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (partner) (void)hipMemcpy(partner, dpa, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (shear) (void)hipMemcpy(shear, dsh, sizeof(double) * 3 * m, hipMemcpyDeviceToHost);
+    }
+    dfree(dpi); dfree(dpj); dfree(dpa); dfree(dsh); dfree(dst);
+    return rc;
+}
+
+int tmps_neigh_get_counts(tmps_system* h, int* num_neighbors, int n) {
+    System& s = *sys(h);
+    if (n > s.n) n = s.n;
+    if (n <= 0) return 0;
+    int* dst = nullptr;
+    int rc = dalloc(&dst, s.n + 1);
+    if (!rc) rc = launch_half_pairs(s, nullptr, nullptr, nullptr, nullptr, dst);
+    if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
+    if (!rc && hipMemcpy(num_neighbors, s.half_count, sizeof(int) * n, hipMemcpyDeviceToHost) !=
+                   hipSuccess)
+        rc = TMPS_ERR_HIP;
+    dfree(dst);
+    return rc;
+}
+
+int tmps_neigh_get_bins(tmps_system* h, int* cell_of, int* cell_start, int* cell_particles) {
+    System& s = *sys(h);
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (s.n > 0 && cell_of)
+        HIP_TRY(hipMemcpy(cell_of, s.cell_of, sizeof(int) * s.n, hipMemcpyDeviceToHost));
+    if (cell_start)
+        HIP_TRY(hipMemcpy(cell_start, s.cell_start, sizeof(int) * (s.grid.ncell + 1),
+                          hipMemcpyDeviceToHost));
+    if (s.n > 0 && cell_particles)
+        HIP_TRY(hipMemcpy(cell_particles, s.cell_particles, sizeof(int) * s.n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tmps_history_reset(tmps_system* h) {
+    System& s = *sys(h);
+    s.have_old = false;
+    return launch_hist_clear(s);
+}
+
+int tmps_history_save(tmps_system* h) { return history_save(*sys(h)); }
+
+int tmps_history_restore(tmps_system* h) { return history_restore(*sys(h)); }
+
+int tmps_history_set_pairs(tmps_system* h, const int* partner, const double* shear, int npairs) {
+    System& s = *sys(h);
+    if (npairs != s.npairs) return tmps_fail(TMPS_ERR_ARG, "history length differs from npairs");
+    if (npairs == 0) return 0;
+    int *dpa = nullptr, *dst = nullptr;
+    double* dsh = nullptr;
+    int rc = dalloc(&dpa, npairs);
+    if (!rc) rc = dalloc(&dsh, 3 * (size_t)npairs);
+    if (!rc) rc = dalloc(&dst, s.n + 1);
+    if (!rc && hipMemcpy(dpa, partner, sizeof(int) * npairs, hipMemcpyHostToDevice) != hipSuccess)
+        rc = TMPS_ERR_HIP;
+    if (!rc && hipMemcpy(dsh, shear, sizeof(double) * 3 * npairs, hipMemcpyHostToDevice) != hipSuccess)
+        rc = TMPS_ERR_HIP;
+    if (!rc) rc = launch_half_pairs(s, nullptr, nullptr, nullptr, nullptr, dst);
+    if (!rc) rc = launch_set_half_history(s, dpa, dsh, dst);
+    if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
+    dfree(dpa); dfree(dsh); dfree(dst);
+    return rc;
+}
+
+int tmps_pair_set(tmps_system* h, const tmps_pair_params* p) {
+    System& s = *sys(h);
+    if (!p) return tmps_fail(TMPS_ERR_ARG, "null pair params");
+    if (p->style < 0 || p->style > 2) return tmps_fail(TMPS_ERR_ARG, "unknown pair style");
+    PairP& q = s.pair;
+    q.style = p->style;
+    q.limit_damping = p->limit_damping;
+    q.kn = p->kn; q.kt = p->kt; q.gamman = p->gamman; q.gammat = p->gammat; q.xmu = p->xmu;
+    q.normal_model = p->normal_model;
+    q.damping_model = p->damping_model;
+    q.tangential_model = p->tangential_model;
+    q.uses_history = (p->style != TMPS_PAIR_GRANULAR) || p->tangential_model == 2 ||
+                     p->tangential_model == 3;
+    q.g_kn = p->g_kn; q.g_damp = p->g_damp; q.g_kt = p->g_kt; q.g_xt = p->g_xt; q.g_mu = p->g_mu;
+    s.have_pair = true;
+    invalidate_graphs(s);
+    return 0;
+}
+
+int tmps_pair_compute(tmps_system* h, double dt, int shearupdate, int accumulate) {
+    return pair_compute(*sys(h), dt, shearupdate, accumulate);
+}
+
+int tmps_nve_initial(tmps_system* h, double dt) {
+    System& s = *sys(h);
+    int rc = ensure_split(s);
+    if (rc) return rc;
+    return split(s) ? launch_nve_initial_split(s, dt) : launch_nve_initial_f64(s, dt);
+}
+
+int tmps_nve_final(tmps_system* h, double dt) {
+    System& s = *sys(h);
+    return split(s) ? launch_nve_final_split(s, dt) : launch_nve_final_f64(s, dt);
+}
+
+int tmps_cundall(tmps_system* h, double gamma_lin, double gamma_ang) {
+    return cundall(*sys(h), gamma_lin, gamma_ang);
+}
+
+int tmps_servo_set(tmps_system* h, const tmps_servo_params* p) {
+    System& s = *sys(h);
+    if (!p) return tmps_fail(TMPS_ERR_ARG, "null servo params");
+    for (int i = 0; i < 3; ++i)
+        if (p->style[i] < 0 || p->style[i] > 9)
+            return tmps_fail(TMPS_ERR_UNSUPPORTED, "unknown deform style code");
+    if (p->couple < 0 || p->couple > 4) return tmps_fail(TMPS_ERR_ARG, "unknown couple code");
+    if (p->remap < 0 || p->remap > 2) return tmps_fail(TMPS_ERR_ARG, "unknown remap code");
+    if (split(s) && p->enabled && p->remap != 0) {
+        for (int i = 0; i < 3; ++i)
+            if (p->style[i] != 0)
+                return tmps_fail(TMPS_ERR_UNSUPPORTED,
+                                 "FP32-split supports a deforming box only with remap x");
+    }
+    s.servo = *p;
+    if (s.servo.nevery < 1) s.servo.nevery = 1;
+    s.have_servo = true;
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    s.hctl->servo.p = s.servo;
+    invalidate_graphs(s);
+    return ctl_push(s);
+}
+
+int tmps_servo_setup(tmps_system* h, int nsteps_total, double dt) {
+    System& s = *sys(h);
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    DevServo& SV = s.hctl->servo;
+    SV.p = s.servo;
+    SV.nsteps = 0;
+    SV.nsteps_total = nsteps_total;
+    DevBox& B = s.hctl->box;
+    for (int i = 0; i < 3; ++i) {
+        SV.lo_start[i] = B.lo[i];
+        SV.hi_start[i] = B.hi[i];
+        SV.cum[i] = 0.0;
+    }
+    double hr[6];
+    strain_h_rate(s.servo, SV, dt, hr);
+    for (int k = 0; k < 6; ++k) B.h_rate[k] = hr[k];
+    B.vremap = s.servo.remap == 1 ? 1 : 0;
+    return ctl_push(s);
+}
+
+int tmps_servo_end_of_step(tmps_system* h, double dt, double tensor_out[6]) {
+    return servo_end_of_step(*sys(h), dt, tensor_out);
+}
+
+int tmps_virial_reduce(tmps_system* h, int keflag, double out6[6]) {
+    System& s = *sys(h);
+    for (int k = 0; k < 6; ++k) out6[k] = 0.0;
+    if (s.n <= 0) return 0;
+    int rc = split(s) ? launch_leaves_split(s, keflag) : 0;
+    if (!split(s)) {
+        rc = launch_leaves_f64(s, keflag);
+        if (!rc) rc = launch_final_tree(s, 6, nblocks_for(s.n));
+    }
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    for (int k = 0; k < 6; ++k) out6[k] = s.hctl->vsum[k];
+    return 0;
+}
+
+int tmps_ke(tmps_system* h, double* ke_trans, double* ke_rot) {
+    System& s = *sys(h);
+    *ke_trans = *ke_rot = 0.0;
+    if (s.n <= 0) return 0;
+    int rc = split(s) ? launch_ke_split(s, nullptr) : launch_ke_f64(s, nullptr);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    *ke_trans = s.hctl->vsum[0];
+    *ke_rot = s.hctl->vsum[1];
+    return 0;
+}
+
+int tmps_run(tmps_system* h, int nsteps, double dt, int run_pre, int use_cundall, double gamma_lin,
+             double gamma_ang) {
+    return run(*sys(h), nsteps, dt, run_pre, use_cundall, gamma_lin, gamma_ang);
+}
+
+int tmps_get_counters(tmps_system* h, tmps_counters* out) {
+    System& s = *sys(h);
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    out->timestep = s.timestep;
+    out->has_built = s.has_built;
+    out->build_count = s.build_count;
+    out->last_build_step = s.last_build_step;
+    out->current_step = s.current_step;
+    out->setup_done = s.setup_done;
+    out->npairs = s.npairs;
+    out->nslots = s.nslots;
+    out->ntouch = s.hctl->ntouch;
+    out->rebuilds = s.rebuilds;
+    out->servo_step_count = s.hctl->servo.step_count;
+    out->servo_nsteps = s.hctl->servo.nsteps;
+    return 0;
+}
+
+int tmps_set_counters(tmps_system* h, const tmps_counters* in) {
+    System& s = *sys(h);
+    s.timestep = in->timestep;
+    s.has_built = in->has_built;
+    s.build_count = in->build_count;
+    s.last_build_step = in->last_build_step;
+    s.current_step = in->current_step;
+    s.setup_done = in->setup_done;
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    s.hctl->servo.step_count = in->servo_step_count;
+    s.hctl->servo.nsteps = in->servo_nsteps;
+    s.hctl->last_build_step = s.last_build_step;
+    return ctl_push(s);
+}
+
+int tmps_last_pressure(tmps_system* h, double out6[6]) {
+    System& s = *sys(h);
+    int rc = ctl_pull(s);
+    if (rc) return rc;
+    for (int k = 0; k < 6; ++k) out6[k] = s.hctl->tensor[k];
+    return 0;
+}
+
+int tmps_set_profiling(tmps_system* h, int enabled) {
+    System& s = *sys(h);
+    s.profiling = enabled != 0;
+    for (int c = 0; c < 3; ++c) {
+        s.prof_ms[c] = 0.0;
+        s.prof_cnt[c] = 0;
+    }
+    return 0;
+}
+
+int tmps_kernel_times(tmps_system* h, double out_ms[3], int counts[3]) {
+    System& s = *sys(h);
+    for (int c = 0; c < 3; ++c) {
+        out_ms[c] = s.prof_cnt[c] ? s.prof_ms[c] / s.prof_cnt[c] : 0.0;
+        if (counts) counts[c] = s.prof_cnt[c];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,821 @@
+// FP64 kernels of the DEM step: contact gather, integrator, damping,
+// reductions, the deform/pressure servo and the fused per-step kernels.
+//
+// Built with -ffp-contract=off: every expression rounds as written, so each
+// result is bitwise equal to the CPU oracle's for the same inputs.
+#include "contact.h"
+#include "tmps_internal.h"
+
+using namespace tmps;
+
+namespace {
+
+struct F64Ptrs {
+    double4* xr;  // x, y, z, radius
+    double4* vm;  // vx, vy, vz, rmass (read by every contact of the step)
+    double4* om;  // omega, 0
+    double4* vm_out;  // where a fused force pass writes v / omega
+    double4* om_out;
+    double4* f;
+    double4* tq;
+    double4* x0;
+    double* vir;  // [n][6]
+    const int* nb_start;
+    const int* nb_j;
+    uint8_t* nb_flag;
+    double4* nb_hist;
+    int n;
+};
+
+F64Ptrs ptrs(System& s, int in_buf = 0, int out_buf = 0) {
+    F64Ptrs p;
+    p.xr = s.xr; p.f = s.f; p.tq = s.tq; p.x0 = s.x0; p.vir = s.vir;
+    p.vm = in_buf ? s.vm2 : s.vm;
+    p.om = in_buf ? s.om2 : s.om;
+    p.vm_out = out_buf ? s.vm2 : s.vm;
+    p.om_out = out_buf ? s.om2 : s.om;
+    p.nb_start = s.nb_start; p.nb_j = s.nb_j; p.nb_flag = s.nb_flag;
+    p.nb_hist = static_cast<double4*>(s.nb_hist);
+    p.n = s.n;
+    return p;
+}
+
+// Domain.minimum_image_and_vshift, orthogonal branch (domain.py:388-462).
+__device__ __forceinline__ void min_image_vshift(const DevBox& B, double d[3], double dv[3]) {
+    double nimg[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (B.per[k] == 1) nimg[k] = round(d[k] / B.prd[k]);
+    dv[0] = dv[1] = dv[2] = 0.0;
+    if (B.vremap != 0) {
+        dv[0] = nimg[0] * B.h_rate[0] + nimg[1] * B.h_rate[5] + nimg[2] * B.h_rate[4];
+        dv[1] = nimg[1] * B.h_rate[1] + nimg[2] * B.h_rate[3];
+        dv[2] = nimg[2] * B.h_rate[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] -= nimg[k] * B.prd[k];
+}
+
+// Sum of every contact of atom i over its full-list row, partners in
+// ascending index order; each pair is evaluated with the lower index first.
+template <int STYLE>
+__device__ __forceinline__ void gather_f64(int i, const double4& xi, const double4& vi,
+                                           const double4& oi, const F64Ptrs& P, const DevBox& B,
+                                           const PairP& pp, double dt, int shearupdate,
+                                           double accF[3], double accT[3], double accV[6],
+                                           int& ntouch) {
+    const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
+    const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
+    for (int s = s0; s < s1; ++s) {
+        const int j = P.nb_j[s];
+        const double4 xj = P.xr[j];
+        const bool first = i < j;
+        const double4& xa = first ? xi : xj;
+        const double4& xb = first ? xj : xi;
+        double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
+        double dvj[3];
+        min_image_vshift(B, d, dvj);
+        const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double ra = xa.w, rb = xb.w;
+        const double radsum = ra + rb;
+        if (rsq < radsum * radsum && rsq > 1e-28) {
+            const double4 vj = P.vm[j];
+            const double4 oj = P.om[j];
+            const double4& va4 = first ? vi : vj;
+            const double4& vb4 = first ? vj : vi;
+            const double4& oa4 = first ? oi : oj;
+            const double4& ob4 = first ? oj : oi;
+            const double va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+            const double oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+            bool before = false;
+            double h[3] = {0.0, 0.0, 0.0};
+            if (uses_history) {
+                before = P.nb_flag[s] != 0;
+                if (before) {
+                    const double4 hh = P.nb_hist[s];
+                    h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
+                }
+            }
+            ContactOut<double> o;
+            if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                legacy_history_contact<double, false>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va,
+                                                      vb, oa, ob, pp.kn, pp.kt, pp.gamman,
+                                                      pp.gammat, pp.xmu, pp.limit_damping, dt,
+                                                      shearupdate, before, h, o);
+            else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                legacy_history_contact<double, true>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va,
+                                                     vb, oa, ob, pp.kn, pp.kt, pp.gamman, pp.gammat,
+                                                     pp.xmu, pp.limit_damping, dt, shearupdate,
+                                                     before, h, o);
+            else
+                granular_contact<double>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob,
+                                         pp, dt, shearupdate, before, h, o);
+            if (uses_history) {
+                if (!before) P.nb_flag[s] = 1;
+                P.nb_hist[s] = make_double4(h[0], h[1], h[2], 0.0);
+            }
+            if (first) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    accF[k] = accF[k] + o.F[k];
+                    accT[k] = accT[k] + o.ta[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    accF[k] = accF[k] + (-o.F[k]);
+                    accT[k] = accT[k] + o.tb[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) accV[k] = accV[k] + o.vir[k];
+            ++ntouch;
+        } else if (uses_history) {
+            if (P.nb_flag[s] != 0) P.nb_flag[s] = 0;
+        }
+    }
+}
+
+__device__ __forceinline__ DevBox load_box(const Ctl* ctl) { return ctl->box; }
+
+// ----------------------------------------------------------- stage kernels
+
+__global__ void k_clear_f64(F64Ptrs P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    P.f[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    P.tq[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = 0.0;
+}
+
+// FixNVESphere.initial_integrate_kernel, nve_sphere.py:43-76
+__global__ void k_nve_initial_f64(F64Ptrs P, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double dtv = dt, dtf = 0.5 * dt;
+    double4 x = P.xr[i], v = P.vm[i], w = P.om[i];
+    const double4 f = P.f[i], t = P.tq[i];
+    const double m = v.w, r = x.w;
+    const double inertia = 0.4 * m * r * r;
+    v.x += dtf * (f.x / m);
+    v.y += dtf * (f.y / m);
+    v.z += dtf * (f.z / m);
+    x.x = x.x + dtv * v.x;
+    x.y = x.y + dtv * v.y;
+    x.z = x.z + dtv * v.z;
+    w.x += dtf * (t.x / inertia);
+    w.y += dtf * (t.y / inertia);
+    w.z += dtf * (t.z / inertia);
+    P.xr[i] = x;
+    P.vm[i] = v;
+    P.om[i] = w;
+}
+
+// FixNVESphere.final_integrate_kernel, nve_sphere.py:78-96
+__global__ void k_nve_final_f64(F64Ptrs P, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double dtf = 0.5 * dt;
+    double4 v = P.vm[i], w = P.om[i];
+    const double4 f = P.f[i], t = P.tq[i];
+    const double m = v.w, r = P.xr[i].w;
+    const double inertia = 0.4 * m * r * r;
+    v.x += dtf * (f.x / m);
+    v.y += dtf * (f.y / m);
+    v.z += dtf * (f.z / m);
+    w.x += dtf * (t.x / inertia);
+    w.y += dtf * (t.y / inertia);
+    w.z += dtf * (t.z / inertia);
+    P.vm[i] = v;
+    P.om[i] = w;
+}
+
+// FixDampingCundall.post_force_kernel, damping_cundall.py:35-53
+__device__ __forceinline__ double cundall_scale(double fv, double g) {
+    const double sgn = fv >= 0.0 ? 1.0 : -1.0;
+    return 1.0 - g * sgn;
+}
+
+__global__ void k_cundall_f64(F64Ptrs P, double gl, double ga) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    double4 f = P.f[i], t = P.tq[i];
+    const double4 v = P.vm[i], w = P.om[i];
+    f.x *= cundall_scale(f.x * v.x, gl);
+    t.x *= cundall_scale(t.x * w.x, ga);
+    f.y *= cundall_scale(f.y * v.y, gl);
+    t.y *= cundall_scale(t.y * w.y, ga);
+    f.z *= cundall_scale(f.z * v.z, gl);
+    t.z *= cundall_scale(t.z * w.z, ga);
+    P.f[i] = f;
+    P.tq[i] = t;
+}
+
+// GranularPair.compute: every contact of atom i, written (or added) to its
+// force, torque and per-atom virial.
+template <int STYLE>
+__global__ void k_pair_f64(F64Ptrs P, const Ctl* ctl, PairP pp, double dt, int shearupdate,
+                           int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const DevBox B = load_box(ctl);
+    double accF[3] = {0.0, 0.0, 0.0}, accT[3] = {0.0, 0.0, 0.0};
+    double accV[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (accumulate) {
+        const double4 f = P.f[i], t = P.tq[i];
+        accF[0] = f.x; accF[1] = f.y; accF[2] = f.z;
+        accT[0] = t.x; accT[1] = t.y; accT[2] = t.z;
+        for (int k = 0; k < 6; ++k) accV[k] = P.vir[6 * i + k];
+    }
+    int nt = 0;
+    gather_f64<STYLE>(i, P.xr[i], P.vm[i], P.om[i], P, B, pp, dt, shearupdate, accF, accT, accV,
+                      nt);
+    P.f[i] = make_double4(accF[0], accF[1], accF[2], 0.0);
+    P.tq[i] = make_double4(accT[0], accT[1], accT[2], 0.0);
+    for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = accV[k];
+}
+
+// NeighborList.check_displacement, neighbor.py:214-245 (a flag, not a stamp:
+// the host clears it before the launch).
+__global__ void k_check_f64(F64Ptrs P, Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double4 x = P.xr[i], x0 = P.x0[i];
+    const double d0 = x.x - x0.x, d1 = x.y - x0.y, d2 = x.z - x0.z;
+    if (d0 * d0 + d1 * d1 + d2 * d2 > ctl->trigger_sq) ctl->moved = 1;
+}
+
+// Per-atom leaves of compute_virial_kernel (thermo.py:62-109), one pairwise
+// tree per block of atoms.
+__global__ void k_leaves_f64(F64Ptrs P, int keflag, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < P.n) {
+        for (int k = 0; k < 6; ++k) leaf[k] = P.vir[6 * i + k];
+        if (keflag) {
+            const double4 v = P.vm[i];
+            const double m = v.w;
+            leaf[0] = leaf[0] + m * v.x * v.x;
+            leaf[1] = leaf[1] + m * v.y * v.y;
+            leaf[2] = leaf[2] + m * v.z * v.z;
+            leaf[3] = leaf[3] + m * v.x * v.y;
+            leaf[4] = leaf[4] + m * v.x * v.z;
+            leaf[5] = leaf[5] + m * v.y * v.z;
+        }
+    }
+    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+// compute_ke_kernel leaves, thermo.py:46-60
+__global__ void k_ke_leaves_f64(F64Ptrs P, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[2] = {0.0, 0.0};
+    if (i < P.n) {
+        const double4 v = P.vm[i], w = P.om[i];
+        const double r = P.xr[i].w, m = v.w;
+        const double vsq = v.x * v.x + v.y * v.y + v.z * v.z;
+        leaf[0] = 0.5 * m * vsq;
+        const double im = 0.4 * m * r * r;
+        const double wsq = w.x * w.x + w.y * w.y + w.z * w.z;
+        leaf[1] = 0.5 * im * wsq;
+    }
+    block_tree_sum<2>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+// ---------------------------------------------------- final tree + servo
+
+// Pairwise tree over nparts block partials (zero-padded to a power of two);
+// thread 0 receives the sums.
+template <int NC>
+__device__ void final_tree(const double* partial, int nparts, double out[NC]) {
+    int P2 = 1;
+    while (P2 < nparts) P2 <<= 1;
+    int L = P2 / TMPS_BLOCK;
+    if (L < 1) L = 1;
+    double stk[16][NC];
+    int sz[16];
+    int top = 0;
+    const int base = threadIdx.x * L;
+    for (int q = 0; q < L; ++q) {
+        const int idx = base + q;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) stk[top][k] = idx < nparts ? partial[(size_t)idx * 8 + k] : 0.0;
+        sz[top] = 1;
+        ++top;
+        while (top >= 2 && sz[top - 1] == sz[top - 2]) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) stk[top - 2][k] = stk[top - 2][k] + stk[top - 1][k];
+            sz[top - 2] *= 2;
+            --top;
+        }
+    }
+    double leaf[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) leaf[k] = stk[0][k];
+    __shared__ double res[NC];
+    block_tree_sum<NC>(leaf, res);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NC; ++k) out[k] = res[k];
+}
+
+template <int NC>
+__global__ void k_final_tree(Ctl* ctl, const double* partial, int nparts) {
+    double out[NC];
+    final_tree<NC>(partial, nparts, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NC; ++k) ctl->vsum[k] = out[k];
+}
+
+// _limit_rate_device, deform_pressure.py:582-596
+__device__ double limit_rate(const tmps_servo_params& S, double rate, double ptarget) {
+    double out = rate;
+    if (S.normalize == 1) {
+        if (ptarget == 0.0) {
+            if (out != 0.0) out = S.max_rate * (out > 0.0 ? 1.0 : -1.0);
+        } else {
+            out = out / fabs(ptarget);
+        }
+    }
+    if (S.max_rate != 0.0 && fabs(out) > S.max_rate) out = S.max_rate * (out > 0.0 ? 1.0 : -1.0);
+    return out;
+}
+
+// FixDeformPressure.end_of_step for x/y/z styles (deform_pressure.py:1113-1219
+// host path, :598-702 device path): pressure tensor from the reduced
+// virial(+kinetic) sum, couple, per-axis targets, remap record, box publish.
+__device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms) {
+    DevServo& SV = ctl->servo;
+    const tmps_servo_params& S = SV.p;
+    DevBox& B = ctl->box;
+    SV.step_count += 1;
+    SV.nsteps += 1;
+    ctl->remap.step = -1;
+    if (SV.step_count % S.nevery != 0) return;
+
+    bool needs_pressure = false;
+    for (int i = 0; i < 3; ++i)
+        if (S.style[i] == 8 || S.style[i] == 9) needs_pressure = true;
+    double p[3] = {0.0, 0.0, 0.0};
+    if (needs_pressure) {
+        const double vol = B.prd[0] * B.prd[1] * B.prd[2];
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (natoms > 0 && vol > 0.0)
+            for (int k = 0; k < 6; ++k) t[k] = vs[k] / vol;
+        for (int k = 0; k < 6; ++k) ctl->tensor[k] = t[k];
+        const double scalar = (t[0] + t[1] + t[2]) / 3.0;
+        if (S.couple == 1) {
+            p[0] = p[1] = p[2] = scalar;
+        } else if (S.couple >= 2) {
+            int a = 0, b = 1;
+            if (S.couple == 3) { a = 1; b = 2; }
+            if (S.couple == 4) { a = 0; b = 2; }
+            const double mean = (t[a] + t[b]) / 2.0;
+            for (int i = 0; i < 3; ++i) p[i] = t[i];
+            p[a] = mean;
+            p[b] = mean;
+        } else {
+            for (int i = 0; i < 3; ++i) {
+                if (S.style[i] == 8) p[i] = t[i];
+                else if (S.style[i] == 9) p[i] = scalar;
+            }
+        }
+    }
+    double old_prd[3], old_lo[3], lo_t[3], hi_t[3];
+    for (int i = 0; i < 3; ++i) {
+        old_prd[i] = B.prd[i];
+        old_lo[i] = B.lo[i];
+        lo_t[i] = B.lo[i];
+        hi_t[i] = B.hi[i];
+    }
+    const double elapsed = SV.nsteps * dt;
+    const double frac = SV.nsteps_total > 0 ? (double)SV.nsteps / (double)SV.nsteps_total : 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double ls = SV.lo_start[i], hs = SV.hi_start[i];
+        switch (S.style[i]) {
+            case 2: {  // trate
+                const double mid = 0.5 * (ls + hs);
+                const double half = 0.5 * (hs - ls) * exp(S.rate[i] * elapsed);
+                lo_t[i] = mid - half; hi_t[i] = mid + half;
+            } break;
+            case 1: {  // erate
+                const double shift = 0.5 * elapsed * S.rate[i] * (hs - ls);
+                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
+            } break;
+            case 3: {  // vel
+                const double shift = 0.5 * elapsed * S.vel[i];
+                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
+            } break;
+            case 4: {  // wiggle
+                const double shift =
+                    0.5 * S.amp[i] * sin(2.0 * 3.14159265358979323846 * elapsed / S.tper[i]);
+                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
+            } break;
+            case 5:
+            case 6:
+            case 7: {  // final, delta, scale
+                double lo_stop, hi_stop;
+                if (S.style[i] == 5) { lo_stop = S.flo[i]; hi_stop = S.fhi[i]; }
+                else if (S.style[i] == 6) { lo_stop = ls + S.dlo[i]; hi_stop = hs + S.dhi[i]; }
+                else {
+                    const double mid = 0.5 * (ls + hs);
+                    const double half = 0.5 * S.scale[i] * (hs - ls);
+                    lo_stop = mid - half; hi_stop = mid + half;
+                }
+                lo_t[i] = ls + frac * (lo_stop - ls);
+                hi_t[i] = hs + frac * (hi_stop - hs);
+            } break;
+            case 8:
+            case 9: {  // pressure, pressure/mean
+                const double rate = limit_rate(S, S.pgain[i] * (p[i] - S.ptarget[i]), S.ptarget[i]);
+                SV.cum[i] += dt * rate * old_prd[i];
+                lo_t[i] = ls - 0.5 * SV.cum[i];
+                hi_t[i] = hs + 0.5 * SV.cum[i];
+            } break;
+            default: break;
+        }
+    }
+    double new_prd[3];
+    for (int i = 0; i < 3; ++i) new_prd[i] = hi_t[i] - lo_t[i];
+    for (int i = 0; i < 3; ++i)
+        if (new_prd[i] <= 0.0) {
+            ctl->error = TMPS_ERR_BOX;
+            ctl->abort = 1;
+            return;
+        }
+    if (S.remap == 0) {
+        RemapRec& R = ctl->remap;
+        const double xprd = old_prd[0], yprd = old_prd[1], zprd = old_prd[2];
+        R.oh[0] = -0.0 / (xprd * yprd);
+        R.oh[1] = (0.0 * 0.0 - yprd * 0.0) / (xprd * yprd * zprd);
+        R.oh[2] = -0.0 / (yprd * zprd);
+        for (int i = 0; i < 3; ++i) {
+            R.old_lo[i] = old_lo[i];
+            R.old_prd[i] = old_prd[i];
+            R.new_lo[i] = lo_t[i];
+            R.new_prd[i] = new_prd[i];
+        }
+        R.step = ctl->timestep + 1;
+    }
+    for (int i = 0; i < 3; ++i) {
+        B.h_rate[i] = (new_prd[i] - old_prd[i]) / dt;
+        B.lo[i] = lo_t[i];
+        B.hi[i] = hi_t[i];
+        B.prd[i] = B.hi[i] - B.lo[i];
+    }
+    B.h_rate[3] = B.h_rate[4] = B.h_rate[5] = (0.0 - 0.0) / dt;
+    B.vremap = S.remap == 1 ? 1 : 0;
+}
+
+// Tree-sums the force pass's block partials and, when enabled, runs the
+// servo. `advance` marks a fused step: it honours the segment's abort and
+// rebuild-trigger flags and completes the step's bookkeeping.
+__global__ void k_step_end(Ctl* ctl, const double* partial, int nparts, double dt, int advance,
+                           int apply_servo, int natoms) {
+    if (advance) {
+        if (ctl->abort) return;
+        const int t = ctl->timestep + 1;
+        if (ctl->trigger == t) {
+            if (threadIdx.x == 0) {
+                ctl->abort = 1;
+            }
+            return;
+        }
+    }
+    double out[7];
+    final_tree<7>(partial, nparts, out);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; ++k) ctl->vsum[k] = out[k];
+        ctl->ntouch = (int)out[6];
+        if (apply_servo) {
+            servo_update(ctl, out, dt, natoms);
+            if (!ctl->abort) update_split_consts(ctl, 0);
+        }
+        if (advance && !ctl->abort) ctl->timestep = ctl->timestep + 1;
+    }
+}
+
+// remap_positions, deform_pressure.py:493-530 (zero tilt)
+__device__ __forceinline__ void remap_point(const RemapRec& R, double4& x) {
+    const double d0 = x.x - R.old_lo[0], d1 = x.y - R.old_lo[1], d2 = x.z - R.old_lo[2];
+    const double l0 = d0 / R.old_prd[0] + R.oh[0] * d1 + R.oh[1] * d2;
+    const double l1 = d1 / R.old_prd[1] + R.oh[2] * d2;
+    const double l2 = d2 / R.old_prd[2];
+    x.x = R.new_prd[0] * l0 + 0.0 * l1 + 0.0 * l2 + R.new_lo[0];
+    x.y = R.new_prd[1] * l1 + 0.0 * l2 + R.new_lo[1];
+    x.z = R.new_prd[2] * l2 + R.new_lo[2];
+}
+
+// Applies a pending remap: in stage mode whatever is pending, in fused mode
+// only the one recorded by the step just completed.
+__global__ void k_remap_apply(F64Ptrs P, const Ctl* ctl, int fused) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const RemapRec& R = ctl->remap;
+    if (R.step == -1) return;
+    if (fused && (ctl->abort || R.step != ctl->timestep)) return;
+    double4 x = P.xr[i];
+    remap_point(R, x);
+    P.xr[i] = x;
+}
+
+__global__ void k_remap_clear(Ctl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->remap.step = -1;
+}
+
+__global__ void k_ctl_rebuilt(Ctl* ctl, int step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->trigger = -1;
+        ctl->abort = 0;
+        ctl->last_build_step = step;
+    }
+}
+
+// ---------------------------------------------------- fused step kernels
+
+// Start of step t: the pending end-of-step remap of step t-1, the
+// initial_integrate half-kick (only for a segment's first step; later steps
+// receive it from the previous force pass), the drift, and the skin check.
+__global__ void k_step_begin_f64(F64Ptrs P, Ctl* ctl, double dt, int kick) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int t = ctl->timestep + 1;
+    double4 x = P.xr[i];
+    const RemapRec& R = ctl->remap;
+    if (R.step != -1 && R.step == ctl->timestep) remap_point(R, x);
+    double4 v = P.vm[i];
+    const double dtf = 0.5 * dt;
+    if (kick) {
+        const double4 f = P.f[i];
+        const double m = v.w;
+        v.x += dtf * (f.x / m);
+        v.y += dtf * (f.y / m);
+        v.z += dtf * (f.z / m);
+    }
+    x.x = x.x + dt * v.x;
+    x.y = x.y + dt * v.y;
+    x.z = x.z + dt * v.z;
+    if (kick) {
+        double4 w = P.om[i];
+        const double4 tq = P.tq[i];
+        const double m = v.w, r = x.w;
+        const double inertia = 0.4 * m * r * r;
+        w.x += dtf * (tq.x / inertia);
+        w.y += dtf * (tq.y / inertia);
+        w.z += dtf * (tq.z / inertia);
+        P.om[i] = w;
+        P.vm[i] = v;
+    }
+    P.xr[i] = x;
+    const int since = t - ctl->last_build_step;
+    if (since >= ctl->delay && since % ctl->every == 0) {
+        if (!ctl->check) {
+            if (i == 0) ctl->trigger = t;
+        } else {
+            const double4 x0 = P.x0[i];
+            const double d0 = x.x - x0.x, d1 = x.y - x0.y, d2 = x.z - x0.z;
+            if (d0 * d0 + d1 * d1 + d2 * d2 > ctl->trigger_sq) ctl->trigger = t;
+        }
+    }
+}
+
+// Step t after the drift: contact forces, damping/cundall, final_integrate,
+// the block's share of the virial(+kinetic) tree, and either the next step's
+// half-kick (mid-segment) or the stored f/torque/virial (segment end).
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, double gl, double ga,
+                                 double* partial) {
+    if (ctl->abort) return;
+    const int t = ctl->timestep + 1;
+    if (ctl->trigger == t) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < P.n) {
+        const DevBox B = load_box(ctl);
+        const double4 xi = P.xr[i];
+        double4 v = P.vm[i];
+        double4 w = P.om[i];
+        double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+        double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int nt = 0;
+        gather_f64<STYLE>(i, xi, v, w, P, B, pp, dt, 1, F, T, V, nt);
+        if (CUNDALL) {
+            F[0] *= cundall_scale(F[0] * v.x, gl);
+            T[0] *= cundall_scale(T[0] * w.x, ga);
+            F[1] *= cundall_scale(F[1] * v.y, gl);
+            T[1] *= cundall_scale(T[1] * w.y, ga);
+            F[2] *= cundall_scale(F[2] * v.z, gl);
+            T[2] *= cundall_scale(T[2] * w.z, ga);
+        }
+        const double dtf = 0.5 * dt;
+        const double m = v.w, r = xi.w;
+        const double inertia = 0.4 * m * r * r;
+        const double am[3] = {F[0] / m, F[1] / m, F[2] / m};
+        const double ai[3] = {T[0] / inertia, T[1] / inertia, T[2] / inertia};
+        v.x += dtf * am[0];
+        v.y += dtf * am[1];
+        v.z += dtf * am[2];
+        w.x += dtf * ai[0];
+        w.y += dtf * ai[1];
+        w.z += dtf * ai[2];
+        leaf[0] = V[0] + m * v.x * v.x;
+        leaf[1] = V[1] + m * v.y * v.y;
+        leaf[2] = V[2] + m * v.z * v.z;
+        leaf[3] = V[3] + m * v.x * v.y;
+        leaf[4] = V[4] + m * v.x * v.z;
+        leaf[5] = V[5] + m * v.y * v.z;
+        leaf[6] = (double)nt;
+        if (FINAL) {
+            P.f[i] = make_double4(F[0], F[1], F[2], 0.0);
+            P.tq[i] = make_double4(T[0], T[1], T[2], 0.0);
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+        } else {
+            v.x += dtf * am[0];
+            v.y += dtf * am[1];
+            v.z += dtf * am[2];
+            w.x += dtf * ai[0];
+            w.y += dtf * ai[1];
+            w.z += dtf * ai[2];
+        }
+        P.vm_out[i] = v;
+        P.om_out[i] = w;
+    }
+    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+__global__ void k_copy_vw(double4* vdst, double4* wdst, const double4* vsrc, const double4* wsrc,
+                          int n, const Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vdst[i] = vsrc[i];
+    wdst[i] = wsrc[i];
+}
+
+}  // namespace
+
+// ================================================================ launchers
+
+#define LAUNCH_N(kernel, n, ...)                                                          \
+    do {                                                                                  \
+        if ((n) > 0) hipLaunchKernelGGL(kernel, dim3(nblocks_for(n)), dim3(TMPS_BLOCK), 0, \
+                                        s.stream, __VA_ARGS__);                           \
+        HIP_TRY(hipGetLastError());                                                       \
+    } while (0)
+
+int launch_clear_f64(System& s) {
+    LAUNCH_N(k_clear_f64, s.n, ptrs(s));
+    return 0;
+}
+
+int launch_nve_initial_f64(System& s, double dt) {
+    LAUNCH_N(k_nve_initial_f64, s.n, ptrs(s), dt);
+    return 0;
+}
+
+int launch_nve_final_f64(System& s, double dt) {
+    LAUNCH_N(k_nve_final_f64, s.n, ptrs(s), dt);
+    return 0;
+}
+
+int launch_cundall_f64(System& s, double gl, double ga) {
+    LAUNCH_N(k_cundall_f64, s.n, ptrs(s), gl, ga);
+    return 0;
+}
+
+int launch_pair_f64(System& s, double dt, int shearupdate, int accumulate) {
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            LAUNCH_N(k_pair_f64<TMPS_PAIR_HOOKE_HISTORY>, s.n, ptrs(s), s.ctl, s.pair, dt,
+                     shearupdate, accumulate);
+            break;
+        case TMPS_PAIR_HERTZ_HISTORY:
+            LAUNCH_N(k_pair_f64<TMPS_PAIR_HERTZ_HISTORY>, s.n, ptrs(s), s.ctl, s.pair, dt,
+                     shearupdate, accumulate);
+            break;
+        default:
+            LAUNCH_N(k_pair_f64<TMPS_PAIR_GRANULAR>, s.n, ptrs(s), s.ctl, s.pair, dt, shearupdate,
+                     accumulate);
+            break;
+    }
+    return 0;
+}
+
+int launch_check_f64(System& s) {
+    LAUNCH_N(k_check_f64, s.n, ptrs(s), s.ctl);
+    return 0;
+}
+
+int launch_final_tree(System& s, int nc, int nparts) {
+    if (nc == 2)
+        hipLaunchKernelGGL(k_final_tree<2>, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
+                           nparts);
+    else
+        hipLaunchKernelGGL(k_final_tree<6>, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
+                           nparts);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Block partials of the per-atom virial (+ kinetic) leaves; the caller
+// finishes the tree (launch_final_tree) or runs the servo (launch_step_end).
+int launch_leaves_f64(System& s, int keflag) {
+    const int nb = nblocks_for(s.n);
+    hipLaunchKernelGGL(k_leaves_f64, dim3(nb > 0 ? nb : 1), dim3(TMPS_BLOCK), 0, s.stream, ptrs(s),
+                       keflag, s.partial);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ke_f64(System& s, double* out2) {
+    (void)out2;
+    const int nb = nblocks_for(s.n);
+    hipLaunchKernelGGL(k_ke_leaves_f64, dim3(nb > 0 ? nb : 1), dim3(TMPS_BLOCK), 0, s.stream,
+                       ptrs(s), s.partial);
+    HIP_TRY(hipGetLastError());
+    return launch_final_tree(s, 2, nb > 0 ? nb : 1);
+}
+
+int launch_step_begin_f64(System& s, double dt, int kick, int buf) {
+    LAUNCH_N(k_step_begin_f64, s.n, ptrs(s, buf, buf), s.ctl, dt, kick);
+    return 0;
+}
+
+int launch_copy_vw(System& s, int honor_abort) {
+    LAUNCH_N(k_copy_vw, s.n, s.vm, s.om, s.vm2, s.om2, s.n, s.ctl, honor_abort);
+    return 0;
+}
+
+template <int STYLE>
+static int launch_force_style(System& s, double dt, int final_mode, int cundall, double gl,
+                              double ga, int in_buf, int out_buf) {
+    const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
+    const F64Ptrs P = ptrs(s, in_buf, out_buf);
+    if (final_mode) {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 1>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 1>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+    } else {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 0>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 0>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, double gl,
+                          double ga, int in_buf, int out_buf) {
+    if (s.n <= 0) return 0;
+    if (in_buf == out_buf) return tmps_fail(TMPS_ERR_ARG, "force pass cannot update v in place");
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            return launch_force_style<TMPS_PAIR_HOOKE_HISTORY>(s, dt, final_mode, cundall, gl, ga,
+                                                               in_buf, out_buf);
+        case TMPS_PAIR_HERTZ_HISTORY:
+            return launch_force_style<TMPS_PAIR_HERTZ_HISTORY>(s, dt, final_mode, cundall, gl, ga,
+                                                               in_buf, out_buf);
+        default:
+            return launch_force_style<TMPS_PAIR_GRANULAR>(s, dt, final_mode, cundall, gl, ga,
+                                                          in_buf, out_buf);
+    }
+}
+
+int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo) {
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
+                       nparts > 0 ? nparts : 1, dt, advance, apply_servo, s.n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_remap_apply(System& s) {
+    if (s.precision == TMPS_FP64) {
+        LAUNCH_N(k_remap_apply, s.n, ptrs(s), s.ctl, 0);
+    }
+    return 0;
+}
+
+int launch_remap_apply_fused(System& s) {
+    if (s.precision == TMPS_FP64) {
+        LAUNCH_N(k_remap_apply, s.n, ptrs(s), s.ctl, 1);
+    }
+    return 0;
+}
+
+int launch_remap_clear(System& s) {
+    hipLaunchKernelGGL(k_remap_clear, dim3(1), dim3(64), 0, s.stream, s.ctl);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ctl_rebuilt(System& s, int step) {
+    hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,518 @@
+// Neighbor-list rebuild: PBC wrap, counting-sort binning, exclusive scans,
+// the full per-atom neighbor list and the contact-history carry-over.
+//
+// Built with -ffp-contract=off: the bin of every atom and the cutoff test of
+// every pair round exactly as the CPU oracle's, so bins and neighbor sets are
+// bitwise identical to it.
+#include "tmps_internal.h"
+
+namespace {
+
+// ------------------------------------------------------------------ PBC
+
+// Domain.pbc_wrap + _pbc_vremap_kernel, orthogonal branch (domain.py:502-571)
+__global__ void k_pbc(double4* xr, double4* vm, int n, const Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevBox& B = ctl->box;
+    double4 x = xr[i];
+    double xs[3] = {x.x, x.y, x.z};
+    if (B.vremap) {
+        double nimg[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 3; ++k)
+            if (B.per[k] == 1) nimg[k] = floor((xs[k] - B.lo[k]) / B.prd[k]);
+        if (nimg[0] != 0.0 || nimg[1] != 0.0 || nimg[2] != 0.0) {
+            const double* h = B.h_rate;
+            double4 v = vm[i];
+            v.x += -(nimg[0] * h[0] + nimg[1] * h[5] + nimg[2] * h[4]);
+            v.y += -(nimg[1] * h[1] + nimg[2] * h[3]);
+            v.z += -(nimg[2] * h[2]);
+            vm[i] = v;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (B.per[k] == 1) {
+            const double sl = (xs[k] - B.lo[k]) / B.prd[k];
+            xs[k] = B.lo[k] + (sl - floor(sl)) * B.prd[k];
+        }
+    }
+    x.x = xs[0]; x.y = xs[1]; x.z = xs[2];
+    xr[i] = x;
+}
+
+// ------------------------------------------------------------------ bins
+
+// NeighborList.get_cell_coord, neighbor.py:191-206 (lamda with zero tilt)
+__device__ __forceinline__ void cell_coord(const DevBox& B, const GridP& G, const double4& p,
+                                           int c[3]) {
+    const double d0 = p.x - B.lo[0], d1 = p.y - B.lo[1], d2 = p.z - B.lo[2];
+    double lam[3];
+    lam[0] = d0 / B.prd[0] + 0.0 * d1 + 0.0 * d2;
+    lam[1] = d1 / B.prd[1] + 0.0 * d2;
+    lam[2] = d2 / B.prd[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ck = (int)floor(lam[k] / G.csize[k]);
+        c[k] = max(0, min(ck, G.g[k] - 1));
+    }
+}
+
+__global__ void k_zero_int(int* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// build_grid count phase, neighbor.py:264-268
+__global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, int* cell_of,
+                            int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c[3];
+    cell_coord(ctl->box, G, xr[i], c);
+    const int idx = c[0] + c[1] * G.g[0] + c[2] * G.g[0] * G.g[1];
+    cell_of[i] = idx;
+    atomicAdd(&count[idx], 1);
+}
+
+// build_grid scatter phase, neighbor.py:276-282
+__global__ void k_bin_fill(int n, const int* cell_of, const int* start, int* fill, int* particles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    const int slot = atomicAdd(&fill[c], 1);
+    particles[start[c] + slot] = i;
+}
+
+// Bin members in ascending atom index (the atomic scatter leaves them in
+// arrival order).
+__global__ void k_bin_sort(int ncell, const int* start, int* particles) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int a = start[c], b = start[c + 1];
+    for (int p = a + 1; p < b; ++p) {
+        const int v = particles[p];
+        int q = p - 1;
+        while (q >= a && particles[q] > v) {
+            particles[q + 1] = particles[q];
+            --q;
+        }
+        particles[q + 1] = v;
+    }
+}
+
+// ------------------------------------------------------------------ scan
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of a block's TMPS_BLOCK values; returns the block total.
+__device__ __forceinline__ int block_excl_scan(int v, int* excl) {
+    __shared__ int wtot[TMPS_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    int woff = 0, total = 0;
+    for (int w = 0; w < TMPS_BLOCK / 64; ++w) {
+        if (w < wave) woff += wtot[w];
+        total += wtot[w];
+    }
+    *excl = woff + inc - v;
+    __syncthreads();
+    return total;
+}
+
+#define SCAN_PER_THREAD 4
+#define SCAN_TILE (TMPS_BLOCK * SCAN_PER_THREAD)
+
+__global__ void k_scan_tiles(const int* in, int* out, int n, int* tile_sums) {
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    int v[SCAN_PER_THREAD];
+    int sum = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        sum += v[k];
+    }
+    int excl;
+    const int total = block_excl_scan(sum, &excl);
+    int run = excl;
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile sums in place; writes the grand total.
+__global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out) {
+    const int per = (ntiles + TMPS_BLOCK - 1) / TMPS_BLOCK;
+    const int a = threadIdx.x * per;
+    int sum = 0;
+    for (int k = 0; k < per; ++k)
+        if (a + k < ntiles) sum += tile_sums[a + k];
+    int excl;
+    const int total = block_excl_scan(sum, &excl);
+    int run = excl;
+    for (int k = 0; k < per; ++k)
+        if (a + k < ntiles) {
+            const int t = tile_sums[a + k];
+            tile_sums[a + k] = run;
+            run += t;
+        }
+    if (threadIdx.x == 0) *total_out = total;
+}
+
+__global__ void k_scan_add(int* out, int n, const int* tile_sums) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] += tile_sums[i / SCAN_TILE];
+}
+
+// ------------------------------------------------------------------ list
+
+// The stencil walk of NeighborList.build_neighbor_list (neighbor.py:315-369),
+// visiting cells in the reference's (dx, dy, dz) order. MODE 0 counts the
+// full row and the half row (j > i); MODE 1 writes the full row.
+template <int MODE>
+__global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const int* cell_start,
+                       const int* particles, double skin, int max_nb, int* full_count,
+                       int* half_count, int* overflow, const int* nb_start, int* nb_j) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevBox& B = ctl->box;
+    const double4 pi = xr[i];
+    int c[3];
+    cell_coord(B, G, pi, c);
+    const int gx = G.g[0], gy = G.g[1], gz = G.g[2], gxy = gx * gy;
+    const int px = B.per[0], py = B.per[1], pz = B.per[2];
+    int nfull = 0, nhalf = 0;
+    int w = MODE == 1 ? nb_start[i] : 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                int nx = c[0] + dx, ny = c[1] + dy, nz = c[2] + dz;
+                const bool skip = (px == 1 && gx == 1 && dx != 0) ||
+                                  (py == 1 && gy == 1 && dy != 0) ||
+                                  (pz == 1 && gz == 1 && dz != 0);
+                if (px == 1) nx = (nx % gx + gx) % gx;
+                if (py == 1) ny = (ny % gy + gy) % gy;
+                if (pz == 1) nz = (nz % gz + gz) % gz;
+                if (skip || nx < 0 || nx >= gx || ny < 0 || ny >= gy || nz < 0 || nz >= gz)
+                    continue;
+                const int cidx = nx + ny * gx + nz * gxy;
+                const int pa = cell_start[cidx], pb = cell_start[cidx + 1];
+                for (int p = pa; p < pb; ++p) {
+                    const int j = particles[p];
+                    if (j == i) continue;
+                    const double4 pj = xr[j];
+                    double d[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+                    for (int k = 0; k < 3; ++k)
+                        if (B.per[k] == 1) d[k] -= B.prd[k] * round(d[k] / B.prd[k]);
+                    const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    const double rad_sum = pi.w + pj.w + skin;
+                    if (rsq < rad_sum * rad_sum) {
+                        if (MODE == 1) nb_j[w++] = j;
+                        ++nfull;
+                        if (j > i) ++nhalf;
+                    }
+                }
+            }
+    if (MODE == 0) {
+        full_count[i] = nfull;
+        half_count[i] = nhalf;
+        if (nhalf > max_nb) *overflow = 1;
+    } else {
+        // rows ascending in j
+        const int a = nb_start[i], b = a + nfull;
+        for (int p = a + 1; p < b; ++p) {
+            const int v = nb_j[p];
+            int q = p - 1;
+            while (q >= a && nb_j[q] > v) {
+                nb_j[q + 1] = nb_j[q];
+                --q;
+            }
+            nb_j[q + 1] = v;
+        }
+    }
+}
+
+// ContactHistory save/restore (contact_history.py:91-139) as a merge of the
+// atom's old and new rows, both ascending in j: a new slot inherits the
+// touching flag and shear of the same partner in the old row, if it was
+// touching there.
+template <typename H>
+__global__ void k_hist_restore(int n, const int* nb_start, const int* nb_j, uint8_t* nb_flag,
+                               H* nb_hist, const int* onb_start, const int* onb_j,
+                               const uint8_t* onb_flag, const H* onb_hist, int old_n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s0 = nb_start[i], s1 = nb_start[i + 1];
+    int o = 0, o1 = 0;
+    if (i < old_n) {
+        o = onb_start[i];
+        o1 = onb_start[i + 1];
+    }
+    for (int s = s0; s < s1; ++s) {
+        const int j = nb_j[s];
+        while (o < o1 && onb_j[o] < j) ++o;
+        if (o < o1 && onb_j[o] == j && onb_flag[o]) {
+            nb_flag[s] = 1;
+            nb_hist[s] = onb_hist[o];
+        } else {
+            nb_flag[s] = 0;
+        }
+    }
+}
+
+template <typename H>
+__global__ void k_hist_clear(int nslots, uint8_t* nb_flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslots) nb_flag[s] = 0;
+}
+
+__global__ void k_store_x0(const double4* xr, double4* x0, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x0[i] = xr[i];
+}
+
+// Rows' half entries (j > i) in the reference's flat pair layout.
+template <typename H>
+__global__ void k_half_pairs(int n, const int* nb_start, const int* nb_j, const uint8_t* nb_flag,
+                             const H* nb_hist, const int* tag, const int* half_start, int* pi,
+                             int* pj, int* partner, double* shear) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int h = half_start[i];
+    for (int s = nb_start[i]; s < nb_start[i + 1]; ++s) {
+        const int j = nb_j[s];
+        if (j <= i) continue;
+        pi[h] = i;
+        pj[h] = j;
+        const bool on = nb_flag[s] != 0;
+        partner[h] = on ? tag[j] : -1;
+        const H hv = nb_hist[s];
+        shear[3 * h + 0] = on ? (double)hv.x : 0.0;
+        shear[3 * h + 1] = on ? (double)hv.y : 0.0;
+        shear[3 * h + 2] = on ? (double)hv.z : 0.0;
+        ++h;
+    }
+}
+
+// Seed both copies of every slot from a flat half-list history
+// (partner tag or -1, shear), as the pair kernels would read it.
+template <typename H>
+__global__ void k_set_half(int n, const int* nb_start, const int* nb_j, uint8_t* nb_flag, H* nb_hist,
+                           const int* tag, const int* half_start, const int* partner,
+                           const double* shear) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int s = nb_start[i]; s < nb_start[i + 1]; ++s) {
+        const int j = nb_j[s];
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        // position of b among a's entries greater than a
+        const int r0 = nb_start[a], r1 = nb_start[a + 1];
+        int lo = r0, hi = r1;
+        while (lo < hi) {  // first entry > a
+            const int mid = (lo + hi) >> 1;
+            if (nb_j[mid] > a) hi = mid; else lo = mid + 1;
+        }
+        const int first_gt = lo;
+        lo = first_gt; hi = r1;
+        while (lo < hi) {  // entry == b
+            const int mid = (lo + hi) >> 1;
+            if (nb_j[mid] < b) lo = mid + 1; else hi = mid;
+        }
+        const int h = half_start[a] + (lo - first_gt);
+        const bool on = partner[h] == tag[b];
+        nb_flag[s] = on ? 1 : 0;
+        H hv;
+        hv.x = shear[3 * h + 0];
+        hv.y = shear[3 * h + 1];
+        hv.z = shear[3 * h + 2];
+        hv.w = 0;
+        nb_hist[s] = hv;
+    }
+}
+
+__global__ void k_half_count(int n, const int* nb_start, const int* nb_j, int* half_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    for (int s = nb_start[i]; s < nb_start[i + 1]; ++s)
+        if (nb_j[s] > i) ++c;
+    half_count[i] = c;
+}
+
+}  // namespace
+
+// ================================================================ launchers
+
+static inline dim3 grid_for(int n) { return dim3(nblocks_for(n > 0 ? n : 1)); }
+
+int launch_pbc(System& s) {
+    if (s.n <= 0) return 0;
+    hipLaunchKernelGGL(k_pbc, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.vm, s.n, s.ctl);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_scan(System& s, const int* in, int* out, int n) {
+    const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int nt = ntiles > 0 ? ntiles : 1;
+    if (s.scan_tmp_cap < nt + 1) return tmps_fail(TMPS_ERR_ARG, "scan workspace too small");
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TMPS_BLOCK), 0, s.stream, in, out, n, s.scan_tmp);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.scan_tmp, nt, out + n);
+    HIP_TRY(hipGetLastError());
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_add, grid_for(n), dim3(TMPS_BLOCK), 0, s.stream, out, n, s.scan_tmp);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int launch_bins(System& s, const double4* xr) {
+    const int ncell = s.grid.ncell;
+    hipLaunchKernelGGL(k_zero_int, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_count,
+                       ncell + 1);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_int, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_fill,
+                       ncell + 1);
+    HIP_TRY(hipGetLastError());
+    if (s.n > 0) {
+        hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
+                           s.grid, s.cell_of, s.cell_count);
+        HIP_TRY(hipGetLastError());
+    }
+    int rc = launch_scan(s, s.cell_count, s.cell_start, ncell);
+    if (rc) return rc;
+    if (s.n > 0) {
+        hipLaunchKernelGGL(k_bin_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n, s.cell_of,
+                           s.cell_start, s.cell_fill, s.cell_particles);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bin_sort, grid_for(ncell), dim3(TMPS_BLOCK), 0, s.stream, ncell,
+                       s.cell_start, s.cell_particles);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Counts, scans and fills the full list. *total_out receives the slot count
+// (a host synchronisation, once per rebuild).
+int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s.stream, s.flags, 4);
+    HIP_TRY(hipGetLastError());
+    if (s.n > 0) {
+        hipLaunchKernelGGL(k_list<0>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
+                           s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb, s.full_count,
+                           s.half_count, s.flags, (const int*)nullptr, (int*)nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    int rc = launch_scan(s, s.full_count, s.nb_start, s.n);
+    if (rc) return rc;
+    int hv[2];
+    HIP_TRY(hipMemcpyAsync(&hv[0], s.flags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemcpyAsync(&hv[1], s.nb_start + s.n, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    *overflow_out = hv[0];
+    *total_out = hv[1];
+    if (hv[0]) return 0;
+    const size_t need = (size_t)hv[1];
+    if (need > s.nb_cap) {
+        // grow the current list (the old list keeps its own buffers)
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        size_t cap = need + need / 4 + 1024;
+        const size_t hsz = s.precision == TMPS_FP64 ? sizeof(double4) : sizeof(float4);
+        if (s.nb_j) HIP_TRY(hipFree(s.nb_j));
+        if (s.nb_flag) HIP_TRY(hipFree(s.nb_flag));
+        if (s.nb_hist) HIP_TRY(hipFree(s.nb_hist));
+        HIP_TRY(hipMalloc(&s.nb_j, cap * sizeof(int)));
+        HIP_TRY(hipMalloc(&s.nb_flag, cap));
+        HIP_TRY(hipMalloc(&s.nb_hist, cap * hsz));
+        s.nb_cap = cap;
+        drop_graphs(s);
+    }
+    if (s.n > 0) {
+        hipLaunchKernelGGL(k_list<1>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
+                           s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb, s.full_count,
+                           s.half_count, s.flags, (const int*)s.nb_start, s.nb_j);
+        HIP_TRY(hipGetLastError());
+    }
+    s.nslots = (int)need;
+    return 0;
+}
+
+int launch_hist_restore(System& s) {
+    if (s.n <= 0) return 0;
+    if (!s.have_old) return launch_hist_clear(s);
+    const int old_n = s.onb_start ? s.n : 0;
+    if (s.precision == TMPS_FP64)
+        hipLaunchKernelGGL(k_hist_restore<double4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (double4*)s.nb_hist, s.onb_start, s.onb_j,
+                           s.onb_flag, (const double4*)s.onb_hist, old_n);
+    else
+        hipLaunchKernelGGL(k_hist_restore<float4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (float4*)s.nb_hist, s.onb_start, s.onb_j,
+                           s.onb_flag, (const float4*)s.onb_hist, old_n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_hist_clear(System& s) {
+    if (s.nslots <= 0) return 0;
+    hipLaunchKernelGGL(k_hist_clear<int>, grid_for(s.nslots), dim3(TMPS_BLOCK), 0, s.stream, s.nslots,
+                       s.nb_flag);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_store_x0(System& s) {
+    if (s.n <= 0) return 0;
+    hipLaunchKernelGGL(k_store_x0, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.x0, s.n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// d_counts: n+1 ints of workspace (half counts, scanned in place to offsets).
+int launch_half_pairs(System& s, int* d_pair_i, int* d_pair_j, int* d_partner, double* d_shear,
+                      int* d_counts) {
+    if (s.n <= 0) return 0;
+    hipLaunchKernelGGL(k_half_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n, s.nb_start,
+                       s.nb_j, s.half_count);
+    HIP_TRY(hipGetLastError());
+    int rc = launch_scan(s, s.half_count, d_counts, s.n);
+    if (rc) return rc;
+    if (d_pair_i == nullptr) return 0;
+    if (s.precision == TMPS_FP64)
+        hipLaunchKernelGGL(k_half_pairs<double4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (const double4*)s.nb_hist, s.tag, d_counts,
+                           d_pair_i, d_pair_j, d_partner, d_shear);
+    else
+        hipLaunchKernelGGL(k_half_pairs<float4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (const float4*)s.nb_hist, s.tag, d_counts,
+                           d_pair_i, d_pair_j, d_partner, d_shear);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_set_half_history(System& s, const int* d_partner, const double* d_shear,
+                            const int* d_half_start) {
+    if (s.n <= 0) return 0;
+    if (s.precision == TMPS_FP64)
+        hipLaunchKernelGGL(k_set_half<double4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (double4*)s.nb_hist, s.tag, d_half_start,
+                           d_partner, d_shear);
+    else
+        hipLaunchKernelGGL(k_set_half<float4>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (float4*)s.nb_hist, s.tag, d_half_start,
+                           d_partner, d_shear);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,559 @@
+// FP32-split kernels of the DEM step.
+//
+// Position encoding:  x = lo + (prd / g) * (c + 0.5 + u)
+//   c  integer bin of the last neighbor build (packed 11/11/10 bits),
+//   u  FP32 offset in bin widths (|u| <= 0.5 + drift since the build).
+// The affine `remap x` of fix deform/pressure keeps every atom's fractional
+// position, so under it the encoding needs no per-atom update at all: the new
+// box alone moves the atoms. Contact and integration arithmetic is FP32; the
+// per-atom virial, the kinetic tensor, their reduction and the servo stay FP64.
+#include "contact.h"
+#include "tmps_internal.h"
+
+using namespace tmps;
+
+namespace {
+
+struct SplitPtrs {
+    float4* up;   // u.xyz, radius
+    float4* vm;   // v.xyz, rmass (read by every contact of the step)
+    float4* om;   // omega.xyz, 1 / (0.4 m r^2)
+    float4* vm_out;  // where a fused force pass writes v / omega
+    float4* om_out;
+    float4* f;
+    float4* tq;
+    float4* u0;   // u at the last build
+    const uint32_t* cell;
+    double* vir;  // [n][6]
+    const int* nb_start;
+    const int* nb_j;
+    uint8_t* nb_flag;
+    float4* nb_hist;
+    int n;
+};
+
+SplitPtrs sptrs(System& s, int in_buf = 0, int out_buf = 0) {
+    SplitPtrs p;
+    p.up = s.up; p.f = s.ff; p.tq = s.tqf; p.u0 = s.u0;
+    p.vm = in_buf ? s.vmf2 : s.vmf;
+    p.om = in_buf ? s.omf2 : s.omf;
+    p.vm_out = out_buf ? s.vmf2 : s.vmf;
+    p.om_out = out_buf ? s.omf2 : s.omf;
+    p.cell = s.cell; p.vir = s.vir;
+    p.nb_start = s.nb_start; p.nb_j = s.nb_j; p.nb_flag = s.nb_flag;
+    p.nb_hist = static_cast<float4*>(s.nb_hist);
+    p.n = s.n;
+    return p;
+}
+
+__device__ __forceinline__ void unpack_cell(uint32_t c, int& cx, int& cy, int& cz) {
+    cx = (int)(c & 0x7FFu);
+    cy = (int)((c >> 11) & 0x7FFu);
+    cz = (int)(c >> 22);
+}
+
+// Separation of a minus b in length units, minimum image taken in bin units.
+__device__ __forceinline__ void split_delta(const SplitConst& sc, const int per[3], uint32_t ca,
+                                            const float4& ua, uint32_t cb, const float4& ub,
+                                            float d[3]) {
+    int ax, ay, az, bx, by, bz;
+    unpack_cell(ca, ax, ay, az);
+    unpack_cell(cb, bx, by, bz);
+    float e[3] = {(float)(ax - bx) + (ua.x - ub.x), (float)(ay - by) + (ua.y - ub.y),
+                  (float)(az - bz) + (ua.z - ub.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (per[k] == 1) e[k] -= sc.g[k] * rintf(e[k] / sc.g[k]);
+        d[k] = e[k] * sc.cw[k];
+    }
+}
+
+template <int STYLE>
+__device__ __forceinline__ void gather_split(int i, uint32_t ci, const float4& ui, const float4& vi,
+                                             const float4& oi, const SplitPtrs& P,
+                                             const SplitConst& sc, const int per[3],
+                                             const PairP& pp, float dt, int shearupdate,
+                                             float accF[3], float accT[3], double accV[6],
+                                             int& ntouch) {
+    const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
+    const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
+    const float zero3[3] = {0.f, 0.f, 0.f};
+    for (int s = s0; s < s1; ++s) {
+        const int j = P.nb_j[s];
+        const float4 uj = P.up[j];
+        const uint32_t cj = P.cell[j];
+        const bool first = i < j;
+        float d[3];
+        if (first) split_delta(sc, per, ci, ui, cj, uj, d);
+        else split_delta(sc, per, cj, uj, ci, ui, d);
+        const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
+        const float radsum = ra + rb;
+        if (rsq < radsum * radsum && rsq > 1e-28f) {
+            const float4 vj = P.vm[j];
+            const float4 oj = P.om[j];
+            const float4& va4 = first ? vi : vj;
+            const float4& vb4 = first ? vj : vi;
+            const float4& oa4 = first ? oi : oj;
+            const float4& ob4 = first ? oj : oi;
+            const float va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+            const float oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+            bool before = false;
+            float h[3] = {0.f, 0.f, 0.f};
+            if (uses_history) {
+                before = P.nb_flag[s] != 0;
+                if (before) {
+                    const float4 hh = P.nb_hist[s];
+                    h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
+                }
+            }
+            ContactOut<float> o;
+            if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                legacy_history_contact<float, false>(
+                    d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                    (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                    pp.limit_damping, dt, shearupdate, before, h, o);
+            else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                legacy_history_contact<float, true>(
+                    d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                    (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                    pp.limit_damping, dt, shearupdate, before, h, o);
+            else
+                granular_contact<float>(d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob,
+                                        pp, dt, shearupdate, before, h, o);
+            if (uses_history) {
+                if (!before) P.nb_flag[s] = 1;
+                P.nb_hist[s] = make_float4(h[0], h[1], h[2], 0.f);
+            }
+            if (first) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    accF[k] += o.F[k];
+                    accT[k] += o.ta[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    accF[k] -= o.F[k];
+                    accT[k] += o.tb[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) accV[k] += (double)o.vir[k];
+            ++ntouch;
+        } else if (uses_history) {
+            if (P.nb_flag[s] != 0) P.nb_flag[s] = 0;
+        }
+    }
+}
+
+__device__ __forceinline__ float cundall_scale_f(float fv, float g) {
+    return 1.0f - g * (fv >= 0.0f ? 1.0f : -1.0f);
+}
+
+// ---------------------------------------------------------- conversions
+
+// x (FP64) of every atom from its split encoding, into xr (x, y, z, r).
+__global__ void k_split_to_x64(SplitPtrs P, double4* xr, const Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const DevBox& B = ctl->box;
+    const SplitConst& sc = ctl->sc;
+    int c[3];
+    unpack_cell(P.cell[i], c[0], c[1], c[2]);
+    const float4 u = P.up[i];
+    const float uu[3] = {u.x, u.y, u.z};
+    double x[3];
+    for (int k = 0; k < 3; ++k) {
+        const double cw = B.prd[k] / (double)sc.gi[k];
+        x[k] = B.lo[k] + cw * ((double)c[k] + 0.5 + (double)uu[k]);
+    }
+    xr[i] = make_double4(x[0], x[1], x[2], (double)u.w);
+}
+
+// Split encoding of every atom from xr on grid G; u0 encodes x0 (the
+// positions of the last build) in the same bins, relative to the build box.
+__global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0, const Ctl* ctl,
+                               GridP G, uint32_t* cell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const DevBox& B = ctl->box;
+    const double4 p = xr[i], q = x0[i];
+    const double xs[3] = {p.x, p.y, p.z}, x0s[3] = {q.x, q.y, q.z};
+    int c[3];
+    float u[3], u0[3];
+    for (int k = 0; k < 3; ++k) {
+        const double lam = (xs[k] - B.lo[k]) / B.prd[k];
+        int ck = (int)floor(lam / G.csize[k]);
+        ck = max(0, min(ck, G.g[k] - 1));
+        c[k] = ck;
+        const double g = (double)G.g[k];
+        u[k] = (float)(lam * g - ((double)ck + 0.5));
+        const double lam0 = (x0s[k] - ctl->lo_build[k]) / ctl->prd_build[k];
+        u0[k] = (float)(lam0 * g - ((double)ck + 0.5));
+    }
+    cell[i] = (uint32_t)c[0] | ((uint32_t)c[1] << 11) | ((uint32_t)c[2] << 22);
+    float4 up = P.up[i];
+    up.x = u[0]; up.y = u[1]; up.z = u[2];
+    P.up[i] = up;
+    P.u0[i] = make_float4(u0[0], u0[1], u0[2], 0.f);
+}
+
+// ------------------------------------------------------------ stage kernels
+
+__global__ void k_clear_split(SplitPtrs P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    P.f[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.tq[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = 0.0;
+}
+
+__global__ void k_nve_initial_split(SplitPtrs P, const Ctl* ctl, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const SplitConst& sc = ctl->sc;
+    const float dtf = 0.5f * dt;
+    float4 u = P.up[i], v = P.vm[i], w = P.om[i];
+    const float4 f = P.f[i], t = P.tq[i];
+    const float im = 1.0f / v.w;
+    v.x += dtf * f.x * im; v.y += dtf * f.y * im; v.z += dtf * f.z * im;
+    u.x += dt * sc.inv_cw[0] * v.x; u.y += dt * sc.inv_cw[1] * v.y; u.z += dt * sc.inv_cw[2] * v.z;
+    w.x += dtf * t.x * w.w; w.y += dtf * t.y * w.w; w.z += dtf * t.z * w.w;
+    P.up[i] = u; P.vm[i] = v; P.om[i] = w;
+}
+
+__global__ void k_nve_final_split(SplitPtrs P, float dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const float dtf = 0.5f * dt;
+    float4 v = P.vm[i], w = P.om[i];
+    const float4 f = P.f[i], t = P.tq[i];
+    const float im = 1.0f / v.w;
+    v.x += dtf * f.x * im; v.y += dtf * f.y * im; v.z += dtf * f.z * im;
+    w.x += dtf * t.x * w.w; w.y += dtf * t.y * w.w; w.z += dtf * t.z * w.w;
+    P.vm[i] = v; P.om[i] = w;
+}
+
+__global__ void k_cundall_split(SplitPtrs P, float gl, float ga) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    float4 f = P.f[i], t = P.tq[i];
+    const float4 v = P.vm[i], w = P.om[i];
+    f.x *= cundall_scale_f(f.x * v.x, gl); t.x *= cundall_scale_f(t.x * w.x, ga);
+    f.y *= cundall_scale_f(f.y * v.y, gl); t.y *= cundall_scale_f(t.y * w.y, ga);
+    f.z *= cundall_scale_f(f.z * v.z, gl); t.z *= cundall_scale_f(t.z * w.z, ga);
+    P.f[i] = f; P.tq[i] = t;
+}
+
+template <int STYLE>
+__global__ void k_pair_split(SplitPtrs P, const Ctl* ctl, PairP pp, float dt, int shearupdate,
+                             int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const SplitConst sc = ctl->sc;
+    const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
+    float F[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+    double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (accumulate) {
+        const float4 f = P.f[i], t = P.tq[i];
+        F[0] = f.x; F[1] = f.y; F[2] = f.z;
+        T[0] = t.x; T[1] = t.y; T[2] = t.z;
+        for (int k = 0; k < 6; ++k) V[k] = P.vir[6 * i + k];
+    }
+    int nt = 0;
+    gather_split<STYLE>(i, P.cell[i], P.up[i], P.vm[i], P.om[i], P, sc, per, pp, dt, shearupdate, F,
+                        T, V, nt);
+    P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
+    P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
+    for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+}
+
+// |x - x0|^2 in the split encoding: the box change since the build plus the
+// atom's own drift.
+__device__ __forceinline__ float split_disp_sq(const SplitConst& sc, uint32_t c, const float4& u,
+                                               const float4& u0) {
+    int cc[3];
+    unpack_cell(c, cc[0], cc[1], cc[2]);
+    const float uu[3] = {u.x, u.y, u.z}, u00[3] = {u0.x, u0.y, u0.z};
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float d = sc.dlo[k] + sc.dcw[k] * ((float)cc[k] + 0.5f + uu[k]) +
+                        sc.cw_build[k] * (uu[k] - u00[k]);
+        s += d * d;
+    }
+    return s;
+}
+
+__global__ void k_check_split(SplitPtrs P, Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    if (split_disp_sq(ctl->sc, P.cell[i], P.up[i], P.u0[i]) > (float)ctl->trigger_sq) ctl->moved = 1;
+}
+
+__global__ void k_leaves_split(SplitPtrs P, int keflag, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < P.n) {
+        for (int k = 0; k < 6; ++k) leaf[k] = P.vir[6 * i + k];
+        if (keflag) {
+            const float4 v4 = P.vm[i];
+            const double m = v4.w, vx = v4.x, vy = v4.y, vz = v4.z;
+            leaf[0] += m * vx * vx; leaf[1] += m * vy * vy; leaf[2] += m * vz * vz;
+            leaf[3] += m * vx * vy; leaf[4] += m * vx * vz; leaf[5] += m * vy * vz;
+        }
+    }
+    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+__global__ void k_ke_leaves_split(SplitPtrs P, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[2] = {0.0, 0.0};
+    if (i < P.n) {
+        const float4 v4 = P.vm[i], w4 = P.om[i];
+        const double m = v4.w, r = P.up[i].w;
+        const double vsq = (double)v4.x * v4.x + (double)v4.y * v4.y + (double)v4.z * v4.z;
+        leaf[0] = 0.5 * m * vsq;
+        const double wsq = (double)w4.x * w4.x + (double)w4.y * w4.y + (double)w4.z * w4.z;
+        leaf[1] = 0.5 * (0.4 * m * r * r) * wsq;
+    }
+    block_tree_sum<2>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+// ------------------------------------------------------------ fused steps
+
+__global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int t = ctl->timestep + 1;
+    const SplitConst& sc = ctl->sc;
+    float4 u = P.up[i];
+    float4 v = P.vm[i];
+    const float dtf = 0.5f * dt;
+    if (kick) {
+        const float4 f = P.f[i], tq = P.tq[i];
+        const float im = 1.0f / v.w;
+        v.x += dtf * f.x * im; v.y += dtf * f.y * im; v.z += dtf * f.z * im;
+        float4 w = P.om[i];
+        w.x += dtf * tq.x * w.w; w.y += dtf * tq.y * w.w; w.z += dtf * tq.z * w.w;
+        P.om[i] = w;
+        P.vm[i] = v;
+    }
+    u.x += dt * sc.inv_cw[0] * v.x;
+    u.y += dt * sc.inv_cw[1] * v.y;
+    u.z += dt * sc.inv_cw[2] * v.z;
+    P.up[i] = u;
+    const int since = t - ctl->last_build_step;
+    if (since >= ctl->delay && since % ctl->every == 0) {
+        if (!ctl->check) {
+            if (i == 0) ctl->trigger = t;
+        } else if (split_disp_sq(sc, P.cell[i], u, P.u0[i]) > (float)ctl->trigger_sq) {
+            ctl->trigger = t;
+        }
+    }
+}
+
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void k_step_force_split(SplitPtrs P, Ctl* ctl, PairP pp, float dt, float gl, float ga,
+                                   double* partial) {
+    if (ctl->abort) return;
+    const int t = ctl->timestep + 1;
+    if (ctl->trigger == t) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < P.n) {
+        const SplitConst sc = ctl->sc;
+        const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
+        const float4 ui = P.up[i];
+        float4 v = P.vm[i];
+        float4 w = P.om[i];
+        float F[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+        double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int nt = 0;
+        gather_split<STYLE>(i, P.cell[i], ui, v, w, P, sc, per, pp, dt, 1, F, T, V, nt);
+        if (CUNDALL) {
+            F[0] *= cundall_scale_f(F[0] * v.x, gl); T[0] *= cundall_scale_f(T[0] * w.x, ga);
+            F[1] *= cundall_scale_f(F[1] * v.y, gl); T[1] *= cundall_scale_f(T[1] * w.y, ga);
+            F[2] *= cundall_scale_f(F[2] * v.z, gl); T[2] *= cundall_scale_f(T[2] * w.z, ga);
+        }
+        const float dtf = 0.5f * dt;
+        const float im = 1.0f / v.w;
+        const float am[3] = {F[0] * im, F[1] * im, F[2] * im};
+        const float ai[3] = {T[0] * w.w, T[1] * w.w, T[2] * w.w};
+        v.x += dtf * am[0]; v.y += dtf * am[1]; v.z += dtf * am[2];
+        w.x += dtf * ai[0]; w.y += dtf * ai[1]; w.z += dtf * ai[2];
+        const double m = v.w, vx = v.x, vy = v.y, vz = v.z;
+        leaf[0] = V[0] + m * vx * vx;
+        leaf[1] = V[1] + m * vy * vy;
+        leaf[2] = V[2] + m * vz * vz;
+        leaf[3] = V[3] + m * vx * vy;
+        leaf[4] = V[4] + m * vx * vz;
+        leaf[5] = V[5] + m * vy * vz;
+        leaf[6] = (double)nt;
+        if (FINAL) {
+            P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
+            P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+        } else {
+            v.x += dtf * am[0]; v.y += dtf * am[1]; v.z += dtf * am[2];
+            w.x += dtf * ai[0]; w.y += dtf * ai[1]; w.z += dtf * ai[2];
+        }
+        P.vm_out[i] = v;
+        P.om_out[i] = w;
+    }
+    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+__global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, const float4* wsrc,
+                                int n, const Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vdst[i] = vsrc[i];
+    wdst[i] = wsrc[i];
+}
+
+}  // namespace
+
+// ================================================================ launchers
+
+#define LAUNCH_S(kernel, n, ...)                                                           \
+    do {                                                                                   \
+        if ((n) > 0) hipLaunchKernelGGL(kernel, dim3(nblocks_for(n)), dim3(TMPS_BLOCK), 0, \
+                                        s.stream, __VA_ARGS__);                            \
+        HIP_TRY(hipGetLastError());                                                        \
+    } while (0)
+
+int launch_split_to_x64(System& s) {
+    LAUNCH_S(k_split_to_x64, s.n, sptrs(s), s.xr, s.ctl);
+    return 0;
+}
+
+int launch_x64_to_split(System& s) {
+    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, s.cell);
+    return 0;
+}
+
+int launch_clear_split(System& s) {
+    LAUNCH_S(k_clear_split, s.n, sptrs(s));
+    return 0;
+}
+
+int launch_nve_initial_split(System& s, double dt) {
+    LAUNCH_S(k_nve_initial_split, s.n, sptrs(s), s.ctl, (float)dt);
+    return 0;
+}
+
+int launch_nve_final_split(System& s, double dt) {
+    LAUNCH_S(k_nve_final_split, s.n, sptrs(s), (float)dt);
+    return 0;
+}
+
+int launch_cundall_split(System& s, double gl, double ga) {
+    LAUNCH_S(k_cundall_split, s.n, sptrs(s), (float)gl, (float)ga);
+    return 0;
+}
+
+int launch_pair_split(System& s, double dt, int shearupdate, int accumulate) {
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            LAUNCH_S(k_pair_split<TMPS_PAIR_HOOKE_HISTORY>, s.n, sptrs(s), s.ctl, s.pair, (float)dt,
+                     shearupdate, accumulate);
+            break;
+        case TMPS_PAIR_HERTZ_HISTORY:
+            LAUNCH_S(k_pair_split<TMPS_PAIR_HERTZ_HISTORY>, s.n, sptrs(s), s.ctl, s.pair, (float)dt,
+                     shearupdate, accumulate);
+            break;
+        default:
+            LAUNCH_S(k_pair_split<TMPS_PAIR_GRANULAR>, s.n, sptrs(s), s.ctl, s.pair, (float)dt,
+                     shearupdate, accumulate);
+            break;
+    }
+    return 0;
+}
+
+int launch_check_split(System& s) {
+    LAUNCH_S(k_check_split, s.n, sptrs(s), s.ctl);
+    return 0;
+}
+
+int launch_leaves_split(System& s, int keflag) {
+    const int nb = nblocks_for(s.n);
+    hipLaunchKernelGGL(k_leaves_split, dim3(nb > 0 ? nb : 1), dim3(TMPS_BLOCK), 0, s.stream,
+                       sptrs(s), keflag, s.partial);
+    HIP_TRY(hipGetLastError());
+    return launch_final_tree(s, 6, nb > 0 ? nb : 1);
+}
+
+int launch_ke_split(System& s, double* out2) {
+    (void)out2;
+    const int nb = nblocks_for(s.n);
+    hipLaunchKernelGGL(k_ke_leaves_split, dim3(nb > 0 ? nb : 1), dim3(TMPS_BLOCK), 0, s.stream,
+                       sptrs(s), s.partial);
+    HIP_TRY(hipGetLastError());
+    return launch_final_tree(s, 2, nb > 0 ? nb : 1);
+}
+
+__global__ void k_split_consts(Ctl* ctl, int set_build) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) update_split_consts(ctl, set_build);
+}
+
+int launch_split_consts(System& s, int set_build) {
+    hipLaunchKernelGGL(k_split_consts, dim3(1), dim3(64), 0, s.stream, s.ctl, set_build);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_step_begin_split(System& s, double dt, int kick, int buf) {
+    LAUNCH_S(k_step_begin_split, s.n, sptrs(s, buf, buf), s.ctl, (float)dt, kick);
+    return 0;
+}
+
+int launch_copy_vw_split(System& s, int honor_abort) {
+    LAUNCH_S(k_copy_vw_split, s.n, s.vmf, s.omf, s.vmf2, s.omf2, s.n, s.ctl, honor_abort);
+    return 0;
+}
+
+template <int STYLE>
+static int launch_force_split_style(System& s, double dt, int final_mode, int cundall, double gl,
+                                    double ga, int in_buf, int out_buf) {
+    const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
+    const SplitPtrs P = sptrs(s, in_buf, out_buf);
+    const float fdt = (float)dt, fgl = (float)gl, fga = (float)ga;
+    if (final_mode) {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 1>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, fdt, fgl, fga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 1>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, fdt, fgl, fga, s.partial);
+    } else {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 0>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, fdt, fgl, fga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 0>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, fdt, fgl, fga, s.partial);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_step_force_split(System& s, double dt, int final_mode, int cundall, double gl,
+                            double ga, int in_buf, int out_buf) {
+    if (s.n <= 0) return 0;
+    if (in_buf == out_buf) return tmps_fail(TMPS_ERR_ARG, "force pass cannot update v in place");
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            return launch_force_split_style<TMPS_PAIR_HOOKE_HISTORY>(s, dt, final_mode, cundall, gl,
+                                                                     ga, in_buf, out_buf);
+        case TMPS_PAIR_HERTZ_HISTORY:
+            return launch_force_split_style<TMPS_PAIR_HERTZ_HISTORY>(s, dt, final_mode, cundall, gl,
+                                                                     ga, in_buf, out_buf);
+        default:
+            return launch_force_split_style<TMPS_PAIR_GRANULAR>(s, dt, final_mode, cundall, gl, ga,
+                                                                in_buf, out_buf);
+    }
+}

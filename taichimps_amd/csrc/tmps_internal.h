@@ -1,0 +1,250 @@
+// Internal structures shared by the HIP kernels and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tmps.h"
+
+#define TMPS_BLOCK 256
+#define TMPS_MAX_SEG 64  // steps per captured graph segment
+
+// ---------------------------------------------------------------- device state
+
+// Domain device fields (domain.py:55-68), orthogonal box.
+struct DevBox {
+    double lo[3], hi[3], prd[3];
+    double h_rate[6];
+    int per[3];
+    int vremap;
+};
+
+// FixDeformPressure state held on the device (deform_pressure.py:216-226).
+struct DevServo {
+    tmps_servo_params p;
+    double lo_start[3], hi_start[3], cum[3];
+    int nsteps, step_count, nsteps_total;
+};
+
+// A pending affine remap (remap_positions, deform_pressure.py:493-530): the
+// box before and after the end-of-step update of `step`.
+struct RemapRec {
+    int step;  // -1: none pending
+    int pad;
+    double old_lo[3], old_prd[3], oh[3], new_lo[3], new_prd[3];
+};
+
+// Per-step constants of the FP32-split position encoding
+//   x = lo + (prd / g) * (c + 0.5 + u)      (c: integer cell, u: FP32)
+struct SplitConst {
+    float cw[3];       // prd / g of the current box
+    float inv_cw[3];   // g / prd (length -> bin units)
+    float dlo[3];      // lo - lo_build           (displacement check)
+    float dcw[3];      // (prd - prd_build) / g   (displacement check)
+    float cw_build[3]; // prd_build / g
+    float g[3];        // grid dimensions as floats
+    int gi[3];
+    float h_rate[6];
+    int pad;
+};
+
+struct Ctl {
+    DevBox box;
+    DevServo servo;
+    RemapRec remap;
+    SplitConst sc;
+    double tensor[6];   // pressure tensor used by the last servo update
+    double vsum[6];     // last reduced virial(+kinetic) sum, or KE pair
+    double lo_build[3], prd_build[3];
+    int timestep;       // last completed step
+    int trigger;        // step whose displacement check asked for a rebuild
+    int abort;          // nonzero: the rest of the segment is skipped
+    int error;          // TMPS_ERR_BOX after a collapsed box
+    int last_build_step;
+    int delay, every, check;
+    double trigger_sq;
+    int moved;          // stage-mode displacement flag
+    int ntouch_acc;     // touching slots counted by the force pass
+    int ntouch;
+    int servo_on;       // end-of-step servo enabled for fused steps
+    int pad2;
+};
+
+// Pair parameters as the kernels read them.
+struct PairP {
+    int style, limit_damping;
+    double kn, kt, gamman, gammat, xmu;
+    int normal_model, damping_model, tangential_model, uses_history;
+    double g_kn, g_damp, g_kt, g_xt, g_mu;
+};
+
+struct GridP {
+    int g[3];
+    double csize[3];
+    int ncell;
+};
+
+// ---------------------------------------------------------------- host system
+
+struct System {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = TMPS_FP64;
+    int max_atoms = 0, max_nb = 64, n = 0;
+
+    // FP64 per-atom state (double4 packing: x|r, v|m, omega|0)
+    double4 *xr = nullptr, *vm = nullptr, *om = nullptr, *f = nullptr, *tq = nullptr, *x0 = nullptr;
+    // second v/omega buffers: a fused force pass reads one and writes the other,
+    // since other atoms' contacts still read this step's half-step velocities
+    double4 *vm2 = nullptr, *om2 = nullptr;
+    double* vir = nullptr;  // [n][6]
+    int* tag = nullptr;
+    // FP32-split per-atom state
+    float4 *up = nullptr, *vmf = nullptr, *omf = nullptr, *ff = nullptr, *tqf = nullptr, *u0 = nullptr;
+    float4 *vmf2 = nullptr, *omf2 = nullptr;
+    uint32_t* cell = nullptr;
+
+    // full neighbor list (CSR) and its history; o* = previous list for the remap
+    int *nb_start = nullptr, *nb_j = nullptr, *onb_start = nullptr, *onb_j = nullptr;
+    uint8_t *nb_flag = nullptr, *onb_flag = nullptr;
+    void *nb_hist = nullptr, *onb_hist = nullptr;  // double4 (FP64) or float4 (split)
+    size_t nb_cap = 0, onb_cap = 0;
+    int nslots = 0, onslots = 0, onb_n = 0;
+    bool have_old = false;
+    bool x64_valid = false;  // FP32-split: xr holds the authoritative positions
+
+    // bins
+    int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr,
+        *cell_particles = nullptr;
+    int cell_cap = 0;
+    int *full_count = nullptr, *half_count = nullptr, *scan_tmp = nullptr, *flags = nullptr;
+    int scan_tmp_cap = 0;
+
+    // reductions
+    double* partial = nullptr;  // [blocks][8]
+    int partial_cap = 0;
+
+    Ctl* ctl = nullptr;   // device control block
+    Ctl* hctl = nullptr;  // pinned host mirror
+
+    // host mirrors
+    std::vector<double> h_radius;
+    double max_r = 0.0;
+    DevBox box{};
+    bool box_on_device_newer = false;
+
+    double skin = 0.001;
+    int delay = 0, every = 1, check = 1, max_grid_cells = 200000;
+    GridP grid{};
+    int has_built = 0, build_count = 0, last_build_step = -1, current_step = 0;
+    int timestep = 0, setup_done = 0, rebuilds = 0, npairs = 0;
+
+    PairP pair{};
+    bool have_pair = false;
+    tmps_servo_params servo{};
+    bool have_servo = false;
+    int servo_step_count = 0, servo_nsteps = 0;
+
+    // captured step segments, keyed by their launch configuration
+    static const int kGraphSlots = 4;
+    hipGraphExec_t seg_exec[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t seg_key[kGraphSlots] = {0, 0, 0, 0};
+    int seg_next = 0;
+
+    // profiling
+    bool profiling = false;
+    double prof_ms[3] = {0, 0, 0};
+    int prof_cnt[3] = {0, 0, 0};
+};
+
+// ---------------------------------------------------------------- launchers
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t _e = (expr);                                         \
+        if (_e != hipSuccess) return tmps_fail_hip(_e, #expr, __LINE__); \
+    } while (0)
+
+int tmps_fail_hip(hipError_t e, const char* what, int line);
+int tmps_fail(int code, const std::string& msg);
+
+inline int nblocks_for(int n) { return (n + TMPS_BLOCK - 1) / TMPS_BLOCK; }
+
+// rebuild (k_rebuild.hip)
+int launch_pbc(System& s);
+int launch_bins(System& s, const double4* xr);
+int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out);
+int launch_hist_restore(System& s);
+int launch_store_x0(System& s);
+int launch_scan(System& s, const int* in, int* out, int n);  // exclusive, out has n+1
+int launch_half_pairs(System& s, int* d_pair_i, int* d_pair_j, int* d_partner, double* d_shear,
+                      int* d_counts);
+int launch_set_half_history(System& s, const int* d_partner, const double* d_shear,
+                            const int* d_half_start);
+int launch_hist_clear(System& s);
+
+// FP64 path (k_fp64.hip)
+int launch_clear_f64(System& s);
+int launch_nve_initial_f64(System& s, double dt);
+int launch_nve_final_f64(System& s, double dt);
+int launch_cundall_f64(System& s, double gl, double ga);
+int launch_pair_f64(System& s, double dt, int shearupdate, int accumulate);
+int launch_check_f64(System& s);
+int launch_leaves_f64(System& s, int keflag);
+int launch_ke_f64(System& s, double* out2);
+int launch_step_begin_f64(System& s, double dt, int kick, int buf);
+int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, double gl, double ga,
+                          int in_buf, int out_buf);
+// copy v/omega buffer 1 -> buffer 0 (skipped on abort when honor_abort)
+int launch_copy_vw(System& s, int honor_abort);
+
+// shared (k_fp64.hip)
+int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo);
+int launch_remap_apply(System& s);
+int launch_remap_clear(System& s);
+int launch_ctl_rebuilt(System& s, int step);
+int launch_remap_apply_fused(System& s);
+void drop_graphs(System& s);
+int launch_final_tree(System& s, int nc, int nparts);
+
+// FP32-split path (k_split.hip)
+int launch_split_to_x64(System& s);
+int launch_x64_to_split(System& s);
+int launch_step_begin_split(System& s, double dt, int kick, int buf);
+int launch_step_force_split(System& s, double dt, int final_mode, int cundall, double gl, double ga,
+                            int in_buf, int out_buf);
+int launch_copy_vw_split(System& s, int honor_abort);
+int launch_pair_split(System& s, double dt, int shearupdate, int accumulate);
+int launch_nve_initial_split(System& s, double dt);
+int launch_nve_final_split(System& s, double dt);
+int launch_cundall_split(System& s, double gl, double ga);
+int launch_check_split(System& s);
+int launch_leaves_split(System& s, int keflag);
+int launch_ke_split(System& s, double* out2);
+int launch_clear_split(System& s);
+int launch_split_consts(System& s, int set_build);
+int launch_split_fields_to_f64(System& s, int field, double* d_out);
+int launch_split_fields_from_f64(System& s, int field, const double* d_in);
+
+// Per-step constants of the split encoding from the current box; with
+// set_build the current box also becomes the reference of the skin check.
+__device__ __forceinline__ void update_split_consts(Ctl* ctl, int set_build) {
+    SplitConst& sc = ctl->sc;
+    const DevBox& B = ctl->box;
+    for (int k = 0; k < 3; ++k) {
+        if (set_build) {
+            ctl->lo_build[k] = B.lo[k];
+            ctl->prd_build[k] = B.prd[k];
+        }
+        const double g = (double)(sc.gi[k] > 0 ? sc.gi[k] : 1);
+        sc.g[k] = (float)g;
+        sc.cw[k] = (float)(B.prd[k] / g);
+        sc.inv_cw[k] = (float)(g / B.prd[k]);
+        sc.dlo[k] = (float)(B.lo[k] - ctl->lo_build[k]);
+        sc.dcw[k] = (float)((B.prd[k] - ctl->prd_build[k]) / g);
+        sc.cw_build[k] = (float)(ctl->prd_build[k] / g);
+    }
+}
