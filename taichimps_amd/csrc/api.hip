@@ -16,6 +16,9 @@
 
 #include "tmps_internal.h"
 
+// steps per captured graph segment of a device-resident run
+#define TMPS_SEG_LEN 32
+
 namespace {
 thread_local std::string g_err;
 
@@ -125,11 +128,17 @@ int ensure_cells(System& s, int ncell) {
 
 void invalidate_graphs(System& s) { drop_graphs(s); }
 
-double max_radius(const System& s) {
+// max radius of the local atoms (NeighborList.build, neighbor.py:419-420),
+// cached until a radius or the atom count changes
+double max_radius(System& s) {
     if (s.n <= 0) return 1.0;
-    double m = s.h_radius[0];
-    for (int i = 1; i < s.n; ++i) m = std::max(m, s.h_radius[i]);
-    return m;
+    if (!s.max_r_valid) {
+        double m = s.h_radius[0];
+        for (int i = 1; i < s.n; ++i) m = std::max(m, s.h_radius[i]);
+        s.max_r = m;
+        s.max_r_valid = true;
+    }
+    return s.max_r;
 }
 
 // FP32-split: bring the split encoding up to date with xr when xr holds the
@@ -184,7 +193,19 @@ int neigh_build(System& s) {
     rc = ctl_pull(s);
     if (rc) return rc;
     const DevBox B = s.hctl->box;
-    setup_grid(s, B, 2.0 * max_radius(s));
+    const double maxcut = 2.0 * max_radius(s);
+    setup_grid(s, B, maxcut);
+    s.ref_grid = s.grid;
+    // The list is built on the same lamda grid without the reference's
+    // 200 000-bin cap (up to 4 bins per atom): every bin is still at least
+    // one cutoff wide, so the 27-bin stencil finds the same neighbor set while
+    // scanning ~4x fewer candidates at 1M atoms.
+    {
+        const int keep = s.max_grid_cells;
+        s.max_grid_cells = std::max(keep, (int)std::min<long long>(4LL * s.n, 1LL << 26));
+        setup_grid(s, B, maxcut);
+        s.max_grid_cells = keep;
+    }
     rc = ensure_cells(s, s.grid.ncell);
     if (rc) return rc;
     rc = launch_bins(s, s.xr);
@@ -211,6 +232,7 @@ int neigh_build(System& s) {
     rc = ctl_pull(s);
     if (rc) return rc;
     s.hctl->last_build_step = s.last_build_step;
+    for (int k = 0; k < 3; ++k) s.hctl->fast_img[k] = s.grid.g[k] >= 3 ? 1 : 0;
     if (split(s)) {
         if (s.grid.g[0] > 2047 || s.grid.g[1] > 2047 || s.grid.g[2] > 1023)
             return tmps_fail(TMPS_ERR_UNSUPPORTED,
@@ -612,36 +634,58 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     const int t_end = s.timestep + nsteps;
     const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    std::vector<int> seg_start;
-    while (s.timestep < t_end) {
-        seg_start.clear();
-        int t = s.timestep;
-        while (t < t_end) {
-            const int len = std::min(TMPS_MAX_SEG, t_end - t);
+    // Segments run two deep: while one executes, the host waits on the one
+    // before it and reads its status (timestep, trigger, abort, error) from
+    // pinned memory, so a rebuild request costs at most one segment of
+    // skipped kernels.
+    struct Seg {
+        int start, len, slot;
+    };
+    Seg inflight[2];
+    int nin = 0, next_slot = 0;
+    int t = s.timestep;
+    while (true) {
+        while (nin < 2 && t < t_end) {
+            const int len = std::min(TMPS_SEG_LEN, t_end - t);
             rc = launch_segment(s, len, dt, use_cundall, gl, ga);
             if (rc) return rc;
-            seg_start.push_back(t);
+            const int slot = next_slot;
+            next_slot = (next_slot + 1) % System::kGraphSlots;
+            HIP_TRY(hipMemcpyAsync(s.hstatus + 4 * slot, &s.ctl->timestep, 4 * sizeof(int),
+                                   hipMemcpyDeviceToHost, s.stream));
+            HIP_TRY(hipEventRecord(s.seg_event[slot], s.stream));
+            inflight[nin++] = Seg{t, len, slot};
             t += len;
         }
+        if (nin == 0) break;
+        const Seg sg = inflight[0];
+        inflight[0] = inflight[1];
+        --nin;
+        HIP_TRY(hipEventSynchronize(s.seg_event[sg.slot]));
+        const int* st4 = s.hstatus + 4 * sg.slot;
+        const int done = st4[0], aborted = st4[2], err = st4[3];
+        if (!aborted && !err) {
+            s.timestep = done;
+            if (s.has_built) s.current_step = s.timestep;
+            continue;
+        }
+        // skip whatever is still queued; it sees the abort and does nothing
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        nin = 0;
         rc = ctl_pull(s);
         if (rc) return rc;
         rc = check_run_error(s);
         if (rc) return rc;
         s.timestep = s.hctl->timestep;
-        if (s.has_built) s.current_step = s.timestep;
-        if (!s.hctl->abort) continue;
-        // The skin check of step st asked for a rebuild: its drift is done,
+        // The skin check of step stp asked for a rebuild: its drift is done,
         // its force pass is not. Steps of a segment alternate v/omega buffers
         // starting from buffer 0; bring the current one back to buffer 0.
-        const int st = s.hctl->timestep + 1;
-        int t0 = seg_start.empty() ? st : seg_start[0];
-        for (int a : seg_start)
-            if (a <= st - 1) t0 = a;
-        if (((st - 1) - t0) % 2 == 1) {
+        const int stp = s.hctl->timestep + 1;
+        if (((stp - 1) - sg.start) % 2 == 1) {
             rc = split(s) ? launch_copy_vw_split(s, 0) : launch_copy_vw(s, 0);
             if (rc) return rc;
         }
-        s.current_step = st;
+        s.current_step = stp;
         rc = history_save(s);
         if (!rc) rc = pbc(s);
         if (!rc) rc = neigh_build(s);
@@ -649,7 +693,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         if (!rc) rc = ensure_split(s);
         if (rc) return rc;
         s.rebuilds += 1;
-        rc = launch_ctl_rebuilt(s, st);
+        rc = launch_ctl_rebuilt(s, stp);
         if (rc) return rc;
         if (split(s)) {
             rc = launch_split_consts(s, 0);
@@ -666,7 +710,10 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         if (rc) return rc;
         s.timestep = s.hctl->timestep;
         s.current_step = s.timestep;
+        t = s.timestep;
     }
+    rc = ctl_pull(s);
+    if (rc) return rc;
     s.servo_step_count = s.hctl->servo.step_count;
     s.servo_nsteps = s.hctl->servo.nsteps;
     return 0;
@@ -707,6 +754,7 @@ int set_field(System& s, int field, const void* host, int count) {
             if (field == TMPS_RADIUS) {
                 buf[i].w = hd[i];
                 s.h_radius[i] = hd[i];
+                s.max_r_valid = false;
             } else if (field == TMPS_RMASS) {
                 buf[i].w = hd[i];
             } else {
@@ -749,6 +797,7 @@ int set_field(System& s, int field, const void* host, int count) {
         if (field == TMPS_RADIUS) {
             buf[i].w = (float)hd[i];
             s.h_radius[i] = hd[i];
+            s.max_r_valid = false;
         } else if (field == TMPS_RMASS) {
             buf[i].w = (float)hd[i];
         } else {
@@ -864,10 +913,17 @@ void free_all(System& s) {
     dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
     dfree(s.cell_particles);
     dfree(s.full_count); dfree(s.half_count); dfree(s.scan_tmp); dfree(s.flags);
+    dfree(s.row_tmp);
     dfree(s.partial);
     dfree(s.ctl);
     if (s.hctl) (void)hipHostFree(s.hctl);
     s.hctl = nullptr;
+    if (s.hstatus) (void)hipHostFree(s.hstatus);
+    s.hstatus = nullptr;
+    for (int k = 0; k < System::kGraphSlots; ++k) {
+        if (s.seg_event[k]) (void)hipEventDestroy(s.seg_event[k]);
+        s.seg_event[k] = nullptr;
+    }
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
 }
@@ -915,6 +971,10 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     rc = rc ? rc : dalloc(&s.ctl, 1);
     if (rc) return rc;
     HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&s.hstatus, sizeof(int) * 4 * System::kGraphSlots,
+                          hipHostMallocDefault));
+    for (int k = 0; k < System::kGraphSlots; ++k)
+        HIP_TRY(hipEventCreateWithFlags(&s.seg_event[k], hipEventDisableTiming));
     std::memset(s.hctl, 0, sizeof(Ctl));
     s.hctl->remap.step = -1;
     s.hctl->trigger = -1;
@@ -1000,7 +1060,10 @@ int tmps_set_nlocal(tmps_system* h, int nlocal) {
     System& s = *sys(h);
     if (nlocal < 0 || nlocal > s.max_atoms)
         return tmps_fail(TMPS_ERR_CAPACITY, "Exceeded max_atoms capacity");
-    if (nlocal != s.n) invalidate_graphs(s);
+    if (nlocal != s.n) {
+        invalidate_graphs(s);
+        s.max_r_valid = false;
+    }
     s.n = nlocal;
     return 0;
 }
@@ -1087,8 +1150,8 @@ int tmps_neigh_build(tmps_system* h) { return neigh_build(*sys(h)); }
 int tmps_neigh_info(tmps_system* h, int gdim[3], double csize[3], int* npairs) {
     System& s = *sys(h);
     for (int k = 0; k < 3; ++k) {
-        if (gdim) gdim[k] = s.grid.g[k];
-        if (csize) csize[k] = s.grid.csize[k];
+        if (gdim) gdim[k] = s.ref_grid.g[k];
+        if (csize) csize[k] = s.ref_grid.csize[k];
     }
     if (npairs) *npairs = s.npairs;
     return 0;
@@ -1136,11 +1199,24 @@ int tmps_neigh_get_counts(tmps_system* h, int* num_neighbors, int n) {
 
 int tmps_neigh_get_bins(tmps_system* h, int* cell_of, int* cell_start, int* cell_particles) {
     System& s = *sys(h);
+    if (s.ref_grid.ncell == 0) return tmps_fail(TMPS_ERR_ARG, "no neighbor list has been built");
+    // bin the current positions on the reference grid (the list's own bins
+    // are scratch of the build)
+    int rc = split_positions_to_xr(s);
+    if (rc) return rc;
+    const GridP keep = s.grid;
+    s.grid = s.ref_grid;
+    rc = ensure_cells(s, s.grid.ncell);
+    if (!rc) rc = launch_bins(s, s.xr);
+    const GridP ref = s.grid;
+    s.grid = keep;
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s.stream));
+    const GridP& G = ref;
     if (s.n > 0 && cell_of)
         HIP_TRY(hipMemcpy(cell_of, s.cell_of, sizeof(int) * s.n, hipMemcpyDeviceToHost));
     if (cell_start)
-        HIP_TRY(hipMemcpy(cell_start, s.cell_start, sizeof(int) * (s.grid.ncell + 1),
+        HIP_TRY(hipMemcpy(cell_start, s.cell_start, sizeof(int) * (G.ncell + 1),
                           hipMemcpyDeviceToHost));
     if (s.n > 0 && cell_particles)
         HIP_TRY(hipMemcpy(cell_particles, s.cell_particles, sizeof(int) * s.n, hipMemcpyDeviceToHost));

@@ -40,17 +40,43 @@ F64Ptrs ptrs(System& s, int in_buf = 0, int out_buf = 0) {
     return p;
 }
 
+// What the contact gather needs of the box.
+struct ImgP {
+    double prd[3], inv[3];
+    int per[3], fast[3];
+    int vremap;
+    const double* h_rate;
+};
+
+__device__ __forceinline__ ImgP img_params(const Ctl* ctl) {
+    ImgP I;
+    for (int k = 0; k < 3; ++k) {
+        I.prd[k] = ctl->box.prd[k];
+        I.inv[k] = 1.0 / I.prd[k];
+        I.per[k] = ctl->box.per[k];
+        I.fast[k] = ctl->fast_img[k];
+    }
+    I.vremap = ctl->box.vremap;
+    I.h_rate = ctl->box.h_rate;
+    return I;
+}
+
 // Domain.minimum_image_and_vshift, orthogonal branch (domain.py:388-462).
-__device__ __forceinline__ void min_image_vshift(const DevBox& B, double d[3], double dv[3]) {
+// Where the dimension holds at least 3 list bins the image count is taken by
+// reciprocal multiply: it can differ from round(d / prd) only for |d| within
+// rounding of half a box, which is farther than any contact reaches, so every
+// contact sees the reference's separation bit for bit.
+__device__ __forceinline__ void min_image_vshift(const ImgP& B, double d[3], double dv[3]) {
     double nimg[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        if (B.per[k] == 1) nimg[k] = round(d[k] / B.prd[k]);
+        if (B.per[k] == 1) nimg[k] = round(B.fast[k] ? d[k] * B.inv[k] : d[k] / B.prd[k]);
     dv[0] = dv[1] = dv[2] = 0.0;
     if (B.vremap != 0) {
-        dv[0] = nimg[0] * B.h_rate[0] + nimg[1] * B.h_rate[5] + nimg[2] * B.h_rate[4];
-        dv[1] = nimg[1] * B.h_rate[1] + nimg[2] * B.h_rate[3];
-        dv[2] = nimg[2] * B.h_rate[2];
+        const double* h = B.h_rate;
+        dv[0] = nimg[0] * h[0] + nimg[1] * h[5] + nimg[2] * h[4];
+        dv[1] = nimg[1] * h[1] + nimg[2] * h[3];
+        dv[2] = nimg[2] * h[2];
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) d[k] -= nimg[k] * B.prd[k];
@@ -60,7 +86,7 @@ __device__ __forceinline__ void min_image_vshift(const DevBox& B, double d[3], d
 // ascending index order; each pair is evaluated with the lower index first.
 template <int STYLE>
 __device__ __forceinline__ void gather_f64(int i, const double4& xi, const double4& vi,
-                                           const double4& oi, const F64Ptrs& P, const DevBox& B,
+                                           const double4& oi, const F64Ptrs& P, const ImgP& B,
                                            const PairP& pp, double dt, int shearupdate,
                                            double accF[3], double accT[3], double accV[6],
                                            int& ntouch) {
@@ -135,8 +161,6 @@ __device__ __forceinline__ void gather_f64(int i, const double4& xi, const doubl
         }
     }
 }
-
-__device__ __forceinline__ DevBox load_box(const Ctl* ctl) { return ctl->box; }
 
 // ----------------------------------------------------------- stage kernels
 
@@ -218,7 +242,7 @@ __global__ void k_pair_f64(F64Ptrs P, const Ctl* ctl, PairP pp, double dt, int s
                            int accumulate) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
-    const DevBox B = load_box(ctl);
+    const ImgP B = img_params(ctl);
     double accF[3] = {0.0, 0.0, 0.0}, accT[3] = {0.0, 0.0, 0.0};
     double accV[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (accumulate) {
@@ -284,44 +308,60 @@ __global__ void k_ke_leaves_f64(F64Ptrs P, double* partial) {
 
 // ---------------------------------------------------- final tree + servo
 
-// Pairwise tree over nparts block partials (zero-padded to a power of two);
-// thread 0 receives the sums.
-template <int NC>
-__device__ void final_tree(const double* partial, int nparts, double out[NC]) {
-    int P2 = 1;
-    while (P2 < nparts) P2 <<= 1;
-    int L = P2 / TMPS_BLOCK;
-    if (L < 1) L = 1;
-    double stk[16][NC];
-    int sz[16];
-    int top = 0;
+#define FINAL_THREADS 1024
+
+// Pairwise tree over nparts block partials (zero-padded to a power of two),
+// by one block of FINAL_THREADS threads: thread t sums leaves [t*L, t*L+L) as
+// an adjacent-pair tree in registers, then the threads' sums are combined the
+// same way (wavefront butterfly, then the 16 wavefront sums). Thread 0
+// receives the sums.
+template <int L, int NC>
+__device__ void final_tree_L(const double* partial, int nparts, double out[NC]) {
+    __shared__ double ws[FINAL_THREADS / 64][NC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int base = threadIdx.x * L;
-    for (int q = 0; q < L; ++q) {
-        const int idx = base + q;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) stk[top][k] = idx < nparts ? partial[(size_t)idx * 8 + k] : 0.0;
-        sz[top] = 1;
-        ++top;
-        while (top >= 2 && sz[top - 1] == sz[top - 2]) {
+    for (int k = 0; k < NC; ++k) {
+        double t[L];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) stk[top - 2][k] = stk[top - 2][k] + stk[top - 1][k];
-            sz[top - 2] *= 2;
-            --top;
+        for (int q = 0; q < L; ++q)
+            t[q] = (base + q < nparts) ? partial[(size_t)(base + q) * 8 + k] : 0.0;
+#pragma unroll
+        for (int w = 1; w < L; w <<= 1)
+#pragma unroll
+            for (int q = 0; q < L; q += 2 * w) t[q] = t[q] + t[q + w];
+        const double v = wave_tree_sum(t[0]);
+        if (lane == 0) ws[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            double u[FINAL_THREADS / 64];
+#pragma unroll
+            for (int q = 0; q < FINAL_THREADS / 64; ++q) u[q] = ws[q][k];
+#pragma unroll
+            for (int w = 1; w < FINAL_THREADS / 64; w <<= 1)
+#pragma unroll
+                for (int q = 0; q < FINAL_THREADS / 64; q += 2 * w) u[q] = u[q] + u[q + w];
+            out[k] = u[0];
         }
     }
-    double leaf[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) leaf[k] = stk[0][k];
-    __shared__ double res[NC];
-    block_tree_sum<NC>(leaf, res);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < NC; ++k) out[k] = res[k];
+    __syncthreads();
 }
 
-template <int NC>
-__global__ void k_final_tree(Ctl* ctl, const double* partial, int nparts) {
+// Leaves per thread for nparts partials (a power of two, at least 1).
+inline int final_leaves(int nparts) {
+    int p2 = 1;
+    while (p2 < nparts) p2 <<= 1;
+    return p2 <= FINAL_THREADS ? 1 : p2 / FINAL_THREADS;
+}
+
+template <int L, int NC>
+__global__ void __launch_bounds__(FINAL_THREADS) k_final_tree(Ctl* ctl, const double* partial,
+                                                               int nparts) {
     double out[NC];
-    final_tree<NC>(partial, nparts, out);
+    final_tree_L<L, NC>(partial, nparts, out);
     if (threadIdx.x == 0)
         for (int k = 0; k < NC; ++k) ctl->vsum[k] = out[k];
 }
@@ -344,13 +384,19 @@ __device__ double limit_rate(const tmps_servo_params& S, double rate, double pta
 // host path, :598-702 device path): pressure tensor from the reduced
 // virial(+kinetic) sum, couple, per-axis targets, remap record, box publish.
 __device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms) {
-    DevServo& SV = ctl->servo;
+    // work on register copies: the control block is global memory, and a
+    // chain of dependent loads and stores on it costs microseconds
+    DevServo SV = ctl->servo;
     const tmps_servo_params& S = SV.p;
-    DevBox& B = ctl->box;
+    DevBox B = ctl->box;
     SV.step_count += 1;
     SV.nsteps += 1;
     ctl->remap.step = -1;
-    if (SV.step_count % S.nevery != 0) return;
+    if (SV.step_count % S.nevery != 0) {
+        ctl->servo.step_count = SV.step_count;
+        ctl->servo.nsteps = SV.nsteps;
+        return;
+    }
 
     bool needs_pressure = false;
     for (int i = 0; i < 3; ++i)
@@ -436,6 +482,9 @@ __device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms
     }
     double new_prd[3];
     for (int i = 0; i < 3; ++i) new_prd[i] = hi_t[i] - lo_t[i];
+    ctl->servo.step_count = SV.step_count;
+    ctl->servo.nsteps = SV.nsteps;
+    for (int i = 0; i < 3; ++i) ctl->servo.cum[i] = SV.cum[i];
     for (int i = 0; i < 3; ++i)
         if (new_prd[i] <= 0.0) {
             ctl->error = TMPS_ERR_BOX;
@@ -443,7 +492,7 @@ __device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms
             return;
         }
     if (S.remap == 0) {
-        RemapRec& R = ctl->remap;
+        RemapRec R;
         const double xprd = old_prd[0], yprd = old_prd[1], zprd = old_prd[2];
         R.oh[0] = -0.0 / (xprd * yprd);
         R.oh[1] = (0.0 * 0.0 - yprd * 0.0) / (xprd * yprd * zprd);
@@ -455,6 +504,8 @@ __device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms
             R.new_prd[i] = new_prd[i];
         }
         R.step = ctl->timestep + 1;
+        R.pad = 0;
+        ctl->remap = R;
     }
     for (int i = 0; i < 3; ++i) {
         B.h_rate[i] = (new_prd[i] - old_prd[i]) / dt;
@@ -464,25 +515,26 @@ __device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms
     }
     B.h_rate[3] = B.h_rate[4] = B.h_rate[5] = (0.0 - 0.0) / dt;
     B.vremap = S.remap == 1 ? 1 : 0;
+    ctl->box = B;
 }
 
 // Tree-sums the force pass's block partials and, when enabled, runs the
 // servo. `advance` marks a fused step: it honours the segment's abort and
 // rebuild-trigger flags and completes the step's bookkeeping.
-__global__ void k_step_end(Ctl* ctl, const double* partial, int nparts, double dt, int advance,
-                           int apply_servo, int natoms) {
+template <int L>
+__global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const double* partial,
+                                                             int nparts, double dt, int advance,
+                                                             int apply_servo, int natoms) {
     if (advance) {
         if (ctl->abort) return;
         const int t = ctl->timestep + 1;
         if (ctl->trigger == t) {
-            if (threadIdx.x == 0) {
-                ctl->abort = 1;
-            }
+            if (threadIdx.x == 0) ctl->abort = 1;
             return;
         }
     }
     double out[7];
-    final_tree<7>(partial, nparts, out);
+    final_tree_L<L, 7>(partial, nparts, out);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 6; ++k) ctl->vsum[k] = out[k];
         ctl->ntouch = (int)out[6];
@@ -591,7 +643,7 @@ __global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, doubl
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < P.n) {
-        const DevBox B = load_box(ctl);
+        const ImgP B = img_params(ctl);
         const double4 xi = P.xr[i];
         double4 v = P.vm[i];
         double4 w = P.om[i];
@@ -706,13 +758,22 @@ int launch_check_f64(System& s) {
     return 0;
 }
 
+#define TREE_CASE(Lv)                                                                          \
+    case Lv:                                                                                   \
+        if (nc == 2)                                                                           \
+            hipLaunchKernelGGL((k_final_tree<Lv, 2>), dim3(1), dim3(FINAL_THREADS), 0, s.stream, \
+                               s.ctl, s.partial, nparts);                                      \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_final_tree<Lv, 6>), dim3(1), dim3(FINAL_THREADS), 0, s.stream, \
+                               s.ctl, s.partial, nparts);                                      \
+        break;
+
 int launch_final_tree(System& s, int nc, int nparts) {
-    if (nc == 2)
-        hipLaunchKernelGGL(k_final_tree<2>, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
-                           nparts);
-    else
-        hipLaunchKernelGGL(k_final_tree<6>, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
-                           nparts);
+    switch (final_leaves(nparts)) {
+        TREE_CASE(1) TREE_CASE(2) TREE_CASE(4) TREE_CASE(8) TREE_CASE(16) TREE_CASE(32)
+        TREE_CASE(64)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -787,9 +848,18 @@ int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, dou
     }
 }
 
+#define END_CASE(Lv)                                                                         \
+    case Lv:                                                                                 \
+        hipLaunchKernelGGL((k_step_end<Lv>), dim3(1), dim3(FINAL_THREADS), 0, s.stream, s.ctl, \
+                           s.partial, nparts, dt, advance, apply_servo, s.n);                \
+        break;
+
 int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo) {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.partial,
-                       nparts > 0 ? nparts : 1, dt, advance, apply_servo, s.n);
+    if (nparts < 1) nparts = 1;
+    switch (final_leaves(nparts)) {
+        END_CASE(1) END_CASE(2) END_CASE(4) END_CASE(8) END_CASE(16) END_CASE(32) END_CASE(64)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }

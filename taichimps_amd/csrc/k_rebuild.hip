@@ -177,13 +177,26 @@ __global__ void k_scan_add(int* out, int n, const int* tile_sums) {
 
 // ------------------------------------------------------------------ list
 
+// Minimum image of one component for the list build. Where the dimension
+// holds at least 3 bins (so the box is at least two cutoffs wide) the image
+// count comes from a reciprocal multiply: it can differ from round(d / prd)
+// only when d is within rounding of half a box, and then both images are
+// beyond the cutoff, so the neighbor set and every accepted rsq are the
+// reference's.
+__device__ __forceinline__ double list_image(double d, double prd, double inv_prd, bool fast) {
+    return d - prd * round(fast ? d * inv_prd : d / prd);
+}
+
 // The stencil walk of NeighborList.build_neighbor_list (neighbor.py:315-369),
-// visiting cells in the reference's (dx, dy, dz) order. MODE 0 counts the
-// full row and the half row (j > i); MODE 1 writes the full row.
+// visiting bins in the reference's (dx, dy, dz) order. MODE 0 counts the full
+// row and the half row (j > i); MODE 1 writes the full row at nb_start[i];
+// MODE 2 counts and writes the row into a per-atom scratch row of `cap`
+// entries (flags[1] is raised when a row does not fit). Rows end up
+// ascending in j.
 template <int MODE>
 __global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const int* cell_start,
                        const int* particles, double skin, int max_nb, int* full_count,
-                       int* half_count, int* overflow, const int* nb_start, int* nb_j) {
+                       int* half_count, int* flags, const int* nb_start, int* nb_j, int cap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DevBox& B = ctl->box;
@@ -192,55 +205,77 @@ __global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const 
     cell_coord(B, G, pi, c);
     const int gx = G.g[0], gy = G.g[1], gz = G.g[2], gxy = gx * gy;
     const int px = B.per[0], py = B.per[1], pz = B.per[2];
+    const double prd[3] = {B.prd[0], B.prd[1], B.prd[2]};
+    const double inv[3] = {1.0 / prd[0], 1.0 / prd[1], 1.0 / prd[2]};
+    const bool fast[3] = {G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3};
     int nfull = 0, nhalf = 0;
-    int w = MODE == 1 ? nb_start[i] : 0;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
+    int* row = nullptr;
+    if (MODE == 1) row = nb_j + nb_start[i];
+    if (MODE == 2) row = nb_j + (size_t)i * cap;
+    for (int dx = -1; dx <= 1; ++dx) {
+        if (px == 1 && gx == 1 && dx != 0) continue;
+        int nx = c[0] + dx;
+        if (px == 1) nx = nx < 0 ? nx + gx : (nx >= gx ? nx - gx : nx);
+        if (nx < 0 || nx >= gx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            if (py == 1 && gy == 1 && dy != 0) continue;
+            int ny = c[1] + dy;
+            if (py == 1) ny = ny < 0 ? ny + gy : (ny >= gy ? ny - gy : ny);
+            if (ny < 0 || ny >= gy) continue;
             for (int dz = -1; dz <= 1; ++dz) {
-                int nx = c[0] + dx, ny = c[1] + dy, nz = c[2] + dz;
-                const bool skip = (px == 1 && gx == 1 && dx != 0) ||
-                                  (py == 1 && gy == 1 && dy != 0) ||
-                                  (pz == 1 && gz == 1 && dz != 0);
-                if (px == 1) nx = (nx % gx + gx) % gx;
-                if (py == 1) ny = (ny % gy + gy) % gy;
-                if (pz == 1) nz = (nz % gz + gz) % gz;
-                if (skip || nx < 0 || nx >= gx || ny < 0 || ny >= gy || nz < 0 || nz >= gz)
-                    continue;
+                if (pz == 1 && gz == 1 && dz != 0) continue;
+                int nz = c[2] + dz;
+                if (pz == 1) nz = nz < 0 ? nz + gz : (nz >= gz ? nz - gz : nz);
+                if (nz < 0 || nz >= gz) continue;
                 const int cidx = nx + ny * gx + nz * gxy;
                 const int pa = cell_start[cidx], pb = cell_start[cidx + 1];
                 for (int p = pa; p < pb; ++p) {
                     const int j = particles[p];
                     if (j == i) continue;
                     const double4 pj = xr[j];
-                    double d[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-                    for (int k = 0; k < 3; ++k)
-                        if (B.per[k] == 1) d[k] -= B.prd[k] * round(d[k] / B.prd[k]);
-                    const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+                    if (px == 1) d0 = list_image(d0, prd[0], inv[0], fast[0]);
+                    if (py == 1) d1 = list_image(d1, prd[1], inv[1], fast[1]);
+                    if (pz == 1) d2 = list_image(d2, prd[2], inv[2], fast[2]);
+                    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
                     const double rad_sum = pi.w + pj.w + skin;
                     if (rsq < rad_sum * rad_sum) {
-                        if (MODE == 1) nb_j[w++] = j;
+                        if (MODE == 1 || (MODE == 2 && nfull < cap)) row[nfull] = j;
                         ++nfull;
                         if (j > i) ++nhalf;
                     }
                 }
             }
-    if (MODE == 0) {
-        full_count[i] = nfull;
-        half_count[i] = nhalf;
-        if (nhalf > max_nb) *overflow = 1;
-    } else {
-        // rows ascending in j
-        const int a = nb_start[i], b = a + nfull;
-        for (int p = a + 1; p < b; ++p) {
-            const int v = nb_j[p];
-            int q = p - 1;
-            while (q >= a && nb_j[q] > v) {
-                nb_j[q + 1] = nb_j[q];
-                --q;
-            }
-            nb_j[q + 1] = v;
         }
     }
+    if (MODE != 1) {
+        full_count[i] = nfull;
+        half_count[i] = nhalf;
+        if (nhalf > max_nb) flags[0] = 1;
+        if (MODE == 2 && nfull > cap) {
+            flags[1] = 1;
+            return;
+        }
+    }
+    if (MODE != 0) {
+        for (int p = 1; p < nfull; ++p) {
+            const int v = row[p];
+            int q = p - 1;
+            while (q >= 0 && row[q] > v) {
+                row[q + 1] = row[q];
+                --q;
+            }
+            row[q + 1] = v;
+        }
+    }
+}
+
+__global__ void k_list_compact(int n, int cap, const int* rows, const int* nb_start, int* nb_j) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = nb_start[i], m = nb_start[i + 1] - a;
+    const int* src = rows + (size_t)i * cap;
+    for (int k = 0; k < m; ++k) nb_j[a + k] = src[k];
 }
 
 // ContactHistory save/restore (contact_history.py:91-139) as a merge of the
@@ -404,48 +439,68 @@ int launch_bins(System& s, const double4* xr) {
     return 0;
 }
 
-// Counts, scans and fills the full list. *total_out receives the slot count
-// (a host synchronisation, once per rebuild).
+#define LIST_ROW_CAP 32
+
+static int grow_list(System& s, size_t need) {
+    if (need <= s.nb_cap) return 0;
+    // the old list keeps its own buffers; graphs hold the current pointers
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    size_t cap = need + need / 4 + 1024;
+    const size_t hsz = s.precision == TMPS_FP64 ? sizeof(double4) : sizeof(float4);
+    if (s.nb_j) HIP_TRY(hipFree(s.nb_j));
+    if (s.nb_flag) HIP_TRY(hipFree(s.nb_flag));
+    if (s.nb_hist) HIP_TRY(hipFree(s.nb_hist));
+    HIP_TRY(hipMalloc(&s.nb_j, cap * sizeof(int)));
+    HIP_TRY(hipMalloc(&s.nb_flag, cap));
+    HIP_TRY(hipMalloc(&s.nb_hist, cap * hsz));
+    s.nb_cap = cap;
+    drop_graphs(s);
+    return 0;
+}
+
+// Builds the full list: one pass into per-atom scratch rows, a scan, and a
+// compaction (two passes, count then fill, if some row exceeds the scratch).
+// *total_out receives the slot count (a host synchronisation per rebuild).
 int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s.stream, s.flags, 4);
     HIP_TRY(hipGetLastError());
+    const size_t rows_need = (size_t)std::max(s.n, 1) * LIST_ROW_CAP;
+    if (rows_need > s.row_tmp_cap) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        if (s.row_tmp) HIP_TRY(hipFree(s.row_tmp));
+        HIP_TRY(hipMalloc(&s.row_tmp, rows_need * sizeof(int)));
+        s.row_tmp_cap = rows_need;
+    }
     if (s.n > 0) {
-        hipLaunchKernelGGL(k_list<0>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
+        hipLaunchKernelGGL(k_list<2>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
                            s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb, s.full_count,
-                           s.half_count, s.flags, (const int*)nullptr, (int*)nullptr);
+                           s.half_count, s.flags, (const int*)nullptr, s.row_tmp, LIST_ROW_CAP);
         HIP_TRY(hipGetLastError());
     }
     int rc = launch_scan(s, s.full_count, s.nb_start, s.n);
     if (rc) return rc;
-    int hv[2];
-    HIP_TRY(hipMemcpyAsync(&hv[0], s.flags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    HIP_TRY(hipMemcpyAsync(&hv[1], s.nb_start + s.n, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    int hv[3];
+    HIP_TRY(hipMemcpyAsync(&hv[0], s.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemcpyAsync(&hv[2], s.nb_start + s.n, sizeof(int), hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));
     *overflow_out = hv[0];
-    *total_out = hv[1];
+    *total_out = hv[2];
     if (hv[0]) return 0;
-    const size_t need = (size_t)hv[1];
-    if (need > s.nb_cap) {
-        // grow the current list (the old list keeps its own buffers)
-        HIP_TRY(hipStreamSynchronize(s.stream));
-        size_t cap = need + need / 4 + 1024;
-        const size_t hsz = s.precision == TMPS_FP64 ? sizeof(double4) : sizeof(float4);
-        if (s.nb_j) HIP_TRY(hipFree(s.nb_j));
-        if (s.nb_flag) HIP_TRY(hipFree(s.nb_flag));
-        if (s.nb_hist) HIP_TRY(hipFree(s.nb_hist));
-        HIP_TRY(hipMalloc(&s.nb_j, cap * sizeof(int)));
-        HIP_TRY(hipMalloc(&s.nb_flag, cap));
-        HIP_TRY(hipMalloc(&s.nb_hist, cap * hsz));
-        s.nb_cap = cap;
-        drop_graphs(s);
-    }
+    rc = grow_list(s, (size_t)hv[2]);
+    if (rc) return rc;
     if (s.n > 0) {
-        hipLaunchKernelGGL(k_list<1>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
-                           s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb, s.full_count,
-                           s.half_count, s.flags, (const int*)s.nb_start, s.nb_j);
+        if (!hv[1]) {
+            hipLaunchKernelGGL(k_list_compact, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                               LIST_ROW_CAP, (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j);
+        } else {
+            hipLaunchKernelGGL(k_list<1>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
+                               s.ctl, s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb,
+                               s.full_count, s.half_count, s.flags, (const int*)s.nb_start, s.nb_j,
+                               0);
+        }
         HIP_TRY(hipGetLastError());
     }
-    s.nslots = (int)need;
+    s.nslots = hv[2];
     return 0;
 }
 

@@ -63,7 +63,7 @@ __device__ __forceinline__ void split_delta(const SplitConst& sc, const int per[
                   (float)(az - bz) + (ua.z - ub.z)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        if (per[k] == 1) e[k] -= sc.g[k] * rintf(e[k] / sc.g[k]);
+        if (per[k] == 1) e[k] -= sc.g[k] * rintf(e[k] * (1.0f / sc.g[k]));
         d[k] = e[k] * sc.cw[k];
     }
 }

@@ -10,7 +10,6 @@
 #include "../../include/tmps.h"
 
 #define TMPS_BLOCK 256
-#define TMPS_MAX_SEG 64  // steps per captured graph segment
 
 // ---------------------------------------------------------------- device state
 
@@ -70,7 +69,8 @@ struct Ctl {
     int ntouch_acc;     // touching slots counted by the force pass
     int ntouch;
     int servo_on;       // end-of-step servo enabled for fused steps
-    int pad2;
+    int fast_img[3];    // per dim: list bins >= 3 at the last build, so the
+                        // contact kernels may take image counts by reciprocal
 };
 
 // Pair parameters as the kernels read them.
@@ -133,12 +133,16 @@ struct System {
     // host mirrors
     std::vector<double> h_radius;
     double max_r = 0.0;
+    bool max_r_valid = false;
+    int* row_tmp = nullptr;  // single-pass list build: per-atom candidate rows
+    size_t row_tmp_cap = 0;
     DevBox box{};
     bool box_on_device_newer = false;
 
     double skin = 0.001;
     int delay = 0, every = 1, check = 1, max_grid_cells = 200000;
-    GridP grid{};
+    GridP grid{};      // bins of the last list build (also the split encoding's bins)
+    GridP ref_grid{};  // the reference's grid (max_grid_cells cap), reported by the API
     int has_built = 0, build_count = 0, last_build_step = -1, current_step = 0;
     int timestep = 0, setup_done = 0, rebuilds = 0, npairs = 0;
 
@@ -153,6 +157,8 @@ struct System {
     hipGraphExec_t seg_exec[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t seg_key[kGraphSlots] = {0, 0, 0, 0};
     int seg_next = 0;
+    int* hstatus = nullptr;  // pinned: [slot][timestep, trigger, abort, error]
+    hipEvent_t seg_event[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
 
     // profiling
     bool profiling = false;
@@ -232,19 +238,26 @@ int launch_split_fields_from_f64(System& s, int field, const double* d_in);
 // Per-step constants of the split encoding from the current box; with
 // set_build the current box also becomes the reference of the skin check.
 __device__ __forceinline__ void update_split_consts(Ctl* ctl, int set_build) {
-    SplitConst& sc = ctl->sc;
-    const DevBox& B = ctl->box;
+    SplitConst sc = ctl->sc;
+    const DevBox B = ctl->box;
+    double lo_b[3], prd_b[3];
     for (int k = 0; k < 3; ++k) {
-        if (set_build) {
-            ctl->lo_build[k] = B.lo[k];
-            ctl->prd_build[k] = B.prd[k];
-        }
+        lo_b[k] = set_build ? B.lo[k] : ctl->lo_build[k];
+        prd_b[k] = set_build ? B.prd[k] : ctl->prd_build[k];
+    }
+    for (int k = 0; k < 3; ++k) {
         const double g = (double)(sc.gi[k] > 0 ? sc.gi[k] : 1);
         sc.g[k] = (float)g;
         sc.cw[k] = (float)(B.prd[k] / g);
         sc.inv_cw[k] = (float)(g / B.prd[k]);
-        sc.dlo[k] = (float)(B.lo[k] - ctl->lo_build[k]);
-        sc.dcw[k] = (float)((B.prd[k] - ctl->prd_build[k]) / g);
-        sc.cw_build[k] = (float)(ctl->prd_build[k] / g);
+        sc.dlo[k] = (float)(B.lo[k] - lo_b[k]);
+        sc.dcw[k] = (float)((B.prd[k] - prd_b[k]) / g);
+        sc.cw_build[k] = (float)(prd_b[k] / g);
     }
+    ctl->sc = sc;
+    if (set_build)
+        for (int k = 0; k < 3; ++k) {
+            ctl->lo_build[k] = lo_b[k];
+            ctl->prd_build[k] = prd_b[k];
+        }
 }
