@@ -1,0 +1,427 @@
+"""CPU checks that pin the FP64 oracle (no GPU).
+
+Each test fixes one part of the oracle by something other than the oracle's
+own formula: the reference's published literal (LAMMPS-cross-checked), a
+closed form of the mechanics, an invariant the reference's tests assert, or a
+brute-force enumeration. A plausible restatement slip (a dropped damping term,
+a wrong sign in the rotational velocity, a transposed remap, a missed stencil
+cell) fails at least one of them.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from taichimps_amd import workloads
+
+
+def _sim(x, radius, density=1.0, v=None, omega=None, lo=(0.0,) * 3, hi=(10.0,) * 3,
+         periodic=(1, 1, 1), skin=0.1, pair=None, max_nb=64):
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    n = len(x)
+    r = np.broadcast_to(np.asarray(radius, dtype=np.float64), (n,)).copy()
+    rho = np.broadcast_to(np.asarray(density, dtype=np.float64), (n,)).copy()
+    v = np.zeros((n, 3)) if v is None else np.asarray(v, dtype=np.float64)
+    omega = np.zeros((n, 3)) if omega is None else np.asarray(omega, dtype=np.float64)
+    o = O.OracleSim(n, max_nb=max_nb)
+    o.set_atoms(x, v, omega, r, O.mass_from_radius(r, rho))
+    o.set_box(lo, hi, periodic)
+    o.set_neigh(skin)
+    if pair is not None:
+        o.set_pair(pair)
+    return o
+
+
+def _force_once(o, dt, shearupdate=True):
+    o.clear_forces()
+    o.pair_compute(dt, shearupdate)
+    return o.atoms()
+
+
+# --------------------------------------------------------------- normal laws
+
+def test_hooke_normal_force_matches_published_literal():
+    """tests/test_hooke.py:15-55: fx = -4288.790204786391 (LAMMPS run 0)."""
+    pair = O.legacy_pair("hooke/history", kn=1000.0, gamman=10.0, kt=0.0, gammat=0.0,
+                         xmu=0.0, dampflag=0)
+    o = _sim([[0.0, 0.0, 0.0], [1.9, 0.0, 0.0]], radius=1.0, density=1000.0,
+             v=[[0.1, 0.0, 0.0], [-0.1, 0.0, 0.0]], lo=(-10.0,) * 3, hi=(10.0,) * 3,
+             periodic=(0, 0, 0), skin=0.2, pair=pair)
+    assert o.nlist_build() == 0
+    a = _force_once(o, 0.001)
+    expected = 4288.790204786391
+    np.testing.assert_allclose(a["f"][0], [-expected, 0.0, 0.0], rtol=1e-12)
+    np.testing.assert_allclose(a["f"][1], [expected, 0.0, 0.0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("style", ["hooke/history", "hertz/history"])
+def test_static_pair_normal_force_closed_form(style):
+    """At rest: Hooke |F| = kn d, Hertz |F| = kn d sqrt(d R1 R2 / (R1 + R2))."""
+    r1, r2, d = 0.5, 0.4, 0.05
+    kn = 2.0e5
+    pair = O.legacy_pair(style, kn=kn, gamman=50.0, kt=2.0 / 7.0 * kn, gammat=25.0, xmu=0.5)
+    o = _sim([[3.0, 4.0, 5.0], [3.0 + (r1 + r2 - d), 4.0, 5.0]], radius=[r1, r2], pair=pair)
+    o.nlist_build()
+    a = _force_once(o, 1e-4)
+    mag = kn * d if style == "hooke/history" else kn * d * math.sqrt(d * r1 * r2 / (r1 + r2))
+    np.testing.assert_allclose(a["f"][0], [-mag, 0.0, 0.0], rtol=1e-12, atol=1e-12 * mag)
+    np.testing.assert_allclose(a["f"][1], [mag, 0.0, 0.0], rtol=1e-12, atol=1e-12 * mag)
+    # the contact virial is r_ij (x) f_ij, split evenly between the two atoms
+    vir = a["virial"]
+    np.testing.assert_allclose(vir[0, 0] + vir[1, 0], (r1 + r2 - d) * mag, rtol=1e-12)
+
+
+def test_normal_damping_closed_form():
+    """Approaching pair: Hooke normal force kn d + meff gamman vn (dashpot opposes approach)."""
+    r, d, u = 0.5, 0.02, 0.3
+    kn, gn = 2.0e5, 50.0
+    pair = O.legacy_pair("hooke/history", kn=kn, gamman=gn, kt=0.0, gammat=0.0, xmu=0.5)
+    o = _sim([[3.0, 3.0, 3.0], [3.0 + 2 * r - d, 3.0, 3.0]], radius=r,
+             v=[[u, 0.0, 0.0], [0.0, 0.0, 0.0]], pair=pair)
+    o.nlist_build()
+    a = _force_once(o, 1e-4)
+    m = 4.0 / 3.0 * math.pi * r**3
+    mag = kn * d + 0.5 * m * gn * u
+    np.testing.assert_allclose(a["f"][0, 0], -mag, rtol=1e-12)
+
+
+# --------------------------------------------------------------- tangential
+
+def _tangential_case(u, k_calls, style="hooke/history", xmu=0.5):
+    r, d = 0.5, 0.02
+    kn, kt, gn, gt = 2.0e5, 2.0 / 7.0 * 2.0e5, 50.0, 25.0
+    pair = O.legacy_pair(style, kn=kn, gamman=gn, kt=kt, gammat=gt, xmu=xmu)
+    o = _sim([[3.0, 3.0, 3.0], [3.0 + 2 * r - d, 3.0, 3.0]], radius=r,
+             v=[[0.0, 0.0, 0.0], [0.0, u, 0.0]], pair=pair)
+    o.nlist_build()
+    dt = 1e-4
+    for _ in range(k_calls):
+        a = _force_once(o, dt)
+    meff = 0.5 * (4.0 / 3.0 * math.pi * r**3)
+    return a, o, dict(r=r, d=d, kn=kn, kt=kt, gn=gn, gt=gt, dt=dt, meff=meff)
+
+
+def test_tangential_stick_closed_form():
+    """Below slip, k updates at constant relative tangential velocity vt give
+    shear = k dt vt and Fs = kt shear + meff gammat vt, torque = r Fs about z."""
+    u, k = 0.1, 3
+    a, o, c = _tangential_case(u, k)
+    fs = u * (c["kt"] * k * c["dt"] + c["meff"] * c["gt"])
+    assert fs < 0.5 * c["kn"] * c["d"]  # stick regime
+    np.testing.assert_allclose(a["f"][0, 1], fs, rtol=1e-12)
+    np.testing.assert_allclose(a["f"][1, 1], -fs, rtol=1e-12)
+    np.testing.assert_allclose(a["torque"][0, 2], c["r"] * fs, rtol=1e-12)
+    np.testing.assert_allclose(a["torque"][1, 2], c["r"] * fs, rtol=1e-12)
+    _, _, _, shear = o.pairs()
+    np.testing.assert_allclose(shear[0], [0.0, -k * c["dt"] * u, 0.0], rtol=1e-12, atol=1e-300)
+
+
+def test_tangential_slip_caps_at_coulomb_and_rescales_history():
+    """At slip |Fs| = xmu |Fn| and the stored shear reproduces the capped force
+    (hooke_history.py:163-166)."""
+    u, k, xmu = 500.0, 2, 0.5
+    a, o, c = _tangential_case(u, k, xmu=xmu)
+    fn = c["kn"] * c["d"]
+    np.testing.assert_allclose(abs(a["f"][0, 1]), xmu * fn, rtol=1e-12)
+    _, _, _, shear = o.pairs()
+    vtr = np.array([0.0, -u, 0.0])
+    fs_from_history = -(c["kt"] * shear[0] + c["meff"] * c["gt"] * vtr)
+    np.testing.assert_allclose(fs_from_history[1], a["f"][0, 1], rtol=1e-12)
+
+
+def test_hertz_tangential_scales_with_contact_radius():
+    """hertz/history multiplies the tangential spring and dashpot by sqrt(Reff d)."""
+    u, k = 0.1, 2
+    a_hooke, _, c = _tangential_case(u, k, "hooke/history")
+    a_hertz, _, _ = _tangential_case(u, k, "hertz/history")
+    poly = math.sqrt(c["d"] * 0.25)
+    np.testing.assert_allclose(a_hertz["f"][0, 1], poly * a_hooke["f"][0, 1], rtol=1e-12)
+
+
+def test_friction_damps_spin():
+    """tests/test_lammps_parity.py:159-190: friction opposes a spin about z."""
+    radius, overlap, spin = 0.5, 0.01, 3.0
+    d = 2.0 * radius - overlap
+    pair = O.legacy_pair("hooke/history", kn=1.0e5, gamman=10.0, kt=5.0e4, gammat=5.0, xmu=0.5)
+    o = _sim([[3.0, 3.0, 3.0], [3.0 + d, 3.0, 3.0]], radius=radius, density=2650.0,
+             omega=[[0.0, 0.0, spin], [0.0, 0.0, 0.0]], lo=(0.0,) * 3, hi=(6.0,) * 3, pair=pair)
+    assert o.init_simulation(1e-4) == 0
+    assert o.step(1e-4) == 0
+    a = o.atoms()
+    assert a["f"][0, 1] < 0.0
+    assert a["torque"][0, 2] < 0.0
+    np.testing.assert_allclose(a["f"][0] + a["f"][1], 0.0, atol=1e-12 * np.abs(a["f"]).max())
+
+
+# -------------------------------------------------------------- invariants
+
+def _random_packing(rng, n=64, box=6.0):
+    """tests/test_lammps_parity.py:56-70: jittered lattice, wrapped."""
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    spacing = box / side
+    pts = np.array([[(ix + 0.5) * spacing, (iy + 0.5) * spacing, (iz + 0.5) * spacing]
+                    for ix in range(side) for iy in range(side) for iz in range(side)])[:n]
+    pts = pts + rng.uniform(-0.02 * spacing, 0.02 * spacing, size=pts.shape)
+    return pts % box
+
+
+def _parity_pair():
+    return O.legacy_pair("hooke/history", kn=1.0e5, gamman=10.0, kt=5.0e4, gammat=5.0, xmu=0.5)
+
+
+def test_pressure_tensor_is_translation_invariant():
+    """tests/test_lammps_parity.py:97-121."""
+    rng = np.random.default_rng(7)
+    box = 6.0
+    radius = 0.52 * (box / 4)
+    pts = _random_packing(rng, n=64, box=box)
+
+    def pressure(offset):
+        o = _sim((pts + offset) % box, radius=radius, density=2650.0, hi=(box,) * 3,
+                 pair=_parity_pair())
+        assert o.init_simulation(1e-4) == 0
+        return o.virial(keflag=False) / box**3
+
+    p0 = pressure(np.zeros(3))
+    p1 = pressure(np.array([0.37, 0.11, 0.53]) * box)
+    assert np.linalg.norm(p0) > 0.0
+    np.testing.assert_allclose(p1, p0, rtol=1e-9, atol=1e-6 * np.linalg.norm(p0))
+
+
+def test_newton_third_law_and_angular_balance():
+    """Contact forces sum to zero; a free cluster's net force is zero to rounding."""
+    rng = np.random.default_rng(5)
+    box = 6.0
+    pts = _random_packing(rng, n=64, box=box)
+    o = _sim(pts, radius=0.78, density=2650.0, hi=(box,) * 3, pair=_parity_pair(),
+             v=rng.normal(scale=0.05, size=pts.shape), omega=rng.normal(scale=0.5, size=pts.shape))
+    assert o.init_simulation(1e-4) == 0
+    for _ in range(5):
+        assert o.step(1e-4) == 0
+    f = o.atoms()["f"]
+    assert np.abs(f).max() > 0.0
+    np.testing.assert_allclose(f.sum(axis=0), 0.0, atol=1e-11 * np.abs(f).max() * len(f))
+
+
+def test_shear_history_survives_neighbor_rebuild():
+    """tests/test_lammps_parity.py:124-156: rebuild frequency does not change forces."""
+    rng = np.random.default_rng(11)
+    box = 6.0
+    radius = 0.52 * (box / 4)
+    pts = _random_packing(rng, n=64, box=box)
+    vel = rng.normal(scale=0.05, size=pts.shape)
+    omg = rng.normal(scale=0.5, size=pts.shape)
+
+    def run(every_step):
+        o = _sim(pts, radius=radius, density=2650.0, hi=(box,) * 3, v=vel, omega=omg,
+                 pair=_parity_pair())
+        if every_step:
+            o.set_neigh(0.1, delay=0, every=1, check=False)
+        else:
+            o.set_neigh(0.1, delay=0, every=10**6, check=True)
+        assert o.run(40, 1e-4) == 0
+        return o.atoms()["f"], o.counters()["build_count"]
+
+    f_rebuilt, nb_rebuilt = run(True)
+    f_kept, nb_kept = run(False)
+    assert nb_rebuilt > 30 and nb_kept == 1
+    scale = np.abs(f_kept).max()
+    assert scale > 0.0
+    np.testing.assert_allclose(f_rebuilt, f_kept, rtol=1e-8, atol=1e-8 * scale)
+
+
+def test_tree_and_chunked_virial_sums_agree():
+    """The pairwise-tree sum and the reference's chunked sum (thermo.py:62-109)
+    are two orders of the same sum."""
+    w = workloads.case("isotropic_hooke_8k", dims=(8, 8, 8))
+    o = O.from_workload(w)
+    o.set_neigh(0.1)
+    o.set_pair(O.legacy_pair("hooke/history", kn=2e5, gamman=50.0, kt=2e5 * 2 / 7,
+                             gammat=25.0, xmu=0.5))
+    assert o.init_simulation(1e-4) == 0
+    t0 = o.virial(keflag=True, mode=0)
+    t1 = o.virial(keflag=True, mode=1)
+    np.testing.assert_allclose(t0, t1, rtol=1e-12, atol=1e-12 * np.abs(t0).max())
+
+
+# ------------------------------------------------------------ neighbor list
+
+def _brute_pairs(x, r, skin, lo, hi, periodic):
+    prd = np.asarray(hi) - np.asarray(lo)
+    d = x[:, None, :] - x[None, :, :]
+    for k in range(3):
+        if periodic[k]:
+            d[..., k] -= prd[k] * np.round(d[..., k] / prd[k])
+    rsq = (d**2).sum(-1)
+    cut = (r[:, None] + r[None, :] + skin) ** 2
+    i, j = np.nonzero(np.triu(rsq < cut, k=1))
+    return set(zip(i.tolist(), j.tolist()))
+
+
+@pytest.mark.parametrize("box,periodic", [
+    ((12.0, 12.0, 12.0), (1, 1, 1)),
+    ((14.0, 3.1, 9.0), (1, 1, 1)),    # a periodic dimension of two bins collapses to one
+    ((10.0, 10.0, 10.0), (0, 1, 0)),  # non-periodic faces
+    ((2.5, 2.5, 40.0), (1, 1, 1)),    # single-bin dimensions skip the +-1 stencil
+])
+def test_neighbor_set_equals_brute_force(box, periodic):
+    rng = np.random.default_rng(17)
+    n = 1200
+    lo = np.zeros(3)
+    hi = np.asarray(box)
+    x = rng.uniform(0.0, 1.0, size=(n, 3)) * hi
+    r = rng.uniform(0.2, 0.45, size=n)
+    o = _sim(x, radius=r, lo=lo, hi=hi, periodic=periodic, skin=0.1, max_nb=128)
+    assert o.nlist_build() == 0
+    pi, pj, _, _ = o.pairs()
+    got = set(zip(pi.tolist(), pj.tolist()))
+    assert len(got) == len(pi)
+    assert all(i < j for i, j in got)
+    assert got == _brute_pairs(x, r, 0.1, lo, hi, periodic)
+    # half-list rows are ascending in j, rows ordered by i
+    order = np.lexsort((pj, pi))
+    assert np.array_equal(order, np.arange(len(pi)))
+
+
+@pytest.mark.parametrize("name,dims,expect", [
+    ("isotropic_hooke_8k", None, 18),   # 22^3 box, cut 1.2 -> 18 bins per side
+    ("adjoint_32k", None, 29),          # 35.2^3 box -> 29 bins per side
+])
+def test_grid_dimensions_known_answers(name, dims, expect):
+    w = workloads.case(name, dims=dims)
+    o = O.from_workload(w)
+    o.set_neigh(0.1)
+    assert o.nlist_build() == 0
+    g, cs, cell_of = o.grid()
+    assert g.tolist() == [expect] * 3
+    np.testing.assert_allclose(cs, 1.0 / expect, rtol=0, atol=0)
+    assert cell_of.min() >= 0 and cell_of.max() < expect**3
+
+
+def test_grid_cap_known_answer_at_one_million_atoms():
+    """91^3 bins would exceed the 200 000 cap; the cap formula lands on 58^3."""
+    w = workloads.case("isotropic_hertz_1m")
+    o = O.from_workload(w)
+    o.set_neigh(0.1)
+    assert o.nlist_build() == 0
+    g, _, _ = o.grid()
+    assert g.tolist() == [58, 58, 58]
+    assert 58**3 <= 200000
+
+
+def test_skin_displacement_trigger():
+    """tests/test_domain_neighbor.py:58-100: rebuild past half the skin."""
+    x = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [5.0, 0.0, 0.0]])
+    o = _sim(x, radius=1.0, density=1000.0, lo=(-10.0,) * 3, hi=(10.0,) * 3,
+             periodic=(0, 0, 0), skin=0.2)
+    r = np.ones(3)
+    m = O.mass_from_radius(r, 1000.0)
+    assert o.decide(0)
+    o.nlist_build()
+    assert o.counters()["build_count"] == 1
+    assert not o.decide(1)
+    for pos, expect in ((0.08, False), (0.12, True)):
+        x2 = x.copy()
+        x2[0, 0] = pos
+        o.set_atoms(x2, np.zeros((3, 3)), np.zeros((3, 3)), r, m)
+        assert o.decide(3) is expect
+    pi, pj, _, _ = o.pairs()
+    assert list(zip(pi.tolist(), pj.tolist())) == [(0, 1)]
+
+
+def test_pbc_wraps_into_box():
+    """tests/test_domain_neighbor.py:15-31."""
+    o = _sim([[10.5, -0.2, 5.0]], radius=0.5, density=1000.0)
+    o.pbc()
+    np.testing.assert_allclose(o.atoms()["x"][0], [0.5, 9.8, 5.0], rtol=1e-12)
+
+
+# --------------------------------------------------------------- integrator
+
+def test_free_flight_and_spin_up():
+    """No contacts: x advances by dt v per step; a torque spins the sphere up by
+    dt tau / (0.4 m r^2) over one initial + final integrate."""
+    v = np.array([[0.3, -0.2, 0.1]])
+    o = _sim([[5.0, 5.0, 5.0]], radius=0.5, v=v, pair=_parity_pair())
+    assert o.run(100, 1e-3) == 0
+    np.testing.assert_allclose(o.atoms()["x"][0], 5.0 + 100 * 1e-3 * v[0], rtol=0, atol=1e-12)
+
+    tau = np.array([[0.0, 0.0, 2.0]])
+    o2 = _sim([[5.0, 5.0, 5.0]], radius=0.5)
+    o2.set_forces(np.zeros((1, 3)), tau)
+    o2.nve_initial(1e-3)
+    o2.nve_final(1e-3)
+    m = 4.0 / 3.0 * math.pi * 0.125
+    np.testing.assert_allclose(o2.atoms()["omega"][0, 2], 1e-3 * 2.0 / (0.4 * m * 0.25), rtol=1e-14)
+
+
+def test_cundall_damping_sign_rule():
+    """damping_cundall.py:35-53: f *= 1 - g sign(f v), per component."""
+    o = _sim([[5.0, 5.0, 5.0], [1.0, 1.0, 1.0]], radius=0.5,
+             v=[[1.0, -1.0, 0.0], [0.0, 0.0, 0.0]], omega=[[0.0, 2.0, -1.0], [0.0, 0.0, 0.0]])
+    o.set_forces(np.array([[2.0, 2.0, 3.0], [1.0, 1.0, 1.0]]),
+                 np.array([[1.0, 1.0, 1.0], [0.0, 0.0, 0.0]]))
+    o.set_cundall(0.1, 0.2)
+    o.cundall()
+    a = o.atoms()
+    np.testing.assert_allclose(a["f"][0], [2.0 * 0.9, 2.0 * 1.1, 3.0 * 0.9], rtol=1e-15)
+    np.testing.assert_allclose(a["torque"][0], [0.8, 0.8, 1.2], rtol=1e-15)
+
+
+# -------------------------------------------------------------------- servo
+
+def test_kinetic_tensor_closed_form():
+    """tests/test_dump_computes.py:17-40: translational KE of two spheres."""
+    o = _sim([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], radius=0.1, density=1000.0,
+             v=[[2.0, 0.0, 0.0], [0.0, 4.0, 0.0]])
+    t = o.virial(keflag=True)
+    m1 = 4.0 / 3.0 * np.pi * (0.1**3) * 1000.0
+    np.testing.assert_allclose(0.5 * t[:3].sum(), 10.0 * m1, rtol=1e-14)
+    np.testing.assert_allclose(t, [4 * m1, 16 * m1, 0, 0, 0, 0], rtol=1e-14, atol=0)
+
+
+def test_pressure_servo_one_update_by_hand():
+    """Isotropic pressure control, couple xyz, no contacts: one update moves
+    each face by dt * pgain * (P - Ptarget) * L / 2 and remap x keeps every
+    atom's fractional position."""
+    L = 10.0
+    rng = np.random.default_rng(3)
+    x = rng.uniform(1.0, 9.0, size=(6, 3))
+    v = rng.normal(scale=0.7, size=(6, 3))
+    o = _sim(x, radius=0.1, v=v, hi=(L,) * 3)
+    o.set_servo(O.servo_params(p_target=1.0, p_gain=1e-4))
+    o.servo_setup(0, 1e-3)
+    assert o.servo_end_of_step(1e-3) == 0
+    m = 4.0 / 3.0 * math.pi * 0.001
+    p = m * (v**2).sum() / 3.0 / L**3
+    cum = 1e-3 * (1e-4 * (p - 1.0)) * L
+    lo, hi, hr = o.box()
+    np.testing.assert_allclose(lo, -0.5 * cum, rtol=1e-13)
+    np.testing.assert_allclose(hi, L + 0.5 * cum, rtol=1e-15)
+    # h_rate = (new prd - old prd) / dt loses ~log10(L/cum) digits to cancellation
+    np.testing.assert_allclose(hr[:3], cum / 1e-3, rtol=1e-7)
+    xn = o.atoms()["x"]
+    np.testing.assert_allclose((xn - lo) / (hi - lo), x / L, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("style,expect", [
+    ("erate", lambda L, r, t: L * (1.0 + r * t)),
+    ("trate", lambda L, r, t: L * math.exp(r * t)),
+])
+def test_strain_styles_box_lengths_and_remap(style, expect):
+    """tests/test_deform_pressure.py:223-255 and test_deform_remap_x.py:74-101."""
+    L, rate, dt, steps = 0.02, -1.0e3, 1.0e-6, 50
+    x = np.array([[0.5 * L] * 3, [0.25 * L] * 3])
+    o = _sim(x, radius=1e-4, density=2500.0, hi=(L,) * 3)
+    o.set_servo(O.servo_params(axes={d: {"style": style, "rate": rate} for d in "xyz"}))
+    o.servo_setup(0, dt)
+    for _ in range(steps):
+        assert o.servo_end_of_step(dt) == 0
+    lo, hi, _ = o.box()
+    np.testing.assert_allclose(hi - lo, expect(L, rate, steps * dt), rtol=1e-12)
+    xn = o.atoms()["x"]
+    np.testing.assert_allclose((xn - lo) / (hi - lo), [[0.5] * 3, [0.25] * 3], atol=1e-12)
