@@ -11,7 +11,7 @@ For each precision (FP64 plain, FP32-split) the script runs `--warmup` steps
 `--steps` device-resident steps (Simulation.run with `pre no`) between two
 stream synchronisations. `value` is the FP32-split throughput in atom-steps/s;
 the FP64 figure and their ratio ride along. A second pass of the same steps
-with per-kernel HIP events gives the force kernel's average duration for the
+with per-kernel HIP events gives the step kernel's average duration for the
 roofline; the CPU baseline times the single-threaded CPU oracle on the same
 workload.
 
@@ -51,20 +51,15 @@ def build_sim(w, precision):
     return sim
 
 
-# Algorithmic bytes per atom of each fused kernel (own-atom streams, list slots
-# and touching-slot history; neighbour gathers are cache traffic and not
-# counted): B = base + per_slot * n_slot + per_touch * n_touch.
-BYTES = {
-    # force pass: x|r r, v r+w, omega r+w, m, list offset (FP64: 24+8 + 48 + 48 + 8 + 4)
-    ("f64", "force"): (140.0, 5.0, 48.0),
-    # begin pass: x r+w, v r, x0 r (FP64: 48 + 24 + 24)
-    ("f64", "begin"): (96.0, 0.0, 0.0),
-    # force pass: u|r r, cell r, v r+w, omega r+w, m, list offset (12+4 + 4 + 24 + 24 + 4 + 4)
-    ("f32split", "force"): (76.0, 5.0, 24.0),
-    # begin pass: u r+w, cell r, v r, u0 r (24 + 4 + 12 + 12)
-    ("f32split", "begin"): (52.0, 0.0, 0.0),
-}
-# Whole-step model (per atom-step): B = B_atom + 6 n_slot + B_hist n_touch
+# Algorithmic bytes per atom-step of the one-kernel step (own-atom streams,
+# list slots and touching-slot history; neighbour gathers are cache traffic
+# and not counted): B = B_atom + 6 n_slot + B_hist n_touch, with
+#   FP64   B_atom = x r+w 48 + v r+w 48 + omega r+w 48 + radius, mass 16
+#                   + x0 r 24 + list offset 4 = 188,   B_hist = shear r+w 48
+#   split  B_atom = offset r+w 24 + cell 4 + v r+w 24 + omega r+w 24
+#                   + radius, mass 8 + x0 offset 12 + list offset 4 = 100,
+#                   B_hist = 24
+# and 6 B per slot (4 B index, 1 B touch flag read and written).
 STEP_MODEL = {"f64": (188.0, 6.0, 48.0), "f32split": (100.0, 6.0, 24.0)}
 
 
@@ -95,18 +90,35 @@ def run_precision(w, precision, steps, warmup, prof_steps):
     n_slot = c["nslots"] / n
     n_touch = c["ntouch"] / n
     # per-kernel HIP-event timings over the same number of steps, same stream
-    N.set_profiling(True)
-    sim.run(prof_steps)
-    ms, cnt = N.kernel_times()
-    N.set_profiling(False)
+    ms, cnt = np.zeros(3), np.zeros(3, dtype=int)
+    if prof_steps > 0:
+        N.set_profiling(True)
+        sim.run(prof_steps)
+        ms, cnt = N.kernel_times()
+        N.set_profiling(False)
     return {
         "n": n, "elapsed_s": elapsed, "ms_per_step": elapsed / steps * 1e3,
         "atom_steps_per_s": n * steps / elapsed, "setup_s": t_setup, "rebuilds": rebuilds,
         "n_slot": n_slot, "n_touch": n_touch, "pressure": pressure.tolist(),
         "mean_pressure": float(pressure[:3].mean()),
-        "kernel_ms": {"begin": ms[0], "force": ms[1], "end": ms[2]},
+        "kernel_ms": {"kick": ms[0], "step": ms[1], "tail": ms[2]},
         "kernel_counts": cnt.tolist(), "x": x_final,
     }
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size: the profiler's FETCH_SIZE / WRITE_SIZE
+    for it, divided by the bytes it must move, is the counters' scale factor."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 5}))
+    return 0
 
 
 def cpu_baseline(w, warmup, steps):
@@ -139,7 +151,14 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dims", type=int, nargs=3, default=None,
                     help="lattice dimensions (default 100 100 100)")
+    ap.add_argument("--precision", choices=("both", "f64", "f32split"), default="both",
+                    help="run one precision only (for profiler passes); the JSON line needs both")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (a known-size "
+                         "kernel that scales the profiler's HBM byte counters), then exit")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
@@ -149,23 +168,24 @@ def main(argv=None):
     w = workloads.case("isotropic_hertz_1m", dims=tuple(args.dims) if args.dims else None)
     device = _native.device_name()
     res = {}
-    for prec in ("f64", "f32split"):
+    precisions = ("f64", "f32split") if args.precision == "both" else (args.precision,)
+    for prec in precisions:
         res[prec] = run_precision(w, prec, args.steps, args.warmup, args.prof_steps)
         r = res[prec]
         print(f"# {prec}: {r['ms_per_step']:.4f} ms/step, {r['atom_steps_per_s']:.4e} atom-steps/s, "
               f"kernels {r['kernel_ms']}, n_slot {r['n_slot']:.3f}, n_touch {r['n_touch']:.3f}, "
               f"rebuilds {r['rebuilds']}, P {r['mean_pressure']:.10g}", file=sys.stderr, flush=True)
+    if args.precision != "both":
+        return 0
 
     head = res["f32split"]
     f64 = res["f64"]
     n = head["n"]
-    force_bytes = model_bytes(BYTES[("f32split", "force")], head["n_slot"], head["n_touch"]) * n
-    force_ms = head["kernel_ms"]["force"]
-    achieved = force_bytes / (force_ms * 1e-3) / 1e9
     step_bytes = model_bytes(STEP_MODEL["f32split"], head["n_slot"], head["n_touch"]) * n
+    kernel_ms = head["kernel_ms"]["step"]
+    achieved = step_bytes / (kernel_ms * 1e-3) / 1e9
     step_achieved = step_bytes / (head["ms_per_step"] * 1e-3) / 1e9
     f64_step_bytes = model_bytes(STEP_MODEL["f64"], f64["n_slot"], f64["n_touch"]) * n
-    f64_force_bytes = model_bytes(BYTES[("f64", "force")], f64["n_slot"], f64["n_touch"]) * n
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -195,17 +215,18 @@ def main(argv=None):
         "fp64": {"value": f64["atom_steps_per_s"], "ms_per_step": f64["ms_per_step"],
                  "kernel_ms": f64["kernel_ms"], "mean_pressure": f64["mean_pressure"],
                  "step_model_GBs": f64_step_bytes / (f64["ms_per_step"] * 1e-3) / 1e9,
-                 "force_achieved_GBs": f64_force_bytes / (f64["kernel_ms"]["force"] * 1e-3) / 1e9},
+                 "kernel_achieved_GBs": f64_step_bytes / (f64["kernel_ms"]["step"] * 1e-3) / 1e9},
         "speedup_fp32split_vs_fp64": head["atom_steps_per_s"] / f64["atom_steps_per_s"],
         "mean_pressure_rel_err_vs_fp64": abs(head["mean_pressure"] - f64["mean_pressure"])
         / abs(f64["mean_pressure"]),
-        "roofline": {"bound": "hbm", "kernel": "k_step_force_split (force pass)",
+        "roofline": {"bound": "hbm", "kernel": "k_step_split (one kernel per step)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": force_bytes, "avg_ms": force_ms},
+                     "bytes_per_launch": step_bytes, "avg_ms": kernel_ms},
         "step_roofline": {"achieved": step_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_achieved / HBM_PEAK_GBS,
-                          "bytes_per_atom_step": step_bytes / n},
+                          "bytes_per_atom_step": step_bytes / n,
+                          "note": "whole run incl. rebuilds and segment overhead"},
         "n_slot": head["n_slot"], "n_touch": head["n_touch"], "rebuilds": head["rebuilds"],
         "kernel_ms": head["kernel_ms"],
         "cpu_baseline": cpu,

@@ -465,76 +465,68 @@ uint64_t dbits(double d) {
 
 // One fused step on v/omega buffer `buf`: the force pass writes the other
 // buffer. Returns the buffer that holds v/omega afterwards.
-int step_kernels(System& s, int buf, int kick, int fin, double dt, int use_cundall, double gl,
-                 double ga, int nparts, int servo_on, int* out_buf) {
-    int rc = split(s) ? launch_step_begin_split(s, dt, kick, buf)
-                      : launch_step_begin_f64(s, dt, kick, buf);
-    if (rc) return rc;
-    rc = split(s) ? launch_step_force_split(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf)
-                  : launch_step_force_f64(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf);
-    if (rc) return rc;
-    rc = launch_step_end(s, dt, nparts, 1, servo_on);
-    *out_buf = 1 - buf;
-    return rc;
+// One fused step of a segment: it reads buffer set `buf`, writes 1 - buf.
+int fused_step(System& s, int buf, int fin, double dt, int use_cundall, double gl, double ga,
+               int servo_on) {
+    return split(s) ? launch_fused_split(s, buf, dt, fin, use_cundall, gl, ga, servo_on)
+                    : launch_fused_f64(s, buf, dt, fin, use_cundall, gl, ga, servo_on);
 }
 
-// The end of a segment: v/omega back into buffer 0, the last step's remap.
+// The end of a segment whose last step wrote buffer set `buf`: the results
+// back into set 0, the last step's remap applied to the stored positions.
+// Both keep their hands off an aborted segment: its redo needs set p and the
+// pending remap record as the aborted step found them.
 int segment_tail(System& s, int buf) {
     int rc = 0;
-    if (buf == 1) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
+    if (buf == 1) rc = launch_copy_state(s, 1);
     if (rc) return rc;
     if (!split(s)) {
         rc = launch_remap_apply_fused(s);
         if (rc) return rc;
     }
-    return launch_remap_clear(s);
+    return launch_remap_clear(s, 1);
 }
 
-// Enqueue the kernels of `len` fused steps (no synchronisation).
+// Enqueue a segment of `len` steps (no synchronisation): the first step's
+// half-kick, one kernel per step alternating buffer sets, the tail.
 int enqueue_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
-    const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    int buf = 0;
-    for (int k = 0; k < len; ++k) {
-        int rc = step_kernels(s, buf, k == 0, k == len - 1, dt, use_cundall, gl, ga, nparts,
-                              servo_on, &buf);
-        if (rc) return rc;
-    }
-    return segment_tail(s, buf);
+    int rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
+    for (int k = 0; k < len && !rc; ++k)
+        rc = fused_step(s, k & 1, k == len - 1, dt, use_cundall, gl, ga, servo_on);
+    if (rc) return rc;
+    return segment_tail(s, len & 1);
 }
 
-// Same as enqueue_steps, timing every kernel with HIP events.
+// Same as enqueue_steps, timing each kernel with HIP events on the stream:
+// class 0 the kick, class 1 the step kernels, class 2 the tail.
 int profiled_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
-    const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    std::vector<hipEvent_t> ev(3 * len + 1);
+    std::vector<hipEvent_t> ev(len + 3);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], s.stream));
-    int rc = 0, buf = 0;
+    int rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
+    if (!rc) rc = hipEventRecord(ev[1], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     for (int k = 0; k < len && !rc; ++k) {
-        const int kick = k == 0 ? 1 : 0;
-        const int fin = k == len - 1 ? 1 : 0;
-        rc = split(s) ? launch_step_begin_split(s, dt, kick, buf)
-                      : launch_step_begin_f64(s, dt, kick, buf);
-        if (!rc) rc = hipEventRecord(ev[3 * k + 1], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
-        if (!rc)
-            rc = split(s) ? launch_step_force_split(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf)
-                          : launch_step_force_f64(s, dt, fin, use_cundall, gl, ga, buf, 1 - buf);
-        if (!rc) rc = hipEventRecord(ev[3 * k + 2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
-        if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
-        if (!rc) rc = hipEventRecord(ev[3 * k + 3], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
-        buf = 1 - buf;
+        rc = fused_step(s, k & 1, k == len - 1, dt, use_cundall, gl, ga, servo_on);
+        if (!rc) rc = hipEventRecord(ev[2 + k], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     }
-    if (!rc) rc = segment_tail(s, buf);
+    if (!rc) rc = segment_tail(s, len & 1);
+    if (!rc) rc = hipEventRecord(ev[len + 2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     HIP_TRY(hipStreamSynchronize(s.stream));
     if (!rc) {
-        for (int k = 0; k < len; ++k)
-            for (int c = 0; c < 3; ++c) {
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, ev[3 * k + c], ev[3 * k + c + 1]));
-                s.prof_ms[c] += ms;
-                s.prof_cnt[c] += 1;
-            }
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        s.prof_ms[0] += ms;
+        s.prof_cnt[0] += 1;
+        for (int k = 0; k < len; ++k) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[1 + k], ev[2 + k]));
+            s.prof_ms[1] += ms;
+            s.prof_cnt[1] += 1;
+        }
+        HIP_TRY(hipEventElapsedTime(&ms, ev[len + 1], ev[len + 2]));
+        s.prof_ms[2] += ms;
+        s.prof_cnt[2] += 1;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
@@ -677,14 +669,23 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         rc = check_run_error(s);
         if (rc) return rc;
         s.timestep = s.hctl->timestep;
-        // The skin check of step stp asked for a rebuild: its drift is done,
-        // its force pass is not. Steps of a segment alternate v/omega buffers
-        // starting from buffer 0; bring the current one back to buffer 0.
+        // The skin check of step stp asked for a rebuild. Its pass wrote
+        // only buffer set 1 - p; set p still holds step stp-1 (positions
+        // before stp-1's remap, half-kicked v, omega, touch flags, history).
+        // Bring that set to buffer 0, redo the step's start there with the
+        // stage kernel (remap, drift), rebuild, then the step's force pass,
+        // final_integrate and end.
         const int stp = s.hctl->timestep + 1;
         if (((stp - 1) - sg.start) % 2 == 1) {
-            rc = split(s) ? launch_copy_vw_split(s, 0) : launch_copy_vw(s, 0);
+            rc = launch_copy_state(s, 0);
             if (rc) return rc;
         }
+        s.hctl->abort = 0;
+        s.hctl->trigger = -1;
+        rc = ctl_push(s);
+        if (rc) return rc;
+        rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
+        if (rc) return rc;
         s.current_step = stp;
         rc = history_save(s);
         if (!rc) rc = pbc(s);
@@ -702,7 +703,10 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
                       : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
         if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
-        if (!rc) rc = segment_tail(s, 1);
+        // the force pass left x, flags and history in set 0 and v, omega in set 1
+        if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
+        if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
+        if (!rc) rc = launch_remap_clear(s, 1);
         if (rc) return rc;
         rc = ctl_pull(s);
         if (rc) return rc;
@@ -902,11 +906,13 @@ int get_field(System& s, int field, void* host, int count) {
 void free_all(System& s) {
     invalidate_graphs(s);
     dfree(s.xr); dfree(s.vm); dfree(s.om); dfree(s.f); dfree(s.tq); dfree(s.x0); dfree(s.vir);
-    dfree(s.vm2); dfree(s.om2); dfree(s.vmf2); dfree(s.omf2);
+    dfree(s.vm2); dfree(s.om2); dfree(s.vmf2); dfree(s.omf2); dfree(s.xr2); dfree(s.up2);
     dfree(s.tag);
     dfree(s.up); dfree(s.vmf); dfree(s.omf); dfree(s.ff); dfree(s.tqf); dfree(s.u0); dfree(s.cell);
     dfree(s.nb_start); dfree(s.nb_j); dfree(s.nb_flag);
-    dfree(s.onb_start); dfree(s.onb_j); dfree(s.onb_flag);
+    dfree(s.onb_start); dfree(s.onb_j); dfree(s.onb_flag); dfree(s.nb_flag2);
+    if (s.nb_hist2) (void)hipFree(s.nb_hist2);
+    s.nb_hist2 = nullptr;
     if (s.nb_hist) (void)hipFree(s.nb_hist);
     if (s.onb_hist) (void)hipFree(s.onb_hist);
     s.nb_hist = s.onb_hist = nullptr;
@@ -915,6 +921,8 @@ void free_all(System& s) {
     dfree(s.full_count); dfree(s.half_count); dfree(s.scan_tmp); dfree(s.flags);
     dfree(s.row_tmp);
     dfree(s.partial);
+    dfree(s.red_buf);
+    dfree(s.red_cnt);
     dfree(s.ctl);
     if (s.hctl) (void)hipHostFree(s.hctl);
     s.hctl = nullptr;
@@ -926,6 +934,44 @@ void free_all(System& s) {
     }
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
+}
+
+// The fused step's reduction levels for up to max_atoms atoms: level 0 one
+// entry per block, level l+1 one per 64 entries of level l, until one.
+int alloc_reduction(System& s) {
+    size_t entries[TMPS_RED_LEVELS + 1] = {0};
+    size_t count = (size_t)nblocks_for(std::max(s.max_atoms, 1));
+    int levels = 0;
+    size_t nd = 0, nc = 0;
+    while (true) {
+        if (levels == TMPS_RED_LEVELS)
+            return tmps_fail(TMPS_ERR_CAPACITY, "max_atoms too large for the step reduction");
+        entries[levels] = count;
+        nd += count * 8;
+        if (count == 1) break;
+        const size_t groups = (count + 63) / 64;
+        nc += groups;
+        count = groups;
+        ++levels;
+    }
+    int rc = dalloc(&s.red_buf, nd);
+    if (!rc) rc = dalloc(&s.red_cnt, std::max<size_t>(nc, 1));
+    if (rc) return rc;
+    size_t od = 0, oc = 0;
+    for (int l = 0; l < TMPS_RED_LEVELS; ++l) {
+        s.red.lvl[l] = nullptr;
+        s.red.cnt[l] = nullptr;
+    }
+    for (int l = 0; l <= levels; ++l) {
+        s.red.lvl[l] = s.red_buf + od;
+        od += entries[l] * 8;
+        if (l < levels) {
+            s.red.cnt[l] = s.red_cnt + oc;
+            oc += (entries[l] + 63) / 64;
+        }
+    }
+    s.red.count0 = 0;
+    return 0;
 }
 
 int create(System& s, int max_atoms, int max_nb, int precision) {
@@ -945,6 +991,7 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
         rc = rc ? rc : dalloc(&s.om, na);
         rc = rc ? rc : dalloc(&s.vm2, na);
         rc = rc ? rc : dalloc(&s.om2, na);
+        rc = rc ? rc : dalloc(&s.xr2, na);
         rc = rc ? rc : dalloc(&s.f, na);
         rc = rc ? rc : dalloc(&s.tq, na);
     } else {
@@ -953,6 +1000,7 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
         rc = rc ? rc : dalloc(&s.omf, na);
         rc = rc ? rc : dalloc(&s.vmf2, na);
         rc = rc ? rc : dalloc(&s.omf2, na);
+        rc = rc ? rc : dalloc(&s.up2, na);
         rc = rc ? rc : dalloc(&s.ff, na);
         rc = rc ? rc : dalloc(&s.tqf, na);
         rc = rc ? rc : dalloc(&s.u0, na);
@@ -968,6 +1016,7 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     const int nparts = nblocks_for(std::max(max_atoms, 1)) + 1;
     rc = rc ? rc : dalloc(&s.partial, (size_t)nparts * 8);
     s.partial_cap = nparts;
+    rc = rc ? rc : alloc_reduction(s);
     rc = rc ? rc : dalloc(&s.ctl, 1);
     if (rc) return rc;
     HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));

@@ -3,7 +3,11 @@
 //
 // Built with -ffp-contract=off: every expression rounds as written, so each
 // result is bitwise equal to the CPU oracle's for the same inputs.
+#include <algorithm>
+
 #include "contact.h"
+#include "fused.h"
+#include "servo.h"
 #include "tmps_internal.h"
 
 using namespace tmps;
@@ -366,158 +370,6 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_final_tree(Ctl* ctl, const do
         for (int k = 0; k < NC; ++k) ctl->vsum[k] = out[k];
 }
 
-// _limit_rate_device, deform_pressure.py:582-596
-__device__ double limit_rate(const tmps_servo_params& S, double rate, double ptarget) {
-    double out = rate;
-    if (S.normalize == 1) {
-        if (ptarget == 0.0) {
-            if (out != 0.0) out = S.max_rate * (out > 0.0 ? 1.0 : -1.0);
-        } else {
-            out = out / fabs(ptarget);
-        }
-    }
-    if (S.max_rate != 0.0 && fabs(out) > S.max_rate) out = S.max_rate * (out > 0.0 ? 1.0 : -1.0);
-    return out;
-}
-
-// FixDeformPressure.end_of_step for x/y/z styles (deform_pressure.py:1113-1219
-// host path, :598-702 device path): pressure tensor from the reduced
-// virial(+kinetic) sum, couple, per-axis targets, remap record, box publish.
-__device__ void servo_update(Ctl* ctl, const double vs[6], double dt, int natoms) {
-    // work on register copies: the control block is global memory, and a
-    // chain of dependent loads and stores on it costs microseconds
-    DevServo SV = ctl->servo;
-    const tmps_servo_params& S = SV.p;
-    DevBox B = ctl->box;
-    SV.step_count += 1;
-    SV.nsteps += 1;
-    ctl->remap.step = -1;
-    if (SV.step_count % S.nevery != 0) {
-        ctl->servo.step_count = SV.step_count;
-        ctl->servo.nsteps = SV.nsteps;
-        return;
-    }
-
-    bool needs_pressure = false;
-    for (int i = 0; i < 3; ++i)
-        if (S.style[i] == 8 || S.style[i] == 9) needs_pressure = true;
-    double p[3] = {0.0, 0.0, 0.0};
-    if (needs_pressure) {
-        const double vol = B.prd[0] * B.prd[1] * B.prd[2];
-        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (natoms > 0 && vol > 0.0)
-            for (int k = 0; k < 6; ++k) t[k] = vs[k] / vol;
-        for (int k = 0; k < 6; ++k) ctl->tensor[k] = t[k];
-        const double scalar = (t[0] + t[1] + t[2]) / 3.0;
-        if (S.couple == 1) {
-            p[0] = p[1] = p[2] = scalar;
-        } else if (S.couple >= 2) {
-            int a = 0, b = 1;
-            if (S.couple == 3) { a = 1; b = 2; }
-            if (S.couple == 4) { a = 0; b = 2; }
-            const double mean = (t[a] + t[b]) / 2.0;
-            for (int i = 0; i < 3; ++i) p[i] = t[i];
-            p[a] = mean;
-            p[b] = mean;
-        } else {
-            for (int i = 0; i < 3; ++i) {
-                if (S.style[i] == 8) p[i] = t[i];
-                else if (S.style[i] == 9) p[i] = scalar;
-            }
-        }
-    }
-    double old_prd[3], old_lo[3], lo_t[3], hi_t[3];
-    for (int i = 0; i < 3; ++i) {
-        old_prd[i] = B.prd[i];
-        old_lo[i] = B.lo[i];
-        lo_t[i] = B.lo[i];
-        hi_t[i] = B.hi[i];
-    }
-    const double elapsed = SV.nsteps * dt;
-    const double frac = SV.nsteps_total > 0 ? (double)SV.nsteps / (double)SV.nsteps_total : 0.0;
-    for (int i = 0; i < 3; ++i) {
-        const double ls = SV.lo_start[i], hs = SV.hi_start[i];
-        switch (S.style[i]) {
-            case 2: {  // trate
-                const double mid = 0.5 * (ls + hs);
-                const double half = 0.5 * (hs - ls) * exp(S.rate[i] * elapsed);
-                lo_t[i] = mid - half; hi_t[i] = mid + half;
-            } break;
-            case 1: {  // erate
-                const double shift = 0.5 * elapsed * S.rate[i] * (hs - ls);
-                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
-            } break;
-            case 3: {  // vel
-                const double shift = 0.5 * elapsed * S.vel[i];
-                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
-            } break;
-            case 4: {  // wiggle
-                const double shift =
-                    0.5 * S.amp[i] * sin(2.0 * 3.14159265358979323846 * elapsed / S.tper[i]);
-                lo_t[i] = ls - shift; hi_t[i] = hs + shift;
-            } break;
-            case 5:
-            case 6:
-            case 7: {  // final, delta, scale
-                double lo_stop, hi_stop;
-                if (S.style[i] == 5) { lo_stop = S.flo[i]; hi_stop = S.fhi[i]; }
-                else if (S.style[i] == 6) { lo_stop = ls + S.dlo[i]; hi_stop = hs + S.dhi[i]; }
-                else {
-                    const double mid = 0.5 * (ls + hs);
-                    const double half = 0.5 * S.scale[i] * (hs - ls);
-                    lo_stop = mid - half; hi_stop = mid + half;
-                }
-                lo_t[i] = ls + frac * (lo_stop - ls);
-                hi_t[i] = hs + frac * (hi_stop - hs);
-            } break;
-            case 8:
-            case 9: {  // pressure, pressure/mean
-                const double rate = limit_rate(S, S.pgain[i] * (p[i] - S.ptarget[i]), S.ptarget[i]);
-                SV.cum[i] += dt * rate * old_prd[i];
-                lo_t[i] = ls - 0.5 * SV.cum[i];
-                hi_t[i] = hs + 0.5 * SV.cum[i];
-            } break;
-            default: break;
-        }
-    }
-    double new_prd[3];
-    for (int i = 0; i < 3; ++i) new_prd[i] = hi_t[i] - lo_t[i];
-    ctl->servo.step_count = SV.step_count;
-    ctl->servo.nsteps = SV.nsteps;
-    for (int i = 0; i < 3; ++i) ctl->servo.cum[i] = SV.cum[i];
-    for (int i = 0; i < 3; ++i)
-        if (new_prd[i] <= 0.0) {
-            ctl->error = TMPS_ERR_BOX;
-            ctl->abort = 1;
-            return;
-        }
-    if (S.remap == 0) {
-        RemapRec R;
-        const double xprd = old_prd[0], yprd = old_prd[1], zprd = old_prd[2];
-        R.oh[0] = -0.0 / (xprd * yprd);
-        R.oh[1] = (0.0 * 0.0 - yprd * 0.0) / (xprd * yprd * zprd);
-        R.oh[2] = -0.0 / (yprd * zprd);
-        for (int i = 0; i < 3; ++i) {
-            R.old_lo[i] = old_lo[i];
-            R.old_prd[i] = old_prd[i];
-            R.new_lo[i] = lo_t[i];
-            R.new_prd[i] = new_prd[i];
-        }
-        R.step = ctl->timestep + 1;
-        R.pad = 0;
-        ctl->remap = R;
-    }
-    for (int i = 0; i < 3; ++i) {
-        B.h_rate[i] = (new_prd[i] - old_prd[i]) / dt;
-        B.lo[i] = lo_t[i];
-        B.hi[i] = hi_t[i];
-        B.prd[i] = B.hi[i] - B.lo[i];
-    }
-    B.h_rate[3] = B.h_rate[4] = B.h_rate[5] = (0.0 - 0.0) / dt;
-    B.vremap = S.remap == 1 ? 1 : 0;
-    ctl->box = B;
-}
-
 // Tree-sums the force pass's block partials and, when enabled, runs the
 // servo. `advance` marks a fused step: it honours the segment's abort and
 // rebuild-trigger flags and completes the step's bookkeeping.
@@ -570,7 +422,8 @@ __global__ void k_remap_apply(F64Ptrs P, const Ctl* ctl, int fused) {
     P.xr[i] = x;
 }
 
-__global__ void k_remap_clear(Ctl* ctl) {
+__global__ void k_remap_clear(Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->remap.step = -1;
 }
 
@@ -702,6 +555,228 @@ __global__ void k_copy_vw(double4* vdst, double4* wdst, const double4* vsrc, con
     if (i >= n) return;
     vdst[i] = vsrc[i];
     wdst[i] = wsrc[i];
+}
+
+// ------------------------------------------------ one kernel per step
+
+// Buffers of one fused step: it reads set `in` (step t-1's results) and
+// writes set `out`.
+struct Fused64 {
+    const double4 *xr_in, *vm_in, *om_in;
+    double4 *xr_out, *vm_out, *om_out;
+    const uint8_t* fl_in;
+    uint8_t* fl_out;
+    const double4* h_in;
+    double4* h_out;
+    double4 *f, *tq;
+    const double4* x0;
+    double* vir;
+    const int *nb_start, *nb_j;
+    int n;
+};
+
+// Where atom j is at step t: step t-1's pending remap, then the drift with
+// its half-step velocity -- the arithmetic of k_step_begin_f64, so every
+// thread that needs atom j computes the same bits.
+__device__ __forceinline__ double4 step_position(const RemapRec& R, bool remap, double dt,
+                                                 double4 x, const double4& v) {
+    if (remap) remap_point(R, x);
+    x.x = x.x + dt * v.x;
+    x.y = x.y + dt * v.y;
+    x.z = x.z + dt * v.z;
+    return x;
+}
+
+// Step t in one pass: drift (own atom and, on the fly, each neighbour), skin
+// check, contact gather, damping/cundall, final_integrate, the next step's
+// half-kick (or, at a segment's end, the stored f/torque/virial), and the
+// step's end in the last block (finish_step). Nothing of set `in` is written.
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_F64))) k_step_f64(Fused64 P, Ctl* ctl, PairP pp, RedP R,
+                                                          double dt, double gl, double ga,
+                                                          int servo_on) {
+    if (ctl->abort) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * TMPS_BLOCK + threadIdx.x;
+    const int t = ctl->timestep + 1;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int trig = 0;
+    if (i < P.n) {
+        const RemapRec RM = ctl->remap;
+        const bool remap = RM.step != -1 && RM.step == ctl->timestep;
+        const ImgP B = img_params(ctl);
+        double4 v = P.vm_in[i];
+        double4 w = P.om_in[i];
+        const double4 xi = step_position(RM, remap, dt, P.xr_in[i], v);
+        if (check_due(ctl, t)) {
+            if (!ctl->check) {
+                trig = 1;
+            } else {
+                const double4 x0 = P.x0[i];
+                const double d0 = xi.x - x0.x, d1 = xi.y - x0.y, d2 = xi.z - x0.z;
+                if (d0 * d0 + d1 * d1 + d2 * d2 > ctl->trigger_sq) trig = 1;
+            }
+        }
+        double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+        double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int nt = 0;
+        const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
+        const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
+        for (int s = s0; s < s1; ++s) {
+            const int j = P.nb_j[s];
+            const double4 vj = P.vm_in[j];
+            const double4 xj = step_position(RM, remap, dt, P.xr_in[j], vj);
+            const bool first = i < j;
+            const double4& xa = first ? xi : xj;
+            const double4& xb = first ? xj : xi;
+            double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
+            double dvj[3];
+            min_image_vshift(B, d, dvj);
+            const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double ra = xa.w, rb = xb.w;
+            const double radsum = ra + rb;
+            uint8_t touch = 0;
+            if (rsq < radsum * radsum && rsq > 1e-28) {
+                const double4 oj = P.om_in[j];
+                const double4& va4 = first ? v : vj;
+                const double4& vb4 = first ? vj : v;
+                const double4& oa4 = first ? w : oj;
+                const double4& ob4 = first ? oj : w;
+                const double va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+                const double oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+                bool before = false;
+                double h[3] = {0.0, 0.0, 0.0};
+                if (uses_history) {
+                    before = P.fl_in[s] != 0;
+                    if (before) {
+                        const double4 hh = P.h_in[s];
+                        h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
+                    }
+                }
+                ContactOut<double> o;
+                if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                    legacy_history_contact<double, false>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w,
+                                                          va, vb, oa, ob, pp.kn, pp.kt, pp.gamman,
+                                                          pp.gammat, pp.xmu, pp.limit_damping, dt,
+                                                          1, before, h, o);
+                else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                    legacy_history_contact<double, true>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w,
+                                                         va, vb, oa, ob, pp.kn, pp.kt, pp.gamman,
+                                                         pp.gammat, pp.xmu, pp.limit_damping, dt, 1,
+                                                         before, h, o);
+                else
+                    granular_contact<double>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa,
+                                             ob, pp, dt, 1, before, h, o);
+                if (uses_history) {
+                    touch = 1;
+                    P.h_out[s] = make_double4(h[0], h[1], h[2], 0.0);
+                }
+                if (first) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        F[k] = F[k] + o.F[k];
+                        T[k] = T[k] + o.ta[k];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        F[k] = F[k] + (-o.F[k]);
+                        T[k] = T[k] + o.tb[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) V[k] = V[k] + o.vir[k];
+                ++nt;
+            }
+            if (uses_history) P.fl_out[s] = touch;
+        }
+        if (CUNDALL) {
+            F[0] *= cundall_scale(F[0] * v.x, gl);
+            T[0] *= cundall_scale(T[0] * w.x, ga);
+            F[1] *= cundall_scale(F[1] * v.y, gl);
+            T[1] *= cundall_scale(T[1] * w.y, ga);
+            F[2] *= cundall_scale(F[2] * v.z, gl);
+            T[2] *= cundall_scale(T[2] * w.z, ga);
+        }
+        const double dtf = 0.5 * dt;
+        const double m = v.w, r = xi.w;
+        const double inertia = 0.4 * m * r * r;
+        const double am[3] = {F[0] / m, F[1] / m, F[2] / m};
+        const double ai[3] = {T[0] / inertia, T[1] / inertia, T[2] / inertia};
+        v.x += dtf * am[0];
+        v.y += dtf * am[1];
+        v.z += dtf * am[2];
+        w.x += dtf * ai[0];
+        w.y += dtf * ai[1];
+        w.z += dtf * ai[2];
+        leaf[0] = V[0] + m * v.x * v.x;
+        leaf[1] = V[1] + m * v.y * v.y;
+        leaf[2] = V[2] + m * v.z * v.z;
+        leaf[3] = V[3] + m * v.x * v.y;
+        leaf[4] = V[4] + m * v.x * v.z;
+        leaf[5] = V[5] + m * v.y * v.z;
+        leaf[6] = (double)nt;
+        if (FINAL) {
+            P.f[i] = make_double4(F[0], F[1], F[2], 0.0);
+            P.tq[i] = make_double4(T[0], T[1], T[2], 0.0);
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+        } else {
+            v.x += dtf * am[0];
+            v.y += dtf * am[1];
+            v.z += dtf * am[2];
+            w.x += dtf * ai[0];
+            w.y += dtf * ai[1];
+            w.z += dtf * ai[2];
+        }
+        P.xr_out[i] = xi;
+        P.vm_out[i] = v;
+        P.om_out[i] = w;
+    }
+    finish_step(ctl, R, leaf, trig, lb, dt, servo_on, P.n);
+}
+
+// initial_integrate's half-kick of a segment's first step (the later steps
+// receive theirs from the previous step's pass), in place.
+__global__ void k_kick_f64(double4* vm, double4* om, const double4* xr, const double4* f,
+                           const double4* tq, int n, const Ctl* ctl, double dt) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dtf = 0.5 * dt;
+    double4 v = vm[i];
+    const double4 ff = f[i];
+    const double m = v.w;
+    v.x += dtf * (ff.x / m);
+    v.y += dtf * (ff.y / m);
+    v.z += dtf * (ff.z / m);
+    double4 w = om[i];
+    const double4 t = tq[i];
+    const double r = xr[i].w;
+    const double inertia = 0.4 * m * r * r;
+    w.x += dtf * (t.x / inertia);
+    w.y += dtf * (t.y / inertia);
+    w.z += dtf * (t.z / inertia);
+    vm[i] = v;
+    om[i] = w;
+}
+
+// Buffer set 1 -> set 0: per-atom 16/32-byte records and per-slot flags and
+// history, one grid-stride pass.
+template <class T4>
+__global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1, const T4* w1,
+                             int n, uint8_t* fl0, const uint8_t* fl1, T4* h0, const T4* h1,
+                             int nslots, const Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        x0[i] = x1[i];
+        v0[i] = v1[i];
+        w0[i] = w1[i];
+    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nslots; k += stride) {
+        fl0[k] = fl1[k];
+        h0[k] = h1[k];
+    }
 }
 
 }  // namespace
@@ -878,14 +953,95 @@ int launch_remap_apply_fused(System& s) {
     return 0;
 }
 
-int launch_remap_clear(System& s) {
-    hipLaunchKernelGGL(k_remap_clear, dim3(1), dim3(64), 0, s.stream, s.ctl);
+int launch_remap_clear(System& s, int honor_abort) {
+    hipLaunchKernelGGL(k_remap_clear, dim3(1), dim3(64), 0, s.stream, s.ctl, honor_abort);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 int launch_ctl_rebuilt(System& s, int step) {
     hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------ one kernel per step
+
+template <int STYLE, int CUNDALL, int FINAL>
+static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, double ga,
+                        int servo_on) {
+    RedP R = s.red;
+    R.count0 = nblocks_for(s.n);
+    hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK), 0,
+                       s.stream, P, s.ctl, s.pair, R, dt, gl, ga, servo_on);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int STYLE>
+static int fused_f64_style(System& s, const Fused64& P, double dt, int fin, int cundall, double gl,
+                           double ga, int servo_on) {
+    if (fin) {
+        if (cundall) return fused_f64_go<STYLE, 1, 1>(s, P, dt, gl, ga, servo_on);
+        return fused_f64_go<STYLE, 0, 1>(s, P, dt, gl, ga, servo_on);
+    }
+    if (cundall) return fused_f64_go<STYLE, 1, 0>(s, P, dt, gl, ga, servo_on);
+    return fused_f64_go<STYLE, 0, 0>(s, P, dt, gl, ga, servo_on);
+}
+
+int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, double gl, double ga,
+                     int servo_on) {
+    if (s.n <= 0) return 0;
+    if (nblocks_for(s.n) > (1 << 30) / TMPS_BLOCK)
+        return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for one step launch");
+    const int ob = 1 - in_buf;
+    Fused64 P;
+    P.xr_in = in_buf ? s.xr2 : s.xr;
+    P.vm_in = in_buf ? s.vm2 : s.vm;
+    P.om_in = in_buf ? s.om2 : s.om;
+    P.xr_out = ob ? s.xr2 : s.xr;
+    P.vm_out = ob ? s.vm2 : s.vm;
+    P.om_out = ob ? s.om2 : s.om;
+    P.fl_in = in_buf ? s.nb_flag2 : s.nb_flag;
+    P.fl_out = ob ? s.nb_flag2 : s.nb_flag;
+    P.h_in = static_cast<const double4*>(in_buf ? s.nb_hist2 : s.nb_hist);
+    P.h_out = static_cast<double4*>(ob ? s.nb_hist2 : s.nb_hist);
+    P.f = s.f;
+    P.tq = s.tq;
+    P.x0 = s.x0;
+    P.vir = s.vir;
+    P.nb_start = s.nb_start;
+    P.nb_j = s.nb_j;
+    P.n = s.n;
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            return fused_f64_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga, servo_on);
+        case TMPS_PAIR_HERTZ_HISTORY:
+            return fused_f64_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga, servo_on);
+        default:
+            return fused_f64_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga, servo_on);
+    }
+}
+
+int launch_kick_f64(System& s, double dt) {
+    LAUNCH_N(k_kick_f64, s.n, s.vm, s.om, s.xr, s.f, s.tq, s.n, s.ctl, dt);
+    return 0;
+}
+
+int launch_copy_state(System& s, int honor_abort) {
+    const int work = std::max(s.n, s.nslots);
+    if (work <= 0) return 0;
+    const int nb = std::min(nblocks_for(work), 4096);
+    if (s.precision == TMPS_FP64)
+        hipLaunchKernelGGL(k_copy_state<double4>, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.xr,
+                           s.vm, s.om, s.xr2, s.vm2, s.om2, s.n, s.nb_flag, s.nb_flag2,
+                           static_cast<double4*>(s.nb_hist), static_cast<const double4*>(s.nb_hist2),
+                           s.nslots, s.ctl, honor_abort);
+    else
+        hipLaunchKernelGGL(k_copy_state<float4>, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.up,
+                           s.vmf, s.omf, s.up2, s.vmf2, s.omf2, s.n, s.nb_flag, s.nb_flag2,
+                           static_cast<float4*>(s.nb_hist), static_cast<const float4*>(s.nb_hist2),
+                           s.nslots, s.ctl, honor_abort);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -450,9 +450,13 @@ static int grow_list(System& s, size_t need) {
     if (s.nb_j) HIP_TRY(hipFree(s.nb_j));
     if (s.nb_flag) HIP_TRY(hipFree(s.nb_flag));
     if (s.nb_hist) HIP_TRY(hipFree(s.nb_hist));
+    if (s.nb_flag2) HIP_TRY(hipFree(s.nb_flag2));
+    if (s.nb_hist2) HIP_TRY(hipFree(s.nb_hist2));
     HIP_TRY(hipMalloc(&s.nb_j, cap * sizeof(int)));
     HIP_TRY(hipMalloc(&s.nb_flag, cap));
     HIP_TRY(hipMalloc(&s.nb_hist, cap * hsz));
+    HIP_TRY(hipMalloc(&s.nb_flag2, cap));
+    HIP_TRY(hipMalloc(&s.nb_hist2, cap * hsz));
     s.nb_cap = cap;
     drop_graphs(s);
     return 0;

@@ -8,6 +8,7 @@
 // box alone moves the atoms. Contact and integration arithmetic is FP32; the
 // per-atom virial, the kinetic tensor, their reduction and the servo stay FP64.
 #include "contact.h"
+#include "fused.h"
 #include "tmps_internal.h"
 
 using namespace tmps;
@@ -415,6 +416,178 @@ __global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, 
     wdst[i] = wsrc[i];
 }
 
+// ------------------------------------------------ one kernel per step
+
+struct FusedS {
+    const float4 *up_in, *vm_in, *om_in;
+    float4 *up_out, *vm_out, *om_out;
+    const uint8_t* fl_in;
+    uint8_t* fl_out;
+    const float4* h_in;
+    float4* h_out;
+    float4 *f, *tq;
+    const float4* u0;
+    const uint32_t* cell;
+    double* vir;
+    const int *nb_start, *nb_j;
+    int n;
+};
+
+// Where atom j is at step t in bin units: under remap x the offsets need no
+// update, so the drift is all (dtc = dt / bin width of the current box).
+__device__ __forceinline__ float4 step_offset(const float dtc[3], float4 u, const float4& v) {
+    u.x = fmaf(dtc[0], v.x, u.x);
+    u.y = fmaf(dtc[1], v.y, u.y);
+    u.z = fmaf(dtc[2], v.z, u.z);
+    return u;
+}
+
+// Step t in one pass (see k_step_f64): FP32 contacts and integration, FP64
+// leaves, tree and servo.
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, RedP R,
+                                                            float dt, float gl, float ga,
+                                                            double ddt, int servo_on) {
+    if (ctl->abort) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = lb * TMPS_BLOCK + threadIdx.x;
+    const int t = ctl->timestep + 1;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int trig = 0;
+    if (i < P.n) {
+        const SplitConst sc = ctl->sc;
+        const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
+        const float dtc[3] = {dt * sc.inv_cw[0], dt * sc.inv_cw[1], dt * sc.inv_cw[2]};
+        float4 v = P.vm_in[i];
+        float4 w = P.om_in[i];
+        const uint32_t ci = P.cell[i];
+        const float4 ui = step_offset(dtc, P.up_in[i], v);
+        if (check_due(ctl, t)) {
+            if (!ctl->check) trig = 1;
+            else if (split_disp_sq(sc, ci, ui, P.u0[i]) > (float)ctl->trigger_sq) trig = 1;
+        }
+        float F[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+        double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int nt = 0;
+        const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
+        const float zero3[3] = {0.f, 0.f, 0.f};
+        const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
+        for (int s = s0; s < s1; ++s) {
+            const int j = P.nb_j[s];
+            const float4 vj = P.vm_in[j];
+            const float4 uj = step_offset(dtc, P.up_in[j], vj);
+            const uint32_t cj = P.cell[j];
+            const bool first = i < j;
+            float d[3];
+            if (first) split_delta(sc, per, ci, ui, cj, uj, d);
+            else split_delta(sc, per, cj, uj, ci, ui, d);
+            const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
+            const float radsum = ra + rb;
+            uint8_t touch = 0;
+            if (rsq < radsum * radsum && rsq > 1e-28f) {
+                const float4 oj = P.om_in[j];
+                const float4& va4 = first ? v : vj;
+                const float4& vb4 = first ? vj : v;
+                const float4& oa4 = first ? w : oj;
+                const float4& ob4 = first ? oj : w;
+                const float va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+                const float oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+                bool before = false;
+                float h[3] = {0.f, 0.f, 0.f};
+                if (uses_history) {
+                    before = P.fl_in[s] != 0;
+                    if (before) {
+                        const float4 hh = P.h_in[s];
+                        h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
+                    }
+                }
+                ContactOut<float> o;
+                if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                    legacy_history_contact<float, false>(
+                        d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                        (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                        pp.limit_damping, dt, 1, before, h, o);
+                else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                    legacy_history_contact<float, true>(
+                        d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                        (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                        pp.limit_damping, dt, 1, before, h, o);
+                else
+                    granular_contact<float>(d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa,
+                                            ob, pp, dt, 1, before, h, o);
+                if (uses_history) {
+                    touch = 1;
+                    P.h_out[s] = make_float4(h[0], h[1], h[2], 0.f);
+                }
+                if (first) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        F[k] += o.F[k];
+                        T[k] += o.ta[k];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        F[k] -= o.F[k];
+                        T[k] += o.tb[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) V[k] += (double)o.vir[k];
+                ++nt;
+            }
+            if (uses_history) P.fl_out[s] = touch;
+        }
+        if (CUNDALL) {
+            F[0] *= cundall_scale_f(F[0] * v.x, gl); T[0] *= cundall_scale_f(T[0] * w.x, ga);
+            F[1] *= cundall_scale_f(F[1] * v.y, gl); T[1] *= cundall_scale_f(T[1] * w.y, ga);
+            F[2] *= cundall_scale_f(F[2] * v.z, gl); T[2] *= cundall_scale_f(T[2] * w.z, ga);
+        }
+        const float dtf = 0.5f * dt;
+        const float im = 1.0f / v.w;
+        const float am[3] = {F[0] * im, F[1] * im, F[2] * im};
+        const float ai[3] = {T[0] * w.w, T[1] * w.w, T[2] * w.w};
+        v.x += dtf * am[0]; v.y += dtf * am[1]; v.z += dtf * am[2];
+        w.x += dtf * ai[0]; w.y += dtf * ai[1]; w.z += dtf * ai[2];
+        const double m = v.w, vx = v.x, vy = v.y, vz = v.z;
+        leaf[0] = V[0] + m * vx * vx;
+        leaf[1] = V[1] + m * vy * vy;
+        leaf[2] = V[2] + m * vz * vz;
+        leaf[3] = V[3] + m * vx * vy;
+        leaf[4] = V[4] + m * vx * vz;
+        leaf[5] = V[5] + m * vy * vz;
+        leaf[6] = (double)nt;
+        if (FINAL) {
+            P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
+            P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
+            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+        } else {
+            v.x += dtf * am[0]; v.y += dtf * am[1]; v.z += dtf * am[2];
+            w.x += dtf * ai[0]; w.y += dtf * ai[1]; w.z += dtf * ai[2];
+        }
+        P.up_out[i] = ui;
+        P.vm_out[i] = v;
+        P.om_out[i] = w;
+    }
+    finish_step(ctl, R, leaf, trig, lb, ddt, servo_on, P.n);
+}
+
+__global__ void k_kick_split(float4* vm, float4* om, const float4* f, const float4* tq, int n,
+                             const Ctl* ctl, float dt) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dtf = 0.5f * dt;
+    float4 v = vm[i], w = om[i];
+    const float4 ff = f[i], tq4 = tq[i];
+    const float im = 1.0f / v.w;
+    v.x += dtf * ff.x * im; v.y += dtf * ff.y * im; v.z += dtf * ff.z * im;
+    w.x += dtf * tq4.x * w.w; w.y += dtf * tq4.y * w.w; w.z += dtf * tq4.z * w.w;
+    vm[i] = v;
+    om[i] = w;
+}
+
 }  // namespace
 
 // ================================================================ launchers
@@ -556,4 +729,69 @@ int launch_step_force_split(System& s, double dt, int final_mode, int cundall, d
             return launch_force_split_style<TMPS_PAIR_GRANULAR>(s, dt, final_mode, cundall, gl, ga,
                                                                 in_buf, out_buf);
     }
+}
+
+// ------------------------------------------------ one kernel per step
+
+template <int STYLE, int CUNDALL, int FINAL>
+static int fused_split_go(System& s, const FusedS& P, double dt, double gl, double ga,
+                          int servo_on) {
+    RedP R = s.red;
+    R.count0 = nblocks_for(s.n);
+    hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK), 0,
+                       s.stream, P, s.ctl, s.pair, R, (float)dt, (float)gl, (float)ga, dt,
+                       servo_on);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <int STYLE>
+static int fused_split_style(System& s, const FusedS& P, double dt, int fin, int cundall,
+                             double gl, double ga, int servo_on) {
+    if (fin) {
+        if (cundall) return fused_split_go<STYLE, 1, 1>(s, P, dt, gl, ga, servo_on);
+        return fused_split_go<STYLE, 0, 1>(s, P, dt, gl, ga, servo_on);
+    }
+    if (cundall) return fused_split_go<STYLE, 1, 0>(s, P, dt, gl, ga, servo_on);
+    return fused_split_go<STYLE, 0, 0>(s, P, dt, gl, ga, servo_on);
+}
+
+int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, double gl,
+                       double ga, int servo_on) {
+    if (s.n <= 0) return 0;
+    const int ob = 1 - in_buf;
+    FusedS P;
+    P.up_in = in_buf ? s.up2 : s.up;
+    P.vm_in = in_buf ? s.vmf2 : s.vmf;
+    P.om_in = in_buf ? s.omf2 : s.omf;
+    P.up_out = ob ? s.up2 : s.up;
+    P.vm_out = ob ? s.vmf2 : s.vmf;
+    P.om_out = ob ? s.omf2 : s.omf;
+    P.fl_in = in_buf ? s.nb_flag2 : s.nb_flag;
+    P.fl_out = ob ? s.nb_flag2 : s.nb_flag;
+    P.h_in = static_cast<const float4*>(in_buf ? s.nb_hist2 : s.nb_hist);
+    P.h_out = static_cast<float4*>(ob ? s.nb_hist2 : s.nb_hist);
+    P.f = s.ff;
+    P.tq = s.tqf;
+    P.u0 = s.u0;
+    P.cell = s.cell;
+    P.vir = s.vir;
+    P.nb_start = s.nb_start;
+    P.nb_j = s.nb_j;
+    P.n = s.n;
+    switch (s.pair.style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            return fused_split_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga,
+                                                              servo_on);
+        case TMPS_PAIR_HERTZ_HISTORY:
+            return fused_split_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga,
+                                                              servo_on);
+        default:
+            return fused_split_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga, servo_on);
+    }
+}
+
+int launch_kick_split(System& s, double dt) {
+    LAUNCH_S(k_kick_split, s.n, s.vmf, s.omf, s.ff, s.tqf, s.n, s.ctl, (float)dt);
+    return 0;
 }

@@ -81,6 +81,16 @@ struct PairP {
     double g_kn, g_damp, g_kt, g_xt, g_mu;
 };
 
+// The fused step's reduction scratch: lvl[0] holds one 8-double entry per
+// block (7 tree components, rebuild flag), lvl[l] one per group of 64 entries
+// of level l-1; cnt[l] counts the arrivals of level-l entries per group.
+#define TMPS_RED_LEVELS 6
+struct RedP {
+    double* lvl[TMPS_RED_LEVELS];
+    unsigned* cnt[TMPS_RED_LEVELS];
+    int count0;  // blocks of the launch
+};
+
 struct GridP {
     int g[3];
     double csize[3];
@@ -100,17 +110,23 @@ struct System {
     // second v/omega buffers: a fused force pass reads one and writes the other,
     // since other atoms' contacts still read this step's half-step velocities
     double4 *vm2 = nullptr, *om2 = nullptr;
+    double4* xr2 = nullptr;  // second position buffer of the one-kernel step
     double* vir = nullptr;  // [n][6]
     int* tag = nullptr;
     // FP32-split per-atom state
     float4 *up = nullptr, *vmf = nullptr, *omf = nullptr, *ff = nullptr, *tqf = nullptr, *u0 = nullptr;
     float4 *vmf2 = nullptr, *omf2 = nullptr;
+    float4* up2 = nullptr;
     uint32_t* cell = nullptr;
 
     // full neighbor list (CSR) and its history; o* = previous list for the remap
     int *nb_start = nullptr, *nb_j = nullptr, *onb_start = nullptr, *onb_j = nullptr;
     uint8_t *nb_flag = nullptr, *onb_flag = nullptr;
     void *nb_hist = nullptr, *onb_hist = nullptr;  // double4 (FP64) or float4 (split)
+    // second touch-flag / history buffers: a one-kernel step reads one set
+    // and writes the other, so a rebuild request leaves the first intact
+    uint8_t* nb_flag2 = nullptr;
+    void* nb_hist2 = nullptr;
     size_t nb_cap = 0, onb_cap = 0;
     int nslots = 0, onslots = 0, onb_n = 0;
     bool have_old = false;
@@ -126,6 +142,9 @@ struct System {
     // reductions
     double* partial = nullptr;  // [blocks][8]
     int partial_cap = 0;
+    double* red_buf = nullptr;  // the fused step's reduction levels
+    unsigned* red_cnt = nullptr;
+    RedP red{};
 
     Ctl* ctl = nullptr;   // device control block
     Ctl* hctl = nullptr;  // pinned host mirror
@@ -207,10 +226,25 @@ int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, dou
 // copy v/omega buffer 1 -> buffer 0 (skipped on abort when honor_abort)
 int launch_copy_vw(System& s, int honor_abort);
 
+// one kernel per step (k_fp64.hip, k_split.hip): in_buf selects the buffer
+// set the step reads (the other one receives its results)
+int launch_fused_f64(System& s, int in_buf, double dt, int final_mode, int cundall, double gl,
+                     double ga, int servo_on);
+int launch_fused_split(System& s, int in_buf, double dt, int final_mode, int cundall, double gl,
+                       double ga, int servo_on);
+// the initial_integrate half-kick of a segment's first step, in buffer 0
+int launch_kick_f64(System& s, double dt);
+int launch_kick_split(System& s, double dt);
+// buffer set 1 -> buffer set 0 (positions, v, omega, touch flags, history);
+// skipped on abort when honor_abort
+int launch_copy_state(System& s, int honor_abort);
+
 // shared (k_fp64.hip)
 int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo);
 int launch_remap_apply(System& s);
-int launch_remap_clear(System& s);
+// clears the pending remap record (kept when honor_abort and the step aborted:
+// the redo of that step still needs it)
+int launch_remap_clear(System& s, int honor_abort = 0);
 int launch_ctl_rebuilt(System& s, int step);
 int launch_remap_apply_fused(System& s);
 void drop_graphs(System& s);
