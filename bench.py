@@ -68,9 +68,10 @@ def model_bytes(key, n_slot, n_touch):
     return base + per_slot * n_slot + per_touch * n_touch
 
 
-def run_precision(w, precision, steps, warmup, prof_steps):
+def run_precision(w, precision, steps, warmup, prof_steps, direct=False):
     sim = build_sim(w, precision)
     N = sim.atom._native
+    N.set_launch_mode(direct)
     n = sim.atom.nlocal
     t0 = time.perf_counter()
     sim.run(warmup)
@@ -153,6 +154,8 @@ def main(argv=None):
                     help="lattice dimensions (default 100 100 100)")
     ap.add_argument("--precision", choices=("both", "f64", "f32split"), default="both",
                     help="run one precision only (for profiler passes); the JSON line needs both")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch each step kernel directly instead of replaying captured graphs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a known-size "
                          "kernel that scales the profiler's HBM byte counters), then exit")
@@ -170,7 +173,7 @@ def main(argv=None):
     res = {}
     precisions = ("f64", "f32split") if args.precision == "both" else (args.precision,)
     for prec in precisions:
-        res[prec] = run_precision(w, prec, args.steps, args.warmup, args.prof_steps)
+        res[prec] = run_precision(w, prec, args.steps, args.warmup, args.prof_steps, args.direct)
         r = res[prec]
         print(f"# {prec}: {r['ms_per_step']:.4f} ms/step, {r['atom_steps_per_s']:.4e} atom-steps/s, "
               f"kernels {r['kernel_ms']}, n_slot {r['n_slot']:.3f}, n_touch {r['n_touch']:.3f}, "

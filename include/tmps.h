@@ -191,9 +191,16 @@ int tmps_set_counters(tmps_system* s, const tmps_counters* in);
 int tmps_last_pressure(tmps_system* s, double out6[6]);
 
 /* ---- measurement ----------------------------------------------------------- */
-/* Average duration (ms) of each kernel class over the last tmps_run, timed
- * with HIP events on the system's stream when profiling is enabled.
- * names: "integrate", "force", "reduce"; out_ms[3]. */
+/* How tmps_run issues a segment's kernels: replayed from a captured HIP graph
+ * (the default) or launched one by one on the system's stream. No reference
+ * counterpart; the reference leaves kernel dispatch to Taichi. */
+#define TMPS_LAUNCH_GRAPH 0
+#define TMPS_LAUNCH_DIRECT 1
+int tmps_set_launch_mode(tmps_system* s, int mode);
+/* Average duration (ms) of each kernel class of tmps_run while profiling is
+ * enabled (direct launches, HIP events on the system's stream between
+ * kernels): [0] a segment's first half-kick, [1] the one-kernel step,
+ * [2] a segment's tail. */
 int tmps_set_profiling(tmps_system* s, int enabled);
 int tmps_kernel_times(tmps_system* s, double out_ms[3], int counts[3]);
 

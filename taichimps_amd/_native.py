@@ -49,6 +49,7 @@ SYMBOLS = (
     "tmps_nve_initial", "tmps_nve_final", "tmps_cundall", "tmps_servo_set", "tmps_servo_setup",
     "tmps_servo_end_of_step", "tmps_virial_reduce", "tmps_ke", "tmps_run", "tmps_get_counters",
     "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
+    "tmps_set_launch_mode",
 )
 
 
@@ -152,6 +153,7 @@ def load(path: str | None = None):
         "tmps_last_pressure": (C.c_int, [vp, dp]),
         "tmps_set_profiling": (C.c_int, [vp, C.c_int]),
         "tmps_kernel_times": (C.c_int, [vp, dp, ip]),
+        "tmps_set_launch_mode": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -381,6 +383,10 @@ class NativeSystem:
 
     def set_profiling(self, on: bool):
         check(self._L.tmps_set_profiling(self._h, 1 if on else 0))
+
+    def set_launch_mode(self, direct: bool):
+        """Graph replay (default) or direct kernel launches for tmps_run."""
+        check(self._L.tmps_set_launch_mode(self._h, 1 if direct else 0))
 
     def kernel_times(self):
         ms = np.zeros(3)

@@ -534,6 +534,7 @@ int profiled_steps(System& s, int len, double dt, int use_cundall, double gl, do
 
 int launch_segment(System& s, int len, double dt, int use_cundall, double gl, double ga) {
     if (s.profiling) return profiled_steps(s, len, dt, use_cundall, gl, ga);
+    if (s.launch_mode == TMPS_LAUNCH_DIRECT) return enqueue_steps(s, len, dt, use_cundall, gl, ga);
     uint64_t key = 1469598103934665603ULL;
     key = mix(key, (uint64_t)len);
     key = mix(key, dbits(dt));
@@ -1474,6 +1475,14 @@ int tmps_set_profiling(tmps_system* h, int enabled) {
         s.prof_ms[c] = 0.0;
         s.prof_cnt[c] = 0;
     }
+    return 0;
+}
+
+int tmps_set_launch_mode(tmps_system* h, int mode) {
+    System& s = *sys(h);
+    if (mode != TMPS_LAUNCH_GRAPH && mode != TMPS_LAUNCH_DIRECT)
+        return tmps_fail(TMPS_ERR_ARG, "launch mode must be TMPS_LAUNCH_GRAPH or TMPS_LAUNCH_DIRECT");
+    s.launch_mode = mode;
     return 0;
 }
 

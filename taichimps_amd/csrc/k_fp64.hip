@@ -100,8 +100,8 @@ __device__ __forceinline__ void gather_f64(int i, const double4& xi, const doubl
         const int j = P.nb_j[s];
         const double4 xj = P.xr[j];
         const bool first = i < j;
-        const double4& xa = first ? xi : xj;
-        const double4& xb = first ? xj : xi;
+        const double4 xa = first ? xi : xj;
+        const double4 xb = first ? xj : xi;
         double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
         double dvj[3];
         min_image_vshift(B, d, dvj);
@@ -111,10 +111,10 @@ __device__ __forceinline__ void gather_f64(int i, const double4& xi, const doubl
         if (rsq < radsum * radsum && rsq > 1e-28) {
             const double4 vj = P.vm[j];
             const double4 oj = P.om[j];
-            const double4& va4 = first ? vi : vj;
-            const double4& vb4 = first ? vj : vi;
-            const double4& oa4 = first ? oi : oj;
-            const double4& ob4 = first ? oj : oi;
+            const double4 va4 = first ? vi : vj;
+            const double4 vb4 = first ? vj : vi;
+            const double4 oa4 = first ? oi : oj;
+            const double4 ob4 = first ? oj : oi;
             const double va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
             const double oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
             bool before = false;
@@ -559,6 +559,10 @@ __global__ void k_copy_vw(double4* vdst, double4* wdst, const double4* vsrc, con
 
 // ------------------------------------------------ one kernel per step
 
+#ifndef F64_BATCH
+#define F64_BATCH 4
+#endif
+
 // Buffers of one fused step: it reads set `in` (step t-1's results) and
 // writes set `out`.
 struct Fused64 {
@@ -622,73 +626,109 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         int nt = 0;
         const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
         const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
-        for (int s = s0; s < s1; ++s) {
+        // Pass 1, every slot: separation and contact test, neighbour states
+        // requested a batch at a time (see k_step_split). Touching slots go
+        // into `tmask` (first 64 of the row; the rest stay in the flags).
+        uint64_t tmask = 0;
+        for (int sb = s0; sb < s1; sb += F64_BATCH) {
+            int jb[F64_BATCH];
+#pragma unroll
+            for (int q = 0; q < F64_BATCH; ++q) jb[q] = sb + q < s1 ? P.nb_j[sb + q] : i;
+            double4 vq[F64_BATCH], xq[F64_BATCH];
+#pragma unroll
+            for (int q = 0; q < F64_BATCH; ++q) {
+                vq[q] = P.vm_in[jb[q]];
+                xq[q] = P.xr_in[jb[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < F64_BATCH; ++q) {
+                if (sb + q < s1) {
+                    const double4 xj = step_position(RM, remap, dt, xq[q], vq[q]);
+                    const bool first = i < jb[q];
+                    const double4 xa = first ? xi : xj;
+                    const double4 xb = first ? xj : xi;
+                    double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
+                    double dvj[3];
+                    min_image_vshift(B, d, dvj);
+                    const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    const double radsum = xa.w + xb.w;
+                    const bool touch = rsq < radsum * radsum && rsq > 1e-28;
+                    const int k = sb + q - s0;
+                    if (touch && k < 64) tmask |= 1ull << k;
+                    P.fl_out[sb + q] = touch ? 1 : 0;
+                }
+            }
+        }
+        // Pass 2, touching slots in ascending order: the contact law with the
+        // neighbour fetched again (the same bits as in pass 1).
+        const int nlong = s1 - s0 > 64 ? s1 - s0 - 64 : 0;
+        for (int k = 0; tmask != 0 || k < nlong; ) {
+            int s;
+            if (tmask != 0) {
+                s = s0 + __builtin_ctzll(tmask);
+                tmask &= tmask - 1;
+            } else {
+                s = s0 + 64 + k++;
+                if (P.fl_out[s] == 0) continue;
+            }
             const int j = P.nb_j[s];
             const double4 vj = P.vm_in[j];
             const double4 xj = step_position(RM, remap, dt, P.xr_in[j], vj);
             const bool first = i < j;
-            const double4& xa = first ? xi : xj;
-            const double4& xb = first ? xj : xi;
+            const double4 xa = first ? xi : xj;
+            const double4 xb = first ? xj : xi;
             double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
             double dvj[3];
             min_image_vshift(B, d, dvj);
             const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             const double ra = xa.w, rb = xb.w;
             const double radsum = ra + rb;
-            uint8_t touch = 0;
-            if (rsq < radsum * radsum && rsq > 1e-28) {
-                const double4 oj = P.om_in[j];
-                const double4& va4 = first ? v : vj;
-                const double4& vb4 = first ? vj : v;
-                const double4& oa4 = first ? w : oj;
-                const double4& ob4 = first ? oj : w;
-                const double va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
-                const double oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
-                bool before = false;
-                double h[3] = {0.0, 0.0, 0.0};
-                if (uses_history) {
-                    before = P.fl_in[s] != 0;
-                    if (before) {
-                        const double4 hh = P.h_in[s];
-                        h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
-                    }
+            const double4 oj = P.om_in[j];
+            const double4 va4 = first ? v : vj;
+            const double4 vb4 = first ? vj : v;
+            const double4 oa4 = first ? w : oj;
+            const double4 ob4 = first ? oj : w;
+            const double va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+            const double oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+            bool before = false;
+            double h[3] = {0.0, 0.0, 0.0};
+            if (uses_history) {
+                before = P.fl_in[s] != 0;
+                if (before) {
+                    const double4 hh = P.h_in[s];
+                    h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
                 }
-                ContactOut<double> o;
-                if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
-                    legacy_history_contact<double, false>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w,
-                                                          va, vb, oa, ob, pp.kn, pp.kt, pp.gamman,
-                                                          pp.gammat, pp.xmu, pp.limit_damping, dt,
-                                                          1, before, h, o);
-                else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
-                    legacy_history_contact<double, true>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w,
-                                                         va, vb, oa, ob, pp.kn, pp.kt, pp.gamman,
-                                                         pp.gammat, pp.xmu, pp.limit_damping, dt, 1,
-                                                         before, h, o);
-                else
-                    granular_contact<double>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa,
-                                             ob, pp, dt, 1, before, h, o);
-                if (uses_history) {
-                    touch = 1;
-                    P.h_out[s] = make_double4(h[0], h[1], h[2], 0.0);
-                }
-                if (first) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        F[k] = F[k] + o.F[k];
-                        T[k] = T[k] + o.ta[k];
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        F[k] = F[k] + (-o.F[k]);
-                        T[k] = T[k] + o.tb[k];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 6; ++k) V[k] = V[k] + o.vir[k];
-                ++nt;
             }
-            if (uses_history) P.fl_out[s] = touch;
+            ContactOut<double> o;
+            if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                legacy_history_contact<double, false>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va,
+                                                      vb, oa, ob, pp.kn, pp.kt, pp.gamman,
+                                                      pp.gammat, pp.xmu, pp.limit_damping, dt, 1,
+                                                      before, h, o);
+            else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                legacy_history_contact<double, true>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va,
+                                                     vb, oa, ob, pp.kn, pp.kt, pp.gamman, pp.gammat,
+                                                     pp.xmu, pp.limit_damping, dt, 1, before, h, o);
+            else
+                granular_contact<double>(d, dvj, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob,
+                                         pp, dt, 1, before, h, o);
+            if (uses_history) P.h_out[s] = make_double4(h[0], h[1], h[2], 0.0);
+            if (first) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    F[c] = F[c] + o.F[c];
+                    T[c] = T[c] + o.ta[c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    F[c] = F[c] + (-o.F[c]);
+                    T[c] = T[c] + o.tb[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) V[c] = V[c] + o.vir[c];
+            ++nt;
         }
         if (CUNDALL) {
             F[0] *= cundall_scale(F[0] * v.x, gl);

@@ -93,10 +93,10 @@ __device__ __forceinline__ void gather_split(int i, uint32_t ci, const float4& u
         if (rsq < radsum * radsum && rsq > 1e-28f) {
             const float4 vj = P.vm[j];
             const float4 oj = P.om[j];
-            const float4& va4 = first ? vi : vj;
-            const float4& vb4 = first ? vj : vi;
-            const float4& oa4 = first ? oi : oj;
-            const float4& ob4 = first ? oj : oi;
+            const float4 va4 = first ? vi : vj;
+            const float4 vb4 = first ? vj : vi;
+            const float4 oa4 = first ? oi : oj;
+            const float4 ob4 = first ? oj : oi;
             const float va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
             const float oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
             bool before = false;
@@ -418,6 +418,10 @@ __global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, 
 
 // ------------------------------------------------ one kernel per step
 
+#ifndef SPLIT_BATCH
+#define SPLIT_BATCH 4
+#endif
+
 struct FusedS {
     const float4 *up_in, *vm_in, *om_in;
     float4 *up_out, *vm_out, *om_out;
@@ -472,7 +476,53 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
         const float zero3[3] = {0.f, 0.f, 0.f};
         const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
-        for (int s = s0; s < s1; ++s) {
+        // Pass 1, every slot: separation and the contact test only. Indices
+        // of a batch, then all their states, are requested before any is
+        // used, so a row costs two memory round trips, not two per slot.
+        // Touching slots are marked in `tmask` (the row's first 64; a longer
+        // row keeps the rest in the touch flags just written).
+        uint64_t tmask = 0;
+        for (int sb = s0; sb < s1; sb += SPLIT_BATCH) {
+            int jb[SPLIT_BATCH];
+#pragma unroll
+            for (int q = 0; q < SPLIT_BATCH; ++q) jb[q] = sb + q < s1 ? P.nb_j[sb + q] : i;
+            float4 vq[SPLIT_BATCH], uq[SPLIT_BATCH];
+            uint32_t cq[SPLIT_BATCH];
+#pragma unroll
+            for (int q = 0; q < SPLIT_BATCH; ++q) {
+                vq[q] = P.vm_in[jb[q]];
+                uq[q] = P.up_in[jb[q]];
+                cq[q] = P.cell[jb[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < SPLIT_BATCH; ++q) {
+                if (sb + q < s1) {
+                    const float4 uj = step_offset(dtc, uq[q], vq[q]);
+                    const bool first = i < jb[q];
+                    float d[3];
+                    if (first) split_delta(sc, per, ci, ui, cq[q], uj, d);
+                    else split_delta(sc, per, cq[q], uj, ci, ui, d);
+                    const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    const float radsum = ui.w + uj.w;
+                    const bool touch = rsq < radsum * radsum && rsq > 1e-28f;
+                    const int k = sb + q - s0;
+                    if (touch && k < 64) tmask |= 1ull << k;
+                    P.fl_out[sb + q] = touch ? 1 : 0;
+                }
+            }
+        }
+        // Pass 2, touching slots in ascending order: the contact law, with the
+        // neighbour's state fetched again (the same bits as in pass 1).
+        const int nlong = s1 - s0 > 64 ? s1 - s0 - 64 : 0;
+        for (int k = 0; tmask != 0 || k < nlong; ) {
+            int s;
+            if (tmask != 0) {
+                s = s0 + __builtin_ctzll(tmask);
+                tmask &= tmask - 1;
+            } else {
+                s = s0 + 64 + k++;
+                if (P.fl_out[s] == 0) continue;
+            }
             const int j = P.nb_j[s];
             const float4 vj = P.vm_in[j];
             const float4 uj = step_offset(dtc, P.up_in[j], vj);
@@ -484,60 +534,53 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
             const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
             const float radsum = ra + rb;
-            uint8_t touch = 0;
-            if (rsq < radsum * radsum && rsq > 1e-28f) {
-                const float4 oj = P.om_in[j];
-                const float4& va4 = first ? v : vj;
-                const float4& vb4 = first ? vj : v;
-                const float4& oa4 = first ? w : oj;
-                const float4& ob4 = first ? oj : w;
-                const float va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
-                const float oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
-                bool before = false;
-                float h[3] = {0.f, 0.f, 0.f};
-                if (uses_history) {
-                    before = P.fl_in[s] != 0;
-                    if (before) {
-                        const float4 hh = P.h_in[s];
-                        h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
-                    }
+            const float4 oj = P.om_in[j];
+            const float4 va4 = first ? v : vj;
+            const float4 vb4 = first ? vj : v;
+            const float4 oa4 = first ? w : oj;
+            const float4 ob4 = first ? oj : w;
+            const float va[3] = {va4.x, va4.y, va4.z}, vb[3] = {vb4.x, vb4.y, vb4.z};
+            const float oa[3] = {oa4.x, oa4.y, oa4.z}, ob[3] = {ob4.x, ob4.y, ob4.z};
+            bool before = false;
+            float h[3] = {0.f, 0.f, 0.f};
+            if (uses_history) {
+                before = P.fl_in[s] != 0;
+                if (before) {
+                    const float4 hh = P.h_in[s];
+                    h[0] = hh.x; h[1] = hh.y; h[2] = hh.z;
                 }
-                ContactOut<float> o;
-                if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
-                    legacy_history_contact<float, false>(
-                        d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
-                        (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
-                        pp.limit_damping, dt, 1, before, h, o);
-                else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
-                    legacy_history_contact<float, true>(
-                        d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
-                        (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
-                        pp.limit_damping, dt, 1, before, h, o);
-                else
-                    granular_contact<float>(d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa,
-                                            ob, pp, dt, 1, before, h, o);
-                if (uses_history) {
-                    touch = 1;
-                    P.h_out[s] = make_float4(h[0], h[1], h[2], 0.f);
-                }
-                if (first) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        F[k] += o.F[k];
-                        T[k] += o.ta[k];
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        F[k] -= o.F[k];
-                        T[k] += o.tb[k];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 6; ++k) V[k] += (double)o.vir[k];
-                ++nt;
             }
-            if (uses_history) P.fl_out[s] = touch;
+            ContactOut<float> o;
+            if (STYLE == TMPS_PAIR_HOOKE_HISTORY)
+                legacy_history_contact<float, false>(
+                    d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                    (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                    pp.limit_damping, dt, 1, before, h, o);
+            else if (STYLE == TMPS_PAIR_HERTZ_HISTORY)
+                legacy_history_contact<float, true>(
+                    d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa, ob, (float)pp.kn,
+                    (float)pp.kt, (float)pp.gamman, (float)pp.gammat, (float)pp.xmu,
+                    pp.limit_damping, dt, 1, before, h, o);
+            else
+                granular_contact<float>(d, zero3, rsq, radsum, ra, rb, va4.w, vb4.w, va, vb, oa,
+                                        ob, pp, dt, 1, before, h, o);
+            if (uses_history) P.h_out[s] = make_float4(h[0], h[1], h[2], 0.f);
+            if (first) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    F[c] += o.F[c];
+                    T[c] += o.ta[c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    F[c] -= o.F[c];
+                    T[c] += o.tb[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) V[c] += (double)o.vir[c];
+            ++nt;
         }
         if (CUNDALL) {
             F[0] *= cundall_scale_f(F[0] * v.x, gl); T[0] *= cundall_scale_f(T[0] * w.x, ga);

@@ -179,6 +179,8 @@ struct System {
     int* hstatus = nullptr;  // pinned: [slot][timestep, trigger, abort, error]
     hipEvent_t seg_event[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
 
+    int launch_mode = TMPS_LAUNCH_GRAPH;
+
     // profiling
     bool profiling = false;
     double prof_ms[3] = {0, 0, 0};
