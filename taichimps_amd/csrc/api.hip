@@ -17,7 +17,10 @@
 #include "tmps_internal.h"
 
 // steps per captured graph segment of a device-resident run
-#define TMPS_SEG_LEN 32
+// Steps per captured segment. A rebuild request wastes on average half a
+// segment plus the one queued behind it (their kernels exit at once), while
+// each segment costs one status read; 16 balances the two at ~70 us/step.
+#define TMPS_SEG_LEN 16
 
 namespace {
 thread_local std::string g_err;
@@ -487,56 +490,79 @@ int segment_tail(System& s, int buf) {
     return launch_remap_clear(s, 1);
 }
 
-// Enqueue a segment of `len` steps (no synchronisation): the first step's
-// half-kick, one kernel per step alternating buffer sets, the tail.
-int enqueue_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
+// Enqueue a segment of `len` steps (no synchronisation). `kick`: the segment
+// opens a run (or follows a rebuild) and applies the first step's half-kick
+// from the stored forces. `last`: it closes the run, so its last step stores
+// f/torque/virial instead of the next half-kick and the tail brings the
+// results to buffer set 0. Inner segments have even length and need neither.
+int enqueue_steps(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
+                  double ga) {
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    int rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
+    int rc = 0;
+    if (kick) rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
     for (int k = 0; k < len && !rc; ++k)
-        rc = fused_step(s, k & 1, k == len - 1, dt, use_cundall, gl, ga, servo_on);
-    if (rc) return rc;
+        rc = fused_step(s, k & 1, last && k == len - 1, dt, use_cundall, gl, ga, servo_on);
+    if (rc || !last) return rc;
     return segment_tail(s, len & 1);
 }
 
-// Same as enqueue_steps, timing each kernel with HIP events on the stream:
-// class 0 the kick, class 1 the step kernels, class 2 the tail.
-int profiled_steps(System& s, int len, double dt, int use_cundall, double gl, double ga) {
+// Same as enqueue_steps, timed: class 0 the kick and class 2 the tail with
+// HIP events between kernels, class 1 the step kernels from the first step
+// kernel's own start to the last one's own stop (hipExtLaunchKernel events),
+// averaged over the segment's steps.
+int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
+                   double ga) {
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    std::vector<hipEvent_t> ev(len + 3);
+    hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[0], s.stream));
-    int rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
+    int rc = 0;
+    if (kick) rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
     if (!rc) rc = hipEventRecord(ev[1], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     for (int k = 0; k < len && !rc; ++k) {
-        rc = fused_step(s, k & 1, k == len - 1, dt, use_cundall, gl, ga, servo_on);
-        if (!rc) rc = hipEventRecord(ev[2 + k], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+        if (k == 0) s.kstart = ev[4];
+        if (k == len - 1) s.kstop = ev[5];
+        rc = fused_step(s, k & 1, last && k == len - 1, dt, use_cundall, gl, ga, servo_on);
     }
-    if (!rc) rc = segment_tail(s, len & 1);
-    if (!rc) rc = hipEventRecord(ev[len + 2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+    s.kstart = s.kstop = nullptr;
+    if (!rc) rc = hipEventRecord(ev[2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
+    if (!rc && last) rc = segment_tail(s, len & 1);
+    if (!rc) rc = hipEventRecord(ev[3], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     HIP_TRY(hipStreamSynchronize(s.stream));
     if (!rc) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        s.prof_ms[0] += ms;
-        s.prof_cnt[0] += 1;
-        for (int k = 0; k < len; ++k) {
-            HIP_TRY(hipEventElapsedTime(&ms, ev[1 + k], ev[2 + k]));
-            s.prof_ms[1] += ms;
-            s.prof_cnt[1] += 1;
+        if (kick) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            s.prof_ms[0] += ms;
+            s.prof_cnt[0] += 1;
         }
-        HIP_TRY(hipEventElapsedTime(&ms, ev[len + 1], ev[len + 2]));
-        s.prof_ms[2] += ms;
-        s.prof_cnt[2] += 1;
+        // a segment cut short by a rebuild request is not a measurement of
+        // the step kernel (its later kernels exit at once)
+        int aborted = 0;
+        HIP_TRY(hipMemcpy(&aborted, &s.ctl->abort, sizeof(int), hipMemcpyDeviceToHost));
+        if (len > 0 && !aborted) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[4], ev[5]));
+            s.prof_ms[1] += ms;
+            s.prof_cnt[1] += len;
+        }
+        if (last) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3]));
+            s.prof_ms[2] += ms;
+            s.prof_cnt[2] += 1;
+        }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 
-int launch_segment(System& s, int len, double dt, int use_cundall, double gl, double ga) {
-    if (s.profiling) return profiled_steps(s, len, dt, use_cundall, gl, ga);
-    if (s.launch_mode == TMPS_LAUNCH_DIRECT) return enqueue_steps(s, len, dt, use_cundall, gl, ga);
+int launch_segment(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
+                   double ga) {
+    if (s.profiling) return profiled_steps(s, len, kick, last, dt, use_cundall, gl, ga);
+    if (s.launch_mode == TMPS_LAUNCH_DIRECT)
+        return enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga);
     uint64_t key = 1469598103934665603ULL;
     key = mix(key, (uint64_t)len);
+    key = mix(key, (uint64_t)(kick ? 1 : 0) | (uint64_t)(last ? 2 : 0));
     key = mix(key, dbits(dt));
     key = mix(key, (uint64_t)use_cundall);
     key = mix(key, dbits(gl));
@@ -559,7 +585,7 @@ int launch_segment(System& s, int len, double dt, int use_cundall, double gl, do
             s.seg_exec[slot] = nullptr;
         }
         HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(s, len, dt, use_cundall, gl, ga);
+        int rc = enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s.stream, &g);
         if (rc) {
@@ -637,10 +663,14 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     Seg inflight[2];
     int nin = 0, next_slot = 0;
     int t = s.timestep;
+    int kick = 1;
     while (true) {
         while (nin < 2 && t < t_end) {
-            const int len = std::min(TMPS_SEG_LEN, t_end - t);
-            rc = launch_segment(s, len, dt, use_cundall, gl, ga);
+            int len = std::min(TMPS_SEG_LEN, t_end - t);
+            const int last = t + len == t_end;
+            if (!last && (len & 1)) --len;
+            rc = launch_segment(s, len, kick, last, dt, use_cundall, gl, ga);
+            kick = 0;
             if (rc) return rc;
             const int slot = next_slot;
             next_slot = (next_slot + 1) % System::kGraphSlots;
@@ -716,6 +746,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         s.timestep = s.hctl->timestep;
         s.current_step = s.timestep;
         t = s.timestep;
+        kick = 1;  // the redone step ended with final_integrate
     }
     rc = ctl_pull(s);
     if (rc) return rc;

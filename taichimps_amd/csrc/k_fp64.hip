@@ -5,6 +5,8 @@
 // result is bitwise equal to the CPU oracle's for the same inputs.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "contact.h"
 #include "fused.h"
 #include "servo.h"
@@ -1012,8 +1014,15 @@ static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, doubl
                         int servo_on) {
     RedP R = s.red;
     R.count0 = nblocks_for(s.n);
-    hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK), 0,
-                       s.stream, P, s.ctl, s.pair, R, dt, gl, ga, servo_on);
+    if (s.kstart || s.kstop) {
+        hipExtLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
+                              0, s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, R, dt, gl, ga,
+                              servo_on);
+        s.kstart = s.kstop = nullptr;
+    } else {
+        hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
+                           0, s.stream, P, s.ctl, s.pair, R, dt, gl, ga, servo_on);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }

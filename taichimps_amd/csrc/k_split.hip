@@ -7,6 +7,8 @@
 // position, so under it the encoding needs no per-atom update at all: the new
 // box alone moves the atoms. Contact and integration arithmetic is FP32; the
 // per-atom virial, the kinetic tensor, their reduction and the servo stay FP64.
+#include <hip/hip_ext.h>
+
 #include "contact.h"
 #include "fused.h"
 #include "tmps_internal.h"
@@ -781,9 +783,16 @@ static int fused_split_go(System& s, const FusedS& P, double dt, double gl, doub
                           int servo_on) {
     RedP R = s.red;
     R.count0 = nblocks_for(s.n);
-    hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK), 0,
-                       s.stream, P, s.ctl, s.pair, R, (float)dt, (float)gl, (float)ga, dt,
-                       servo_on);
+    if (s.kstart || s.kstop) {
+        hipExtLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0),
+                              dim3(TMPS_BLOCK), 0, s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair,
+                              R, (float)dt, (float)gl, (float)ga, dt, servo_on);
+        s.kstart = s.kstop = nullptr;
+    } else {
+        hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
+                           0, s.stream, P, s.ctl, s.pair, R, (float)dt, (float)gl, (float)ga, dt,
+                           servo_on);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -183,6 +183,8 @@ struct System {
 
     // profiling
     bool profiling = false;
+    // one-shot: the next step kernel records its own start / stop time
+    hipEvent_t kstart = nullptr, kstop = nullptr;
     double prof_ms[3] = {0, 0, 0};
     int prof_cnt[3] = {0, 0, 0};
 };
