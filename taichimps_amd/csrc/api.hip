@@ -942,7 +942,7 @@ void free_all(System& s) {
     dfree(s.tag);
     dfree(s.up); dfree(s.vmf); dfree(s.omf); dfree(s.ff); dfree(s.tqf); dfree(s.u0); dfree(s.cell);
     dfree(s.nb_start); dfree(s.nb_j); dfree(s.nb_flag);
-    dfree(s.onb_start); dfree(s.onb_j); dfree(s.onb_flag); dfree(s.nb_flag2);
+    dfree(s.onb_start); dfree(s.onb_j); dfree(s.onb_flag); dfree(s.nb_flag2); dfree(s.nb_code);
     if (s.nb_hist2) (void)hipFree(s.nb_hist2);
     s.nb_hist2 = nullptr;
     if (s.nb_hist) (void)hipFree(s.nb_hist);

@@ -35,6 +35,13 @@ __device__ __forceinline__ T tsqrt(T v) { return sqrt(v); }
 template <>
 __device__ __forceinline__ float tsqrt<float>(float v) { return sqrtf(v); }
 
+// a / b: correctly rounded in FP64 (the parity path); in FP32 the hardware
+// reciprocal (about 1 ulp), which the FP32-split error budget absorbs.
+template <typename T>
+__device__ __forceinline__ T tdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float tdiv<float>(float a, float b) { return __fdividef(a, b); }
+
 template <typename T>
 struct ContactOut {
     T F[3];    // force on a (b receives -F)
@@ -56,12 +63,12 @@ __device__ __forceinline__ void legacy_history_contact(
     const T delta = radsum - r;
     T polyhertz = T(1);
     if (HERTZ) {
-        const T reff = (ri * rj) / (ri + rj);
+        const T reff = tdiv(ri * rj, ri + rj);
         polyhertz = tsqrt(reff * delta);
     }
-    const T meff = (mi * mj) / (mi + mj);
-    const T rsqinv = T(1) / rsq;
-    const T rinv = T(1) / r;
+    const T meff = tdiv(mi * mj, mi + mj);
+    const T rsqinv = tdiv(T(1), rsq);
+    const T rinv = tdiv(T(1), r);
     T vrel[3];
     for (int k = 0; k < 3; ++k) vrel[k] = vi[k] - (vj[k] + dvj[k]);
     const T vnnr = dot3(vrel, dpos);
@@ -103,9 +110,9 @@ __device__ __forceinline__ void legacy_history_contact(
     const T fn_coulomb = xmu * fabs(ccel * r);
     if (fs_mag > fn_coulomb) {
         if (shrmag != T(0)) {
-            const T ratio = fn_coulomb / fs_mag;
+            const T ratio = tdiv(fn_coulomb, fs_mag);
             for (int k = 0; k < 3; ++k) {
-                const T damp_corr = gammat_eff * vtr[k] / kt;
+                const T damp_corr = tdiv(gammat_eff * vtr[k], kt);
                 shear[k] = ratio * (shear[k] + damp_corr) - damp_corr;
                 fs[k] = ratio * fs[k];
             }
@@ -139,7 +146,7 @@ __device__ __forceinline__ void rotate_rescale(T h[3], const T n[3]) {
     for (int k = 0; k < 3; ++k) out[k] = h[k] - hn * n[k];
     const T prjmag = tsqrt(dot3(out, out));
     T scale = T(0);
-    if (prjmag > T(0)) scale = shrmag / prjmag;
+    if (prjmag > T(0)) scale = tdiv(shrmag, prjmag);
     for (int k = 0; k < 3; ++k) h[k] = out[k] * scale;
 }
 
@@ -150,11 +157,11 @@ __device__ __forceinline__ void granular_contact(
     const T vj[3], const T oi[3], const T oj[3], const PairP& P, T dt, int history_update,
     bool touching_before, T hist[3], ContactOut<T>& o) {
     const T r = tsqrt(rsq);
-    const T rinv = T(1) / r;
+    const T rinv = tdiv(T(1), r);
     T nrm[3] = {dpos[0] * rinv, dpos[1] * rinv, dpos[2] * rinv};
     const T delta = radsum - r;
-    const T reff = (ri * rj) / radsum;
-    const T meff = (mi * mj) / (mi + mj);
+    const T reff = tdiv(ri * rj, radsum);
+    const T meff = tdiv(mi * mj, mi + mj);
     const T a = tsqrt(delta * reff);
     T vr[3];
     for (int k = 0; k < 3; ++k) vr[k] = vi[k] - (vj[k] + dvj[k]);
@@ -181,7 +188,7 @@ __device__ __forceinline__ void granular_contact(
         case 4:
         case 5: {
             T arg = T(0);
-            if (delta > T(0)) arg = fmax(T(0), meff * fne / delta);
+            if (delta > T(0)) arg = fmax(T(0), tdiv(meff * fne, delta));
             damp_prefactor = gdamp * tsqrt(arg);
         } break;
         default: break;
@@ -194,7 +201,7 @@ __device__ __forceinline__ void granular_contact(
     if (P.tangential_model == 1) {
         const T damp_t = T(P.g_xt) * damp_prefactor;
         T ft = T(0);
-        if (vrel != T(0)) ft = fmin(fscrit, damp_t * vrel) / vrel;
+        if (vrel != T(0)) ft = tdiv(fmin(fscrit, damp_t * vrel), vrel);
         for (int k = 0; k < 3; ++k) fs[k] = -ft * vtr[k];
     } else if (P.uses_history) {
         T h[3];
@@ -219,8 +226,8 @@ __device__ __forceinline__ void granular_contact(
         const T magfs = tsqrt(dot3(fs, fs));
         if (magfs > fscrit) {
             if (tsqrt(dot3(h, h)) != T(0)) {
-                for (int k = 0; k < 3; ++k) fs[k] = fs[k] * (fscrit / magfs);
-                for (int k = 0; k < 3; ++k) h[k] = -(fs[k] - fdamp[k]) / k_eff;
+                for (int k = 0; k < 3; ++k) fs[k] = fs[k] * tdiv(fscrit, magfs);
+                for (int k = 0; k < 3; ++k) h[k] = tdiv(-(fs[k] - fdamp[k]), k_eff);
             } else {
                 fs[0] = fs[1] = fs[2] = T(0);
             }
@@ -248,10 +255,35 @@ __device__ __forceinline__ void granular_contact(
 // Pairwise sum of one leaf per lane over a 64-lane wavefront: lanes combine in
 // adjacent pairs, then pairs of pairs, ... (xor offsets 1, 2, 4, ..., 32), the
 // same binary tree a serial pairwise sum over the 64 leaves forms.
+// One DPP move of a double across lanes (CTRL: a gfx9 dpp_ctrl code).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Pairwise tree over the 64 lanes of a wavefront in lane order (all lanes
+// active): adjacent pairs, then pairs of pairs, ... -- the same additions as a
+// pairwise tree over those 64 leaves. Lane 8k+4's value reaches lane 8k by
+// row_ror:12, lane 16k+8's lane 16k by row_ror:8; the four 16-lane sums are
+// combined as uniform values. Every lane receives the sum.
 __device__ __forceinline__ double wave_tree_sum(double v) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) v = v + __shfl_xor(v, off, 64);
-    return v;
+    v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]: lane i^1
+    v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]: lane i^2
+    v = v + dpp_f64<0x12C>(v);  // row_ror:12: lane i+4 within the row
+    v = v + dpp_f64<0x128>(v);  // row_ror:8: lane i+8 within the row
+    const double a = readlane_f64(v, 0), b = readlane_f64(v, 16);
+    const double c = readlane_f64(v, 32), d = readlane_f64(v, 48);
+    return (a + b) + (c + d);
 }
 
 // Block of TMPS_BLOCK (= 4 wavefronts) leaves -> one pairwise-tree sum per

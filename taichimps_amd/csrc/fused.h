@@ -37,12 +37,6 @@ __device__ __forceinline__ double ld_agent(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // Ends step t = ctl->timestep + 1 of a fused pass. Every thread of every
 // block calls it once, after its own atom's work. `leaf` holds the atom's
 // 7 tree leaves (6 virial+kinetic components, touching-slot count), `trig`
@@ -88,14 +82,14 @@ __device__ inline void finish_step(Ctl* ctl, const RedP& R, const double leaf[7]
             prev = __hip_atomic_fetch_add(R.cnt[level] + g, 1u, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
         }
-        prev = __shfl(prev, 0, 64);
+        prev = (unsigned)__builtin_amdgcn_readlane((int)prev, 0);
         if (prev != (unsigned)(gsize - 1)) return;
         if (lane == 0)
             __hip_atomic_store(R.cnt[level] + g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const size_t m = (size_t)(g << 6) + lane;
 #pragma unroll
         for (int k = 0; k < 7; ++k) e[k] = wave_tree_sum(lane < gsize ? ld_agent(arr + m * 8 + k) : 0.0);
-        e[7] = wave_max(lane < gsize ? ld_agent(arr + m * 8 + 7) : 0.0);
+        e[7] = __any(lane < gsize && ld_agent(arr + m * 8 + 7) != 0.0) ? 1.0 : 0.0;
         idx = g;
         count = (count + 63) >> 6;
     }

@@ -646,14 +646,13 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
             for (int q = 0; q < F64_BATCH; ++q) {
                 if (sb + q < s1) {
                     const double4 xj = step_position(RM, remap, dt, xq[q], vq[q]);
-                    const bool first = i < jb[q];
-                    const double4 xa = first ? xi : xj;
-                    const double4 xb = first ? xj : xi;
-                    double d[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
+                    // i - j: the exact negation of the canonical (lower index
+                    // first) separation, so rsq and the test are the same bits
+                    double d[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
                     double dvj[3];
                     min_image_vshift(B, d, dvj);
                     const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                    const double radsum = xa.w + xb.w;
+                    const double radsum = xi.w + xj.w;
                     const bool touch = rsq < radsum * radsum && rsq > 1e-28;
                     const int k = sb + q - s0;
                     if (touch && k < 64) tmask |= 1ull << k;

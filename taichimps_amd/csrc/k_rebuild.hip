@@ -452,11 +452,13 @@ static int grow_list(System& s, size_t need) {
     if (s.nb_hist) HIP_TRY(hipFree(s.nb_hist));
     if (s.nb_flag2) HIP_TRY(hipFree(s.nb_flag2));
     if (s.nb_hist2) HIP_TRY(hipFree(s.nb_hist2));
+    if (s.nb_code) HIP_TRY(hipFree(s.nb_code));
     HIP_TRY(hipMalloc(&s.nb_j, cap * sizeof(int)));
     HIP_TRY(hipMalloc(&s.nb_flag, cap));
     HIP_TRY(hipMalloc(&s.nb_hist, cap * hsz));
     HIP_TRY(hipMalloc(&s.nb_flag2, cap));
     HIP_TRY(hipMalloc(&s.nb_hist2, cap * hsz));
+    HIP_TRY(hipMalloc(&s.nb_code, cap * sizeof(uint16_t)));
     s.nb_cap = cap;
     drop_graphs(s);
     return 0;

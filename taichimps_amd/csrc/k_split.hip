@@ -55,6 +55,13 @@ __device__ __forceinline__ void unpack_cell(uint32_t c, int& cx, int& cy, int& c
     cz = (int)(c >> 22);
 }
 
+// A cell difference reduced to the periodic image nearest zero.
+__device__ __forceinline__ int wrap_cells(int dc, int g) {
+    if (2 * dc > g) dc -= g;
+    else if (2 * dc < -g) dc += g;
+    return dc;
+}
+
 // Separation of a minus b in length units, minimum image taken in bin units.
 __device__ __forceinline__ void split_delta(const SplitConst& sc, const int per[3], uint32_t ca,
                                             const float4& ua, uint32_t cb, const float4& ub,
@@ -62,8 +69,14 @@ __device__ __forceinline__ void split_delta(const SplitConst& sc, const int per[
     int ax, ay, az, bx, by, bz;
     unpack_cell(ca, ax, ay, az);
     unpack_cell(cb, bx, by, bz);
-    float e[3] = {(float)(ax - bx) + (ua.x - ub.x), (float)(ay - by) + (ua.y - ub.y),
-                  (float)(az - bz) + (ua.z - ub.z)};
+    // the cell difference is wrapped as an integer first, so the FP32 offset
+    // difference is added to a small number and keeps its precision
+    int dc[3] = {ax - bx, ay - by, az - bz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (per[k] == 1) dc[k] = wrap_cells(dc[k], sc.gi[k]);
+    float e[3] = {(float)dc[0] + (ua.x - ub.x), (float)dc[1] + (ua.y - ub.y),
+                  (float)dc[2] + (ua.z - ub.z)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (per[k] == 1) e[k] -= sc.g[k] * rintf(e[k] * (1.0f / sc.g[k]));
@@ -200,6 +213,34 @@ __global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0
     up.x = u[0]; up.y = u[1]; up.z = u[2];
     P.up[i] = up;
     P.u0[i] = make_float4(u0[0], u0[1], u0[2], 0.f);
+}
+
+// Per slot of the neighbor list: the cell of i minus the cell of j, wrapped
+// to the nearest periodic image, 3 bits per axis (offset by 3). Between two
+// builds the split cells do not change, so the one-kernel step reads this
+// instead of gathering and unpacking the neighbour's cell. List partners sit
+// within one cell of each other; a re-encode without a rebuild moves an atom
+// by at most one more, so |difference| <= 3 always fits.
+__global__ void k_slot_codes(int n, const int* nb_start, const int* nb_j, const uint32_t* cell,
+                             const Ctl* ctl, uint16_t* code) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ci[3];
+    unpack_cell(cell[i], ci[0], ci[1], ci[2]);
+    const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
+    const int g[3] = {ctl->sc.gi[0], ctl->sc.gi[1], ctl->sc.gi[2]};
+    for (int s = nb_start[i]; s < nb_start[i + 1]; ++s) {
+        int cj[3];
+        unpack_cell(cell[nb_j[s]], cj[0], cj[1], cj[2]);
+        unsigned c = 0;
+        for (int k = 0; k < 3; ++k) {
+            int dc = ci[k] - cj[k];
+            if (per[k] == 1) dc = wrap_cells(dc, g[k]);
+            dc = max(-3, min(3, dc));
+            c |= (unsigned)(dc + 3) << (3 * k);
+        }
+        code[s] = (uint16_t)c;
+    }
 }
 
 // ------------------------------------------------------------ stage kernels
@@ -424,6 +465,21 @@ __global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, 
 #define SPLIT_BATCH 4
 #endif
 
+// Separation i - j in length units from the slot's cell offset (i's cell
+// minus j's, nearest image) and the two drifted offsets.
+__device__ __forceinline__ void slot_separation(const SplitConst& sc, const bool fix[3],
+                                                const float ginv[3], unsigned code,
+                                                const float4& ui, const float4& uj, float d[3]) {
+    float e[3] = {(float)((int)(code & 7u) - 3) + (ui.x - uj.x),
+                  (float)((int)((code >> 3) & 7u) - 3) + (ui.y - uj.y),
+                  (float)((int)((code >> 6) & 7u) - 3) + (ui.z - uj.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (fix[k]) e[k] -= sc.g[k] * rintf(e[k] * ginv[k]);
+        d[k] = e[k] * sc.cw[k];
+    }
+}
+
 struct FusedS {
     const float4 *up_in, *vm_in, *om_in;
     float4 *up_out, *vm_out, *om_out;
@@ -436,6 +492,7 @@ struct FusedS {
     const uint32_t* cell;
     double* vir;
     const int *nb_start, *nb_j;
+    const uint16_t* nb_code;
     int n;
 };
 
@@ -478,32 +535,42 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
         const float zero3[3] = {0.f, 0.f, 0.f};
         const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
-        // Pass 1, every slot: separation and the contact test only. Indices
-        // of a batch, then all their states, are requested before any is
-        // used, so a row costs two memory round trips, not two per slot.
-        // Touching slots are marked in `tmask` (the row's first 64; a longer
-        // row keeps the rest in the touch flags just written).
+        // Per axis: offsets of list partners can be wider than half the box
+        // only where the box holds fewer than 9 bins.
+        bool fix[3];
+        float ginv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            fix[k] = per[k] == 1 && sc.gi[k] < 9;
+            ginv[k] = fix[k] ? 1.0f / sc.g[k] : 0.0f;
+        }
+        // Pass 1, every slot: separation (i minus j) and the contact test.
+        // Indices and cell offsets of a batch, then all the partners' states,
+        // are requested before any is used, so a row costs two memory round
+        // trips, not two per slot. Touching slots are marked in `tmask` (the
+        // row's first 64; a longer row keeps the rest in the touch flags).
         uint64_t tmask = 0;
         for (int sb = s0; sb < s1; sb += SPLIT_BATCH) {
             int jb[SPLIT_BATCH];
+            uint16_t cb[SPLIT_BATCH];
 #pragma unroll
-            for (int q = 0; q < SPLIT_BATCH; ++q) jb[q] = sb + q < s1 ? P.nb_j[sb + q] : i;
+            for (int q = 0; q < SPLIT_BATCH; ++q) {
+                const bool in = sb + q < s1;
+                jb[q] = in ? P.nb_j[sb + q] : i;
+                cb[q] = in ? P.nb_code[sb + q] : (uint16_t)0xDB;
+            }
             float4 vq[SPLIT_BATCH], uq[SPLIT_BATCH];
-            uint32_t cq[SPLIT_BATCH];
 #pragma unroll
             for (int q = 0; q < SPLIT_BATCH; ++q) {
                 vq[q] = P.vm_in[jb[q]];
                 uq[q] = P.up_in[jb[q]];
-                cq[q] = P.cell[jb[q]];
             }
 #pragma unroll
             for (int q = 0; q < SPLIT_BATCH; ++q) {
                 if (sb + q < s1) {
                     const float4 uj = step_offset(dtc, uq[q], vq[q]);
-                    const bool first = i < jb[q];
                     float d[3];
-                    if (first) split_delta(sc, per, ci, ui, cq[q], uj, d);
-                    else split_delta(sc, per, cq[q], uj, ci, ui, d);
+                    slot_separation(sc, fix, ginv, cb[q], ui, uj, d);
                     const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                     const float radsum = ui.w + uj.w;
                     const bool touch = rsq < radsum * radsum && rsq > 1e-28f;
@@ -513,8 +580,10 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
                 }
             }
         }
-        // Pass 2, touching slots in ascending order: the contact law, with the
-        // neighbour's state fetched again (the same bits as in pass 1).
+        // Pass 2, touching slots in ascending order: the contact law for the
+        // pair in canonical orientation (lower index first; the separation of
+        // the other orientation is the exact negation), the partner's state
+        // fetched again.
         const int nlong = s1 - s0 > 64 ? s1 - s0 - 64 : 0;
         for (int k = 0; tmask != 0 || k < nlong; ) {
             int s;
@@ -528,11 +597,14 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
             const int j = P.nb_j[s];
             const float4 vj = P.vm_in[j];
             const float4 uj = step_offset(dtc, P.up_in[j], vj);
-            const uint32_t cj = P.cell[j];
             const bool first = i < j;
             float d[3];
-            if (first) split_delta(sc, per, ci, ui, cj, uj, d);
-            else split_delta(sc, per, cj, uj, ci, ui, d);
+            slot_separation(sc, fix, ginv, P.nb_code[s], ui, uj, d);
+            if (!first) {
+                d[0] = -d[0];
+                d[1] = -d[1];
+                d[2] = -d[2];
+            }
             const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
             const float radsum = ra + rb;
@@ -651,6 +723,9 @@ int launch_split_to_x64(System& s) {
 
 int launch_x64_to_split(System& s) {
     LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, s.cell);
+    // the cells may have changed: refresh the list's per-slot cell offsets
+    if (s.nslots > 0 && s.nb_code)
+        LAUNCH_S(k_slot_codes, s.n, s.n, s.nb_start, s.nb_j, s.cell, s.ctl, s.nb_code);
     return 0;
 }
 
@@ -830,6 +905,7 @@ int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, d
     P.vir = s.vir;
     P.nb_start = s.nb_start;
     P.nb_j = s.nb_j;
+    P.nb_code = s.nb_code;
     P.n = s.n;
     switch (s.pair.style) {
         case TMPS_PAIR_HOOKE_HISTORY:

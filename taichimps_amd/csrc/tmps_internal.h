@@ -127,6 +127,8 @@ struct System {
     // and writes the other, so a rebuild request leaves the first intact
     uint8_t* nb_flag2 = nullptr;
     void* nb_hist2 = nullptr;
+    // FP32-split: per slot, i's cell minus j's (nearest image), 3 bits per axis
+    uint16_t* nb_code = nullptr;
     size_t nb_cap = 0, onb_cap = 0;
     int nslots = 0, onslots = 0, onb_n = 0;
     bool have_old = false;
