@@ -107,6 +107,23 @@ def run_precision(w, precision, steps, warmup, prof_steps, direct=False):
     }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_split_isotropic_hertz_1m_hbm_traffic.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the FP32-split step kernel as measured by the
+    FETCH_SIZE / WRITE_SIZE counter passes recorded in the profile summary
+    (each counter scaled by its plain-copy calibration factor), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_calibration(mib):
     """A plain device copy of known size: the profiler's FETCH_SIZE / WRITE_SIZE
     for it, divided by the bytes it must move, is the counters' scale factor."""
@@ -224,7 +241,9 @@ def main(argv=None):
         / abs(f64["mean_pressure"]),
         "roofline": {"bound": "hbm", "kernel": "k_step_split (one kernel per step)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-calibrated; profiles/)",
                      "bytes_per_launch": step_bytes, "avg_ms": kernel_ms},
         "step_roofline": {"achieved": step_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_achieved / HBM_PEAK_GBS,
