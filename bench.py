@@ -91,7 +91,7 @@ def run_precision(w, precision, steps, warmup, prof_steps, direct=False):
     n_slot = c["nslots"] / n
     n_touch = c["ntouch"] / n
     # per-kernel HIP-event timings over the same number of steps, same stream
-    ms, cnt = np.zeros(3), np.zeros(3, dtype=int)
+    ms, cnt = np.zeros(4), np.zeros(4, dtype=int)
     if prof_steps > 0:
         N.set_profiling(True)
         sim.run(prof_steps)
@@ -102,7 +102,8 @@ def run_precision(w, precision, steps, warmup, prof_steps, direct=False):
         "atom_steps_per_s": n * steps / elapsed, "setup_s": t_setup, "rebuilds": rebuilds,
         "n_slot": n_slot, "n_touch": n_touch, "pressure": pressure.tolist(),
         "mean_pressure": float(pressure[:3].mean()),
-        "kernel_ms": {"kick": ms[0], "step": ms[1], "tail": ms[2]},
+        "kernel_ms": {"kick": float(ms[0]), "step": float(ms[1]), "tail": float(ms[2]),
+                      "close": float(ms[3])},
         "kernel_counts": cnt.tolist(), "x": x_final,
     }
 

@@ -198,11 +198,14 @@ int tmps_last_pressure(tmps_system* s, double out6[6]);
 #define TMPS_LAUNCH_DIRECT 1
 int tmps_set_launch_mode(tmps_system* s, int mode);
 /* Average duration (ms) of each kernel class of tmps_run while profiling is
- * enabled (direct launches, HIP events on the system's stream between
- * kernels): [0] a segment's first half-kick, [1] the one-kernel step,
- * [2] a segment's tail. */
+ * enabled (direct launches, HIP events on the system's stream):
+ * [0] a segment's first half-kick, [1] the step pass kernel (its own start to
+ * its own stop), [2] a segment's tail, [3] the step's close kernel (tree and
+ * servo, measured from the pass's stop, so it includes the launch gap). */
+#define TMPS_PROF_CLASSES 4
 int tmps_set_profiling(tmps_system* s, int enabled);
-int tmps_kernel_times(tmps_system* s, double out_ms[3], int counts[3]);
+int tmps_kernel_times(tmps_system* s, double out_ms[TMPS_PROF_CLASSES],
+                      int counts[TMPS_PROF_CLASSES]);
 
 #ifdef __cplusplus
 }

@@ -389,8 +389,10 @@ class NativeSystem:
         check(self._L.tmps_set_launch_mode(self._h, 1 if direct else 0))
 
     def kernel_times(self):
-        ms = np.zeros(3)
-        cnt = np.zeros(3, dtype=np.int32)
+        """Average ms and counts per kernel class: kick, step pass, segment
+        tail, step close."""
+        ms = np.zeros(4)
+        cnt = np.zeros(4, dtype=np.int32)
         check(self._L.tmps_kernel_times(self._h, _dp(ms), _ip(cnt)))
         return ms, cnt
 

@@ -469,10 +469,13 @@ uint64_t dbits(double d) {
 // One fused step on v/omega buffer `buf`: the force pass writes the other
 // buffer. Returns the buffer that holds v/omega afterwards.
 // One fused step of a segment: it reads buffer set `buf`, writes 1 - buf.
+// The step pass writes the block partials; k_step_close reduces them, runs the
+// servo and advances the step (or records a rebuild request).
 int fused_step(System& s, int buf, int fin, double dt, int use_cundall, double gl, double ga,
                int servo_on) {
-    return split(s) ? launch_fused_split(s, buf, dt, fin, use_cundall, gl, ga, servo_on)
-                    : launch_fused_f64(s, buf, dt, fin, use_cundall, gl, ga, servo_on);
+    const int rc = split(s) ? launch_fused_split(s, buf, dt, fin, use_cundall, gl, ga)
+                            : launch_fused_f64(s, buf, dt, fin, use_cundall, gl, ga);
+    return rc ? rc : launch_step_close(s, dt, servo_on);
 }
 
 // The end of a segment whose last step wrote buffer set `buf`: the results
@@ -506,25 +509,31 @@ int enqueue_steps(System& s, int len, int kick, int last, double dt, int use_cun
     return segment_tail(s, len & 1);
 }
 
-// Same as enqueue_steps, timed: class 0 the kick and class 2 the tail with
-// HIP events between kernels, class 1 the step kernels from the first step
-// kernel's own start to the last one's own stop (hipExtLaunchKernel events),
-// averaged over the segment's steps.
+// Same as enqueue_steps, timed with HIP events on the system's stream: class 0
+// the kick, class 1 each step pass kernel from its own start to its own stop
+// (hipExtLaunchKernel events), class 3 each step's close kernel, class 2 the
+// tail. A segment cut short by a rebuild request is not counted (its later
+// kernels exit at once).
 int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
                    double ga) {
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
-    hipEvent_t ev[6];
+    std::vector<hipEvent_t> ev(4 + 3 * (size_t)std::max(len, 0), nullptr);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t* se = ev.data() + 4;  // per step: pass start, pass stop, close stop
     HIP_TRY(hipEventRecord(ev[0], s.stream));
     int rc = 0;
     if (kick) rc = split(s) ? launch_kick_split(s, dt) : launch_kick_f64(s, dt);
     if (!rc) rc = hipEventRecord(ev[1], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     for (int k = 0; k < len && !rc; ++k) {
-        if (k == 0) s.kstart = ev[4];
-        if (k == len - 1) s.kstop = ev[5];
-        rc = fused_step(s, k & 1, last && k == len - 1, dt, use_cundall, gl, ga, servo_on);
+        const int fin = last && k == len - 1;
+        s.kstart = se[3 * k];
+        s.kstop = se[3 * k + 1];
+        rc = split(s) ? launch_fused_split(s, k & 1, dt, fin, use_cundall, gl, ga)
+                      : launch_fused_f64(s, k & 1, dt, fin, use_cundall, gl, ga);
+        s.kstart = s.kstop = nullptr;
+        if (!rc) rc = launch_step_close(s, dt, servo_on);
+        if (!rc) rc = hipEventRecord(se[3 * k + 2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     }
-    s.kstart = s.kstop = nullptr;
     if (!rc) rc = hipEventRecord(ev[2], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
     if (!rc && last) rc = segment_tail(s, len & 1);
     if (!rc) rc = hipEventRecord(ev[3], s.stream) == hipSuccess ? 0 : TMPS_ERR_HIP;
@@ -536,14 +545,17 @@ int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cu
             s.prof_ms[0] += ms;
             s.prof_cnt[0] += 1;
         }
-        // a segment cut short by a rebuild request is not a measurement of
-        // the step kernel (its later kernels exit at once)
         int aborted = 0;
         HIP_TRY(hipMemcpy(&aborted, &s.ctl->abort, sizeof(int), hipMemcpyDeviceToHost));
         if (len > 0 && !aborted) {
-            HIP_TRY(hipEventElapsedTime(&ms, ev[4], ev[5]));
-            s.prof_ms[1] += ms;
+            for (int k = 0; k < len; ++k) {
+                HIP_TRY(hipEventElapsedTime(&ms, se[3 * k], se[3 * k + 1]));
+                s.prof_ms[1] += ms;
+                HIP_TRY(hipEventElapsedTime(&ms, se[3 * k + 1], se[3 * k + 2]));
+                s.prof_ms[3] += ms;
+            }
             s.prof_cnt[1] += len;
+            s.prof_cnt[3] += len;
         }
         if (last) {
             HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3]));
@@ -950,11 +962,11 @@ void free_all(System& s) {
     s.nb_hist = s.onb_hist = nullptr;
     dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
     dfree(s.cell_particles);
+    dfree(s.bin_x);
     dfree(s.full_count); dfree(s.half_count); dfree(s.scan_tmp); dfree(s.flags);
     dfree(s.row_tmp);
     dfree(s.partial);
-    dfree(s.red_buf);
-    dfree(s.red_cnt);
+    dfree(s.level);
     dfree(s.ctl);
     if (s.hctl) (void)hipHostFree(s.hctl);
     s.hctl = nullptr;
@@ -966,44 +978,6 @@ void free_all(System& s) {
     }
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
-}
-
-// The fused step's reduction levels for up to max_atoms atoms: level 0 one
-// entry per block, level l+1 one per 64 entries of level l, until one.
-int alloc_reduction(System& s) {
-    size_t entries[TMPS_RED_LEVELS + 1] = {0};
-    size_t count = (size_t)nblocks_for(std::max(s.max_atoms, 1));
-    int levels = 0;
-    size_t nd = 0, nc = 0;
-    while (true) {
-        if (levels == TMPS_RED_LEVELS)
-            return tmps_fail(TMPS_ERR_CAPACITY, "max_atoms too large for the step reduction");
-        entries[levels] = count;
-        nd += count * 8;
-        if (count == 1) break;
-        const size_t groups = (count + 63) / 64;
-        nc += groups;
-        count = groups;
-        ++levels;
-    }
-    int rc = dalloc(&s.red_buf, nd);
-    if (!rc) rc = dalloc(&s.red_cnt, std::max<size_t>(nc, 1));
-    if (rc) return rc;
-    size_t od = 0, oc = 0;
-    for (int l = 0; l < TMPS_RED_LEVELS; ++l) {
-        s.red.lvl[l] = nullptr;
-        s.red.cnt[l] = nullptr;
-    }
-    for (int l = 0; l <= levels; ++l) {
-        s.red.lvl[l] = s.red_buf + od;
-        od += entries[l] * 8;
-        if (l < levels) {
-            s.red.cnt[l] = s.red_cnt + oc;
-            oc += (entries[l] + 63) / 64;
-        }
-    }
-    s.red.count0 = 0;
-    return 0;
 }
 
 int create(System& s, int max_atoms, int max_nb, int precision) {
@@ -1042,13 +1016,15 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     rc = rc ? rc : dalloc(&s.onb_start, na + 1);
     rc = rc ? rc : dalloc(&s.cell_of, na);
     rc = rc ? rc : dalloc(&s.cell_particles, na);
+    rc = rc ? rc : dalloc(&s.bin_x, na);
     rc = rc ? rc : dalloc(&s.full_count, na + 1);
     rc = rc ? rc : dalloc(&s.half_count, na + 1);
     rc = rc ? rc : dalloc(&s.flags, 4);
     const int nparts = nblocks_for(std::max(max_atoms, 1)) + 1;
     rc = rc ? rc : dalloc(&s.partial, (size_t)nparts * 8);
     s.partial_cap = nparts;
-    rc = rc ? rc : alloc_reduction(s);
+    s.level_cap = nparts / 4096 + 1;
+    rc = rc ? rc : dalloc(&s.level, (size_t)s.level_cap * 8);
     rc = rc ? rc : dalloc(&s.ctl, 1);
     if (rc) return rc;
     HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));
@@ -1502,7 +1478,7 @@ int tmps_last_pressure(tmps_system* h, double out6[6]) {
 int tmps_set_profiling(tmps_system* h, int enabled) {
     System& s = *sys(h);
     s.profiling = enabled != 0;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < TMPS_PROF_CLASSES; ++c) {
         s.prof_ms[c] = 0.0;
         s.prof_cnt[c] = 0;
     }
@@ -1517,9 +1493,10 @@ int tmps_set_launch_mode(tmps_system* h, int mode) {
     return 0;
 }
 
-int tmps_kernel_times(tmps_system* h, double out_ms[3], int counts[3]) {
+int tmps_kernel_times(tmps_system* h, double out_ms[TMPS_PROF_CLASSES],
+                      int counts[TMPS_PROF_CLASSES]) {
     System& s = *sys(h);
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < TMPS_PROF_CLASSES; ++c) {
         out_ms[c] = s.prof_cnt[c] ? s.prof_ms[c] / s.prof_cnt[c] : 0.0;
         if (counts) counts[c] = s.prof_cnt[c];
     }

@@ -1,6 +1,6 @@
-// Pieces shared by the one-kernel-per-step passes of both precisions: the
-// XCD-aware block order and the end of the step, where the last block to
-// finish reduces the virial(+kinetic) tree and runs the servo.
+// Pieces shared by the fused step passes of both precisions: the XCD-aware
+// block order and the publication of each block's tree partial, which the
+// one-block k_step_close reduces before it runs the servo.
 #pragma once
 
 #include "contact.h"
@@ -27,89 +27,21 @@ __device__ __forceinline__ int xcd_block(int w, int nb) {
     return x * q + (x < r ? x : r) + k;
 }
 
-// Device-coherent (agent scope) accesses: write-through stores and loads that
-// see them, so one block can read what others published without flushing or
-// invalidating the caches.
-__device__ __forceinline__ void st_agent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Ends step t = ctl->timestep + 1 of a fused pass. Every thread of every
-// block calls it once, after its own atom's work. `leaf` holds the atom's
-// 7 tree leaves (6 virial+kinetic components, touching-slot count), `trig`
-// whether the atom asked for a neighbor rebuild.
-//
-// Tree: the block's 256 leaves, then groups of 64 entries level by level
-// (RedP levels), each as a pairwise tree in lane order. Blocks are 256-atom
-// aligned and groups 64-entry aligned, so every partial is an aligned subtree
-// of the pairwise tree over all atoms zero-padded to a power of two: the sum
-// is bitwise the one the CPU oracle forms.
-//
-// Publication: an entry is written with agent-scope stores; its writer waits
-// for them to complete before counting itself into its group, and the group's
-// last arrival (the only reader) loads with agent-scope loads.
-__device__ inline void finish_step(Ctl* ctl, const RedP& R, const double leaf[7], int trig, int lb,
-                                   double dt, int servo_on, int natoms) {
-    __shared__ double wsum[TMPS_BLOCK / 64][8];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Ends a block's share of a fused step pass: the pairwise-tree partial of the
+// block's 256 atoms for the 7 tree leaves (6 virial+kinetic components,
+// touching-slot count) and, as an 8th component, how many of its atoms asked
+// for a neighbor rebuild, written to entry `lb` of `partial` (8 doubles).
+// Blocks are 256-atom aligned, so every entry is an aligned subtree of the
+// pairwise tree over all atoms zero-padded to a power of two, and
+// k_step_close's tree over the entries is bitwise the sum the CPU oracle
+// forms. Plain stores: the kernel boundary publishes them.
+__device__ __forceinline__ void publish_block(const double leaf[7], int trig, int lb,
+                                              double* partial) {
+    double l8[8];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        const double t = wave_tree_sum(leaf[k]);
-        if (lane == 0) wsum[wave][k] = t;
-    }
-    const int any = __any(trig);
-    if (lane == 0) wsum[wave][7] = any ? 1.0 : 0.0;
-    __syncthreads();
-    if (wave != 0) return;
-    double e[8];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) e[k] = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
-    e[7] = fmax(fmax(wsum[0][7], wsum[1][7]), fmax(wsum[2][7], wsum[3][7]));
-
-    int idx = lb, count = R.count0;
-    for (int level = 0; count > 1; ++level) {
-        const int g = idx >> 6;
-        const int gsize = min(64, count - (g << 6));
-        double* arr = R.lvl[level];
-        unsigned prev = 0;
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) st_agent(arr + (size_t)idx * 8 + k, e[k]);
-            __builtin_amdgcn_s_waitcnt(0);
-            prev = __hip_atomic_fetch_add(R.cnt[level] + g, 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        }
-        prev = (unsigned)__builtin_amdgcn_readlane((int)prev, 0);
-        if (prev != (unsigned)(gsize - 1)) return;
-        if (lane == 0)
-            __hip_atomic_store(R.cnt[level] + g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const size_t m = (size_t)(g << 6) + lane;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) e[k] = wave_tree_sum(lane < gsize ? ld_agent(arr + m * 8 + k) : 0.0);
-        e[7] = __any(lane < gsize && ld_agent(arr + m * 8 + 7) != 0.0) ? 1.0 : 0.0;
-        idx = g;
-        count = (count + 63) >> 6;
-    }
-    if (lane != 0) return;
-    // the last block of the step
-    const int t = ctl->timestep + 1;
-    if (e[7] != 0.0) {
-        // the skin check asked for a rebuild: the step's outputs are
-        // discarded and the host redoes it after the build
-        ctl->trigger = t;
-        ctl->abort = 1;
-        return;
-    }
-    for (int k = 0; k < 6; ++k) ctl->vsum[k] = e[k];
-    ctl->ntouch = (int)e[6];
-    if (servo_on) {
-        servo_update(ctl, e, dt, natoms);
-        if (!ctl->abort) update_split_consts(ctl, 0);
-    }
-    if (!ctl->abort) ctl->timestep = t;
+    for (int k = 0; k < 7; ++k) l8[k] = leaf[k];
+    l8[7] = trig ? 1.0 : 0.0;
+    block_tree_sum<8>(l8, partial + (size_t)lb * 8);
 }
 
 // neigh_modify delay / every for step t (neighbor.py:371-403)

@@ -321,9 +321,9 @@ __global__ void k_ke_leaves_f64(F64Ptrs P, double* partial) {
 // an adjacent-pair tree in registers, then the threads' sums are combined the
 // same way (wavefront butterfly, then the 16 wavefront sums). Thread 0
 // receives the sums.
-template <int L, int NC>
+template <int L, int NC, int THREADS = FINAL_THREADS>
 __device__ void final_tree_L(const double* partial, int nparts, double out[NC]) {
-    __shared__ double ws[FINAL_THREADS / 64][NC];
+    __shared__ double ws[THREADS / 64][NC];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int base = threadIdx.x * L;
 #pragma unroll
@@ -343,24 +343,122 @@ __device__ void final_tree_L(const double* partial, int nparts, double out[NC]) 
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-            double u[FINAL_THREADS / 64];
+            double u[THREADS / 64];
 #pragma unroll
-            for (int q = 0; q < FINAL_THREADS / 64; ++q) u[q] = ws[q][k];
+            for (int q = 0; q < THREADS / 64; ++q) u[q] = ws[q][k];
 #pragma unroll
-            for (int w = 1; w < FINAL_THREADS / 64; w <<= 1)
+            for (int w = 1; w < THREADS / 64; w <<= 1)
 #pragma unroll
-                for (int q = 0; q < FINAL_THREADS / 64; q += 2 * w) u[q] = u[q] + u[q + w];
+                for (int q = 0; q < THREADS / 64; q += 2 * w) u[q] = u[q] + u[q + w];
             out[k] = u[0];
         }
     }
     __syncthreads();
 }
 
-// Leaves per thread for nparts partials (a power of two, at least 1).
-inline int final_leaves(int nparts) {
+// Leaves per thread for nparts partials over `threads` threads (a power of
+// two, at least 1). Leaf chunks, wavefronts and the block are all aligned
+// power-of-two subtrees, so the thread count does not change the tree.
+inline int final_leaves(int nparts, int threads = FINAL_THREADS) {
     int p2 = 1;
     while (p2 < nparts) p2 <<= 1;
-    return p2 <= FINAL_THREADS ? 1 : p2 / FINAL_THREADS;
+    return p2 <= threads ? 1 : p2 / threads;
+}
+
+// The step close kernel's block (two wavefronts per SIMD) and how many rounds
+// of one block partial per thread it loads at once.
+#define CLOSE_THREADS 512
+#define CLOSE_BATCH 8
+
+// Partials one close block reduces: CLOSE_BATCH rounds of CLOSE_THREADS.
+#define CLOSE_SPAN (CLOSE_THREADS * CLOSE_BATCH)
+
+// Orders LDS accesses of one wavefront: its lanes' earlier writes are visible
+// to its later reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The tree of final_tree_L over 8 components for up to CLOSE_SPAN partials.
+// Wavefront w owns partials [w*64L, (w+1)*64L) and loads them in L coalesced
+// rounds (lane l: partial w*64L + 64r + l, one 64-byte row); per component an
+// LDS transpose hands lane l the L consecutive partials [w*64L + L*l, +L),
+// which it sums pairwise in registers, then one wavefront butterfly and the
+// pairwise sum over the wavefronts. Thread, wavefront and block ranges are
+// aligned power-of-two subtrees, so this is final_tree_L's tree. Thread 0
+// receives the sums.
+template <int L>
+__device__ void close_tree(const double* partial, int nparts, double out[8]) {
+    static_assert(L <= CLOSE_BATCH, "a close block reduces at most CLOSE_SPAN partials");
+    constexpr int NWAVE = CLOSE_THREADS / 64;
+    __shared__ double xp[NWAVE][64 * L];
+    __shared__ double ws[8][NWAVE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double4* rows = reinterpret_cast<const double4*>(partial);
+    double e[L][8];
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        const int idx = wave * 64 * L + 64 * r + lane;
+        double4 lo = make_double4(0.0, 0.0, 0.0, 0.0), hi = lo;
+        if (idx < nparts) {
+            lo = rows[2 * (size_t)idx];
+            hi = rows[2 * (size_t)idx + 1];
+        }
+        e[r][0] = lo.x; e[r][1] = lo.y; e[r][2] = lo.z; e[r][3] = lo.w;
+        e[r][4] = hi.x; e[r][5] = hi.y; e[r][6] = hi.z; e[r][7] = hi.w;
+    }
+    double* t = xp[wave];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double v[L];
+        if (L == 1) {
+            v[0] = e[0][k];
+        } else {
+#pragma unroll
+            for (int r = 0; r < L; ++r) t[64 * r + lane] = e[r][k];
+            wave_lds_sync();
+#pragma unroll
+            for (int q = 0; q < L; ++q) v[q] = t[L * lane + q];
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int w = 1; w < L; w <<= 1)
+#pragma unroll
+            for (int q = 0; q < L; q += 2 * w) v[q] = v[q] + v[q + w];
+        const double sum = wave_tree_sum(v[0]);
+        if (lane == 0) ws[k][wave] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double u[NWAVE];
+#pragma unroll
+            for (int q = 0; q < NWAVE; ++q) u[q] = ws[k][q];
+#pragma unroll
+            for (int w = 1; w < NWAVE; w <<= 1)
+#pragma unroll
+                for (int q = 0; q < NWAVE; q += 2 * w) u[q] = u[q] + u[q + w];
+            out[k] = u[0];
+        }
+    }
+    __syncthreads();
+}
+
+// Above CLOSE_SPAN block partials, one block per aligned group of CLOSE_SPAN
+// reduces the group to one 8-double entry of `level` first; groups are
+// aligned subtrees, so the close kernel's tree over the entries is the same
+// tree.
+__global__ void __launch_bounds__(CLOSE_THREADS) k_tree_level(const double* partial, int nparts,
+                                                               double* level) {
+    const int g = blockIdx.x;
+    const int count = min(CLOSE_SPAN, nparts - g * CLOSE_SPAN);
+    double out[8];
+    close_tree<CLOSE_BATCH>(partial + (size_t)g * CLOSE_SPAN * 8, count, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) level[(size_t)g * 8 + k] = out[k];
 }
 
 template <int L, int NC>
@@ -397,6 +495,57 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const doub
             if (!ctl->abort) update_split_consts(ctl, 0);
         }
         if (advance && !ctl->abort) ctl->timestep = ctl->timestep + 1;
+    }
+}
+
+// End of a fused step, in one block: the pairwise tree over the step pass's
+// block partials (publish_block), the rebuild request they carry, and the
+// servo. The control block is copied into LDS and written back whole, both
+// as one parallel round trip, so the serial servo arithmetic in between waits
+// on no global memory. A step whose atoms asked for a rebuild records the
+// trigger and the abort that skips the rest of its segment; an aborted
+// segment is left untouched.
+template <int L>
+__global__ void __launch_bounds__(CLOSE_THREADS) k_step_close(Ctl* ctl, const double* partial,
+                                                               int nparts, double dt,
+                                                               int servo_on, int natoms) {
+    static_assert(sizeof(Ctl) % 4 == 0, "the control block is copied in 4-byte words");
+    constexpr int NW = (int)(sizeof(Ctl) / 4);
+    __shared__ Ctl cs;
+    __shared__ int writeback;
+    // a segment cut short by a rebuild request skips its remaining closes
+    if (ctl->abort) return;
+    const unsigned* src = reinterpret_cast<const unsigned*>(ctl);
+    unsigned* lds = reinterpret_cast<unsigned*>(&cs);
+    for (int k = threadIdx.x; k < NW; k += CLOSE_THREADS) lds[k] = src[k];
+    double out[8];
+    close_tree<L>(partial, nparts, out);  // ends with a barrier: cs is complete
+    if (threadIdx.x == 0) {
+        int wb = 0;
+        if (!cs.abort) {
+            wb = 1;
+            const int t = cs.timestep + 1;
+            if (out[7] != 0.0) {
+                // the skin check asked for a rebuild: the step's outputs are
+                // discarded and the host redoes it after the build
+                cs.trigger = t;
+                cs.abort = 1;
+            } else {
+                for (int k = 0; k < 6; ++k) cs.vsum[k] = out[k];
+                cs.ntouch = (int)out[6];
+                if (servo_on) {
+                    servo_update(&cs, out, dt, natoms);
+                    if (!cs.abort) update_split_consts(&cs, 0);
+                }
+                if (!cs.abort) cs.timestep = t;
+            }
+        }
+        writeback = wb;
+    }
+    __syncthreads();
+    if (writeback) {
+        unsigned* dst = reinterpret_cast<unsigned*>(ctl);
+        for (int k = threadIdx.x; k < NW; k += CLOSE_THREADS) dst[k] = lds[k];
     }
 }
 
@@ -596,14 +745,22 @@ __device__ __forceinline__ double4 step_position(const RemapRec& R, bool remap, 
 // Step t in one pass: drift (own atom and, on the fly, each neighbour), skin
 // check, contact gather, damping/cundall, final_integrate, the next step's
 // half-kick (or, at a segment's end, the stored f/torque/virial), and the
-// step's end in the last block (finish_step). Nothing of set `in` is written.
+// block's tree partial (publish_block); k_step_close ends the step. Nothing of
+// set `in` is written.
 template <int STYLE, int CUNDALL, int FINAL>
-__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_F64))) k_step_f64(Fused64 P, Ctl* ctl, PairP pp, RedP R,
-                                                          double dt, double gl, double ga,
-                                                          int servo_on) {
-    if (ctl->abort) return;
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_F64))) k_step_f64(Fused64 P, Ctl* ctl, PairP pp, double* partial,
+                                                          double dt, double gl, double ga) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * TMPS_BLOCK + threadIdx.x;
+    // the atom's own streams and row bounds are requested before the control
+    // block is read, so the two memory round trips overlap
+    const int ia = i < P.n ? i : P.n - 1;
+    double4 v = P.vm_in[ia];
+    double4 w = P.om_in[ia];
+    const double4 xr_i = P.xr_in[ia];
+    const double4 x0 = P.x0[ia];
+    const int s0 = P.nb_start[ia], s1 = P.nb_start[ia + 1];
+    if (ctl->abort) return;
     const int t = ctl->timestep + 1;
     double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int trig = 0;
@@ -611,14 +768,11 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         const RemapRec RM = ctl->remap;
         const bool remap = RM.step != -1 && RM.step == ctl->timestep;
         const ImgP B = img_params(ctl);
-        double4 v = P.vm_in[i];
-        double4 w = P.om_in[i];
-        const double4 xi = step_position(RM, remap, dt, P.xr_in[i], v);
+        const double4 xi = step_position(RM, remap, dt, xr_i, v);
         if (check_due(ctl, t)) {
             if (!ctl->check) {
                 trig = 1;
             } else {
-                const double4 x0 = P.x0[i];
                 const double d0 = xi.x - x0.x, d1 = xi.y - x0.y, d2 = xi.z - x0.z;
                 if (d0 * d0 + d1 * d1 + d2 * d2 > ctl->trigger_sq) trig = 1;
             }
@@ -627,7 +781,6 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int nt = 0;
         const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
-        const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
         // Pass 1, every slot: separation and contact test, neighbour states
         // requested a batch at a time (see k_step_split). Touching slots go
         // into `tmask` (first 64 of the row; the rest stay in the flags).
@@ -773,7 +926,7 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         P.vm_out[i] = v;
         P.om_out[i] = w;
     }
-    finish_step(ctl, R, leaf, trig, lb, dt, servo_on, P.n);
+    publish_block(leaf, trig, lb, partial);
 }
 
 // initial_integrate's half-kick of a segment's first step (the later steps
@@ -980,6 +1133,33 @@ int launch_step_end(System& s, double dt, int nparts, int advance, int apply_ser
     return 0;
 }
 
+#define CLOSE_CASE(Lv)                                                                          \
+    case Lv:                                                                                    \
+        hipLaunchKernelGGL((k_step_close<Lv>), dim3(1), dim3(CLOSE_THREADS), 0, s.stream, s.ctl, \
+                           src, nparts, dt, servo_on, s.n);                                     \
+        break;
+
+int launch_step_close(System& s, double dt, int servo_on) {
+    int nparts = nblocks_for(std::max(s.n, 1));
+    const double* src = s.partial;
+    if (nparts > CLOSE_SPAN) {
+        const int groups = (nparts + CLOSE_SPAN - 1) / CLOSE_SPAN;
+        if (groups > CLOSE_SPAN || groups > s.level_cap)
+            return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+        hipLaunchKernelGGL(k_tree_level, dim3(groups), dim3(CLOSE_THREADS), 0, s.stream, s.partial,
+                           nparts, s.level);
+        HIP_TRY(hipGetLastError());
+        src = s.level;
+        nparts = groups;
+    }
+    switch (final_leaves(nparts, CLOSE_THREADS)) {
+        CLOSE_CASE(1) CLOSE_CASE(2) CLOSE_CASE(4) CLOSE_CASE(8)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 int launch_remap_apply(System& s) {
     if (s.precision == TMPS_FP64) {
         LAUNCH_N(k_remap_apply, s.n, ptrs(s), s.ctl, 0);
@@ -1009,18 +1189,16 @@ int launch_ctl_rebuilt(System& s, int step) {
 // ------------------------------------------------ one kernel per step
 
 template <int STYLE, int CUNDALL, int FINAL>
-static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, double ga,
-                        int servo_on) {
-    RedP R = s.red;
-    R.count0 = nblocks_for(s.n);
+static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, double ga) {
+    const int nb = nblocks_for(s.n);
     if (s.kstart || s.kstop) {
-        hipExtLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
-                              0, s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, R, dt, gl, ga,
-                              servo_on);
+        hipExtLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
+                              s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, s.partial, dt, gl,
+                              ga);
         s.kstart = s.kstop = nullptr;
     } else {
-        hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
-                           0, s.stream, P, s.ctl, s.pair, R, dt, gl, ga, servo_on);
+        hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
+                           s.stream, P, s.ctl, s.pair, s.partial, dt, gl, ga);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1028,17 +1206,17 @@ static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, doubl
 
 template <int STYLE>
 static int fused_f64_style(System& s, const Fused64& P, double dt, int fin, int cundall, double gl,
-                           double ga, int servo_on) {
+                           double ga) {
     if (fin) {
-        if (cundall) return fused_f64_go<STYLE, 1, 1>(s, P, dt, gl, ga, servo_on);
-        return fused_f64_go<STYLE, 0, 1>(s, P, dt, gl, ga, servo_on);
+        if (cundall) return fused_f64_go<STYLE, 1, 1>(s, P, dt, gl, ga);
+        return fused_f64_go<STYLE, 0, 1>(s, P, dt, gl, ga);
     }
-    if (cundall) return fused_f64_go<STYLE, 1, 0>(s, P, dt, gl, ga, servo_on);
-    return fused_f64_go<STYLE, 0, 0>(s, P, dt, gl, ga, servo_on);
+    if (cundall) return fused_f64_go<STYLE, 1, 0>(s, P, dt, gl, ga);
+    return fused_f64_go<STYLE, 0, 0>(s, P, dt, gl, ga);
 }
 
-int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, double gl, double ga,
-                     int servo_on) {
+int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, double gl,
+                     double ga) {
     if (s.n <= 0) return 0;
     if (nblocks_for(s.n) > (1 << 30) / TMPS_BLOCK)
         return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for one step launch");
@@ -1063,11 +1241,11 @@ int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, dou
     P.n = s.n;
     switch (s.pair.style) {
         case TMPS_PAIR_HOOKE_HISTORY:
-            return fused_f64_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga, servo_on);
+            return fused_f64_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga);
         case TMPS_PAIR_HERTZ_HISTORY:
-            return fused_f64_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga, servo_on);
+            return fused_f64_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga);
         default:
-            return fused_f64_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga, servo_on);
+            return fused_f64_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga);
     }
 }
 

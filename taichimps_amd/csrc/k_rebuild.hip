@@ -187,6 +187,10 @@ __device__ __forceinline__ double list_image(double d, double prd, double inv_pr
     return d - prd * round(fast ? d * inv_prd : d / prd);
 }
 
+// Scratch row length of the single-pass list build (longer rows take the
+// two-pass count-then-fill path).
+#define LIST_ROW_CAP 32
+
 // The stencil walk of NeighborList.build_neighbor_list (neighbor.py:315-369),
 // visiting bins in the reference's (dx, dy, dz) order. MODE 0 counts the full
 // row and the half row (j > i); MODE 1 writes the full row at nb_start[i];
@@ -270,12 +274,133 @@ __global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const 
     }
 }
 
-__global__ void k_list_compact(int n, int cap, const int* rows, const int* nb_start, int* nb_j) {
+// Positions in bin order (after k_bin_sort): bx[p] = xr[particles[p]], so the
+// list build reads a bin's candidates and their positions as two contiguous
+// streams instead of a dependent gather per candidate.
+__global__ void k_bin_gather(int n, const int* particles, const double4* xr, double4* bx) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) bx[p] = xr[particles[p]];
+}
+
+// One bin-order range [a, b) of candidates for atom i: the cutoff test of
+// k_list (same expressions, so the same accepted set), each accepted j
+// inserted in ascending order into the thread's LDS row (column of a
+// [LIST_ROW_CAP][TMPS_BLOCK] array). Candidates are read four at a time.
+__device__ __forceinline__ void list_range(int a, int b, int i, const double4& pi,
+                                           const int* particles, const double4* bx,
+                                           const int per[3], const double prd[3],
+                                           const double inv[3], const bool fast[3], double skin,
+                                           int* row, int& nfull, int& nhalf) {
+    for (int p0 = a; p0 < b; p0 += 4) {
+        int jq[4];
+        double4 xq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = p0 + q < b ? p0 + q : b - 1;
+            jq[q] = particles[p];
+            xq[q] = bx[p];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = jq[q];
+            if (p0 + q >= b || j == i) continue;
+            const double4 pj = xq[q];
+            double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+            if (per[0] == 1) d0 = list_image(d0, prd[0], inv[0], fast[0]);
+            if (per[1] == 1) d1 = list_image(d1, prd[1], inv[1], fast[1]);
+            if (per[2] == 1) d2 = list_image(d2, prd[2], inv[2], fast[2]);
+            const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+            const double rad_sum = pi.w + pj.w + skin;
+            if (rsq < rad_sum * rad_sum) {
+                if (nfull < LIST_ROW_CAP) {
+                    int q2 = nfull - 1;
+                    while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
+                        row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
+                        --q2;
+                    }
+                    row[(q2 + 1) * TMPS_BLOCK] = j;
+                }
+                ++nfull;
+                if (j > i) ++nhalf;
+            }
+        }
+    }
+}
+
+// The stencil walk of NeighborList.build_neighbor_list (neighbor.py:315-369)
+// as a set: for each of the 9 (dy, dz) bin rows, the x-adjacent bins are one
+// contiguous range of bin-ordered candidates (two where the row wraps a
+// periodic x boundary). The row is kept sorted in LDS and written, slot-major,
+// to rows[k * n + i] (k < LIST_ROW_CAP); full and half counts go to
+// full_count / half_count, flags[0] is raised past max_nb half entries and
+// flags[1] when a row exceeds LIST_ROW_CAP (k_list<1> then writes the rows
+// directly).
+__global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
+                                                          GridP G, const int* cell_start,
+                                                          const int* particles, const double4* bx,
+                                                          double skin, int max_nb, int* full_count,
+                                                          int* half_count, int* flags, int* rows) {
+    __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int* row = lrow + threadIdx.x;
+    const DevBox& B = ctl->box;
+    const double4 pi = xr[i];
+    int c[3];
+    cell_coord(B, G, pi, c);
+    const int gx = G.g[0], gy = G.g[1], gz = G.g[2], gxy = gx * gy;
+    const int per[3] = {B.per[0], B.per[1], B.per[2]};
+    const double prd[3] = {B.prd[0], B.prd[1], B.prd[2]};
+    const double inv[3] = {1.0 / prd[0], 1.0 / prd[1], 1.0 / prd[2]};
+    const bool fast[3] = {G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3};
+    // x-adjacent bins [x0, x1] of every row, plus a wrapped piece [w, w]
+    int x0 = c[0] - 1, x1 = c[0] + 1, w = -1;
+    if (gx == 1) {
+        x0 = 0; x1 = 0;
+    } else if (per[0] == 1) {
+        // periodic with at least 3 bins (setup_grid collapses a periodic
+        // 2-bin axis to 1)
+        if (x0 < 0) { x0 = 0; w = gx - 1; }
+        if (x1 >= gx) { x1 = gx - 1; w = 0; }
+    } else {
+        x0 = max(x0, 0);
+        x1 = min(x1, gx - 1);
+    }
+    int nfull = 0, nhalf = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        if (per[2] == 1 && gz == 1 && dz != 0) continue;
+        int nz = c[2] + dz;
+        if (per[2] == 1) nz = nz < 0 ? nz + gz : (nz >= gz ? nz - gz : nz);
+        if (nz < 0 || nz >= gz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            if (per[1] == 1 && gy == 1 && dy != 0) continue;
+            int ny = c[1] + dy;
+            if (per[1] == 1) ny = ny < 0 ? ny + gy : (ny >= gy ? ny - gy : ny);
+            if (ny < 0 || ny >= gy) continue;
+            const int rb = ny * gx + nz * gxy;
+            list_range(cell_start[rb + x0], cell_start[rb + x1 + 1], i, pi, particles, bx, per,
+                       prd, inv, fast, skin, row, nfull, nhalf);
+            if (w >= 0)
+                list_range(cell_start[rb + w], cell_start[rb + w + 1], i, pi, particles, bx, per,
+                           prd, inv, fast, skin, row, nfull, nhalf);
+        }
+    }
+    full_count[i] = nfull;
+    half_count[i] = nhalf;
+    if (nhalf > max_nb) flags[0] = 1;
+    if (nfull > LIST_ROW_CAP) {
+        flags[1] = 1;
+        return;
+    }
+    for (int k = 0; k < nfull; ++k) rows[(size_t)k * n + i] = row[k * TMPS_BLOCK];
+}
+
+// Rows from the slot-major scratch of k_list_rows into the CSR list.
+__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int a = nb_start[i], m = nb_start[i + 1] - a;
-    const int* src = rows + (size_t)i * cap;
-    for (int k = 0; k < m; ++k) nb_j[a + k] = src[k];
+    for (int k = 0; k < m; ++k) nb_j[a + k] = rows[(size_t)k * n + i];
 }
 
 // ContactHistory save/restore (contact_history.py:91-139) as a merge of the
@@ -439,7 +564,6 @@ int launch_bins(System& s, const double4* xr) {
     return 0;
 }
 
-#define LIST_ROW_CAP 32
 
 static int grow_list(System& s, size_t need) {
     if (need <= s.nb_cap) return 0;
@@ -478,9 +602,12 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
         s.row_tmp_cap = rows_need;
     }
     if (s.n > 0) {
-        hipLaunchKernelGGL(k_list<2>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
-                           s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb, s.full_count,
-                           s.half_count, s.flags, (const int*)nullptr, s.row_tmp, LIST_ROW_CAP);
+        hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.cell_particles, xr, s.bin_x);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
+                           s.ctl, s.grid, s.cell_start, s.cell_particles, s.bin_x, s.skin,
+                           s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp);
         HIP_TRY(hipGetLastError());
     }
     int rc = launch_scan(s, s.full_count, s.nb_start, s.n);
@@ -497,7 +624,7 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
     if (s.n > 0) {
         if (!hv[1]) {
             hipLaunchKernelGGL(k_list_compact, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
-                               LIST_ROW_CAP, (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j);
+                               (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j);
         } else {
             hipLaunchKernelGGL(k_list<1>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
                                s.ctl, s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb,

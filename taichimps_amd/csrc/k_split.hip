@@ -506,14 +506,22 @@ __device__ __forceinline__ float4 step_offset(const float dtc[3], float4 u, cons
 }
 
 // Step t in one pass (see k_step_f64): FP32 contacts and integration, FP64
-// leaves, tree and servo.
+// leaves; k_step_close finishes the tree and runs the servo.
 template <int STYLE, int CUNDALL, int FINAL>
-__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, RedP R,
-                                                            float dt, float gl, float ga,
-                                                            double ddt, int servo_on) {
-    if (ctl->abort) return;
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
+                                                            float dt, float gl, float ga) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int i = lb * TMPS_BLOCK + threadIdx.x;
+    // The atom's own streams and row bounds are requested before the control
+    // block is read, so the two memory round trips overlap.
+    const int ia = i < P.n ? i : P.n - 1;
+    float4 v = P.vm_in[ia];
+    float4 w = P.om_in[ia];
+    const uint32_t ci = P.cell[ia];
+    const float4 up_i = P.up_in[ia];
+    const float4 u0_i = P.u0[ia];
+    const int s0 = P.nb_start[ia], s1 = P.nb_start[ia + 1];
+    if (ctl->abort) return;
     const int t = ctl->timestep + 1;
     double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int trig = 0;
@@ -521,20 +529,16 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         const SplitConst sc = ctl->sc;
         const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
         const float dtc[3] = {dt * sc.inv_cw[0], dt * sc.inv_cw[1], dt * sc.inv_cw[2]};
-        float4 v = P.vm_in[i];
-        float4 w = P.om_in[i];
-        const uint32_t ci = P.cell[i];
-        const float4 ui = step_offset(dtc, P.up_in[i], v);
+        const float4 ui = step_offset(dtc, up_i, v);
         if (check_due(ctl, t)) {
             if (!ctl->check) trig = 1;
-            else if (split_disp_sq(sc, ci, ui, P.u0[i]) > (float)ctl->trigger_sq) trig = 1;
+            else if (split_disp_sq(sc, ci, ui, u0_i) > (float)ctl->trigger_sq) trig = 1;
         }
         float F[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
         double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int nt = 0;
         const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
         const float zero3[3] = {0.f, 0.f, 0.f};
-        const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
         // Per axis: offsets of list partners can be wider than half the box
         // only where the box holds fewer than 9 bins.
         bool fix[3];
@@ -687,7 +691,7 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         P.vm_out[i] = v;
         P.om_out[i] = w;
     }
-    finish_step(ctl, R, leaf, trig, lb, ddt, servo_on, P.n);
+    publish_block(leaf, trig, lb, partial);
 }
 
 __global__ void k_kick_split(float4* vm, float4* om, const float4* f, const float4* tq, int n,
@@ -854,19 +858,16 @@ int launch_step_force_split(System& s, double dt, int final_mode, int cundall, d
 // ------------------------------------------------ one kernel per step
 
 template <int STYLE, int CUNDALL, int FINAL>
-static int fused_split_go(System& s, const FusedS& P, double dt, double gl, double ga,
-                          int servo_on) {
-    RedP R = s.red;
-    R.count0 = nblocks_for(s.n);
+static int fused_split_go(System& s, const FusedS& P, double dt, double gl, double ga) {
+    const int nb = nblocks_for(s.n);
     if (s.kstart || s.kstop) {
-        hipExtLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0),
-                              dim3(TMPS_BLOCK), 0, s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair,
-                              R, (float)dt, (float)gl, (float)ga, dt, servo_on);
+        hipExtLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
+                              s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, s.partial,
+                              (float)dt, (float)gl, (float)ga);
         s.kstart = s.kstop = nullptr;
     } else {
-        hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(R.count0), dim3(TMPS_BLOCK),
-                           0, s.stream, P, s.ctl, s.pair, R, (float)dt, (float)gl, (float)ga, dt,
-                           servo_on);
+        hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
+                           s.stream, P, s.ctl, s.pair, s.partial, (float)dt, (float)gl, (float)ga);
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -874,17 +875,17 @@ static int fused_split_go(System& s, const FusedS& P, double dt, double gl, doub
 
 template <int STYLE>
 static int fused_split_style(System& s, const FusedS& P, double dt, int fin, int cundall,
-                             double gl, double ga, int servo_on) {
+                             double gl, double ga) {
     if (fin) {
-        if (cundall) return fused_split_go<STYLE, 1, 1>(s, P, dt, gl, ga, servo_on);
-        return fused_split_go<STYLE, 0, 1>(s, P, dt, gl, ga, servo_on);
+        if (cundall) return fused_split_go<STYLE, 1, 1>(s, P, dt, gl, ga);
+        return fused_split_go<STYLE, 0, 1>(s, P, dt, gl, ga);
     }
-    if (cundall) return fused_split_go<STYLE, 1, 0>(s, P, dt, gl, ga, servo_on);
-    return fused_split_go<STYLE, 0, 0>(s, P, dt, gl, ga, servo_on);
+    if (cundall) return fused_split_go<STYLE, 1, 0>(s, P, dt, gl, ga);
+    return fused_split_go<STYLE, 0, 0>(s, P, dt, gl, ga);
 }
 
 int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, double gl,
-                       double ga, int servo_on) {
+                       double ga) {
     if (s.n <= 0) return 0;
     const int ob = 1 - in_buf;
     FusedS P;
@@ -909,13 +910,11 @@ int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, d
     P.n = s.n;
     switch (s.pair.style) {
         case TMPS_PAIR_HOOKE_HISTORY:
-            return fused_split_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga,
-                                                              servo_on);
+            return fused_split_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga);
         case TMPS_PAIR_HERTZ_HISTORY:
-            return fused_split_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga,
-                                                              servo_on);
+            return fused_split_style<TMPS_PAIR_HERTZ_HISTORY>(s, P, dt, fin, cundall, gl, ga);
         default:
-            return fused_split_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga, servo_on);
+            return fused_split_style<TMPS_PAIR_GRANULAR>(s, P, dt, fin, cundall, gl, ga);
     }
 }
 

@@ -81,16 +81,6 @@ struct PairP {
     double g_kn, g_damp, g_kt, g_xt, g_mu;
 };
 
-// The fused step's reduction scratch: lvl[0] holds one 8-double entry per
-// block (7 tree components, rebuild flag), lvl[l] one per group of 64 entries
-// of level l-1; cnt[l] counts the arrivals of level-l entries per group.
-#define TMPS_RED_LEVELS 6
-struct RedP {
-    double* lvl[TMPS_RED_LEVELS];
-    unsigned* cnt[TMPS_RED_LEVELS];
-    int count0;  // blocks of the launch
-};
-
 struct GridP {
     int g[3];
     double csize[3];
@@ -144,9 +134,8 @@ struct System {
     // reductions
     double* partial = nullptr;  // [blocks][8]
     int partial_cap = 0;
-    double* red_buf = nullptr;  // the fused step's reduction levels
-    unsigned* red_cnt = nullptr;
-    RedP red{};
+    double* level = nullptr;  // [groups of 4096 partials][8], large systems only
+    int level_cap = 0;
 
     Ctl* ctl = nullptr;   // device control block
     Ctl* hctl = nullptr;  // pinned host mirror
@@ -155,7 +144,8 @@ struct System {
     std::vector<double> h_radius;
     double max_r = 0.0;
     bool max_r_valid = false;
-    int* row_tmp = nullptr;  // single-pass list build: per-atom candidate rows
+    int* row_tmp = nullptr;  // single-pass list build: slot-major candidate rows
+    double4* bin_x = nullptr;  // list build: positions in bin order
     size_t row_tmp_cap = 0;
     DevBox box{};
     bool box_on_device_newer = false;
@@ -187,8 +177,9 @@ struct System {
     bool profiling = false;
     // one-shot: the next step kernel records its own start / stop time
     hipEvent_t kstart = nullptr, kstop = nullptr;
-    double prof_ms[3] = {0, 0, 0};
-    int prof_cnt[3] = {0, 0, 0};
+    // per class (kick, step pass, segment tail, step close): summed ms, count
+    double prof_ms[TMPS_PROF_CLASSES] = {0, 0, 0, 0};
+    int prof_cnt[TMPS_PROF_CLASSES] = {0, 0, 0, 0};
 };
 
 // ---------------------------------------------------------------- launchers
@@ -235,9 +226,10 @@ int launch_copy_vw(System& s, int honor_abort);
 // one kernel per step (k_fp64.hip, k_split.hip): in_buf selects the buffer
 // set the step reads (the other one receives its results)
 int launch_fused_f64(System& s, int in_buf, double dt, int final_mode, int cundall, double gl,
-                     double ga, int servo_on);
+                     double ga);
 int launch_fused_split(System& s, int in_buf, double dt, int final_mode, int cundall, double gl,
-                       double ga, int servo_on);
+                       double ga);
+int launch_step_close(System& s, double dt, int servo_on);
 // the initial_integrate half-kick of a segment's first step, in buffer 0
 int launch_kick_f64(System& s, double dt);
 int launch_kick_split(System& s, double dt);

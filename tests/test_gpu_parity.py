@@ -361,3 +361,19 @@ def test_reference_remap_x_keeps_fractional_positions():
     np.testing.assert_allclose((pos[0] - lo) / prd, [0.5] * 3, atol=1e-12)
     np.testing.assert_allclose((pos[1] - lo) / prd, [0.25] * 3, atol=1e-12)
     assert prd[2] == pytest.approx(BOX * (1.0 - 1.0e3 * 50 * DT), rel=1e-9)
+
+
+def test_large_system_reduction_levels_bitexact():
+    """More than 4096 blocks of 256 atoms (1,061,208 atoms): the step close
+    first reduces aligned groups of 4096 block partials in parallel. The
+    servo-driven trajectory and box stay bitwise the oracle's."""
+    w = workloads.lattice(102, 102, 102, 1.1, 1002, "uniform")
+    assert len(w["x"]) > 4096 * 256
+    sim = H.device_sim(w, style="hertz/history")
+    o = H.oracle_sim(w, style="hertz/history")
+    sim.run(4)
+    assert sim.last_run_device_resident
+    assert o.run(4, 1e-4) == 0
+    dev, orc = H.device_state(sim), H.oracle_state(o)
+    _assert_state_equal(dev, orc, ("x", "v", "omega"))
+    assert np.array_equal(dev["boxlo"], orc["boxlo"]) and np.array_equal(dev["boxhi"], orc["boxhi"])
