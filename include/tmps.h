@@ -190,6 +190,28 @@ int tmps_set_counters(tmps_system* s, const tmps_counters* in);
 /* pressure tensor of the last servo update (virial+kinetic over volume) */
 int tmps_last_pressure(tmps_system* s, double out6[6]);
 
+/* ---- adjoint ---------------------------------------------------------------- */
+/* No reference counterpart: the reference computes no gradients. A
+ * hand-written reverse of the FP64 step (gran/hooke/history or
+ * gran/hertz/history contacts, nve/sphere, deform/pressure pressure styles with
+ * remap x and nevery 1) for calibration against measured stress.
+ *
+ * tmps_adjoint_forward runs Verlet::setup and `nsteps` steps as tmps_run does
+ * (the fix setup must have been done by the caller), checkpointing the state
+ * every `every` steps. It returns the final pressure tensor (xx yy zz xy xz yz,
+ * virial plus kinetic over volume, as the servo saw it) and the coordination
+ * number (touching contacts per atom, nullable).
+ *
+ * tmps_adjoint_backward takes dL/d(final pressure tensor) and returns
+ * dL/d(kn, kt, gamman, gammat, xmu), dL/d(radius) per atom at fixed density
+ * (n doubles) and, each nullable, dL/d(initial x, v, omega) (3n doubles each).
+ * It replays each segment from its checkpoint and leaves the system in the
+ * state the forward run ended in. */
+int tmps_adjoint_forward(tmps_system* s, int nsteps, int every, double dt, double tensor_out[6],
+                         double* coordination);
+int tmps_adjoint_backward(tmps_system* s, const double dL_dtensor[6], double grad_coeffs[5],
+                          double* grad_radius, double* grad_x, double* grad_v, double* grad_omega);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* How tmps_run issues a segment's kernels: replayed from a captured HIP graph
  * (the default) or launched one by one on the system's stream. No reference

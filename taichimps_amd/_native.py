@@ -49,7 +49,7 @@ SYMBOLS = (
     "tmps_nve_initial", "tmps_nve_final", "tmps_cundall", "tmps_servo_set", "tmps_servo_setup",
     "tmps_servo_end_of_step", "tmps_virial_reduce", "tmps_ke", "tmps_run", "tmps_get_counters",
     "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
-    "tmps_set_launch_mode",
+    "tmps_set_launch_mode", "tmps_adjoint_forward", "tmps_adjoint_backward",
 )
 
 
@@ -154,6 +154,8 @@ def load(path: str | None = None):
         "tmps_set_profiling": (C.c_int, [vp, C.c_int]),
         "tmps_kernel_times": (C.c_int, [vp, dp, ip]),
         "tmps_set_launch_mode": (C.c_int, [vp, C.c_int]),
+        "tmps_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, dp, dp]),
+        "tmps_adjoint_backward": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -387,6 +389,30 @@ class NativeSystem:
     def set_launch_mode(self, direct: bool):
         """Graph replay (default) or direct kernel launches for tmps_run."""
         check(self._L.tmps_set_launch_mode(self._h, 1 if direct else 0))
+
+    def adjoint_forward(self, nsteps, every, dt):
+        """Setup force plus `nsteps` FP64 steps with a checkpoint every `every`
+        steps; returns (final pressure tensor [6], coordination number)."""
+        t = np.zeros(6)
+        z = C.c_double(0.0)
+        check(self._L.tmps_adjoint_forward(self._h, int(nsteps), int(every), float(dt), _dp(t),
+                                           C.byref(z)))
+        return t, z.value
+
+    def adjoint_backward(self, dL_dtensor, n, initial_state=False):
+        """dL/d(kn, kt, gamman, gammat, xmu), dL/d(radius) and, when asked,
+        dL/d(initial x, v, omega), for dL/d(final pressure tensor)."""
+        g = np.ascontiguousarray(np.asarray(dL_dtensor, dtype=np.float64).reshape(6))
+        gc = np.zeros(5)
+        gr = np.zeros(n)
+        gx = gv = gw = None
+        if initial_state:
+            gx, gv, gw = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        nullp = C.POINTER(C.c_double)()
+        check(self._L.tmps_adjoint_backward(
+            self._h, _dp(g), _dp(gc), _dp(gr), _dp(gx) if gx is not None else nullp,
+            _dp(gv) if gv is not None else nullp, _dp(gw) if gw is not None else nullp))
+        return gc, gr, (gx, gv, gw)
 
     def kernel_times(self):
         """Average ms and counts per kernel class: kick, step pass, segment

@@ -41,7 +41,7 @@ int tmps_fail_hip(hipError_t e, const char* what, int line) {
     return TMPS_ERR_HIP;
 }
 
-namespace {
+namespace tmps_api {
 
 template <typename T>
 int dalloc(T** p, size_t count) {
@@ -948,6 +948,7 @@ int get_field(System& s, int field, void* host, int count) {
 }
 
 void free_all(System& s) {
+    adjoint_free(s);
     invalidate_graphs(s);
     dfree(s.xr); dfree(s.vm); dfree(s.om); dfree(s.f); dfree(s.tq); dfree(s.x0); dfree(s.vir);
     dfree(s.vm2); dfree(s.om2); dfree(s.vmf2); dfree(s.omf2); dfree(s.xr2); dfree(s.up2);
@@ -1054,7 +1055,9 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     return 0;
 }
 
-}  // namespace
+}  // namespace tmps_api
+
+using namespace tmps_api;
 
 void drop_graphs(System& s) {
     bool any = false;

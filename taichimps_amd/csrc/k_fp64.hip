@@ -959,8 +959,11 @@ __global__ void k_kick_f64(double4* vm, double4* om, const double4* xr, const do
 template <class T4>
 __global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1, const T4* w1,
                              int n, uint8_t* fl0, const uint8_t* fl1, T4* h0, const T4* h1,
-                             int nslots, const Ctl* ctl, int honor_abort) {
+                             const int* nb_start, const Ctl* ctl, int honor_abort) {
     if (honor_abort && ctl->abort) return;
+    // the slot count of the current list, read on the device: a captured
+    // segment outlives the list it was captured with
+    const int nslots = nb_start[n];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         x0[i] = x1[i];
@@ -1255,19 +1258,21 @@ int launch_kick_f64(System& s, double dt) {
 }
 
 int launch_copy_state(System& s, int honor_abort) {
-    const int work = std::max(s.n, s.nslots);
-    if (work <= 0) return 0;
+    if (s.n <= 0) return 0;
+    // grid-stride over atoms and slots; sized for the list's capacity, since
+    // the slot count can grow while a captured segment is replayed
+    const int work = (int)std::min<size_t>(std::max<size_t>((size_t)s.n, s.nb_cap), 1u << 30);
     const int nb = std::min(nblocks_for(work), 4096);
     if (s.precision == TMPS_FP64)
         hipLaunchKernelGGL(k_copy_state<double4>, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.xr,
                            s.vm, s.om, s.xr2, s.vm2, s.om2, s.n, s.nb_flag, s.nb_flag2,
                            static_cast<double4*>(s.nb_hist), static_cast<const double4*>(s.nb_hist2),
-                           s.nslots, s.ctl, honor_abort);
+                           (const int*)s.nb_start, s.ctl, honor_abort);
     else
         hipLaunchKernelGGL(k_copy_state<float4>, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.up,
                            s.vmf, s.omf, s.up2, s.vmf2, s.omf2, s.n, s.nb_flag, s.nb_flag2,
                            static_cast<float4*>(s.nb_hist), static_cast<const float4*>(s.nb_hist2),
-                           s.nslots, s.ctl, honor_abort);
+                           (const int*)s.nb_start, s.ctl, honor_abort);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -588,6 +588,8 @@ static int grow_list(System& s, size_t need) {
     return 0;
 }
 
+int ensure_list_capacity(System& s, size_t need) { return grow_list(s, need); }
+
 // Builds the full list: one pass into per-atom scratch rows, a scan, and a
 // compaction (two passes, count then fill, if some row exceeds the scratch).
 // *total_out receives the slot count (a host synchronisation per rebuild).

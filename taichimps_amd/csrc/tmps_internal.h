@@ -173,6 +173,9 @@ struct System {
 
     int launch_mode = TMPS_LAUNCH_GRAPH;
 
+    // the adjoint engine's checkpoints and work buffers (adjoint.hip)
+    struct AdjContext* adj = nullptr;
+
     // profiling
     bool profiling = false;
     // one-shot: the next step kernel records its own start / stop time
@@ -181,6 +184,22 @@ struct System {
     double prof_ms[TMPS_PROF_CLASSES] = {0, 0, 0, 0};
     int prof_cnt[TMPS_PROF_CLASSES] = {0, 0, 0, 0};
 };
+
+// Host-side stages of api.hip used by the adjoint engine (adjoint.hip).
+namespace tmps_api {
+int ctl_pull(System& s);
+int ctl_push(System& s);
+int reneighbor(System& s, bool force, int* rebuilt);
+int clear_forces(System& s);
+int pair_compute(System& s, double dt, int shearupdate, int accumulate);
+int servo_end_of_step(System& s, double dt, double* tensor_out);
+int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double gl, double ga);
+}  // namespace tmps_api
+
+// grows the list buffers to hold `need` slots (k_rebuild.hip)
+int ensure_list_capacity(System& s, size_t need);
+// frees the adjoint engine's buffers (adjoint.hip)
+void adjoint_free(System& s);
 
 // ---------------------------------------------------------------- launchers
 

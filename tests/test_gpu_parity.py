@@ -377,3 +377,39 @@ def test_large_system_reduction_levels_bitexact():
     dev, orc = H.device_state(sim), H.oracle_state(o)
     _assert_state_equal(dev, orc, ("x", "v", "omega"))
     assert np.array_equal(dev["boxlo"], orc["boxlo"]) and np.array_equal(dev["boxhi"], orc["boxhi"])
+
+
+def test_chained_runs_match_the_stage_path():
+    """Device-resident runs of one step each, chained without a new fix
+    setup, follow the stage path bit for bit through list rebuilds that grow
+    the list: the segment tail copies every slot's history even when its
+    captured graph predates the rebuild."""
+    from taichimps_amd import _native
+    w = dense(6, 1.0, seed=43)
+    rng = np.random.default_rng(44)
+    w["v"] = rng.normal(0.0, 3.0, size=w["v"].shape)
+    w["omega"] = rng.normal(0.0, 5.0, size=w["v"].shape)
+    a = H.device_sim(w, style="hertz/history", skin=0.02)
+    b = H.device_sim(w, style="hertz/history", skin=0.02)
+    b.setup_run(30)
+    b.init_simulation()
+    a.setup_run(30)
+    N = a.atom._native
+    a.neighbor._bind(a.atom)
+    a.neighbor._push()
+    a.history._bind(a.atom)
+    a.pair_style._install(a.atom)
+    N.set_counters(timestep=0, setup_done=0)
+    for step in range(30):
+        N.run(1, a.dt, run_pre=step == 0)
+        b.step()
+    c = N.counters()
+    a.timestep = c["timestep"]
+    a.neighbor._pull()
+    a.domain.mark_device_authoritative(a.atom)
+    assert a.neighbor.build_count >= 3
+    sa, sb = H.device_state(a), H.device_state(b)
+    _assert_state_equal(sa, sb, ("x", "v", "omega", "f", "torque"))
+    np_ = a.neighbor.npairs[None]
+    assert np.array_equal(a.history.shear.to_numpy()[:np_], b.history.shear.to_numpy()[:np_])
+    assert _native is not None
