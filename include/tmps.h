@@ -212,6 +212,32 @@ int tmps_adjoint_forward(tmps_system* s, int nsteps, int every, double dt, doubl
 int tmps_adjoint_backward(tmps_system* s, const double dL_dtensor[6], double grad_coeffs[5],
                           double* grad_radius, double* grad_x, double* grad_v, double* grad_omega);
 
+/* ---- ensembles ---------------------------------------------------------------- */
+/* No reference counterpart: the reference runs one system per Simulation.
+ * An ensemble advances many independent FP32-split systems (each with its own
+ * box, atoms, pair style and coefficients, servo) together: per step one
+ * launch per pair style covers every member's atoms and one launch closes
+ * every member's step, captured as HIP graph segments. A member whose skin
+ * check asks for a rebuild is redone alone and rejoins; the others do not
+ * wait. Every member's results are bitwise those tmps_run gives it alone.
+ *
+ * tmps_batch_create takes `n` distinct systems, each FP32-split, with atoms,
+ * a pair style (hooke/history, hertz/history or granular) and, optionally,
+ * deform/pressure; no fix damping/cundall. The systems stay owned by the
+ * caller and must outlive the ensemble; their atom counts must not change.
+ * tmps_batch_run is tmps_run(nsteps, dt, run_pre) on every member. With
+ * `autotune` (and always on the first run) its first segments time each
+ * launch configuration (segment length 8/16/32, members' blocks in sequence
+ * or grouped per XCD); tmps_batch_tuning reports the chosen one, each
+ * configuration's measured rate (atom-steps/s, 6 entries) and the number of
+ * member steps the last run redid after a rebuild request. */
+typedef struct tmps_batch tmps_batch;
+int tmps_batch_create(tmps_system* const* systems, int n, tmps_batch** out);
+int tmps_batch_destroy(tmps_batch* b);
+int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotune);
+int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
+                      int* rebuilds);
+
 /* ---- measurement ----------------------------------------------------------- */
 /* How tmps_run issues a segment's kernels: replayed from a captured HIP graph
  * (the default) or launched one by one on the system's stream. No reference

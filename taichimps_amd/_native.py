@@ -50,6 +50,7 @@ SYMBOLS = (
     "tmps_servo_end_of_step", "tmps_virial_reduce", "tmps_ke", "tmps_run", "tmps_get_counters",
     "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
     "tmps_set_launch_mode", "tmps_adjoint_forward", "tmps_adjoint_backward",
+    "tmps_batch_create", "tmps_batch_destroy", "tmps_batch_run", "tmps_batch_tuning",
 )
 
 
@@ -156,6 +157,10 @@ def load(path: str | None = None):
         "tmps_set_launch_mode": (C.c_int, [vp, C.c_int]),
         "tmps_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, dp, dp]),
         "tmps_adjoint_backward": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+        "tmps_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+        "tmps_batch_destroy": (C.c_int, [vp]),
+        "tmps_batch_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+        "tmps_batch_tuning": (C.c_int, [vp, ip, ip, dp, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -421,6 +426,46 @@ class NativeSystem:
         cnt = np.zeros(4, dtype=np.int32)
         check(self._L.tmps_kernel_times(self._h, _dp(ms), _ip(cnt)))
         return ms, cnt
+
+
+class NativeBatch:
+    """One tmps_batch: an ensemble of FP32-split systems advanced together.
+    The member NativeSystems must outlive it."""
+
+    SEG_LENS = (8, 8, 16, 16, 32, 32)
+
+    def __init__(self, members):
+        L = load()
+        self._L = L
+        self._members = list(members)
+        arr = (C.c_void_p * len(self._members))(*[m._h.value for m in self._members])
+        h = C.c_void_p()
+        check(L.tmps_batch_create(arr, len(self._members), C.byref(h)))
+        self._h = h
+
+    def run(self, nsteps, dt, run_pre, autotune=False):
+        check(self._L.tmps_batch_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0,
+                                     1 if autotune else 0))
+
+    def tuning(self):
+        """(segment length, block order, atom-steps/s per configuration,
+        member steps redone after rebuild requests in the last run)."""
+        seg, order, reb = C.c_int(0), C.c_int(0), C.c_int(0)
+        rates = np.zeros(6)
+        check(self._L.tmps_batch_tuning(self._h, C.byref(seg), C.byref(order), _dp(rates),
+                                        C.byref(reb)))
+        return seg.value, order.value, rates, reb.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.tmps_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_name() -> str:

@@ -626,6 +626,60 @@ int check_run_error(System& s) {
                      "max/rate or strain rate");
 }
 
+// The redo of a step whose skin check asked for a rebuild (after the
+// control block was pulled and the step's abort seen). Its pass wrote only
+// buffer set 1 - par; set par still holds the previous step (positions
+// before its remap, half-kicked v, omega, touch flags, history). That set is
+// brought to buffer 0, the step's start redone there with the stage kernel
+// (remap, drift), the list rebuilt, then the step's force pass,
+// final_integrate and end run. On return the step is complete with its
+// results in buffer 0 and f / torque stored for the next step's half-kick.
+int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga) {
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    const int stp = s.hctl->timestep + 1;
+    int rc = 0;
+    if (par == 1) {
+        rc = launch_copy_state(s, 0);
+        if (rc) return rc;
+    }
+    s.hctl->abort = 0;
+    s.hctl->trigger = -1;
+    rc = ctl_push(s);
+    if (rc) return rc;
+    rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
+    if (rc) return rc;
+    s.current_step = stp;
+    rc = history_save(s);
+    if (!rc) rc = pbc(s);
+    if (!rc) rc = neigh_build(s);
+    if (!rc) rc = history_restore(s);
+    if (!rc) rc = ensure_split(s);
+    if (rc) return rc;
+    s.rebuilds += 1;
+    rc = launch_ctl_rebuilt(s, stp);
+    if (rc) return rc;
+    if (split(s)) {
+        rc = launch_split_consts(s, 0);
+        if (rc) return rc;
+    }
+    rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
+                  : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
+    if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
+    // the force pass left x, flags and history in set 0 and v, omega in set 1
+    if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
+    if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
+    if (!rc) rc = launch_remap_clear(s, 1);
+    if (rc) return rc;
+    rc = ctl_pull(s);
+    if (rc) return rc;
+    rc = check_run_error(s);
+    if (rc) return rc;
+    s.timestep = s.hctl->timestep;
+    s.current_step = s.timestep;
+    return 0;
+}
+
 // Simulation.run(steps) with nve/sphere + pair [+ damping/cundall]
 // [+ deform/pressure], device-resident.
 int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double gl, double ga) {
@@ -663,8 +717,6 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         if (rc) return rc;
     }
     const int t_end = s.timestep + nsteps;
-    const int nparts = nblocks_for(std::max(s.n, 1));
-    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
     // Segments run two deep: while one executes, the host waits on the one
     // before it and reads its status (timestep, trigger, abort, error) from
     // pinned memory, so a rebuild request costs at most one segment of
@@ -712,51 +764,9 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         rc = check_run_error(s);
         if (rc) return rc;
         s.timestep = s.hctl->timestep;
-        // The skin check of step stp asked for a rebuild. Its pass wrote
-        // only buffer set 1 - p; set p still holds step stp-1 (positions
-        // before stp-1's remap, half-kicked v, omega, touch flags, history).
-        // Bring that set to buffer 0, redo the step's start there with the
-        // stage kernel (remap, drift), rebuild, then the step's force pass,
-        // final_integrate and end.
         const int stp = s.hctl->timestep + 1;
-        if (((stp - 1) - sg.start) % 2 == 1) {
-            rc = launch_copy_state(s, 0);
-            if (rc) return rc;
-        }
-        s.hctl->abort = 0;
-        s.hctl->trigger = -1;
-        rc = ctl_push(s);
+        rc = redo_step(s, ((stp - 1) - sg.start) % 2, dt, use_cundall, gl, ga);
         if (rc) return rc;
-        rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
-        if (rc) return rc;
-        s.current_step = stp;
-        rc = history_save(s);
-        if (!rc) rc = pbc(s);
-        if (!rc) rc = neigh_build(s);
-        if (!rc) rc = history_restore(s);
-        if (!rc) rc = ensure_split(s);
-        if (rc) return rc;
-        s.rebuilds += 1;
-        rc = launch_ctl_rebuilt(s, stp);
-        if (rc) return rc;
-        if (split(s)) {
-            rc = launch_split_consts(s, 0);
-            if (rc) return rc;
-        }
-        rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
-                      : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
-        if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
-        // the force pass left x, flags and history in set 0 and v, omega in set 1
-        if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
-        if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
-        if (!rc) rc = launch_remap_clear(s, 1);
-        if (rc) return rc;
-        rc = ctl_pull(s);
-        if (rc) return rc;
-        rc = check_run_error(s);
-        if (rc) return rc;
-        s.timestep = s.hctl->timestep;
-        s.current_step = s.timestep;
         t = s.timestep;
         kick = 1;  // the redone step ended with final_integrate
     }

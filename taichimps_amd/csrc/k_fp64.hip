@@ -506,9 +506,9 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const doub
 // trigger and the abort that skips the rest of its segment; an aborted
 // segment is left untouched.
 template <int L>
-__global__ void __launch_bounds__(CLOSE_THREADS) k_step_close(Ctl* ctl, const double* partial,
-                                                               int nparts, double dt,
-                                                               int servo_on, int natoms) {
+__device__ __forceinline__ void step_close_block(Ctl* ctl, const double* partial, int nparts,
+                                                 double dt, int servo_on, int natoms,
+                                                 bool member) {
     static_assert(sizeof(Ctl) % 4 == 0, "the control block is copied in 4-byte words");
     constexpr int NW = (int)(sizeof(Ctl) / 4);
     __shared__ Ctl cs;
@@ -537,7 +537,16 @@ __global__ void __launch_bounds__(CLOSE_THREADS) k_step_close(Ctl* ctl, const do
                     servo_update(&cs, out, dt, natoms);
                     if (!cs.abort) update_split_consts(&cs, 0);
                 }
-                if (!cs.abort) cs.timestep = t;
+                if (!cs.abort) {
+                    cs.timestep = t;
+                    if (member) {
+                        // an ensemble member: the next step reads the other
+                        // set; its run ends with step bend
+                        cs.bpar ^= 1;
+                        cs.bkick = 0;
+                        if (t == cs.bend) cs.bdone = 1;
+                    }
+                }
             }
         }
         writeback = wb;
@@ -547,6 +556,21 @@ __global__ void __launch_bounds__(CLOSE_THREADS) k_step_close(Ctl* ctl, const do
         unsigned* dst = reinterpret_cast<unsigned*>(ctl);
         for (int k = threadIdx.x; k < NW; k += CLOSE_THREADS) dst[k] = lds[k];
     }
+}
+
+template <int L>
+__global__ void __launch_bounds__(CLOSE_THREADS) k_step_close(Ctl* ctl, const double* partial,
+                                                               int nparts, double dt,
+                                                               int servo_on, int natoms) {
+    step_close_block<L>(ctl, partial, nparts, dt, servo_on, natoms, false);
+}
+
+// The close of every running ensemble member, one block each.
+template <int L>
+__global__ void __launch_bounds__(CLOSE_THREADS) k_batch_close(const CloseEntry* table, double dt) {
+    const CloseEntry& e = table[blockIdx.x];
+    if (e.ctl->bdone) return;
+    step_close_block<L>(e.ctl, e.partial, e.nparts, dt, e.servo_on, e.natoms, true);
 }
 
 // remap_positions, deform_pressure.py:493-530 (zero tilt)
@@ -1158,6 +1182,25 @@ int launch_step_close(System& s, double dt, int servo_on) {
     switch (final_leaves(nparts, CLOSE_THREADS)) {
         CLOSE_CASE(1) CLOSE_CASE(2) CLOSE_CASE(4) CLOSE_CASE(8)
         default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+#define BATCH_CLOSE_CASE(Lv)                                                                 \
+    case Lv:                                                                                 \
+        hipLaunchKernelGGL((k_batch_close<Lv>), dim3(ninst), dim3(CLOSE_THREADS), 0, st, table, \
+                           dt);                                                              \
+        break;
+
+int launch_batch_close(const CloseEntry* table, int ninst, int max_parts, double dt,
+                       hipStream_t st) {
+    if (ninst <= 0) return 0;
+    if (max_parts > CLOSE_SPAN)
+        return tmps_fail(TMPS_ERR_CAPACITY, "an ensemble member has too many atoms");
+    switch (final_leaves(max_parts, CLOSE_THREADS)) {
+        BATCH_CLOSE_CASE(1) BATCH_CLOSE_CASE(2) BATCH_CLOSE_CASE(4) BATCH_CLOSE_CASE(8)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "an ensemble member has too many atoms");
     }
     HIP_TRY(hipGetLastError());
     return 0;

@@ -9,6 +9,8 @@
 // per-atom virial, the kinetic tensor, their reduction and the servo stay FP64.
 #include <hip/hip_ext.h>
 
+#include <cstring>
+
 #include "contact.h"
 #include "fused.h"
 #include "tmps_internal.h"
@@ -505,12 +507,15 @@ __device__ __forceinline__ float4 step_offset(const float dtc[3], float4 u, cons
     return u;
 }
 
-// Step t in one pass (see k_step_f64): FP32 contacts and integration, FP64
-// leaves; k_step_close finishes the tree and runs the servo.
-template <int STYLE, int CUNDALL, int FINAL>
-__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
-                                                            float dt, float gl, float ga) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
+// Step t in one pass (see k_step_f64) for the atoms of block `lb`: FP32
+// contacts and integration, FP64 leaves; k_step_close finishes the tree and
+// runs the servo. `fin`: the run's last step, which stores f / torque /
+// virial instead of applying the next step's half-kick. Shared by the
+// single-system kernel and the batched one (k_batch.hip).
+template <int STYLE, int CUNDALL>
+__device__ __forceinline__ void split_step_block(const FusedS& P, Ctl* ctl, const PairP& pp,
+                                                 double* partial, float dt, float gl, float ga,
+                                                 int lb, bool fin) {
     const int i = lb * TMPS_BLOCK + threadIdx.x;
     // The atom's own streams and row bounds are requested before the control
     // block is read, so the two memory round trips overlap.
@@ -679,7 +684,7 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         leaf[4] = V[4] + m * vx * vz;
         leaf[5] = V[5] + m * vy * vz;
         leaf[6] = (double)nt;
-        if (FINAL) {
+        if (fin) {
             P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
             P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
             for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
@@ -692,6 +697,83 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
         P.om_out[i] = w;
     }
     publish_block(leaf, trig, lb, partial);
+}
+
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
+                                                            float dt, float gl, float ga) {
+    split_step_block<STYLE, CUNDALL>(P, ctl, pp, partial, dt, gl, ga,
+                                     xcd_block(blockIdx.x, gridDim.x), FINAL != 0);
+}
+
+// ------------------------------------------------ ensemble (batch.hip)
+
+// One ensemble member: its step's buffers for either input set, its control
+// block, pair coefficients and tree partials.
+struct BatchSplit {
+    FusedS P[2];  // reading set 0 / set 1
+    Ctl* ctl;
+    PairP pp;
+    double* partial;
+    float4 *vm0, *om0, *f, *tq;  // set 0 and the stored forces (the half-kick)
+    int n;
+};
+
+// Member b.x, its local block b.y: the step of a member that is running and
+// not cut short by a rebuild request, on the set its previous step left. The
+// per-atom work is split_step_block's, so a member's results are bitwise
+// those of its single-system run.
+template <int STYLE>
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_batch_step_split(const BatchSplit* table, const int2* blocks, float dt) {
+    const int2 b = blocks[blockIdx.x];
+    const BatchSplit& E = table[b.x];
+    Ctl* ctl = E.ctl;
+    if (ctl->bdone) return;
+    const int par = ctl->bpar;
+    const bool fin = ctl->timestep + 1 == ctl->bend;
+    split_step_block<STYLE, 0>(E.P[par], ctl, E.pp, E.partial, dt, 0.f, 0.f, b.y, fin);
+}
+
+// The segment-start half-kick of members that need one (their run's start
+// or a redone step), in set 0.
+__global__ void k_batch_kick_split(const BatchSplit* table, const int2* blocks, float dt) {
+    const int2 b = blocks[blockIdx.x];
+    const BatchSplit& E = table[b.x];
+    const Ctl* ctl = E.ctl;
+    if (ctl->bdone || !ctl->bkick || ctl->abort) return;
+    const int i = b.y * TMPS_BLOCK + threadIdx.x;
+    if (i >= E.n) return;
+    const float dtf = 0.5f * dt;
+    float4 v = E.vm0[i], w = E.om0[i];
+    const float4 ff = E.f[i], tq4 = E.tq[i];
+    const float im = 1.0f / v.w;
+    v.x += dtf * ff.x * im; v.y += dtf * ff.y * im; v.z += dtf * ff.z * im;
+    w.x += dtf * tq4.x * w.w; w.y += dtf * tq4.y * w.w; w.z += dtf * tq4.z * w.w;
+    E.vm0[i] = v;
+    E.om0[i] = w;
+}
+
+// A finished member whose last step wrote set 1: its state back into set 0
+// (as the single-system segment tail does).
+__global__ void k_batch_tail_split(const BatchSplit* table, const int2* blocks) {
+    const int2 b = blocks[blockIdx.x];
+    const BatchSplit& E = table[b.x];
+    const Ctl* ctl = E.ctl;
+    if (!ctl->bdone || ctl->bpar != 1) return;
+    const FusedS& Q = E.P[0];  // reads set 0, writes set 1
+    const int i = b.y * TMPS_BLOCK + threadIdx.x;
+    if (i < E.n) {
+        const_cast<float4*>(Q.up_in)[i] = Q.up_out[i];
+        const_cast<float4*>(Q.vm_in)[i] = Q.vm_out[i];
+        const_cast<float4*>(Q.om_in)[i] = Q.om_out[i];
+    }
+    // the member's slots, strided over its blocks
+    const int nb = (E.n + TMPS_BLOCK - 1) / TMPS_BLOCK;
+    const int ns = Q.nb_start[E.n];
+    for (int k = b.y * TMPS_BLOCK + threadIdx.x; k < ns; k += nb * TMPS_BLOCK) {
+        const_cast<uint8_t*>(Q.fl_in)[k] = Q.fl_out[k];
+        const_cast<float4*>(Q.h_in)[k] = Q.h_out[k];
+    }
 }
 
 __global__ void k_kick_split(float4* vm, float4* om, const float4* f, const float4* tq, int n,
@@ -920,5 +1002,91 @@ int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, d
 
 int launch_kick_split(System& s, double dt) {
     LAUNCH_S(k_kick_split, s.n, s.vmf, s.omf, s.ff, s.tqf, s.n, s.ctl, (float)dt);
+    return 0;
+}
+
+// ------------------------------------------------ ensemble (batch.hip)
+
+size_t batch_split_entry_size() { return sizeof(BatchSplit); }
+
+static FusedS fused_split_ptrs(System& s, int in_buf) {
+    const int ob = 1 - in_buf;
+    FusedS P;
+    P.up_in = in_buf ? s.up2 : s.up;
+    P.vm_in = in_buf ? s.vmf2 : s.vmf;
+    P.om_in = in_buf ? s.omf2 : s.omf;
+    P.up_out = ob ? s.up2 : s.up;
+    P.vm_out = ob ? s.vmf2 : s.vmf;
+    P.om_out = ob ? s.omf2 : s.omf;
+    P.fl_in = in_buf ? s.nb_flag2 : s.nb_flag;
+    P.fl_out = ob ? s.nb_flag2 : s.nb_flag;
+    P.h_in = static_cast<const float4*>(in_buf ? s.nb_hist2 : s.nb_hist);
+    P.h_out = static_cast<float4*>(ob ? s.nb_hist2 : s.nb_hist);
+    P.f = s.ff;
+    P.tq = s.tqf;
+    P.u0 = s.u0;
+    P.cell = s.cell;
+    P.vir = s.vir;
+    P.nb_start = s.nb_start;
+    P.nb_j = s.nb_j;
+    P.nb_code = s.nb_code;
+    P.n = s.n;
+    return P;
+}
+
+int batch_split_entry(System& s, void* host_entry) {
+    if (s.precision != TMPS_FP32_SPLIT)
+        return tmps_fail(TMPS_ERR_UNSUPPORTED, "an ensemble runs FP32-split systems");
+    BatchSplit E;
+    E.P[0] = fused_split_ptrs(s, 0);
+    E.P[1] = fused_split_ptrs(s, 1);
+    E.ctl = s.ctl;
+    E.pp = s.pair;
+    E.partial = s.partial;
+    E.vm0 = s.vmf;
+    E.om0 = s.omf;
+    E.f = s.ff;
+    E.tq = s.tqf;
+    E.n = s.n;
+    std::memcpy(host_entry, &E, sizeof E);
+    return 0;
+}
+
+int launch_batch_split_kick(const void* table, const int2* blocks, int nblocks, float dt,
+                            hipStream_t st) {
+    if (nblocks <= 0) return 0;
+    hipLaunchKernelGGL(k_batch_kick_split, dim3(nblocks), dim3(TMPS_BLOCK), 0, st,
+                       static_cast<const BatchSplit*>(table), blocks, dt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_batch_split_step(int style, const void* table, const int2* blocks, int nblocks,
+                            float dt, hipStream_t st) {
+    if (nblocks <= 0) return 0;
+    const BatchSplit* T = static_cast<const BatchSplit*>(table);
+    switch (style) {
+        case TMPS_PAIR_HOOKE_HISTORY:
+            hipLaunchKernelGGL(k_batch_step_split<TMPS_PAIR_HOOKE_HISTORY>, dim3(nblocks),
+                               dim3(TMPS_BLOCK), 0, st, T, blocks, dt);
+            break;
+        case TMPS_PAIR_HERTZ_HISTORY:
+            hipLaunchKernelGGL(k_batch_step_split<TMPS_PAIR_HERTZ_HISTORY>, dim3(nblocks),
+                               dim3(TMPS_BLOCK), 0, st, T, blocks, dt);
+            break;
+        default:
+            hipLaunchKernelGGL(k_batch_step_split<TMPS_PAIR_GRANULAR>, dim3(nblocks),
+                               dim3(TMPS_BLOCK), 0, st, T, blocks, dt);
+            break;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_batch_split_tail(const void* table, const int2* blocks, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return 0;
+    hipLaunchKernelGGL(k_batch_tail_split, dim3(nblocks), dim3(TMPS_BLOCK), 0, st,
+                       static_cast<const BatchSplit*>(table), blocks);
+    HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -71,6 +71,17 @@ struct Ctl {
     int servo_on;       // end-of-step servo enabled for fused steps
     int fast_img[3];    // per dim: list bins >= 3 at the last build, so the
                         // contact kernels may take image counts by reciprocal
+    // an ensemble member's own progress (batch.hip): the buffer set its next
+    // step reads, the step that ends its run, a pending segment-start
+    // half-kick, and whether its run is complete
+    int bpar, bend, bkick, bdone;
+};
+
+// One ensemble member as the batched close kernel sees it.
+struct CloseEntry {
+    Ctl* ctl;
+    const double* partial;
+    int nparts, servo_on, natoms, pad;
 };
 
 // Pair parameters as the kernels read them.
@@ -189,6 +200,10 @@ struct System {
 namespace tmps_api {
 int ctl_pull(System& s);
 int ctl_push(System& s);
+int check_run_error(System& s);
+int ensure_split(System& s);
+int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga);
+int init_simulation(System& s, double dt, int use_cundall, double gl, double ga);
 int reneighbor(System& s, bool force, int* rebuilt);
 int clear_forces(System& s);
 int pair_compute(System& s, double dt, int shearupdate, int accumulate);
@@ -249,6 +264,17 @@ int launch_fused_f64(System& s, int in_buf, double dt, int final_mode, int cunda
 int launch_fused_split(System& s, int in_buf, double dt, int final_mode, int cundall, double gl,
                        double ga);
 int launch_step_close(System& s, double dt, int servo_on);
+
+// ensemble (batch.hip): one launch advances every member that is running
+size_t batch_split_entry_size();
+int batch_split_entry(System& s, void* host_entry);  // fills one table entry
+int launch_batch_split_kick(const void* table, const int2* blocks, int nblocks, float dt,
+                            hipStream_t st);
+int launch_batch_split_step(int style, const void* table, const int2* blocks, int nblocks,
+                            float dt, hipStream_t st);
+int launch_batch_split_tail(const void* table, const int2* blocks, int nblocks, hipStream_t st);
+int launch_batch_close(const CloseEntry* table, int ninst, int max_parts, double dt,
+                       hipStream_t st);
 // the initial_integrate half-kick of a segment's first step, in buffer 0
 int launch_kick_f64(System& s, double dt);
 int launch_kick_split(System& s, double dt);

@@ -18,6 +18,13 @@ Cases:
   triaxial_4m          160 x 125 x 200, a = 1.3, log-normal radii (median 0.5,
                        sigma 0.15, clipped [0.35, 0.7]), seed 1003
   adjoint_32k          32^3, a = 1.1, radii U[0.45, 0.55], seed 1004
+
+Ensembles (ensemble()): 256 members from SeedSequence(1005).spawn(256); member
+k draws, from default_rng(child k), a target count N log-uniform in
+[2048, 65536], kn U[1e5, 1e6], xmu U[0.2, 0.7], target pressure U[0.5, 2.0],
+and a lattice seed; its lattice is nx x ny x nz with nx = ny = round(N^(1/3))
+and nz = round(N / nx^2) (a = 1.1, radii U[0.45, 0.55]), and even members use
+gran/hooke/history, odd ones gran/hertz/history.
 """
 
 from __future__ import annotations
@@ -82,3 +89,23 @@ def case(name: str, dims: tuple[int, int, int] | None = None) -> dict:
     w["style"] = c["style"]
     w["steps"] = c["steps"]
     return w
+
+
+def ensemble(count: int = 256, seed: int = 1005, n_min: int = 2048, n_max: int = 65536) -> list:
+    """The ensemble members' atoms and settings (see the module docstring).
+    Each entry is a lattice() dict plus style, kn, xmu and p_target."""
+    out = []
+    for k, child in enumerate(np.random.SeedSequence(seed).spawn(count)):
+        rng = np.random.default_rng(child)
+        target = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        kn = float(rng.uniform(1.0e5, 1.0e6))
+        xmu = float(rng.uniform(0.2, 0.7))
+        p_target = float(rng.uniform(0.5, 2.0))
+        lat_seed = int(rng.integers(0, 2**31 - 1))
+        nx = max(2, int(round(target ** (1.0 / 3.0))))
+        nz = max(2, int(round(target / (nx * nx))))
+        w = lattice(nx, nx, nz, 1.1, lat_seed, "uniform")
+        w.update(name=f"ensemble_{k}", style="hooke/history" if k % 2 == 0 else "hertz/history",
+                 kn=kn, xmu=xmu, p_target=p_target)
+        out.append(w)
+    return out
