@@ -227,6 +227,126 @@ def adjoint_c4(steps=2000, every=100, fd_rel=1e-6):
     }
 
 
+def build_member(w, precision):
+    """An ensemble member: its lattice, pair style and coefficients (kt = 2/7 kn,
+    gamman 50, gammat 25), nve/sphere and deform/pressure at its own target."""
+    import taichimps_amd as tm
+    from taichimps_amd.workloads import ISOTROPIC as C
+    n = len(w["x"])
+    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "p"))
+    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=C["skin"], max_atoms=n, max_neighbors_per_atom=64)
+    hist = tm.ContactHistory(n, 64)
+    cls = tm.GranHookeHistory if w["style"] == "hooke/history" else tm.GranHertzHistory
+    pair = cls(dom, w["kn"], C["gamman"], 2.0 / 7.0 * w["kn"], C["gammat"], w["xmu"])
+    sim = tm.Simulation(dom, atom, nl, hist, pair, dt=C["dt"])
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixDeformPressure(dom, p_target=w["p_target"], p_gain=C["p_gain"]))
+    return sim
+
+
+def ensemble_c5(steps=500, n_grad=8, every=100):
+    """Ensemble of 256 packings (taichimps_amd.workloads.ensemble: N log-uniform
+    in [2048, 65536], per-member kn, xmu, target P, hooke/hertz alternating),
+    FP32-split, `steps` steps after each member's setup force. Times the
+    ensemble run (its autotuner's probe segments included) and the same
+    members run one Simulation at a time, checks that every member's final
+    state is bitwise the same both ways, and computes per-member adjoint
+    gradients of the final mean pressure for `n_grad` members spread over the
+    size range (FP64 twins: the adjoint engine differentiates the FP64 step)."""
+    import torch
+    from taichimps_amd import adjoint as ADJ
+    from taichimps_amd import workloads
+    from taichimps_amd.ensemble import Ensemble
+    from taichimps_amd.workloads import ISOTROPIC as C
+    ws = workloads.ensemble()
+    natoms = int(sum(len(w["x"]) for w in ws))
+
+    def setup(sims):
+        for s in sims:
+            s.run(0)
+            s.run_pre = False
+
+    sims = [build_member(w, "f32split") for w in ws]
+    ens = Ensemble(sims)
+    ens.run(0)
+    for s in sims:
+        s.run_pre = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps, autotune=True)
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    tuning = ens.tuning()
+    print(f"# ensemble: {natoms} atoms, {steps} steps in {t_ens:.3f} s, {tuning}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ens.run(steps)
+    torch.cuda.synchronize()
+    t_ens2 = time.perf_counter() - t0
+    redone2 = ens.tuning()["redone_steps"]
+
+    solo = [build_member(w, "f32split") for w in ws]
+    setup(solo)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solo:
+        s.run(steps)
+    torch.cuda.synchronize()
+    t_solo = time.perf_counter() - t0
+    print(f"# one at a time: {t_solo:.3f} s", file=sys.stderr, flush=True)
+    for s in solo:
+        s.run(steps)
+    differ = 0
+    for a, b in zip(sims, solo):
+        xa, xb = a.atom.x.to_numpy(), b.atom.x.to_numpy()
+        va, vb = a.atom.v.to_numpy(), b.atom.v.to_numpy()
+        if not (np.array_equal(xa, xb) and np.array_equal(va, vb)
+                and np.array_equal(a.domain.boxhi, b.domain.boxhi)):
+            differ += 1
+    rebuilds = int(sum(s.neighbor.build_count for s in sims))
+    del solo
+
+    order = np.argsort([len(w["x"]) for w in ws])
+    pick = order[np.linspace(0, len(ws) - 1, n_grad).round().astype(int)]
+    mean_p = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0]) / 3.0
+    grads = []
+    t0 = time.perf_counter()
+    for k in pick:
+        w = ws[k]
+        build = ADJ.isotropic_builder(w, style=w["style"], skin=C["skin"], dt=C["dt"],
+                                      p_target=w["p_target"], p_gain=C["p_gain"])
+        theta = np.array([w["kn"], 2.0 / 7.0 * w["kn"], C["gamman"], C["gammat"], w["xmu"]])
+        sim = build(theta, w["radius"])
+        tensor, _ = ADJ.run_forward(sim, steps, every)
+        gc, gr, _ = ADJ.run_backward(sim, mean_p)
+        grads.append({"member": int(k), "atoms": len(w["x"]), "style": w["style"],
+                      "P": float(mean_p @ tensor),
+                      "dP_dkn": float(gc[0]), "dP_dxmu": float(gc[4]),
+                      "dP_dgamman": float(gc[2]),
+                      "dP_dradius_norm": float(np.linalg.norm(gr))})
+    t_grad = time.perf_counter() - t0
+    print(f"# gradients of {len(pick)} members: {t_grad:.1f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": f"{len(ws)} periodic packings, {natoms} atoms (N log-uniform in "
+                    "[2048, 65536]), hooke/hertz history alternating, per-member kn, xmu, "
+                    f"target P; FP32-split, {steps} steps",
+        "atoms": natoms, "members": len(ws), "steps": steps,
+        "atom_steps_per_s": natoms * steps / t_ens,
+        "ms_per_step": t_ens / steps * 1e3,
+        "tuned_rerun_atom_steps_per_s": natoms * steps / t_ens2,
+        "one_at_a_time_atom_steps_per_s": natoms * steps / t_solo,
+        "speedup_vs_one_at_a_time": t_solo / t_ens,
+        "members_bitwise_equal_single_runs": len(ws) - differ,
+        "tuning": tuning, "rerun_redone_steps": redone2, "list_builds": rebuilds,
+        "gradients": {"members": grads, "steps": steps, "checkpoint_every": every,
+                      "seconds": t_grad,
+                      "note": "adjoint of each member's FP64 twin (same inputs, FP64 step)"},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +358,10 @@ def main(argv=None):
                     help="skip the adjoint calibration block (32k spheres, 2000 steps)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the adjoint calibration block and print it")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble block (256 packings, 500 steps)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the ensemble block and print it")
     ap.add_argument("--dims", type=int, nargs=3, default=None,
                     help="lattice dimensions (default 100 100 100)")
     ap.add_argument("--precision", choices=("both", "f64", "f32split"), default="both",
@@ -252,6 +376,9 @@ def main(argv=None):
         return copy_calibration(args.copy_calibration)
     if args.adjoint_only:
         print(json.dumps(adjoint_c4()))
+        return 0
+    if args.ensemble_only:
+        print(json.dumps(ensemble_c5()))
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -329,6 +456,8 @@ def main(argv=None):
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_c4()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_c5()
     if cpu is not None:
         out["mean_pressure_rel_err_vs_cpu_oracle"] = (
             abs(head["mean_pressure"] - cpu["mean_pressure"]) / abs(cpu["mean_pressure"]))

@@ -20,8 +20,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "tmps_internal.h"
@@ -66,6 +70,8 @@ struct tmps_batch_impl {
     int config = -1;  // chosen launch configuration (-1: not tuned yet)
     double config_rate[kNumConfigs] = {};  // measured atom-steps per second
     int rebuilds = 0;
+    // host threads that redo members after rebuild requests
+    int redo_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
 };
 
 namespace {
@@ -297,7 +303,6 @@ int redo_member(Batch& B, int k, double dt) {
     s.timestep = s.hctl->timestep;
     rc = tmps_api::redo_step(s, s.hctl->bpar, dt, 0, 0.0, 0.0);
     if (rc) return rc;
-    B.rebuilds += 1;
     Ctl& H = *s.hctl;
     H.bpar = 0;
     H.bkick = 1;
@@ -305,6 +310,44 @@ int redo_member(Batch& B, int k, double dt) {
     rc = tmps_api::ctl_push(s);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+// Redoes the members in `which`, several at a time: each redo is a chain of
+// small kernels and host round trips on the member's own stream, so host
+// threads (up to `threads`) keep several members' chains in flight at once.
+// The first failure (in member order) is reported.
+int redo_members(Batch& B, const std::vector<int>& which, double dt, int threads) {
+    const int m = (int)which.size();
+    if (m == 0) return 0;
+    const int nt = std::max(1, std::min(threads, m));
+    if (nt == 1) {
+        for (int k : which) {
+            int rc = redo_member(B, k, dt);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::vector<int> rcs(m, 0);
+    std::vector<std::string> msgs(m);
+    auto work = [&]() {
+        for (int q = next.fetch_add(1); q < m && !failed.load(); q = next.fetch_add(1)) {
+            const int rc = redo_member(B, which[q], dt);
+            if (rc) {
+                rcs[q] = rc;
+                msgs[q] = tmps_last_error();
+                failed.store(true);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (int q = 0; q < m; ++q)
+        if (rcs[q]) return tmps_fail(rcs[q], msgs[q]);
     return 0;
 }
 
@@ -344,7 +387,7 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
     int probe = (autotune || B.config < 0) ? 0 : kNumConfigs;
     double best_rate = -1.0;
     int best = B.config >= 0 ? B.config : 2;
-    std::vector<int> prev(ninst);
+    std::vector<int> prev(ninst), redo;
     for (int k = 0; k < ninst; ++k) prev[k] = B.sys[k]->timestep;
     bool running = nsteps > 0;
     while (running) {
@@ -359,25 +402,27 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         running = false;
         double atom_steps = 0.0;
-        bool any_redo = false;
+        redo.clear();
         for (int k = 0; k < ninst; ++k) {
             const int4 st = B.h_status[k];
             atom_steps += (double)B.sys[k]->n * (st.x - prev[k]);
             prev[k] = st.x;
-            if (st.y || st.z) {
-                rc = redo_member(B, k, dt);
-                if (rc) return rc;
-                // the rebuild may have moved the member's list buffers
-                rc = write_entry(B, k);
-                if (rc) return rc;
-                any_redo = true;
-                prev[k] = B.sys[k]->timestep;
-                if (!B.sys[k]->hctl->bdone) running = true;
-            } else if (!st.w) {
+            if (st.y || st.z)
+                redo.push_back(k);
+            else if (!st.w)
                 running = true;
-            }
         }
-        if (any_redo) HIP_TRY(hipStreamSynchronize(B.stream));
+        rc = redo_members(B, redo, dt, B.redo_threads);
+        if (rc) return rc;
+        B.rebuilds += (int)redo.size();
+        for (int k : redo) {
+            // the rebuild may have moved the member's list buffers
+            rc = write_entry(B, k);
+            if (rc) return rc;
+            prev[k] = B.sys[k]->timestep;
+            if (!B.sys[k]->hctl->bdone) running = true;
+        }
+        if (!redo.empty()) HIP_TRY(hipStreamSynchronize(B.stream));
         if (probe < kNumConfigs) {
             const double rate = sec > 0.0 ? atom_steps / sec : 0.0;
             B.config_rate[ci] = rate;
