@@ -113,7 +113,9 @@ int ensure_cells(System& s, int ncell) {
     dfree(s.cell_count);
     dfree(s.cell_start);
     dfree(s.cell_fill);
-    int cap = ncell + 1;
+    // slack, so that a box that grows a little under the servo does not
+    // reallocate at a rebuild
+    int cap = ncell + ncell / 4 + 1;
     int rc = dalloc(&s.cell_count, cap);
     if (!rc) rc = dalloc(&s.cell_start, cap);
     if (!rc) rc = dalloc(&s.cell_fill, cap);
@@ -185,7 +187,9 @@ int split_positions_to_xr(System& s) {
 
 // ------------------------------------------------------------ neighbor build
 
-int neigh_build(System& s) {
+// box_known: the host mirror of the control block is current (the caller
+// pulled it and nothing since changed the box), so no round trip is needed.
+int neigh_build(System& s, bool box_known = false) {
     if (s.n <= 0) return 0;
     int rc = 0;
     if (split(s)) {
@@ -193,8 +197,10 @@ int neigh_build(System& s) {
         if (rc) return rc;
         s.x64_valid = true;
     }
-    rc = ctl_pull(s);
-    if (rc) return rc;
+    if (!box_known) {
+        rc = ctl_pull(s);
+        if (rc) return rc;
+    }
     const DevBox B = s.hctl->box;
     const double maxcut = 2.0 * max_radius(s);
     setup_grid(s, B, maxcut);
@@ -232,17 +238,30 @@ int neigh_build(System& s) {
     s.has_built = 1;
     s.build_count += 1;
     s.last_build_step = s.current_step;
-    rc = ctl_pull(s);
-    if (rc) return rc;
-    s.hctl->last_build_step = s.last_build_step;
-    for (int k = 0; k < 3; ++k) s.hctl->fast_img[k] = s.grid.g[k] >= 3 ? 1 : 0;
-    if (split(s)) {
-        if (s.grid.g[0] > 2047 || s.grid.g[1] > 2047 || s.grid.g[2] > 1023)
-            return tmps_fail(TMPS_ERR_UNSUPPORTED,
-                             "FP32-split encoding holds at most 2047 x 2047 x 1023 bins");
-        for (int k = 0; k < 3; ++k) s.hctl->sc.gi[k] = s.grid.g[k];
+    // the copy of the list the next rebuild saves for the history remap,
+    // allocated with the list (not inside a later rebuild)
+    if (!s.onb_j && s.nb_cap > 0) {
+        const size_t hsz = split(s) ? sizeof(float4) : sizeof(double4);
+        HIP_TRY(hipMalloc(&s.onb_j, s.nb_cap * sizeof(int)));
+        HIP_TRY(hipMalloc(&s.onb_flag, s.nb_cap));
+        HIP_TRY(hipMalloc(&s.onb_hist, s.nb_cap * hsz));
+        s.onb_cap = s.nb_cap;
     }
-    rc = ctl_push(s);
+    // the build's record in the control block (device and host mirror)
+    int fast[3], gi[3];
+    for (int k = 0; k < 3; ++k) {
+        fast[k] = s.grid.g[k] >= 3 ? 1 : 0;
+        gi[k] = s.grid.g[k];
+    }
+    if (split(s) && (s.grid.g[0] > 2047 || s.grid.g[1] > 2047 || s.grid.g[2] > 1023))
+        return tmps_fail(TMPS_ERR_UNSUPPORTED,
+                         "FP32-split encoding holds at most 2047 x 2047 x 1023 bins");
+    s.hctl->last_build_step = s.last_build_step;
+    for (int k = 0; k < 3; ++k) {
+        s.hctl->fast_img[k] = fast[k];
+        if (split(s)) s.hctl->sc.gi[k] = gi[k];
+    }
+    rc = launch_ctl_built(s, s.last_build_step, fast, gi, split(s) ? 1 : 0);
     if (rc) return rc;
     if (split(s)) {
         rc = launch_split_consts(s, 1);
@@ -289,11 +308,15 @@ int history_restore(System& s) {
     return rc;
 }
 
-int pbc(System& s) {
+// ctl_fresh: the host mirror of the control block is current.
+int pbc(System& s, bool ctl_fresh = false) {
     if (s.n <= 0) return 0;
     if (split(s)) {
-        int rc = ctl_pull(s);
-        if (rc) return rc;
+        if (!ctl_fresh) {
+            int rc = ctl_pull(s);
+            if (rc) return rc;
+        }
+        int rc = 0;
         if (s.hctl->box.vremap)
             return tmps_fail(TMPS_ERR_UNSUPPORTED, "remap v is not supported with FP32-split");
         rc = split_positions_to_xr(s);
@@ -643,16 +666,19 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
         rc = launch_copy_state(s, 0);
         if (rc) return rc;
     }
+    // The caller pulled the control block after the abort and nothing below
+    // changes the box before the build, so the host mirror serves the PBC
+    // wrap and the grid; the abort is cleared on the device.
     s.hctl->abort = 0;
     s.hctl->trigger = -1;
-    rc = ctl_push(s);
+    rc = launch_ctl_resume(s);
     if (rc) return rc;
     rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
     if (rc) return rc;
     s.current_step = stp;
     rc = history_save(s);
-    if (!rc) rc = pbc(s);
-    if (!rc) rc = neigh_build(s);
+    if (!rc) rc = pbc(s, true);
+    if (!rc) rc = neigh_build(s, true);
     if (!rc) rc = history_restore(s);
     if (!rc) rc = ensure_split(s);
     if (rc) return rc;

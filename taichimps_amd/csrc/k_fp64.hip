@@ -610,6 +610,30 @@ __global__ void k_ctl_rebuilt(Ctl* ctl, int step) {
     }
 }
 
+// A step cut short by a rebuild request is about to be redone.
+__global__ void k_ctl_resume(Ctl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->trigger = -1;
+        ctl->abort = 0;
+    }
+}
+
+// What a list build records in the control block: its step, per axis whether
+// it has at least 3 bins, and (FP32-split) the bins of the encoding.
+__global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int set_gi) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->last_build_step = step;
+        ctl->fast_img[0] = fast.x;
+        ctl->fast_img[1] = fast.y;
+        ctl->fast_img[2] = fast.z;
+        if (set_gi) {
+            ctl->sc.gi[0] = gi.x;
+            ctl->sc.gi[1] = gi.y;
+            ctl->sc.gi[2] = gi.z;
+        }
+    }
+}
+
 // ---------------------------------------------------- fused step kernels
 
 // Start of step t: the pending end-of-step remap of step t-1, the
@@ -1228,6 +1252,20 @@ int launch_remap_clear(System& s, int honor_abort) {
 
 int launch_ctl_rebuilt(System& s, int step) {
     hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ctl_resume(System& s) {
+    hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, s.stream, s.ctl);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int set_gi) {
+    hipLaunchKernelGGL(k_ctl_built, dim3(1), dim3(64), 0, s.stream, s.ctl, step,
+                       make_int3(fast[0], fast[1], fast[2]), make_int3(gi[0], gi[1], gi[2]),
+                       set_gi);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -289,6 +289,10 @@ int launch_remap_apply(System& s);
 // the redo of that step still needs it)
 int launch_remap_clear(System& s, int honor_abort = 0);
 int launch_ctl_rebuilt(System& s, int step);
+// clears trigger and abort before a redo
+int launch_ctl_resume(System& s);
+// records a list build (last_build_step, fast_img, and sc.gi when set_gi)
+int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int set_gi);
 int launch_remap_apply_fused(System& s);
 void drop_graphs(System& s);
 int launch_final_tree(System& s, int nc, int nparts);
