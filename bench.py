@@ -68,8 +68,62 @@ def model_bytes(key, n_slot, n_touch):
     return base + per_slot * n_slot + per_touch * n_touch
 
 
-def run_precision(w, precision, steps, warmup, prof_steps, direct=False):
-    sim = build_sim(w, precision)
+def build_triaxial(w, precision):
+    """The triaxial test: PairGranular hertz/material E = 1e7, damp 0.5,
+    nu 0.3, viscoelastic damping, mindlin (NULL, 1.0, 0.5); damping/cundall
+    0.1 0.1; x and y pressure servo P = 1.0 (gain 1e-4, couple none), z erate
+    -1e-3; dt 1e-4, skin 0.1."""
+    import taichimps_amd as tm
+    n = len(w["x"])
+    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "p"))
+    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+    hist = tm.ContactHistory(n, 64)
+    pair = tm.PairGranular(dom, normal="hertz/material", normal_coeffs=[1e7, 0.5, 0.3],
+                           tangential="mindlin", tangential_coeffs=[None, 1.0, 0.5],
+                           damping="viscoelastic")
+    sim = tm.Simulation(dom, atom, nl, hist, pair, dt=1e-4)
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixDampingCundall(dom, 0.1, 0.1))
+    sim.add_fix(tm.FixDeformPressure(dom, axes={
+        "x": {"style": "pressure", "ptarget": 1.0, "pgain": 1e-4},
+        "y": {"style": "pressure", "ptarget": 1.0, "pgain": 1e-4},
+        "z": {"style": "erate", "rate": -1e-3}}))
+    return sim
+
+
+def triaxial_c3(steps=1000, warmup=10):
+    """Triaxial test on 4,000,000 polydisperse spheres (160 x 125 x 200
+    lattice, a = 1.3, log-normal radii, seed 1003; build_triaxial), FP32-split
+    and FP64, `steps` timed steps after `warmup`. Rolling resistance is not
+    part of it: the reference's PairGranular raises on rolling models."""
+    from taichimps_amd import workloads
+    w = workloads.case("triaxial_4m")
+    out = {}
+    for prec in ("f32split", "f64"):
+        r = run_precision(w, prec, steps, warmup, 50, builder=build_triaxial)
+        r.pop("x")
+        out[prec] = r
+        print(f"# triaxial {prec}: {r['ms_per_step']:.4f} ms/step, "
+              f"{r['atom_steps_per_s']:.4e} atom-steps/s, kernels {r['kernel_ms']}, "
+              f"rebuilds {r['rebuilds']}", file=sys.stderr, flush=True)
+    ps, pd = np.array(out["f32split"]["pressure"]), np.array(out["f64"]["pressure"])
+    return {
+        "workload": "triaxial, 4,000,000 polydisperse spheres, PairGranular hertz/material + "
+                    "viscoelastic + mindlin, damping/cundall 0.1, x/y pressure 1.0, z erate "
+                    f"-1e-3, {steps} steps",
+        "atoms": out["f32split"]["n"],
+        "f32split": out["f32split"], "f64": out["f64"],
+        "speedup_fp32split_vs_fp64": out["f32split"]["atom_steps_per_s"]
+        / out["f64"]["atom_steps_per_s"],
+        "pressure_xyz_rel_err_vs_fp64": (np.abs(ps[:3] - pd[:3]) / np.abs(pd[:3])).tolist(),
+    }
+
+
+def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder=None):
+    sim = (builder or build_sim)(w, precision)
     N = sim.atom._native
     N.set_launch_mode(direct)
     n = sim.atom.nlocal
@@ -394,6 +448,10 @@ def main(argv=None):
                     help="only run the adjoint calibration block and print it")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble block (256 packings, 500 steps)")
+    ap.add_argument("--triaxial", action="store_true",
+                    help="also run the 4M-sphere triaxial block (FP32-split and FP64)")
+    ap.add_argument("--triaxial-only", action="store_true",
+                    help="only run the triaxial block and print it")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the ensemble block and print it")
     ap.add_argument("--dims", type=int, nargs=3, default=None,
@@ -413,6 +471,9 @@ def main(argv=None):
         return 0
     if args.ensemble_only:
         print(json.dumps(ensemble_c5()))
+        return 0
+    if args.triaxial_only:
+        print(json.dumps(triaxial_c3()))
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -492,6 +553,8 @@ def main(argv=None):
         out["adjoint"] = adjoint_c4()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_c5()
+    if args.triaxial:
+        out["triaxial"] = triaxial_c3()
     if cpu is not None:
         out["mean_pressure_rel_err_vs_cpu_oracle"] = (
             abs(head["mean_pressure"] - cpu["mean_pressure"]) / abs(cpu["mean_pressure"]))
