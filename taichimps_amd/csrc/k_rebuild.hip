@@ -192,15 +192,12 @@ __device__ __forceinline__ double list_image(double d, double prd, double inv_pr
 #define LIST_ROW_CAP 32
 
 // The stencil walk of NeighborList.build_neighbor_list (neighbor.py:315-369),
-// visiting bins in the reference's (dx, dy, dz) order. MODE 0 counts the full
-// row and the half row (j > i); MODE 1 writes the full row at nb_start[i];
-// MODE 2 counts and writes the row into a per-atom scratch row of `cap`
-// entries (flags[1] is raised when a row does not fit). Rows end up
-// ascending in j.
-template <int MODE>
-__global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const int* cell_start,
-                       const int* particles, double skin, int max_nb, int* full_count,
-                       int* half_count, int* flags, const int* nb_start, int* nb_j, int cap) {
+// visiting bins in the reference's (dx, dy, dz) order, writing each atom's
+// full row at nb_start[i] and sorting it ascending in j: the path for lists
+// with a row longer than k_list_rows' scratch (whose counts gave nb_start).
+__global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G,
+                            const int* cell_start, const int* particles, double skin,
+                            const int* nb_start, int* nb_j) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const DevBox& B = ctl->box;
@@ -212,10 +209,8 @@ __global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const 
     const double prd[3] = {B.prd[0], B.prd[1], B.prd[2]};
     const double inv[3] = {1.0 / prd[0], 1.0 / prd[1], 1.0 / prd[2]};
     const bool fast[3] = {G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3};
-    int nfull = 0, nhalf = 0;
-    int* row = nullptr;
-    if (MODE == 1) row = nb_j + nb_start[i];
-    if (MODE == 2) row = nb_j + (size_t)i * cap;
+    int nfull = 0;
+    int* row = nb_j + nb_start[i];
     for (int dx = -1; dx <= 1; ++dx) {
         if (px == 1 && gx == 1 && dx != 0) continue;
         int nx = c[0] + dx;
@@ -243,34 +238,19 @@ __global__ void k_list(const double4* xr, int n, const Ctl* ctl, GridP G, const 
                     if (pz == 1) d2 = list_image(d2, prd[2], inv[2], fast[2]);
                     const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
                     const double rad_sum = pi.w + pj.w + skin;
-                    if (rsq < rad_sum * rad_sum) {
-                        if (MODE == 1 || (MODE == 2 && nfull < cap)) row[nfull] = j;
-                        ++nfull;
-                        if (j > i) ++nhalf;
-                    }
+                    if (rsq < rad_sum * rad_sum) row[nfull++] = j;
                 }
             }
         }
     }
-    if (MODE != 1) {
-        full_count[i] = nfull;
-        half_count[i] = nhalf;
-        if (nhalf > max_nb) flags[0] = 1;
-        if (MODE == 2 && nfull > cap) {
-            flags[1] = 1;
-            return;
+    for (int p = 1; p < nfull; ++p) {
+        const int v = row[p];
+        int q = p - 1;
+        while (q >= 0 && row[q] > v) {
+            row[q + 1] = row[q];
+            --q;
         }
-    }
-    if (MODE != 0) {
-        for (int p = 1; p < nfull; ++p) {
-            const int v = row[p];
-            int q = p - 1;
-            while (q >= 0 && row[q] > v) {
-                row[q + 1] = row[q];
-                --q;
-            }
-            row[q + 1] = v;
-        }
+        row[q + 1] = v;
     }
 }
 
@@ -333,7 +313,7 @@ __device__ __forceinline__ void list_range(int a, int b, int i, const double4& p
 // periodic x boundary). The row is kept sorted in LDS and written, slot-major,
 // to rows[k * n + i] (k < LIST_ROW_CAP); full and half counts go to
 // full_count / half_count, flags[0] is raised past max_nb half entries and
-// flags[1] when a row exceeds LIST_ROW_CAP (k_list<1> then writes the rows
+// flags[1] when a row exceeds LIST_ROW_CAP (k_list_fill then writes the rows
 // directly).
 __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
                                                           GridP G, const int* cell_start,
@@ -633,10 +613,9 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
             hipLaunchKernelGGL(k_list_compact, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                                (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j);
         } else {
-            hipLaunchKernelGGL(k_list<1>, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
-                               s.ctl, s.grid, s.cell_start, s.cell_particles, s.skin, s.max_nb,
-                               s.full_count, s.half_count, s.flags, (const int*)s.nb_start, s.nb_j,
-                               0);
+            hipLaunchKernelGGL(k_list_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
+                               s.ctl, s.grid, s.cell_start, s.cell_particles, s.skin,
+                               (const int*)s.nb_start, s.nb_j);
         }
         HIP_TRY(hipGetLastError());
     }

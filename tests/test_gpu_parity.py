@@ -54,6 +54,27 @@ def test_bins_and_half_list_bitexact():
     assert sim.neighbor.npairs[None] == len(pi)
 
 
+def test_long_rows_take_the_fill_path_bitexact():
+    """A skin wide enough that rows exceed the list kernel's 32-entry scratch
+    (about 45 partners per atom): the list is rebuilt by the stencil-walk fill
+    kernel and still equals the oracle's, and a run over it stays bitwise."""
+    w = dense(9, 1.0, seed=9)
+    sim = H.device_sim(w, servo=False, skin=1.2)
+    sim.neighbor.build(sim.atom)
+    o = H.oracle_sim(w, servo=False, skin=1.2)
+    assert o.nlist_build() == 0
+    pi, pj, _, _ = o.pairs()
+    counts = np.bincount(np.concatenate([pi, pj]), minlength=len(w["x"]))
+    assert counts.max() > 32
+    assert np.array_equal(sim.neighbor.pair_i.to_numpy(), pi)
+    assert np.array_equal(sim.neighbor.pair_j.to_numpy(), pj)
+    sim2 = H.device_sim(w, style="hertz/history", skin=1.2)
+    o2 = H.oracle_sim(w, style="hertz/history", skin=1.2)
+    sim2.run(20)
+    assert o2.run(20, 1e-4) == 0
+    _assert_state_equal(H.device_state(sim2), H.oracle_state(o2))
+
+
 def test_capped_grid_bitexact():
     # a box whose uncapped grid exceeds max_grid_cells, as the 1M-atom case does
     w = dense(10, 1.1, seed=5)
