@@ -150,3 +150,27 @@ def test_workload_generator_is_seeded_and_matches_spec():
     np.testing.assert_allclose(w1["x"], (site + 0.5) * 1.1 + jitter, rtol=0, atol=1e-12)
     assert np.array_equal(w1["radius"], radii)
     assert np.array_equal(w1["tag"], np.arange(1, 8001))
+
+
+def test_ensemble_workload_is_seeded_and_matches_spec():
+    """256 members from SeedSequence(1005).spawn(256): N log-uniform in
+    [2048, 65536] (lattice-rounded), kn, xmu and target P in their ranges,
+    hooke and hertz alternating."""
+    e1 = workloads.ensemble()
+    e2 = workloads.ensemble()
+    assert len(e1) == 256
+    for a, b in zip(e1, e2):
+        assert np.array_equal(a["x"], b["x"]) and a["kn"] == b["kn"]
+    ns = np.array([len(w["x"]) for w in e1])
+    assert ns.min() >= 1800 and ns.max() <= 70000
+    # log-uniform: about half of the members below the geometric midpoint
+    assert 0.35 < np.mean(ns < np.sqrt(2048 * 65536)) < 0.65
+    assert all(1e5 <= w["kn"] <= 1e6 and 0.2 <= w["xmu"] <= 0.7
+               and 0.5 <= w["p_target"] <= 2.0 for w in e1)
+    assert [w["style"] for w in e1[:4]] == ["hooke/history", "hertz/history"] * 2
+    # member k's draws come from its own spawned generator
+    rng = np.random.default_rng(np.random.SeedSequence(1005).spawn(256)[3])
+    target = np.exp(rng.uniform(np.log(2048), np.log(65536)))
+    assert e1[3]["kn"] == rng.uniform(1.0e5, 1.0e6)
+    nx = int(round(target ** (1.0 / 3.0)))
+    assert len(e1[3]["x"]) == nx * nx * max(2, int(round(target / (nx * nx))))

@@ -41,7 +41,9 @@ def test_header_declares_the_expected_entry_points():
     fns = header_functions()
     assert len(fns) >= 40
     for must in ("tmps_create", "tmps_neigh_build", "tmps_pair_compute", "tmps_nve_initial",
-                 "tmps_virial_reduce", "tmps_servo_end_of_step", "tmps_run"):
+                 "tmps_virial_reduce", "tmps_servo_end_of_step", "tmps_run",
+                 "tmps_adjoint_forward", "tmps_adjoint_backward", "tmps_batch_create",
+                 "tmps_batch_run", "tmps_batch_tuning", "tmps_batch_destroy"):
         assert must in fns
 
 
