@@ -228,15 +228,18 @@ int tmps_adjoint_backward(tmps_system* s, const double dL_dtensor[6], double gra
  * tmps_batch_run is tmps_run(nsteps, dt, run_pre) on every member. With
  * `autotune` (and always on the first run) its first segments time each
  * launch configuration (segment length 8/16/32, members' blocks in sequence
- * or grouped per XCD); tmps_batch_tuning reports the chosen one, each
- * configuration's measured rate (atom-steps/s, 6 entries) and the number of
- * member steps the last run redid after a rebuild request. */
+ * or grouped per XCD) and its first segments with at least 8 member redos
+ * time each number of redo threads (4, 8, 16); tmps_batch_tuning reports the
+ * chosen configuration, each configuration's measured rate (atom-steps/s, 6
+ * entries), the number of member steps the last run redid after a rebuild
+ * request, the chosen redo thread count and each count's wall ms per redo
+ * (3 entries; 0 where not measured). */
 typedef struct tmps_batch tmps_batch;
 int tmps_batch_create(tmps_system* const* systems, int n, tmps_batch** out);
 int tmps_batch_destroy(tmps_batch* b);
 int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotune);
 int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
-                      int* rebuilds);
+                      int* rebuilds, int* redo_threads, double redo_ms[3]);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* How tmps_run issues a segment's kernels: replayed from a captured HIP graph

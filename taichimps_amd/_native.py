@@ -160,7 +160,7 @@ def load(path: str | None = None):
         "tmps_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
         "tmps_batch_destroy": (C.c_int, [vp]),
         "tmps_batch_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
-        "tmps_batch_tuning": (C.c_int, [vp, ip, ip, dp, ip]),
+        "tmps_batch_tuning": (C.c_int, [vp, ip, ip, dp, ip, ip, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -449,12 +449,13 @@ class NativeBatch:
 
     def tuning(self):
         """(segment length, block order, atom-steps/s per configuration,
-        member steps redone after rebuild requests in the last run)."""
-        seg, order, reb = C.c_int(0), C.c_int(0), C.c_int(0)
-        rates = np.zeros(6)
+        member steps redone after rebuild requests in the last run, redo
+        threads, wall ms per redo for 4/8/16 threads)."""
+        seg, order, reb, thr = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        rates, redo_ms = np.zeros(6), np.zeros(3)
         check(self._L.tmps_batch_tuning(self._h, C.byref(seg), C.byref(order), _dp(rates),
-                                        C.byref(reb)))
-        return seg.value, order.value, rates, reb.value
+                                        C.byref(reb), C.byref(thr), _dp(redo_ms)))
+        return seg.value, order.value, rates, reb.value, thr.value, redo_ms
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -47,6 +47,12 @@ struct Config {
 const Config kConfigs[] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {32, 0}, {32, 1}};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
+// candidate numbers of redo threads
+const int kRedoThreads[] = {4, 8, 16};
+constexpr int kNumRedoThreads = sizeof(kRedoThreads) / sizeof(kRedoThreads[0]);
+// a segment is a redo-thread probe only if it has at least this many redos
+constexpr int kRedoProbeMin = 8;
+
 }  // namespace
 
 struct tmps_batch_impl {
@@ -70,8 +76,12 @@ struct tmps_batch_impl {
     int config = -1;  // chosen launch configuration (-1: not tuned yet)
     double config_rate[kNumConfigs] = {};  // measured atom-steps per second
     int rebuilds = 0;
-    // host threads that redo members after rebuild requests
-    int redo_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    // host threads that redo members after rebuild requests: the autotuner
+    // times each candidate count on a segment with enough redos and keeps the
+    // fastest (more threads keep more members' redo chains in flight, but
+    // they contend for the runtime and the hardware queues)
+    int redo_threads = 8;
+    double redo_ms[kNumRedoThreads] = {};  // measured wall ms per redo
 };
 
 namespace {
@@ -387,6 +397,7 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
     int probe = (autotune || B.config < 0) ? 0 : kNumConfigs;
     double best_rate = -1.0;
     int best = B.config >= 0 ? B.config : 2;
+    int tprobe = (autotune || B.redo_ms[0] == 0.0) ? 0 : kNumRedoThreads;
     std::vector<int> prev(ninst), redo;
     for (int k = 0; k < ninst; ++k) prev[k] = B.sys[k]->timestep;
     bool running = nsteps > 0;
@@ -412,8 +423,23 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
             else if (!st.w)
                 running = true;
         }
-        rc = redo_members(B, redo, dt, B.redo_threads);
+        const bool tprobe_now = tprobe < kNumRedoThreads && (int)redo.size() >= kRedoProbeMin;
+        const int threads = tprobe_now ? kRedoThreads[tprobe] : B.redo_threads;
+        const auto r0 = std::chrono::steady_clock::now();
+        rc = redo_members(B, redo, dt, threads);
         if (rc) return rc;
+        if (tprobe_now) {
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0)
+                    .count();
+            B.redo_ms[tprobe] = ms / (double)redo.size();
+            if (++tprobe == kNumRedoThreads) {
+                int bt = 0;
+                for (int q = 1; q < kNumRedoThreads; ++q)
+                    if (B.redo_ms[q] < B.redo_ms[bt]) bt = q;
+                B.redo_threads = kRedoThreads[bt];
+            }
+        }
         B.rebuilds += (int)redo.size();
         for (int k : redo) {
             // the rebuild may have moved the member's list buffers
@@ -508,7 +534,7 @@ int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotu
 }
 
 int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
-                      int* rebuilds) {
+                      int* rebuilds, int* redo_threads, double redo_ms[3]) {
     if (!b) return tmps_fail(TMPS_ERR_ARG, "null ensemble");
     Batch& B = *reinterpret_cast<Batch*>(b);
     const int c = B.config >= 0 ? B.config : 2;
@@ -517,6 +543,9 @@ int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rate
     if (rates)
         for (int k = 0; k < kNumConfigs; ++k) rates[k] = B.config_rate[k];
     if (rebuilds) *rebuilds = B.rebuilds;
+    if (redo_threads) *redo_threads = B.redo_threads;
+    if (redo_ms)
+        for (int k = 0; k < kNumRedoThreads; ++k) redo_ms[k] = B.redo_ms[k];
     return 0;
 }
 

@@ -28,6 +28,7 @@ from .EXTRA_FIX.deform_pressure import FixDeformPressure
 
 SEGMENT_LENGTHS = (8, 8, 16, 16, 32, 32)
 BLOCK_ORDERS = ("sequence", "xcd", "sequence", "xcd", "sequence", "xcd")
+REDO_THREADS = (4, 8, 16)
 
 
 class Ensemble:
@@ -113,13 +114,16 @@ class Ensemble:
         """The launch configuration in use and what the autotuner measured."""
         if self._batch is None:
             raise RuntimeError("the ensemble has not run yet")
-        seg, order, rates, redone = self._batch.tuning()
+        seg, order, rates, redone, threads, redo_ms = self._batch.tuning()
         return {
             "segment_length": seg,
             "block_order": "xcd" if order else "sequence",
             "rates_atom_steps_per_s": {f"{SEGMENT_LENGTHS[k]}/{BLOCK_ORDERS[k]}": float(rates[k])
                                        for k in range(len(rates))},
             "redone_steps": redone,
+            "redo_threads": threads,
+            "redo_ms_per_member": {str(t): float(redo_ms[k])
+                                   for k, t in enumerate(REDO_THREADS)},
         }
 
     def close(self) -> None:
