@@ -657,7 +657,8 @@ int check_run_error(System& s) {
 // (remap, drift), the list rebuilt, then the step's force pass,
 // final_integrate and end run. On return the step is complete with its
 // results in buffer 0 and f / torque stored for the next step's half-kick.
-int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga) {
+int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga,
+              bool pull_after) {
     const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
     const int stp = s.hctl->timestep + 1;
@@ -697,6 +698,14 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
     if (!rc) rc = launch_remap_clear(s, 1);
     if (rc) return rc;
+    if (!pull_after) {
+        // the caller learns of an error from the next status it reads; the
+        // step completes as step stp otherwise
+        s.hctl->timestep = stp;
+        s.timestep = stp;
+        s.current_step = stp;
+        return 0;
+    }
     rc = ctl_pull(s);
     if (rc) return rc;
     rc = check_run_error(s);

@@ -39,6 +39,24 @@ __global__ void k_batch_status(const CloseEntry* table, int n, int4* out) {
     out[k] = make_int4(c->timestep, c->abort, c->error, c->bdone);
 }
 
+// Every member's control block into one array (one block per member), read
+// back with the status: the host mirror a member's redo starts from.
+__global__ void k_batch_ctl_gather(const CloseEntry* table, Ctl* out) {
+    static_assert(sizeof(Ctl) % 4 == 0, "the control block is copied in 4-byte words");
+    const unsigned* src = reinterpret_cast<const unsigned*>(table[blockIdx.x].ctl);
+    unsigned* dst = reinterpret_cast<unsigned*>(out + blockIdx.x);
+    for (int w = threadIdx.x; w < (int)(sizeof(Ctl) / 4); w += blockDim.x) dst[w] = src[w];
+}
+
+// A redone member rejoins: its next step reads set 0 after a half-kick.
+__global__ void k_batch_rejoin(Ctl* ctl, int done) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->bpar = 0;
+        ctl->bkick = 1;
+        ctl->bdone = done;
+    }
+}
+
 struct Config {
     int seg_len;
     int order;  // 0: members in sequence, 1: members grouped per XCD
@@ -71,6 +89,8 @@ struct tmps_batch_impl {
     int max_parts = 1;
     int4* d_status = nullptr;
     int4* h_status = nullptr;
+    Ctl* d_ctls = nullptr;  // members' control blocks after the last segment
+    Ctl* h_ctls = nullptr;
     hipGraphExec_t exec[kNumConfigs] = {};
     double exec_dt[kNumConfigs] = {};
     int config = -1;  // chosen launch configuration (-1: not tuned yet)
@@ -101,6 +121,8 @@ void free_batch(Batch& B) {
     }
     if (B.d_status) (void)hipFree(B.d_status);
     if (B.h_status) (void)hipHostFree(B.h_status);
+    if (B.d_ctls) (void)hipFree(B.d_ctls);
+    if (B.h_ctls) (void)hipHostFree(B.h_ctls);
     if (B.stream) (void)hipStreamDestroy(B.stream);
 }
 
@@ -188,6 +210,8 @@ int build(Batch& B) {
     HIP_TRY(hipMalloc((void**)&B.d_close, ninst * sizeof(CloseEntry)));
     HIP_TRY(hipMalloc((void**)&B.d_status, ninst * sizeof(int4)));
     HIP_TRY(hipHostMalloc((void**)&B.h_status, ninst * sizeof(int4), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void**)&B.d_ctls, ninst * sizeof(Ctl)));
+    HIP_TRY(hipHostMalloc((void**)&B.h_ctls, ninst * sizeof(Ctl), hipHostMallocDefault));
     std::vector<int> all, by_style[3];
     for (int k = 0; k < ninst; ++k) {
         System& s = *B.sys[k];
@@ -231,6 +255,9 @@ int enqueue_segment(Batch& B, const Config& c, double dt) {
     hipLaunchKernelGGL(k_batch_status, dim3((ninst + 255) / 256), dim3(256), 0, B.stream, B.d_close,
                        ninst, B.d_status);
     HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_batch_ctl_gather, dim3(ninst), dim3(256), 0, B.stream, B.d_close,
+                       B.d_ctls);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -266,6 +293,8 @@ int launch_segment(Batch& B, int ci, double dt) {
     HIP_TRY(hipGraphLaunch(B.exec[ci], B.stream));
     HIP_TRY(hipMemcpyAsync(B.h_status, B.d_status, B.sys.size() * sizeof(int4),
                            hipMemcpyDeviceToHost, B.stream));
+    HIP_TRY(hipMemcpyAsync(B.h_ctls, B.d_ctls, B.sys.size() * sizeof(Ctl), hipMemcpyDeviceToHost,
+                           B.stream));
     return 0;
 }
 
@@ -304,21 +333,25 @@ int start_member(System& s, int nsteps, double dt, int run_pre) {
 }
 
 // Member k asked for a rebuild: redo its step alone, then it rejoins.
+// Member k asked for a rebuild: redo its step alone, then it rejoins. The
+// control block it starts from is the segment's gathered copy (the device
+// copy has not changed since), and the rejoin is written on the device, so
+// the only host round trips are the list build's slot count and the final
+// wait; an error of the redone step shows in the next segment's status.
 int redo_member(Batch& B, int k, double dt) {
     System& s = *B.sys[k];
-    int rc = tmps_api::ctl_pull(s);
-    if (rc) return rc;
-    rc = tmps_api::check_run_error(s);
+    *s.hctl = B.h_ctls[k];
+    int rc = tmps_api::check_run_error(s);
     if (rc) return rc;
     s.timestep = s.hctl->timestep;
-    rc = tmps_api::redo_step(s, s.hctl->bpar, dt, 0, 0.0, 0.0);
+    rc = tmps_api::redo_step(s, s.hctl->bpar, dt, 0, 0.0, 0.0, false);
     if (rc) return rc;
     Ctl& H = *s.hctl;
     H.bpar = 0;
     H.bkick = 1;
     H.bdone = s.timestep == H.bend ? 1 : 0;
-    rc = tmps_api::ctl_push(s);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_batch_rejoin, dim3(1), dim3(64), 0, s.stream, s.ctl, H.bdone);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s.stream));
     return 0;
 }

@@ -202,7 +202,10 @@ int ctl_pull(System& s);
 int ctl_push(System& s);
 int check_run_error(System& s);
 int ensure_split(System& s);
-int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga);
+// pull_after: read the control block back (timestep, error) when the redo
+// is done; without it the host mirror is current only in the box and step
+int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga,
+              bool pull_after = true);
 int init_simulation(System& s, double dt, int use_cundall, double gl, double ga);
 int reneighbor(System& s, bool force, int* rebuilt);
 int clear_forces(System& s);
