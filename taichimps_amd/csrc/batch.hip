@@ -91,6 +91,7 @@ struct tmps_batch_impl {
     int4* h_status = nullptr;
     Ctl* d_ctls = nullptr;  // members' control blocks after the last segment
     Ctl* h_ctls = nullptr;
+    std::vector<hipStream_t> worker_streams;  // one per redo thread
     hipGraphExec_t exec[kNumConfigs] = {};
     double exec_dt[kNumConfigs] = {};
     int config = -1;  // chosen launch configuration (-1: not tuned yet)
@@ -123,6 +124,10 @@ void free_batch(Batch& B) {
     if (B.h_status) (void)hipHostFree(B.h_status);
     if (B.d_ctls) (void)hipFree(B.d_ctls);
     if (B.h_ctls) (void)hipHostFree(B.h_ctls);
+    for (hipStream_t ws : B.worker_streams) {
+        (void)hipStreamSynchronize(ws);
+        (void)hipStreamDestroy(ws);
+    }
     if (B.stream) (void)hipStreamDestroy(B.stream);
 }
 
@@ -338,7 +343,7 @@ int start_member(System& s, int nsteps, double dt, int run_pre) {
 // copy has not changed since), and the rejoin is written on the device, so
 // the only host round trips are the list build's slot count and the final
 // wait; an error of the redone step shows in the next segment's status.
-int redo_member(Batch& B, int k, double dt) {
+int redo_member_on(Batch& B, int k, double dt) {
     System& s = *B.sys[k];
     *s.hctl = B.h_ctls[k];
     int rc = tmps_api::check_run_error(s);
@@ -356,6 +361,20 @@ int redo_member(Batch& B, int k, double dt) {
     return 0;
 }
 
+// The redo runs on the calling thread's stream `ws` rather than the member's
+// own (nothing is pending on either when it starts, and `ws` is drained when
+// it ends): a few streams shared by many members' redos cost the runtime less
+// than one stream per member.
+int redo_member(Batch& B, int k, double dt, hipStream_t ws) {
+    System& s = *B.sys[k];
+    const hipStream_t own = s.stream;
+    s.stream = ws;
+    const int rc = redo_member_on(B, k, dt);
+    if (rc) (void)hipStreamSynchronize(ws);
+    s.stream = own;
+    return rc;
+}
+
 // Redoes the members in `which`, several at a time: each redo is a chain of
 // small kernels and host round trips on the member's own stream, so host
 // threads (up to `threads`) keep several members' chains in flight at once.
@@ -364,9 +383,14 @@ int redo_members(Batch& B, const std::vector<int>& which, double dt, int threads
     const int m = (int)which.size();
     if (m == 0) return 0;
     const int nt = std::max(1, std::min(threads, m));
+    while ((int)B.worker_streams.size() < nt) {
+        hipStream_t ws = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
+        B.worker_streams.push_back(ws);
+    }
     if (nt == 1) {
         for (int k : which) {
-            int rc = redo_member(B, k, dt);
+            int rc = redo_member(B, k, dt, B.worker_streams[0]);
             if (rc) return rc;
         }
         return 0;
@@ -375,9 +399,9 @@ int redo_members(Batch& B, const std::vector<int>& which, double dt, int threads
     std::atomic<bool> failed{false};
     std::vector<int> rcs(m, 0);
     std::vector<std::string> msgs(m);
-    auto work = [&]() {
+    auto work = [&](int t) {
         for (int q = next.fetch_add(1); q < m && !failed.load(); q = next.fetch_add(1)) {
-            const int rc = redo_member(B, which[q], dt);
+            const int rc = redo_member(B, which[q], dt, B.worker_streams[t]);
             if (rc) {
                 rcs[q] = rc;
                 msgs[q] = tmps_last_error();
@@ -386,8 +410,8 @@ int redo_members(Batch& B, const std::vector<int>& which, double dt, int threads
         }
     };
     std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
     for (auto& t : pool) t.join();
     for (int q = 0; q < m; ++q)
         if (rcs[q]) return tmps_fail(rcs[q], msgs[q]);
