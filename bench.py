@@ -487,7 +487,7 @@ def main(argv=None):
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                          "(multi-GPU sharding is a design only, not implemented)"}))
         return 2
 
     from taichimps_amd import _native, workloads
