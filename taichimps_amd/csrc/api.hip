@@ -264,8 +264,7 @@ int neigh_build(System& s, bool box_known = false) {
     rc = launch_ctl_built(s, s.last_build_step, fast, gi, split(s) ? 1 : 0);
     if (rc) return rc;
     if (split(s)) {
-        rc = launch_split_consts(s, 1);
-        if (!rc) rc = launch_x64_to_split(s);
+        rc = launch_x64_to_split(s);
         if (rc) return rc;
         s.x64_valid = false;
     }
@@ -288,13 +287,8 @@ int history_save(System& s) {
         HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
         s.onb_cap = cap;
     }
-    HIP_TRY(hipMemcpyAsync(s.onb_start, s.nb_start, sizeof(int) * (s.n + 1), hipMemcpyDeviceToDevice,
-                           s.stream));
-    if (need > 0) {
-        HIP_TRY(hipMemcpyAsync(s.onb_j, s.nb_j, sizeof(int) * need, hipMemcpyDeviceToDevice, s.stream));
-        HIP_TRY(hipMemcpyAsync(s.onb_flag, s.nb_flag, need, hipMemcpyDeviceToDevice, s.stream));
-        HIP_TRY(hipMemcpyAsync(s.onb_hist, s.nb_hist, need * hsz, hipMemcpyDeviceToDevice, s.stream));
-    }
+    int rc = launch_list_save(s);
+    if (rc) return rc;
     s.onslots = (int)need;
     s.onb_n = s.n;
     s.have_old = true;
@@ -686,10 +680,6 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     s.rebuilds += 1;
     rc = launch_ctl_rebuilt(s, stp);
     if (rc) return rc;
-    if (split(s)) {
-        rc = launch_split_consts(s, 0);
-        if (rc) return rc;
-    }
     rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
                   : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
     if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);

@@ -177,14 +177,15 @@ int upload(Batch& B, const std::vector<int2>& v, int2** d) {
     return 0;
 }
 
-// (Re)writes member k's table entries (its buffers can move at a rebuild).
-int write_entry(Batch& B, int k) {
-    System& s = *B.sys[k];
-    int rc = batch_split_entry(s, B.h_table.data() + (size_t)k * B.entry_size);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(static_cast<char*>(B.d_table) + (size_t)k * B.entry_size,
-                           B.h_table.data() + (size_t)k * B.entry_size, B.entry_size,
-                           hipMemcpyHostToDevice, B.stream));
+// (Re)writes member k's table entry on the host (its buffers can move at a
+// rebuild); upload_table sends the table to the device.
+int fill_entry(Batch& B, int k) {
+    return batch_split_entry(*B.sys[k], B.h_table.data() + (size_t)k * B.entry_size);
+}
+
+int upload_table(Batch& B) {
+    HIP_TRY(hipMemcpyAsync(B.d_table, B.h_table.data(), B.h_table.size(), hipMemcpyHostToDevice,
+                           B.stream));
     return 0;
 }
 
@@ -441,7 +442,11 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
     }
     // tables: member buffers may have moved since the last run
     for (int k = 0; k < ninst; ++k) {
-        int rc = write_entry(B, k);
+        int rc = fill_entry(B, k);
+        if (rc) return rc;
+    }
+    {
+        int rc = upload_table(B);
         if (rc) return rc;
     }
     int rc0 = write_close_table(B);
@@ -500,12 +505,16 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
         B.rebuilds += (int)redo.size();
         for (int k : redo) {
             // the rebuild may have moved the member's list buffers
-            rc = write_entry(B, k);
+            rc = fill_entry(B, k);
             if (rc) return rc;
             prev[k] = B.sys[k]->timestep;
             if (!B.sys[k]->hctl->bdone) running = true;
         }
-        if (!redo.empty()) HIP_TRY(hipStreamSynchronize(B.stream));
+        if (!redo.empty()) {
+            rc = upload_table(B);
+            if (rc) return rc;
+            HIP_TRY(hipStreamSynchronize(B.stream));
+        }
         if (probe < kNumConfigs) {
             const double rate = sec > 0.0 ? atom_steps / sec : 0.0;
             B.config_rate[ci] = rate;

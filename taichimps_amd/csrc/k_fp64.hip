@@ -602,11 +602,12 @@ __global__ void k_remap_clear(Ctl* ctl, int honor_abort) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->remap.step = -1;
 }
 
-__global__ void k_ctl_rebuilt(Ctl* ctl, int step) {
+__global__ void k_ctl_rebuilt(Ctl* ctl, int step, int split) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->trigger = -1;
         ctl->abort = 0;
         ctl->last_build_step = step;
+        if (split) update_split_consts(ctl, 0);
     }
 }
 
@@ -620,16 +621,19 @@ __global__ void k_ctl_resume(Ctl* ctl) {
 
 // What a list build records in the control block: its step, per axis whether
 // it has at least 3 bins, and (FP32-split) the bins of the encoding.
-__global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int set_gi) {
+// (FP32-split: with the encoding's bins, and the split constants of the
+// build box, which becomes the skin check's reference)
+__global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->last_build_step = step;
         ctl->fast_img[0] = fast.x;
         ctl->fast_img[1] = fast.y;
         ctl->fast_img[2] = fast.z;
-        if (set_gi) {
+        if (split) {
             ctl->sc.gi[0] = gi.x;
             ctl->sc.gi[1] = gi.y;
             ctl->sc.gi[2] = gi.z;
+            update_split_consts(ctl, 1);
         }
     }
 }
@@ -1251,7 +1255,8 @@ int launch_remap_clear(System& s, int honor_abort) {
 }
 
 int launch_ctl_rebuilt(System& s, int step) {
-    hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step);
+    hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step,
+                       s.precision == TMPS_FP32_SPLIT ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1262,10 +1267,10 @@ int launch_ctl_resume(System& s) {
     return 0;
 }
 
-int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int set_gi) {
+int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int split) {
     hipLaunchKernelGGL(k_ctl_built, dim3(1), dim3(64), 0, s.stream, s.ctl, step,
                        make_int3(fast[0], fast[1], fast[2]), make_int3(gi[0], gi[1], gi[2]),
-                       set_gi);
+                       split);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -62,6 +62,35 @@ __global__ void k_zero_int(int* a, int n) {
     if (i < n) a[i] = 0;
 }
 
+// a[0, n) = b[0, n) = 0, and c[0, nc) = 0
+__global__ void k_zero_int3(int* a, int* b, int n, int* c, int nc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+    if (i < nc) c[i] = 0;
+}
+
+// The current list into the copy the history remap merges with (the
+// reference's save_state, contact_history.py:141-145): row starts, partners,
+// touch flags and histories, the slot count read on the device.
+template <typename H>
+__global__ void k_list_save(int n, const int* nb_start, const int* nb_j, const uint8_t* nb_flag,
+                            const H* nb_hist, int* onb_start, int* onb_j, uint8_t* onb_flag,
+                            H* onb_hist, int cap) {
+    const int nslots = nb_start[n];
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= n || k < nslots; k += stride) {
+        if (k <= n) onb_start[k] = nb_start[k];
+        if (k < nslots && k < cap) {
+            onb_j[k] = nb_j[k];
+            onb_flag[k] = nb_flag[k];
+            onb_hist[k] = nb_hist[k];
+        }
+    }
+}
+
 // build_grid count phase, neighbor.py:264-268
 __global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, int* cell_of,
                             int* count) {
@@ -151,7 +180,7 @@ __global__ void k_scan_tiles(const int* in, int* out, int n, int* tile_sums) {
 }
 
 // One block: exclusive scan of the tile sums in place; writes the grand total.
-__global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out) {
+__global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out, int* total_copy) {
     const int per = (ntiles + TMPS_BLOCK - 1) / TMPS_BLOCK;
     const int a = threadIdx.x * per;
     int sum = 0;
@@ -166,7 +195,10 @@ __global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out) {
             tile_sums[a + k] = run;
             run += t;
         }
-    if (threadIdx.x == 0) *total_out = total;
+    if (threadIdx.x == 0) {
+        *total_out = total;
+        if (total_copy) *total_copy = total;
+    }
 }
 
 __global__ void k_scan_add(int* out, int n, const int* tile_sums) {
@@ -508,13 +540,57 @@ int launch_pbc(System& s) {
     return 0;
 }
 
-int launch_scan(System& s, const int* in, int* out, int n) {
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total) in one block,
+// tile after tile with a running carry: one launch for small arrays.
+__global__ void k_scan_one_block(const int* in, int* out, int n, int* total_copy) {
+    __shared__ int carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < n; t0 += SCAN_TILE) {
+        int v[SCAN_PER_THREAD];
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+            const int idx = t0 + threadIdx.x * SCAN_PER_THREAD + q;
+            v[q] = idx < n ? in[idx] : 0;
+            sum += v[q];
+        }
+        int excl = 0;
+        const int tile_total = block_excl_scan(sum, &excl);
+        int run = carry_s + excl;
+#pragma unroll
+        for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+            const int idx = t0 + threadIdx.x * SCAN_PER_THREAD + q;
+            if (idx < n) out[idx] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s += tile_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[n] = carry_s;
+        if (total_copy) *total_copy = carry_s;
+    }
+}
+
+// arrays up to this many entries are scanned by one block
+#define SCAN_ONE_BLOCK_MAX 65536
+
+int launch_scan(System& s, const int* in, int* out, int n, int* total_copy) {
+    if (n <= SCAN_ONE_BLOCK_MAX) {
+        hipLaunchKernelGGL(k_scan_one_block, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, in, out, n,
+                           total_copy);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     const int nt = ntiles > 0 ? ntiles : 1;
     if (s.scan_tmp_cap < nt + 1) return tmps_fail(TMPS_ERR_ARG, "scan workspace too small");
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(TMPS_BLOCK), 0, s.stream, in, out, n, s.scan_tmp);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.scan_tmp, nt, out + n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.scan_tmp, nt, out + n,
+                       total_copy);
     HIP_TRY(hipGetLastError());
     if (n > 0) {
         hipLaunchKernelGGL(k_scan_add, grid_for(n), dim3(TMPS_BLOCK), 0, s.stream, out, n, s.scan_tmp);
@@ -525,18 +601,16 @@ int launch_scan(System& s, const int* in, int* out, int n) {
 
 int launch_bins(System& s, const double4* xr) {
     const int ncell = s.grid.ncell;
-    hipLaunchKernelGGL(k_zero_int, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_count,
-                       ncell + 1);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_zero_int, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_fill,
-                       ncell + 1);
+    // bin counts and fill cursors, and the list build's flags
+    hipLaunchKernelGGL(k_zero_int3, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_count,
+                       s.cell_fill, ncell + 1, s.flags, 4);
     HIP_TRY(hipGetLastError());
     if (s.n > 0) {
         hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
                            s.grid, s.cell_of, s.cell_count);
         HIP_TRY(hipGetLastError());
     }
-    int rc = launch_scan(s, s.cell_count, s.cell_start, ncell);
+    int rc = launch_scan(s, s.cell_count, s.cell_start, ncell, nullptr);
     if (rc) return rc;
     if (s.n > 0) {
         hipLaunchKernelGGL(k_bin_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n, s.cell_of,
@@ -579,8 +653,7 @@ int ensure_list_capacity(System& s, size_t need) { return grow_list(s, need); }
 // compaction (two passes, count then fill, if some row exceeds the scratch).
 // *total_out receives the slot count (a host synchronisation per rebuild).
 int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s.stream, s.flags, 4);
-    HIP_TRY(hipGetLastError());
+    // s.flags was cleared by launch_bins (flags[2] receives the slot count)
     const size_t rows_need = (size_t)std::max(s.n, 1) * LIST_ROW_CAP;
     if (rows_need > s.row_tmp_cap) {
         HIP_TRY(hipStreamSynchronize(s.stream));
@@ -597,11 +670,10 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
                            s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp);
         HIP_TRY(hipGetLastError());
     }
-    int rc = launch_scan(s, s.full_count, s.nb_start, s.n);
+    int rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
     if (rc) return rc;
     int hv[3];
-    HIP_TRY(hipMemcpyAsync(&hv[0], s.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    HIP_TRY(hipMemcpyAsync(&hv[2], s.nb_start + s.n, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipMemcpyAsync(hv, s.flags, 3 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));
     *overflow_out = hv[0];
     *total_out = hv[2];
@@ -620,6 +692,22 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
         HIP_TRY(hipGetLastError());
     }
     s.nslots = hv[2];
+    return 0;
+}
+
+int launch_list_save(System& s) {
+    const int items = std::max(s.n + 1, s.nslots);
+    const int blocks = std::min((items + TMPS_BLOCK - 1) / TMPS_BLOCK, 4096);
+    const int cap = (int)std::min<size_t>(s.onb_cap, (size_t)INT32_MAX);
+    if (s.precision == TMPS_FP64)
+        hipLaunchKernelGGL(k_list_save<double4>, dim3(blocks), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (const double4*)s.nb_hist, s.onb_start,
+                           s.onb_j, s.onb_flag, (double4*)s.onb_hist, cap);
+    else
+        hipLaunchKernelGGL(k_list_save<float4>, dim3(blocks), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                           s.nb_start, s.nb_j, s.nb_flag, (const float4*)s.nb_hist, s.onb_start,
+                           s.onb_j, s.onb_flag, (float4*)s.onb_hist, cap);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -661,7 +749,7 @@ int launch_half_pairs(System& s, int* d_pair_i, int* d_pair_j, int* d_partner, d
     hipLaunchKernelGGL(k_half_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n, s.nb_start,
                        s.nb_j, s.half_count);
     HIP_TRY(hipGetLastError());
-    int rc = launch_scan(s, s.half_count, d_counts, s.n);
+    int rc = launch_scan(s, s.half_count, d_counts, s.n, nullptr);
     if (rc) return rc;
     if (d_pair_i == nullptr) return 0;
     if (s.precision == TMPS_FP64)

@@ -237,8 +237,11 @@ int launch_pbc(System& s);
 int launch_bins(System& s, const double4* xr);
 int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out);
 int launch_hist_restore(System& s);
+// the current list, touch flags and history into the o* copy (save_state)
+int launch_list_save(System& s);
 int launch_store_x0(System& s);
-int launch_scan(System& s, const int* in, int* out, int n);  // exclusive, out has n+1
+// exclusive, out has n+1 (out[n] = total, also written to total_copy when set)
+int launch_scan(System& s, const int* in, int* out, int n, int* total_copy);
 int launch_half_pairs(System& s, int* d_pair_i, int* d_pair_j, int* d_partner, double* d_shear,
                       int* d_counts);
 int launch_set_half_history(System& s, const int* d_partner, const double* d_shear,
@@ -291,11 +294,14 @@ int launch_remap_apply(System& s);
 // clears the pending remap record (kept when honor_abort and the step aborted:
 // the redo of that step still needs it)
 int launch_remap_clear(System& s, int honor_abort = 0);
+// a redone step resumes after its rebuild (FP32-split: with the split
+// constants of the current box)
 int launch_ctl_rebuilt(System& s, int step);
 // clears trigger and abort before a redo
 int launch_ctl_resume(System& s);
-// records a list build (last_build_step, fast_img, and sc.gi when set_gi)
-int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int set_gi);
+// records a list build (last_build_step, fast_img; FP32-split: sc.gi and the
+// split constants with the current box as the skin check's reference)
+int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int split);
 int launch_remap_apply_fused(System& s);
 void drop_graphs(System& s);
 int launch_final_tree(System& s, int nc, int nparts);
