@@ -306,4 +306,36 @@ __device__ __forceinline__ void block_tree_sum(double leaf[NC], double* out) {
     __syncthreads();
 }
 
+// The same sums as block_tree_sum, with each wavefront's levels formed in LDS
+// instead of by DPP moves: for many components this issues far fewer vector
+// instructions. Entry p of a level is the sum of entries 2p and 2p + 1 of the
+// level before (written in place: a round of the loop covers whole
+// components, and a wavefront's reads precede its writes), so every addition
+// is the one the pairwise tree makes.
+template <int NC>
+__device__ __forceinline__ void block_tree_sum_lds(double leaf[NC], double* out) {
+    __shared__ double buf[TMPS_BLOCK / 64][NC * 64];
+    __shared__ double wsum[TMPS_BLOCK / 64][NC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* b = buf[wave];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) b[k * 64 + lane] = leaf[k];
+#pragma unroll
+    for (int size = 32; size >= 1; size >>= 1) {
+        for (int q = lane; q < NC * size; q += 64) {
+            const int k = q / size, p = q - k * size;
+            const double x = b[k * 64 + 2 * p], y = b[k * 64 + 2 * p + 1];
+            b[k * 64 + p] = x + y;
+        }
+    }
+    if (lane < NC) wsum[wave][lane] = b[lane * 64];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            out[k] = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
+    }
+    __syncthreads();
+}
+
 }  // namespace tmps

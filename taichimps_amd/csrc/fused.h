@@ -41,7 +41,7 @@ __device__ __forceinline__ void publish_block(const double leaf[7], int trig, in
 #pragma unroll
     for (int k = 0; k < 7; ++k) l8[k] = leaf[k];
     l8[7] = trig ? 1.0 : 0.0;
-    block_tree_sum<8>(l8, partial + (size_t)lb * 8);
+    block_tree_sum_lds<8>(l8, partial + (size_t)lb * 8);
 }
 
 // neigh_modify delay / every for step t (neighbor.py:371-403)
