@@ -9,11 +9,13 @@ remap x). Synthetic inputs from taichimps_amd.workloads.
 For each precision (FP64 plain, FP32-split) the script runs `--warmup` steps
 (the run's setup force and first neighbor build included), then times exactly
 `--steps` device-resident steps (Simulation.run with `pre no`) between two
-stream synchronisations. `value` is the FP32-split throughput in atom-steps/s;
-the FP64 figure and their ratio ride along. A second pass of the same steps
-with per-kernel HIP events gives the step kernel's average duration for the
-roofline; the CPU baseline times the single-threaded CPU oracle on the same
-workload.
+stream synchronisations, on `--repeats` fresh systems (the median run is
+reported; the runs are bitwise equal). `value` is the FP32-split throughput in
+atom-steps/s; the FP64 figure and their ratio ride along. A further pass of
+50 steps with per-kernel HIP events gives the step kernel's average duration
+for the roofline; the CPU baseline times the single-threaded CPU oracle on the
+same workload. The adjoint (32k spheres), ensemble (256 packings) and, with
+`--triaxial`, the 4M-sphere triaxial blocks follow in the same JSON line.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 """
@@ -122,22 +124,34 @@ def triaxial_c3(steps=1000, warmup=10):
     }
 
 
-def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder=None):
-    sim = (builder or build_sim)(w, precision)
-    N = sim.atom._native
-    N.set_launch_mode(direct)
-    n = sim.atom.nlocal
-    t0 = time.perf_counter()
-    sim.run(warmup)
-    N.sync()
-    t_setup = time.perf_counter() - t0
-    sim.run_pre = False
-    builds0 = sim.neighbor.build_count
-    N.sync()
-    t0 = time.perf_counter()
-    sim.run(steps)
-    N.sync()
-    elapsed = time.perf_counter() - t0
+def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder=None, repeats=1):
+    """`repeats` fresh systems, each with `warmup` steps then exactly `steps`
+    timed steps; the median timed run is reported (every run does the same
+    work: the trajectories are bitwise equal)."""
+    times = []
+    p_first = None
+    for _ in range(max(1, repeats)):
+        sim = (builder or build_sim)(w, precision)
+        N = sim.atom._native
+        N.set_launch_mode(direct)
+        n = sim.atom.nlocal
+        t0 = time.perf_counter()
+        sim.run(warmup)
+        N.sync()
+        t_setup = time.perf_counter() - t0
+        sim.run_pre = False
+        builds0 = sim.neighbor.build_count
+        N.sync()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        N.sync()
+        times.append(time.perf_counter() - t0)
+        p_run = N.last_pressure()
+        if p_first is None:
+            p_first = p_run
+        elif not np.array_equal(p_first, p_run):
+            raise RuntimeError("repeated runs of the same workload differ")
+    elapsed = float(np.median(times))
     c = N.counters()
     rebuilds = sim.neighbor.build_count - builds0
     pressure = N.last_pressure()
@@ -153,6 +167,7 @@ def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder
         N.set_profiling(False)
     return {
         "n": n, "elapsed_s": elapsed, "ms_per_step": elapsed / steps * 1e3,
+        "ms_per_step_runs": [t / steps * 1e3 for t in times],
         "atom_steps_per_s": n * steps / elapsed, "setup_s": t_setup, "rebuilds": rebuilds,
         "n_slot": n_slot, "n_touch": n_touch, "pressure": pressure.tolist(),
         "mean_pressure": float(pressure[:3].mean()),
@@ -451,6 +466,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="fresh systems timed per precision; the median run is reported")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint calibration block (32k spheres, 2000 steps)")
@@ -496,7 +513,8 @@ def main(argv=None):
     res = {}
     precisions = ("f64", "f32split") if args.precision == "both" else (args.precision,)
     for prec in precisions:
-        res[prec] = run_precision(w, prec, args.steps, args.warmup, args.prof_steps, args.direct)
+        res[prec] = run_precision(w, prec, args.steps, args.warmup, args.prof_steps, args.direct,
+                                  repeats=args.repeats)
         r = res[prec]
         print(f"# {prec}: {r['ms_per_step']:.4f} ms/step, {r['atom_steps_per_s']:.4e} atom-steps/s, "
               f"kernels {r['kernel_ms']}, n_slot {r['n_slot']:.3f}, n_touch {r['n_touch']:.3f}, "
@@ -530,6 +548,8 @@ def main(argv=None):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
+        "repeats": args.repeats,
+        "ms_per_step_runs": head["ms_per_step_runs"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
