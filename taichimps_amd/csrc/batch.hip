@@ -9,21 +9,21 @@
 // close launch with a block per member (k_batch_close). Each member carries
 // its own progress on the device (the buffer set its next step reads, a
 // pending half-kick, the step that ends its run), so a member whose skin
-// check asks for a rebuild is redone alone on the host (redo_step) and rejoins
-// the next segment while the others never wait. A member's arithmetic, tree
-// and servo are those of its single-system run, so its results are bitwise
-// the ones tmps_run gives it alone.
+// check asks for a rebuild is redone by the host after the segment (the
+// single-system redo_step, several members at once on host threads) and
+// rejoins the next segment, while the others never wait. A member's
+// arithmetic, tree and servo are those of its single-system run, so its
+// results are bitwise the ones tmps_run gives it alone.
 //
-// Launch configuration (segment length, order of the blocks in the map) does
-// not change any result; tmps_batch_run can pick it by timing the first
-// segments of the run itself (the autotuner).
+// Launch configuration (segment length, order of the blocks in the map) and
+// the number of redo threads change no result; tmps_batch_run picks them by
+// timing the first segments of the run itself (the autotuner).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
