@@ -218,8 +218,10 @@ int tmps_adjoint_backward(tmps_system* s, const double dL_dtensor[6], double gra
  * box, atoms, pair style and coefficients, servo) together: per step one
  * launch per pair style covers every member's atoms and one launch closes
  * every member's step, captured as HIP graph segments. A member whose skin
- * check asks for a rebuild is redone alone and rejoins; the others do not
- * wait. Every member's results are bitwise those tmps_run gives it alone.
+ * check asks for a rebuild stops; after the segment the steps of all such
+ * members are redone with their list rebuilds, each kernel launched once for
+ * all of them, and they rejoin; the others do not wait. Every member's
+ * results are bitwise those tmps_run gives it alone.
  *
  * tmps_batch_create takes `n` distinct systems, each FP32-split, with atoms,
  * a pair style (hooke/history, hertz/history or granular) and, optionally,
@@ -228,18 +230,16 @@ int tmps_adjoint_backward(tmps_system* s, const double dL_dtensor[6], double gra
  * tmps_batch_run is tmps_run(nsteps, dt, run_pre) on every member. With
  * `autotune` (and always on the first run) its first segments time each
  * launch configuration (segment length 8/16/32, members' blocks in sequence
- * or grouped per XCD) and its first segments with at least 8 member redos
- * time each number of redo threads (4, 8, 16); tmps_batch_tuning reports the
- * chosen configuration, each configuration's measured rate (atom-steps/s, 6
- * entries), the number of member steps the last run redid after a rebuild
- * request, the chosen redo thread count and each count's wall ms per redo
- * (3 entries; 0 where not measured). */
+ * or grouped per XCD); tmps_batch_tuning reports the chosen configuration,
+ * each configuration's measured rate (atom-steps/s, 6 entries), the number of
+ * member steps the last run redid after a rebuild request, the number of
+ * redo passes (one per segment with such requests) and their total wall ms. */
 typedef struct tmps_batch tmps_batch;
 int tmps_batch_create(tmps_system* const* systems, int n, tmps_batch** out);
 int tmps_batch_destroy(tmps_batch* b);
 int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotune);
 int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
-                      int* rebuilds, int* redo_threads, double redo_ms[3]);
+                      int* rebuilds, int* redo_passes, double* redo_ms);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* How tmps_run issues a segment's kernels: replayed from a captured HIP graph

@@ -450,12 +450,12 @@ class NativeBatch:
     def tuning(self):
         """(segment length, block order, atom-steps/s per configuration,
         member steps redone after rebuild requests in the last run, redo
-        threads, wall ms per redo for 4/8/16 threads)."""
-        seg, order, reb, thr = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-        rates, redo_ms = np.zeros(6), np.zeros(3)
+        passes of the last run, their total wall ms)."""
+        seg, order, reb, passes = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        rates, redo_ms = np.zeros(6), C.c_double(0.0)
         check(self._L.tmps_batch_tuning(self._h, C.byref(seg), C.byref(order), _dp(rates),
-                                        C.byref(reb), C.byref(thr), _dp(redo_ms)))
-        return seg.value, order.value, rates, reb.value, thr.value, redo_ms
+                                        C.byref(reb), C.byref(passes), C.byref(redo_ms)))
+        return seg.value, order.value, rates, reb.value, passes.value, redo_ms.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

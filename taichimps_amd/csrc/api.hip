@@ -187,20 +187,10 @@ int split_positions_to_xr(System& s) {
 
 // ------------------------------------------------------------ neighbor build
 
-// box_known: the host mirror of the control block is current (the caller
-// pulled it and nothing since changed the box), so no round trip is needed.
-int neigh_build(System& s, bool box_known = false) {
-    if (s.n <= 0) return 0;
-    int rc = 0;
-    if (split(s)) {
-        rc = split_positions_to_xr(s);
-        if (rc) return rc;
-        s.x64_valid = true;
-    }
-    if (!box_known) {
-        rc = ctl_pull(s);
-        if (rc) return rc;
-    }
+// The bins of a list build, from the box in the host mirror: the reference's
+// grid (reported by the API) and the build's own, with bin arrays that hold
+// it.
+int neigh_grid(System& s) {
     const DevBox B = s.hctl->box;
     const double maxcut = 2.0 * max_radius(s);
     setup_grid(s, B, maxcut);
@@ -215,25 +205,22 @@ int neigh_build(System& s, bool box_known = false) {
         setup_grid(s, B, maxcut);
         s.max_grid_cells = keep;
     }
-    rc = ensure_cells(s, s.grid.ncell);
-    if (rc) return rc;
-    rc = launch_bins(s, s.xr);
-    if (rc) return rc;
-    int overflow = 0, total = 0;
-    rc = launch_list(s, s.xr, &overflow, &total);
-    if (rc) return rc;
-    if (overflow) {
-        char buf[256];
-        snprintf(buf, sizeof buf,
-                 "Neighbor list overflow: more than %d neighbors for at least one particle. "
-                 "Increase max_neighbors_per_atom, or reduce the neighbor skin.",
-                 s.max_nb);
-        return tmps_fail(TMPS_ERR_OVERFLOW, buf);
-    }
-    rc = launch_hist_clear(s);
-    if (rc) return rc;
-    rc = launch_store_x0(s);
-    if (rc) return rc;
+    return ensure_cells(s, s.grid.ncell);
+}
+
+// NeighborList.build's overflow error (neighbor.py:426-431)
+int list_overflow_error(const System& s) {
+    char buf[256];
+    snprintf(buf, sizeof buf,
+             "Neighbor list overflow: more than %d neighbors for at least one particle. "
+             "Increase max_neighbors_per_atom, or reduce the neighbor skin.",
+             s.max_nb);
+    return tmps_fail(TMPS_ERR_OVERFLOW, buf);
+}
+
+// The host's record of a list build of `total` slots at s.current_step
+// (counters, host mirror); fast / gi receive what the control block records.
+int neigh_record(System& s, int total, int fast[3], int gi[3]) {
     s.npairs = total / 2;
     s.has_built = 1;
     s.build_count += 1;
@@ -248,7 +235,6 @@ int neigh_build(System& s, bool box_known = false) {
         s.onb_cap = s.nb_cap;
     }
     // the build's record in the control block (device and host mirror)
-    int fast[3], gi[3];
     for (int k = 0; k < 3; ++k) {
         fast[k] = s.grid.g[k] >= 3 ? 1 : 0;
         gi[k] = s.grid.g[k];
@@ -261,6 +247,38 @@ int neigh_build(System& s, bool box_known = false) {
         s.hctl->fast_img[k] = fast[k];
         if (split(s)) s.hctl->sc.gi[k] = gi[k];
     }
+    return 0;
+}
+
+// box_known: the host mirror of the control block is current (the caller
+// pulled it and nothing since changed the box), so no round trip is needed.
+int neigh_build(System& s, bool box_known = false) {
+    if (s.n <= 0) return 0;
+    int rc = 0;
+    if (split(s)) {
+        rc = split_positions_to_xr(s);
+        if (rc) return rc;
+        s.x64_valid = true;
+    }
+    if (!box_known) {
+        rc = ctl_pull(s);
+        if (rc) return rc;
+    }
+    rc = neigh_grid(s);
+    if (rc) return rc;
+    rc = launch_bins(s, s.xr);
+    if (rc) return rc;
+    int overflow = 0, total = 0;
+    rc = launch_list(s, s.xr, &overflow, &total);
+    if (rc) return rc;
+    if (overflow) return list_overflow_error(s);
+    rc = launch_hist_clear(s);
+    if (rc) return rc;
+    rc = launch_store_x0(s);
+    if (rc) return rc;
+    int fast[3], gi[3];
+    rc = neigh_record(s, total, fast, gi);
+    if (rc) return rc;
     rc = launch_ctl_built(s, s.last_build_step, fast, gi, split(s) ? 1 : 0);
     if (rc) return rc;
     if (split(s)) {
@@ -271,8 +289,9 @@ int neigh_build(System& s, bool box_known = false) {
     return 0;
 }
 
-int history_save(System& s) {
-    // keep a copy of the current list: the restore merges the new list with it
+// The copy of the list that the history restore merges with, sized for the
+// current list.
+int history_reserve(System& s) {
     const size_t need = (size_t)std::max(s.nslots, 0);
     const size_t hsz = split(s) ? sizeof(float4) : sizeof(double4);
     if (need > s.onb_cap || s.onb_j == nullptr) {
@@ -287,7 +306,14 @@ int history_save(System& s) {
         HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
         s.onb_cap = cap;
     }
-    int rc = launch_list_save(s);
+    return 0;
+}
+
+int history_save(System& s) {
+    // keep a copy of the current list: the restore merges the new list with it
+    const size_t need = (size_t)std::max(s.nslots, 0);
+    int rc = history_reserve(s);
+    if (!rc) rc = launch_list_save(s);
     if (rc) return rc;
     s.onslots = (int)need;
     s.onb_n = s.n;
@@ -702,6 +728,125 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     if (rc) return rc;
     s.timestep = s.hctl->timestep;
     s.current_step = s.timestep;
+    return 0;
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+
+// The list's buffers in a redo entry (a build can move them).
+void redo_entry_lists(const System& s, RedoEntry& e) {
+    e.nb_start = s.nb_start;
+    e.nb_j = s.nb_j;
+    e.nb_flag = s.nb_flag;
+    e.nb_flag2 = s.nb_flag2;
+    e.nb_hist = static_cast<float4*>(s.nb_hist);
+    e.nb_hist2 = static_cast<float4*>(s.nb_hist2);
+    e.nb_code = s.nb_code;
+    e.onb_start = s.onb_start;
+    e.onb_j = s.onb_j;
+    e.onb_flag = s.onb_flag;
+    e.onb_hist = static_cast<float4*>(s.onb_hist);
+    e.onb_cap = (int)std::min<size_t>(s.onb_cap, (size_t)INT32_MAX);
+}
+
+// redo_step (pull_after false) for an FP32-split member, up to the slot
+// count of its list build: the same host mirror updates, capacities and
+// grid, in the same order; the kernels are the batched ones.
+int redo_batch_begin(System& s, RedoEntry& e) {
+    if (!split(s) || s.n <= 0) return tmps_fail(TMPS_ERR_ARG, "batched redo of a non-split system");
+    Ctl& H = *s.hctl;
+    const int stp = H.timestep + 1;
+    e = RedoEntry{};
+    // the resume (copy of set 1 when H.bpar is 1) and the step's start
+    H.abort = 0;
+    H.trigger = -1;
+    s.current_step = stp;
+    // history_save
+    const int old_slots = std::max(s.nslots, 0);
+    int rc = history_reserve(s);
+    if (rc) return rc;
+    e.save_blocks = std::min((std::max(s.n + 1, old_slots) + TMPS_BLOCK - 1) / TMPS_BLOCK, 4096);
+    e.copy_blocks = std::min(nblocks_for(std::max(s.n, old_slots)), 4096);
+    s.onslots = old_slots;
+    s.onb_n = s.n;
+    s.have_old = true;
+    // pbc(s, true)
+    if (H.box.vremap)
+        return tmps_fail(TMPS_ERR_UNSUPPORTED, "remap v is not supported with FP32-split");
+    e.to_x64 = s.x64_valid ? 0 : 1;
+    s.x64_valid = true;
+    // neigh_build(s, true), up to the list's row counts
+    rc = neigh_grid(s);
+    if (!rc) rc = list_reserve_rows(s);
+    if (rc) return rc;
+    e.ctl = s.ctl;
+    e.n = s.n;
+    e.ncell = s.grid.ncell;
+    e.old_n = s.onb_start ? s.n : 0;
+    e.max_nb = s.max_nb;
+    e.servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    e.grid = s.grid;
+    e.skin = s.skin;
+    e.pair = s.pair;
+    e.partial = s.partial;
+    e.xr = s.xr;
+    e.vm = s.vm;
+    e.x0 = s.x0;
+    e.bin_x = s.bin_x;
+    e.up = s.up;
+    e.up2 = s.up2;
+    e.vmf = s.vmf;
+    e.vmf2 = s.vmf2;
+    e.omf = s.omf;
+    e.omf2 = s.omf2;
+    e.ff = s.ff;
+    e.tqf = s.tqf;
+    e.u0 = s.u0;
+    e.cell = s.cell;
+    e.vir = s.vir;
+    e.cell_of = s.cell_of;
+    e.cell_count = s.cell_count;
+    e.cell_start = s.cell_start;
+    e.cell_fill = s.cell_fill;
+    e.cell_particles = s.cell_particles;
+    e.scan_tmp = s.scan_tmp;
+    e.full_count = s.full_count;
+    e.half_count = s.half_count;
+    e.flags = s.flags;
+    e.row_tmp = s.row_tmp;
+    redo_entry_lists(s, e);
+    return 0;
+}
+
+// The rest of redo_step's host work once the list's flags are known.
+int redo_batch_list(System& s, int4 flags, RedoEntry& e) {
+    if (flags.x) return list_overflow_error(s);
+    int rc = ensure_list_capacity(s, (size_t)flags.z);
+    if (rc) return rc;
+    s.nslots = flags.z;
+    int fast[3], gi[3];
+    rc = neigh_record(s, flags.z, fast, gi);
+    if (rc) return rc;
+    s.x64_valid = false;  // the split encoding from the new bins
+    s.have_old = false;   // history_restore
+    s.rebuilds += 1;
+    Ctl& H = *s.hctl;
+    const int stp = s.current_step;
+    H.timestep = stp;
+    s.timestep = stp;
+    // the member rejoins the segments
+    H.bpar = 0;
+    H.bkick = 1;
+    H.bdone = stp == H.bend ? 1 : 0;
+    e.long_rows = flags.y ? 1 : 0;
+    e.codes = (s.nslots > 0 && s.nb_code) ? 1 : 0;
+    e.step = stp;
+    for (int k = 0; k < 3; ++k) {
+        e.fast[k] = fast[k];
+        e.gi[k] = gi[k];
+    }
+    e.done = H.bdone;
+    redo_entry_lists(s, e);
     return 0;
 }
 

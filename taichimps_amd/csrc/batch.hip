@@ -9,23 +9,23 @@
 // close launch with a block per member (k_batch_close). Each member carries
 // its own progress on the device (the buffer set its next step reads, a
 // pending half-kick, the step that ends its run), so a member whose skin
-// check asks for a rebuild is redone by the host after the segment (the
-// single-system redo_step, several members at once on host threads) and
-// rejoins the next segment, while the others never wait. A member's
-// arithmetic, tree and servo are those of its single-system run, so its
-// results are bitwise the ones tmps_run gives it alone.
+// check asks for a rebuild stops there while the others never wait. After
+// the segment the steps of all such members are redone together, each with
+// its list rebuild: the host work of the single-system redo_step per member,
+// and each of its kernels launched once for all of them over block maps
+// (redo_batched). The members then rejoin the next segment. A member's
+// arithmetic, bins, list, tree and servo are those of its single-system run,
+// so its results are bitwise the ones tmps_run gives it alone.
 //
-// Launch configuration (segment length, order of the blocks in the map) and
-// the number of redo threads change no result; tmps_batch_run picks them by
-// timing the first segments of the run itself (the autotuner).
+// Launch configuration (segment length, order of the blocks in the map)
+// changes no result; tmps_batch_run picks it by timing the first segments of
+// the run itself (the autotuner).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "tmps_internal.h"
@@ -65,11 +65,6 @@ struct Config {
 const Config kConfigs[] = {{8, 0}, {8, 1}, {16, 0}, {16, 1}, {32, 0}, {32, 1}};
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 
-// candidate numbers of redo threads
-const int kRedoThreads[] = {4, 8, 16};
-constexpr int kNumRedoThreads = sizeof(kRedoThreads) / sizeof(kRedoThreads[0]);
-// a segment is a redo-thread probe only if it has at least this many redos
-constexpr int kRedoProbeMin = 8;
 
 }  // namespace
 
@@ -91,18 +86,21 @@ struct tmps_batch_impl {
     int4* h_status = nullptr;
     Ctl* d_ctls = nullptr;  // members' control blocks after the last segment
     Ctl* h_ctls = nullptr;
-    std::vector<hipStream_t> worker_streams;  // one per redo thread
     hipGraphExec_t exec[kNumConfigs] = {};
     double exec_dt[kNumConfigs] = {};
     int config = -1;  // chosen launch configuration (-1: not tuned yet)
     double config_rate[kNumConfigs] = {};  // measured atom-steps per second
     int rebuilds = 0;
-    // host threads that redo members after rebuild requests: the autotuner
-    // times each candidate count on a segment with enough redos and keeps the
-    // fastest (more threads keep more members' redo chains in flight, but
-    // they contend for the runtime and the hardware queues)
-    int redo_threads = 8;
-    double redo_ms[kNumRedoThreads] = {};  // measured wall ms per redo
+    // the batched redo: entries and block maps (pinned staging, one device
+    // buffer), the members' list flags, and the last run's redo passes
+    std::vector<RedoEntry> redo_entries;
+    char* h_redo = nullptr;
+    void* d_redo = nullptr;
+    size_t redo_cap = 0;
+    int4* d_redo_flags = nullptr;
+    int4* h_redo_flags = nullptr;
+    int redo_passes = 0;
+    double redo_ms = 0.0;
 };
 
 namespace {
@@ -124,10 +122,10 @@ void free_batch(Batch& B) {
     if (B.h_status) (void)hipHostFree(B.h_status);
     if (B.d_ctls) (void)hipFree(B.d_ctls);
     if (B.h_ctls) (void)hipHostFree(B.h_ctls);
-    for (hipStream_t ws : B.worker_streams) {
-        (void)hipStreamSynchronize(ws);
-        (void)hipStreamDestroy(ws);
-    }
+    if (B.d_redo) (void)hipFree(B.d_redo);
+    if (B.h_redo) (void)hipHostFree(B.h_redo);
+    if (B.d_redo_flags) (void)hipFree(B.d_redo_flags);
+    if (B.h_redo_flags) (void)hipHostFree(B.h_redo_flags);
     if (B.stream) (void)hipStreamDestroy(B.stream);
 }
 
@@ -218,6 +216,8 @@ int build(Batch& B) {
     HIP_TRY(hipHostMalloc((void**)&B.h_status, ninst * sizeof(int4), hipHostMallocDefault));
     HIP_TRY(hipMalloc((void**)&B.d_ctls, ninst * sizeof(Ctl)));
     HIP_TRY(hipHostMalloc((void**)&B.h_ctls, ninst * sizeof(Ctl), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void**)&B.d_redo_flags, ninst * sizeof(int4)));
+    HIP_TRY(hipHostMalloc((void**)&B.h_redo_flags, ninst * sizeof(int4), hipHostMallocDefault));
     std::vector<int> all, by_style[3];
     for (int k = 0; k < ninst; ++k) {
         System& s = *B.sys[k];
@@ -338,85 +338,108 @@ int start_member(System& s, int nsteps, double dt, int run_pre) {
     return 0;
 }
 
-// Member k asked for a rebuild: redo its step alone, then it rejoins.
-// Member k asked for a rebuild: redo its step alone, then it rejoins. The
-// control block it starts from is the segment's gathered copy (the device
-// copy has not changed since), and the rejoin is written on the device, so
-// the only host round trips are the list build's slot count and the final
-// wait; an error of the redone step shows in the next segment's status.
-int redo_member_on(Batch& B, int k, double dt) {
-    System& s = *B.sys[k];
-    *s.hctl = B.h_ctls[k];
-    int rc = tmps_api::check_run_error(s);
-    if (rc) return rc;
-    s.timestep = s.hctl->timestep;
-    rc = tmps_api::redo_step(s, s.hctl->bpar, dt, 0, 0.0, 0.0, false);
-    if (rc) return rc;
-    Ctl& H = *s.hctl;
-    H.bpar = 0;
-    H.bkick = 1;
-    H.bdone = s.timestep == H.bend ? 1 : 0;
-    hipLaunchKernelGGL(k_batch_rejoin, dim3(1), dim3(64), 0, s.stream, s.ctl, H.bdone);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s.stream));
-    return 0;
-}
-
-// The redo runs on the calling thread's stream `ws` rather than the member's
-// own (nothing is pending on either when it starts, and `ws` is drained when
-// it ends): a few streams shared by many members' redos cost the runtime less
-// than one stream per member.
-int redo_member(Batch& B, int k, double dt, hipStream_t ws) {
-    System& s = *B.sys[k];
-    const hipStream_t own = s.stream;
-    s.stream = ws;
-    const int rc = redo_member_on(B, k, dt);
-    if (rc) (void)hipStreamSynchronize(ws);
-    s.stream = own;
-    return rc;
-}
-
-// Redoes the members in `which`, several at a time: each redo is a chain of
-// small kernels and host round trips on the member's own stream, so host
-// threads (up to `threads`) keep several members' chains in flight at once.
-// The first failure (in member order) is reported.
-int redo_members(Batch& B, const std::vector<int>& which, double dt, int threads) {
+// Redoes the members in `which` together: each member's host work is
+// redo_step's (tmps_api::redo_batch_begin / redo_batch_list), and each
+// kernel of the redo runs once for all of them over block maps. Two host
+// round trips in all: the list builds' flags, and the caller's final wait.
+int redo_batched(Batch& B, const std::vector<int>& which, double dt) {
     const int m = (int)which.size();
     if (m == 0) return 0;
-    const int nt = std::max(1, std::min(threads, m));
-    while ((int)B.worker_streams.size() < nt) {
-        hipStream_t ws = nullptr;
-        HIP_TRY(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking));
-        B.worker_streams.push_back(ws);
+    std::vector<RedoEntry>& E = B.redo_entries;
+    E.resize(m);
+    std::vector<int2> atoms, cells, cell_tiles, atom_tiles, copy, save, style[3];
+    int max_parts = 1;
+    for (int q = 0; q < m; ++q) {
+        System& s = *B.sys[which[q]];
+        *s.hctl = B.h_ctls[which[q]];
+        int rc = tmps_api::check_run_error(s);
+        if (rc) return rc;
+        s.timestep = s.hctl->timestep;
+        const int par = s.hctl->bpar;
+        rc = tmps_api::redo_batch_begin(s, E[q]);
+        if (rc) return rc;
+        const RedoEntry& e = E[q];
+        const int nb = nblocks_for(e.n);
+        for (int b = 0; b < nb; ++b) atoms.push_back(make_int2(q, b));
+        for (int b = 0; b < nb; ++b) style[style_of(s)].push_back(make_int2(q, b));
+        for (int b = 0; b < nblocks_for(e.ncell + 1); ++b) cells.push_back(make_int2(q, b));
+        for (int b = 0; b < (e.ncell + TMPS_SCAN_TILE - 1) / TMPS_SCAN_TILE; ++b)
+            cell_tiles.push_back(make_int2(q, b));
+        for (int b = 0; b < (e.n + TMPS_SCAN_TILE - 1) / TMPS_SCAN_TILE; ++b)
+            atom_tiles.push_back(make_int2(q, b));
+        if (par == 1)
+            for (int b = 0; b < e.copy_blocks; ++b) copy.push_back(make_int2(q, b));
+        for (int b = 0; b < e.save_blocks; ++b) save.push_back(make_int2(q, b));
+        max_parts = std::max(max_parts, nb);
     }
-    if (nt == 1) {
-        for (int k : which) {
-            int rc = redo_member(B, k, dt, B.worker_streams[0]);
-            if (rc) return rc;
-        }
-        return 0;
+    // one staging buffer: the entries, then the maps
+    const size_t entries_bytes = (size_t)m * sizeof(RedoEntry);
+    std::vector<const std::vector<int2>*> lists = {&atoms, &cells, &cell_tiles, &atom_tiles,
+                                                   &copy, &save, &style[0], &style[1], &style[2]};
+    size_t bytes = entries_bytes;
+    for (const auto* l : lists) bytes += l->size() * sizeof(int2);
+    if (bytes > B.redo_cap) {
+        HIP_TRY(hipStreamSynchronize(B.stream));
+        if (B.d_redo) HIP_TRY(hipFree(B.d_redo));
+        if (B.h_redo) HIP_TRY(hipHostFree(B.h_redo));
+        B.d_redo = nullptr;
+        B.h_redo = nullptr;
+        B.redo_cap = 0;
+        const size_t cap = bytes + bytes / 2 + 4096;
+        HIP_TRY(hipMalloc(&B.d_redo, cap));
+        HIP_TRY(hipHostMalloc((void**)&B.h_redo, cap, hipHostMallocDefault));
+        B.redo_cap = cap;
     }
-    std::atomic<int> next{0};
-    std::atomic<bool> failed{false};
-    std::vector<int> rcs(m, 0);
-    std::vector<std::string> msgs(m);
-    auto work = [&](int t) {
-        for (int q = next.fetch_add(1); q < m && !failed.load(); q = next.fetch_add(1)) {
-            const int rc = redo_member(B, which[q], dt, B.worker_streams[t]);
-            if (rc) {
-                rcs[q] = rc;
-                msgs[q] = tmps_last_error();
-                failed.store(true);
-            }
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& t : pool) t.join();
-    for (int q = 0; q < m; ++q)
-        if (rcs[q]) return tmps_fail(rcs[q], msgs[q]);
-    return 0;
+    std::memcpy(B.h_redo, E.data(), entries_bytes);
+    const int2* dmap[9];
+    size_t off = entries_bytes;
+    for (int k = 0; k < 9; ++k) {
+        const size_t nbytes = lists[k]->size() * sizeof(int2);
+        if (nbytes) std::memcpy(B.h_redo + off, lists[k]->data(), nbytes);
+        dmap[k] = reinterpret_cast<const int2*>(static_cast<char*>(B.d_redo) + off);
+        off += nbytes;
+    }
+    RedoMaps M{};
+    M.entries = m;
+    M.max_parts = max_parts;
+    M.atoms = dmap[0]; M.n_atoms = (int)atoms.size();
+    M.cells = dmap[1]; M.n_cells = (int)cells.size();
+    M.cell_tiles = dmap[2]; M.n_cell_tiles = (int)cell_tiles.size();
+    M.atom_tiles = dmap[3]; M.n_atom_tiles = (int)atom_tiles.size();
+    M.copy = dmap[4]; M.n_copy = (int)copy.size();
+    M.save = dmap[5]; M.n_save = (int)save.size();
+    for (int st = 0; st < 3; ++st) {
+        M.style[st] = dmap[6 + st];
+        M.n_style[st] = (int)style[st].size();
+    }
+    const RedoEntry* T = static_cast<const RedoEntry*>(B.d_redo);
+    const hipStream_t st = B.stream;
+    HIP_TRY(hipMemcpyAsync(B.d_redo, B.h_redo, bytes, hipMemcpyHostToDevice, st));
+    int rc = launch_redo_resume(T, M, st);
+    if (!rc) rc = launch_redo_begin_split(T, M, (float)dt, st);
+    if (!rc) rc = launch_redo_save(T, M, st);
+    if (!rc) rc = launch_redo_pbc(T, M, st);
+    if (!rc) rc = launch_redo_bins(T, M, st);
+    if (!rc) rc = launch_redo_rows(T, M, B.d_redo_flags, st);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(B.h_redo_flags, B.d_redo_flags, m * sizeof(int4), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int q = 0; q < m; ++q) {
+        rc = tmps_api::redo_batch_list(*B.sys[which[q]], B.h_redo_flags[q], E[q]);
+        if (rc) return rc;
+    }
+    std::memcpy(B.h_redo, E.data(), entries_bytes);
+    HIP_TRY(hipMemcpyAsync(B.d_redo, B.h_redo, entries_bytes, hipMemcpyHostToDevice, st));
+    rc = launch_redo_list(T, M, st);
+    if (!rc) rc = launch_redo_built(T, M, st);
+    if (!rc) rc = launch_redo_encode_split(T, M, st);
+    if (!rc) rc = launch_redo_restore(T, M, st);
+    if (!rc) rc = launch_redo_force_split(T, M, (float)dt, st);
+    if (!rc) rc = launch_redo_end(T, M, dt, st);
+    if (!rc) rc = launch_redo_copy_vw_split(T, M, st);
+    if (!rc) rc = launch_redo_rejoin(T, M, st);
+    return rc;
 }
 
 int finish_member(System& s) {
@@ -453,13 +476,14 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
     if (rc0) return rc0;
     HIP_TRY(hipStreamSynchronize(B.stream));
     B.rebuilds = 0;
+    B.redo_passes = 0;
+    B.redo_ms = 0.0;
     // the autotuner: the first segments of the run try each configuration
     // once; its rate is the atom-steps that segment completed per second of
     // its launch (host redo work excluded), and the fastest runs the rest
     int probe = (autotune || B.config < 0) ? 0 : kNumConfigs;
     double best_rate = -1.0;
     int best = B.config >= 0 ? B.config : 2;
-    int tprobe = (autotune || B.redo_ms[0] == 0.0) ? 0 : kNumRedoThreads;
     std::vector<int> prev(ninst), redo;
     for (int k = 0; k < ninst; ++k) prev[k] = B.sys[k]->timestep;
     bool running = nsteps > 0;
@@ -485,23 +509,9 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
             else if (!st.w)
                 running = true;
         }
-        const bool tprobe_now = tprobe < kNumRedoThreads && (int)redo.size() >= kRedoProbeMin;
-        const int threads = tprobe_now ? kRedoThreads[tprobe] : B.redo_threads;
         const auto r0 = std::chrono::steady_clock::now();
-        rc = redo_members(B, redo, dt, threads);
+        rc = redo_batched(B, redo, dt);
         if (rc) return rc;
-        if (tprobe_now) {
-            const double ms =
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0)
-                    .count();
-            B.redo_ms[tprobe] = ms / (double)redo.size();
-            if (++tprobe == kNumRedoThreads) {
-                int bt = 0;
-                for (int q = 1; q < kNumRedoThreads; ++q)
-                    if (B.redo_ms[q] < B.redo_ms[bt]) bt = q;
-                B.redo_threads = kRedoThreads[bt];
-            }
-        }
         B.rebuilds += (int)redo.size();
         for (int k : redo) {
             // the rebuild may have moved the member's list buffers
@@ -514,6 +524,10 @@ int run(Batch& B, int nsteps, double dt, int run_pre, int autotune) {
             rc = upload_table(B);
             if (rc) return rc;
             HIP_TRY(hipStreamSynchronize(B.stream));
+            B.redo_passes += 1;
+            B.redo_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0)
+                    .count();
         }
         if (probe < kNumConfigs) {
             const double rate = sec > 0.0 ? atom_steps / sec : 0.0;
@@ -600,7 +614,7 @@ int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotu
 }
 
 int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
-                      int* rebuilds, int* redo_threads, double redo_ms[3]) {
+                      int* rebuilds, int* redo_passes, double* redo_ms) {
     if (!b) return tmps_fail(TMPS_ERR_ARG, "null ensemble");
     Batch& B = *reinterpret_cast<Batch*>(b);
     const int c = B.config >= 0 ? B.config : 2;
@@ -609,9 +623,8 @@ int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rate
     if (rates)
         for (int k = 0; k < kNumConfigs; ++k) rates[k] = B.config_rate[k];
     if (rebuilds) *rebuilds = B.rebuilds;
-    if (redo_threads) *redo_threads = B.redo_threads;
-    if (redo_ms)
-        for (int k = 0; k < kNumRedoThreads; ++k) redo_ms[k] = B.redo_ms[k];
+    if (redo_passes) *redo_passes = B.redo_passes;
+    if (redo_ms) *redo_ms = B.redo_ms;
     return 0;
 }
 

@@ -474,9 +474,9 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_final_tree(Ctl* ctl, const do
 // servo. `advance` marks a fused step: it honours the segment's abort and
 // rebuild-trigger flags and completes the step's bookkeeping.
 template <int L>
-__global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const double* partial,
-                                                             int nparts, double dt, int advance,
-                                                             int apply_servo, int natoms) {
+__device__ __forceinline__ void step_end_block(Ctl* ctl, const double* partial, int nparts,
+                                               double dt, int advance, int apply_servo,
+                                               int natoms) {
     if (advance) {
         if (ctl->abort) return;
         const int t = ctl->timestep + 1;
@@ -496,6 +496,13 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const doub
         }
         if (advance && !ctl->abort) ctl->timestep = ctl->timestep + 1;
     }
+}
+
+template <int L>
+__global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const double* partial,
+                                                             int nparts, double dt, int advance,
+                                                             int apply_servo, int natoms) {
+    step_end_block<L>(ctl, partial, nparts, dt, advance, apply_servo, natoms);
 }
 
 // End of a fused step, in one block: the pairwise tree over the step pass's
@@ -602,13 +609,15 @@ __global__ void k_remap_clear(Ctl* ctl, int honor_abort) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->remap.step = -1;
 }
 
+__device__ __forceinline__ void ctl_rebuilt(Ctl* ctl, int step, int split) {
+    ctl->trigger = -1;
+    ctl->abort = 0;
+    ctl->last_build_step = step;
+    if (split) update_split_consts(ctl, 0);
+}
+
 __global__ void k_ctl_rebuilt(Ctl* ctl, int step, int split) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ctl->trigger = -1;
-        ctl->abort = 0;
-        ctl->last_build_step = step;
-        if (split) update_split_consts(ctl, 0);
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl_rebuilt(ctl, step, split);
 }
 
 // A step cut short by a rebuild request is about to be redone.
@@ -623,19 +632,21 @@ __global__ void k_ctl_resume(Ctl* ctl) {
 // it has at least 3 bins, and (FP32-split) the bins of the encoding.
 // (FP32-split: with the encoding's bins, and the split constants of the
 // build box, which becomes the skin check's reference)
-__global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ctl->last_build_step = step;
-        ctl->fast_img[0] = fast.x;
-        ctl->fast_img[1] = fast.y;
-        ctl->fast_img[2] = fast.z;
-        if (split) {
-            ctl->sc.gi[0] = gi.x;
-            ctl->sc.gi[1] = gi.y;
-            ctl->sc.gi[2] = gi.z;
-            update_split_consts(ctl, 1);
-        }
+__device__ __forceinline__ void ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
+    ctl->last_build_step = step;
+    ctl->fast_img[0] = fast.x;
+    ctl->fast_img[1] = fast.y;
+    ctl->fast_img[2] = fast.z;
+    if (split) {
+        ctl->sc.gi[0] = gi.x;
+        ctl->sc.gi[1] = gi.y;
+        ctl->sc.gi[2] = gi.z;
+        update_split_consts(ctl, 1);
     }
+}
+
+__global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl_built(ctl, step, fast, gi, split);
 }
 
 // ---------------------------------------------------- fused step kernels
@@ -1009,23 +1020,86 @@ __global__ void k_kick_f64(double4* vm, double4* om, const double4* xr, const do
 // Buffer set 1 -> set 0: per-atom 16/32-byte records and per-slot flags and
 // history, one grid-stride pass.
 template <class T4>
-__global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1, const T4* w1,
-                             int n, uint8_t* fl0, const uint8_t* fl1, T4* h0, const T4* h1,
-                             const int* nb_start, const Ctl* ctl, int honor_abort) {
-    if (honor_abort && ctl->abort) return;
+__device__ __forceinline__ void copy_state_items(int first, int stride, T4* x0, T4* v0, T4* w0,
+                                                 const T4* x1, const T4* v1, const T4* w1, int n,
+                                                 uint8_t* fl0, const uint8_t* fl1, T4* h0,
+                                                 const T4* h1, const int* nb_start) {
     // the slot count of the current list, read on the device: a captured
     // segment outlives the list it was captured with
     const int nslots = nb_start[n];
-    const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int i = first; i < n; i += stride) {
         x0[i] = x1[i];
         v0[i] = v1[i];
         w0[i] = w1[i];
     }
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nslots; k += stride) {
+    for (int k = first; k < nslots; k += stride) {
         fl0[k] = fl1[k];
         h0[k] = h1[k];
     }
+}
+
+template <class T4>
+__global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1, const T4* w1,
+                             int n, uint8_t* fl0, const uint8_t* fl1, T4* h0, const T4* h1,
+                             const int* nb_start, const Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
+    copy_state_items<T4>(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, x0, v0, w0,
+                         x1, v1, w1, n, fl0, fl1, h0, h1, nb_start);
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+//
+// The control-block steps, buffer-set copy and step end of ensemble members'
+// redone rebuild steps, many members per launch (the bodies of the
+// single-system kernels above).
+
+// entries that resume from set 1: set 1 -> set 0 (honor_abort 0)
+__global__ void kb_redo_copy_state(const RedoEntry* T, const int2* map) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    copy_state_items<float4>(mb.y * TMPS_BLOCK + threadIdx.x, e.copy_blocks * TMPS_BLOCK, e.up,
+                             e.vmf, e.omf, e.up2, e.vmf2, e.omf2, e.n, e.nb_flag, e.nb_flag2,
+                             e.nb_hist, e.nb_hist2, e.nb_start);
+}
+
+// one block per entry (k_ctl_resume)
+__global__ void kb_redo_resume(const RedoEntry* T) {
+    if (threadIdx.x != 0) return;
+    Ctl* ctl = T[blockIdx.x].ctl;
+    ctl->trigger = -1;
+    ctl->abort = 0;
+}
+
+// The build record (k_ctl_built), then the resume after the rebuild
+// (k_ctl_rebuilt). Between the two, the single-system redo runs the split
+// encoding, the slot codes and the history restore, which read neither
+// what k_ctl_rebuilt writes nor anything it derives from.
+__global__ void kb_redo_built(const RedoEntry* T) {
+    if (threadIdx.x != 0) return;
+    const RedoEntry& e = T[blockIdx.x];
+    ctl_built(e.ctl, e.step, make_int3(e.fast[0], e.fast[1], e.fast[2]),
+              make_int3(e.gi[0], e.gi[1], e.gi[2]), 1);
+    ctl_rebuilt(e.ctl, e.step, 1);
+}
+
+// one block per entry (k_step_end with advance)
+template <int L>
+__global__ void __launch_bounds__(FINAL_THREADS) kb_redo_end(const RedoEntry* T, double dt) {
+    const RedoEntry& e = T[blockIdx.x];
+    const int nparts = ((e.n > 0 ? e.n : 1) + TMPS_BLOCK - 1) / TMPS_BLOCK;
+    step_end_block<L>(e.ctl, e.partial, nparts, dt, 1, e.servo_on, e.n);
+}
+
+// The pending remap cleared unless the step aborted (k_remap_clear), and the
+// member rejoins the segments: its next step reads set 0 after a half-kick.
+__global__ void kb_redo_rejoin(const RedoEntry* T) {
+    if (threadIdx.x != 0) return;
+    const RedoEntry& e = T[blockIdx.x];
+    Ctl* ctl = e.ctl;
+    if (!ctl->abort) ctl->remap.step = -1;
+    ctl->bpar = 0;
+    ctl->bkick = 1;
+    ctl->bdone = e.done;
 }
 
 }  // namespace
@@ -1359,6 +1433,47 @@ int launch_copy_state(System& s, int honor_abort) {
                            s.vmf, s.omf, s.up2, s.vmf2, s.omf2, s.n, s.nb_flag, s.nb_flag2,
                            static_cast<float4*>(s.nb_hist), static_cast<const float4*>(s.nb_hist2),
                            (const int*)s.nb_start, s.ctl, honor_abort);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+
+int launch_redo_resume(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    if (M.n_copy > 0) {
+        hipLaunchKernelGGL(kb_redo_copy_state, dim3(M.n_copy), dim3(TMPS_BLOCK), 0, st, T, M.copy);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kb_redo_resume, dim3(M.entries), dim3(64), 0, st, T);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_redo_built(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    hipLaunchKernelGGL(kb_redo_built, dim3(M.entries), dim3(64), 0, st, T);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+#define REDO_END_CASE(Lv)                                                                       \
+    case Lv:                                                                                    \
+        hipLaunchKernelGGL((kb_redo_end<Lv>), dim3(M.entries), dim3(FINAL_THREADS), 0, st, T, dt); \
+        break;
+
+// The tree's leaves per thread follow the largest member; zero padding and
+// aligned subtrees leave every member's tree its own.
+int launch_redo_end(const RedoEntry* T, const RedoMaps& M, double dt, hipStream_t st) {
+    switch (final_leaves(std::max(M.max_parts, 1))) {
+        REDO_END_CASE(1) REDO_END_CASE(2) REDO_END_CASE(4) REDO_END_CASE(8) REDO_END_CASE(16)
+        REDO_END_CASE(32) REDO_END_CASE(64)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_redo_rejoin(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    hipLaunchKernelGGL(kb_redo_rejoin, dim3(M.entries), dim3(64), 0, st, T);
     HIP_TRY(hipGetLastError());
     return 0;
 }

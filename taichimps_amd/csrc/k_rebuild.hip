@@ -11,9 +11,7 @@ namespace {
 // ------------------------------------------------------------------ PBC
 
 // Domain.pbc_wrap + _pbc_vremap_kernel, orthogonal branch (domain.py:502-571)
-__global__ void k_pbc(double4* xr, double4* vm, int n, const Ctl* ctl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void pbc_atom(int i, double4* xr, double4* vm, const Ctl* ctl) {
     const DevBox& B = ctl->box;
     double4 x = xr[i];
     double xs[3] = {x.x, x.y, x.z};
@@ -38,6 +36,11 @@ __global__ void k_pbc(double4* xr, double4* vm, int n, const Ctl* ctl) {
     }
     x.x = xs[0]; x.y = xs[1]; x.z = xs[2];
     xr[i] = x;
+}
+
+__global__ void k_pbc(double4* xr, double4* vm, int n, const Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pbc_atom(i, xr, vm, ctl);
 }
 
 // ------------------------------------------------------------------ bins
@@ -75,13 +78,14 @@ __global__ void k_zero_int3(int* a, int* b, int n, int* c, int nc) {
 // The current list into the copy the history remap merges with (the
 // reference's save_state, contact_history.py:141-145): row starts, partners,
 // touch flags and histories, the slot count read on the device.
+// Items first, first + stride, ... of the copy.
 template <typename H>
-__global__ void k_list_save(int n, const int* nb_start, const int* nb_j, const uint8_t* nb_flag,
-                            const H* nb_hist, int* onb_start, int* onb_j, uint8_t* onb_flag,
-                            H* onb_hist, int cap) {
+__device__ __forceinline__ void list_save_items(int first, int stride, int n, const int* nb_start,
+                                                const int* nb_j, const uint8_t* nb_flag,
+                                                const H* nb_hist, int* onb_start, int* onb_j,
+                                                uint8_t* onb_flag, H* onb_hist, int cap) {
     const int nslots = nb_start[n];
-    const int stride = gridDim.x * blockDim.x;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= n || k < nslots; k += stride) {
+    for (int k = first; k <= n || k < nslots; k += stride) {
         if (k <= n) onb_start[k] = nb_start[k];
         if (k < nslots && k < cap) {
             onb_j[k] = nb_j[k];
@@ -91,11 +95,17 @@ __global__ void k_list_save(int n, const int* nb_start, const int* nb_j, const u
     }
 }
 
+template <typename H>
+__global__ void k_list_save(int n, const int* nb_start, const int* nb_j, const uint8_t* nb_flag,
+                            const H* nb_hist, int* onb_start, int* onb_j, uint8_t* onb_flag,
+                            H* onb_hist, int cap) {
+    list_save_items<H>(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, n, nb_start,
+                       nb_j, nb_flag, nb_hist, onb_start, onb_j, onb_flag, onb_hist, cap);
+}
+
 // build_grid count phase, neighbor.py:264-268
-__global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, int* cell_of,
-                            int* count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void bin_count_atom(int i, const double4* xr, const Ctl* ctl,
+                                               const GridP& G, int* cell_of, int* count) {
     int c[3];
     cell_coord(ctl->box, G, xr[i], c);
     const int idx = c[0] + c[1] * G.g[0] + c[2] * G.g[0] * G.g[1];
@@ -103,20 +113,28 @@ __global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, i
     atomicAdd(&count[idx], 1);
 }
 
-// build_grid scatter phase, neighbor.py:276-282
-__global__ void k_bin_fill(int n, const int* cell_of, const int* start, int* fill, int* particles) {
+__global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, int* cell_of,
+                            int* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i < n) bin_count_atom(i, xr, ctl, G, cell_of, count);
+}
+
+// build_grid scatter phase, neighbor.py:276-282
+__device__ __forceinline__ void bin_fill_atom(int i, const int* cell_of, const int* start,
+                                              int* fill, int* particles) {
     const int c = cell_of[i];
     const int slot = atomicAdd(&fill[c], 1);
     particles[start[c] + slot] = i;
 }
 
+__global__ void k_bin_fill(int n, const int* cell_of, const int* start, int* fill, int* particles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bin_fill_atom(i, cell_of, start, fill, particles);
+}
+
 // Bin members in ascending atom index (the atomic scatter leaves them in
 // arrival order).
-__global__ void k_bin_sort(int ncell, const int* start, int* particles) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+__device__ __forceinline__ void bin_sort_cell(int c, const int* start, int* particles) {
     const int a = start[c], b = start[c + 1];
     for (int p = a + 1; p < b; ++p) {
         const int v = particles[p];
@@ -127,6 +145,11 @@ __global__ void k_bin_sort(int ncell, const int* start, int* particles) {
         }
         particles[q + 1] = v;
     }
+}
+
+__global__ void k_bin_sort(int ncell, const int* start, int* particles) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell) bin_sort_cell(c, start, particles);
 }
 
 // ------------------------------------------------------------------ scan
@@ -160,9 +183,12 @@ __device__ __forceinline__ int block_excl_scan(int v, int* excl) {
 
 #define SCAN_PER_THREAD 4
 #define SCAN_TILE (TMPS_BLOCK * SCAN_PER_THREAD)
+static_assert(SCAN_TILE == TMPS_SCAN_TILE, "scan tile of the batched redo's maps");
 
-__global__ void k_scan_tiles(const int* in, int* out, int n, int* tile_sums) {
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+// Tile `tile` of the exclusive scan; its sum into tile_sums[tile].
+__device__ __forceinline__ void scan_tile(int tile, const int* in, int* out, int n,
+                                          int* tile_sums) {
+    const int base = tile * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     int v[SCAN_PER_THREAD];
     int sum = 0;
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
@@ -176,11 +202,16 @@ __global__ void k_scan_tiles(const int* in, int* out, int n, int* tile_sums) {
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) tile_sums[tile] = total;
+}
+
+__global__ void k_scan_tiles(const int* in, int* out, int n, int* tile_sums) {
+    scan_tile(blockIdx.x, in, out, n, tile_sums);
 }
 
 // One block: exclusive scan of the tile sums in place; writes the grand total.
-__global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out, int* total_copy) {
+__device__ __forceinline__ void scan_sums(int* tile_sums, int ntiles, int* total_out,
+                                          int* total_copy) {
     const int per = (ntiles + TMPS_BLOCK - 1) / TMPS_BLOCK;
     const int a = threadIdx.x * per;
     int sum = 0;
@@ -199,6 +230,10 @@ __global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out, int* tot
         *total_out = total;
         if (total_copy) *total_copy = total;
     }
+}
+
+__global__ void k_scan_sums(int* tile_sums, int ntiles, int* total_out, int* total_copy) {
+    scan_sums(tile_sums, ntiles, total_out, total_copy);
 }
 
 __global__ void k_scan_add(int* out, int n, const int* tile_sums) {
@@ -227,11 +262,10 @@ __device__ __forceinline__ double list_image(double d, double prd, double inv_pr
 // visiting bins in the reference's (dx, dy, dz) order, writing each atom's
 // full row at nb_start[i] and sorting it ascending in j: the path for lists
 // with a row longer than k_list_rows' scratch (whose counts gave nb_start).
-__global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G,
-                            const int* cell_start, const int* particles, double skin,
-                            const int* nb_start, int* nb_j) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void list_fill_atom(int i, const double4* xr, const Ctl* ctl,
+                                               const GridP& G, const int* cell_start,
+                                               const int* particles, double skin,
+                                               const int* nb_start, int* nb_j) {
     const DevBox& B = ctl->box;
     const double4 pi = xr[i];
     int c[3];
@@ -284,6 +318,13 @@ __global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G,
         }
         row[q + 1] = v;
     }
+}
+
+__global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G,
+                            const int* cell_start, const int* particles, double skin,
+                            const int* nb_start, int* nb_j) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) list_fill_atom(i, xr, ctl, G, cell_start, particles, skin, nb_start, nb_j);
 }
 
 // Positions in bin order (after k_bin_sort): bx[p] = xr[particles[p]], so the
@@ -347,20 +388,19 @@ __device__ __forceinline__ void list_range(int a, int b, int i, const double4& p
 // full_count / half_count, flags[0] is raised past max_nb half entries and
 // flags[1] when a row exceeds LIST_ROW_CAP (k_list_fill then writes the rows
 // directly).
-__global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
-                                                          GridP G, const int* cell_start,
-                                                          const int* particles, const double4* bx,
-                                                          double skin, int max_nb, int* full_count,
-                                                          int* half_count, int* flags, int* rows) {
-    __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
-    // Threads walk the atoms in bin order: a wave's atoms then sit in
-    // neighbouring bins of one bin row, share their 9 candidate rows and read
-    // overlapping candidate ranges (about 2.5x faster than atom order at 1M
-    // atoms). The row written for atom i does not depend on the order.
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+//
+// Threads walk the atoms in bin order: a wave's atoms then sit in
+// neighbouring bins of one bin row, share their 9 candidate rows and read
+// overlapping candidate ranges (about 2.5x faster than atom order at 1M
+// atoms). The row written for atom i does not depend on the order. `row` is
+// the thread's column of the block's LDS rows.
+__device__ __forceinline__ void list_rows_item(int p, int* row, const double4* xr, int n,
+                                               const Ctl* ctl, const GridP& G,
+                                               const int* cell_start, const int* particles,
+                                               const double4* bx, double skin, int max_nb,
+                                               int* full_count, int* half_count, int* flags,
+                                               int* rows) {
     const int i = particles[p];
-    int* row = lrow + threadIdx.x;
     const DevBox& B = ctl->box;
     const double4 pi = bx[p];  // = xr[i]
     int c[3];
@@ -412,12 +452,28 @@ __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int
     for (int k = 0; k < nfull; ++k) rows[(size_t)k * n + i] = row[k * TMPS_BLOCK];
 }
 
+__global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
+                                                          GridP G, const int* cell_start,
+                                                          const int* particles, const double4* bx,
+                                                          double skin, int max_nb, int* full_count,
+                                                          int* half_count, int* flags, int* rows) {
+    __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n)
+        list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, G, cell_start, particles, bx, skin,
+                       max_nb, full_count, half_count, flags, rows);
+}
+
 // Rows from the slot-major scratch of k_list_rows into the CSR list.
-__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void list_compact_atom(int i, int n, const int* rows,
+                                                  const int* nb_start, int* nb_j) {
     const int a = nb_start[i], m = nb_start[i + 1] - a;
     for (int k = 0; k < m; ++k) nb_j[a + k] = rows[(size_t)k * n + i];
+}
+
+__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) list_compact_atom(i, n, rows, nb_start, nb_j);
 }
 
 // ContactHistory save/restore (contact_history.py:91-139) as a merge of the
@@ -425,11 +481,11 @@ __global__ void k_list_compact(int n, const int* rows, const int* nb_start, int*
 // touching flag and shear of the same partner in the old row, if it was
 // touching there.
 template <typename H>
-__global__ void k_hist_restore(int n, const int* nb_start, const int* nb_j, uint8_t* nb_flag,
-                               H* nb_hist, const int* onb_start, const int* onb_j,
-                               const uint8_t* onb_flag, const H* onb_hist, int old_n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void hist_restore_atom(int i, const int* nb_start, const int* nb_j,
+                                                  uint8_t* nb_flag, H* nb_hist,
+                                                  const int* onb_start, const int* onb_j,
+                                                  const uint8_t* onb_flag, const H* onb_hist,
+                                                  int old_n) {
     const int s0 = nb_start[i], s1 = nb_start[i + 1];
     int o = 0, o1 = 0;
     if (i < old_n) {
@@ -446,6 +502,16 @@ __global__ void k_hist_restore(int n, const int* nb_start, const int* nb_j, uint
             nb_flag[s] = 0;
         }
     }
+}
+
+template <typename H>
+__global__ void k_hist_restore(int n, const int* nb_start, const int* nb_j, uint8_t* nb_flag,
+                               H* nb_hist, const int* onb_start, const int* onb_j,
+                               const uint8_t* onb_flag, const H* onb_hist, int old_n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        hist_restore_atom<H>(i, nb_start, nb_j, nb_flag, nb_hist, onb_start, onb_j, onb_flag,
+                             onb_hist, old_n);
 }
 
 template <typename H>
@@ -525,6 +591,137 @@ __global__ void k_half_count(int n, const int* nb_start, const int* nb_j, int* h
     for (int s = nb_start[i]; s < nb_start[i + 1]; ++s)
         if (nb_j[s] > i) ++c;
     half_count[i] = c;
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+//
+// The rebuild of many ensemble members at once: each kernel below runs the
+// per-item body of its single-system counterpart above for every entry in a
+// block map of (entry, local block) pairs, so each member's bins, list and
+// history are exactly those of its own rebuild.
+
+#define REDO_ITEM(map)                         \
+    const int2 mb = (map)[blockIdx.x];         \
+    const RedoEntry& e = T[mb.x];              \
+    const int i = mb.y * TMPS_BLOCK + threadIdx.x;
+
+__global__ void kb_list_save(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    list_save_items<float4>(i, e.save_blocks * TMPS_BLOCK, e.n, e.nb_start, e.nb_j, e.nb_flag,
+                            e.nb_hist, e.onb_start, e.onb_j, e.onb_flag, e.onb_hist, e.onb_cap);
+}
+
+__global__ void kb_pbc(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.n) pbc_atom(i, e.xr, e.vm, e.ctl);
+}
+
+// bin counts and fill cursors, and the list build's flags (k_zero_int3)
+__global__ void kb_bins_zero(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.ncell + 1) {
+        e.cell_count[i] = 0;
+        e.cell_fill[i] = 0;
+    }
+    if (i < 4) e.flags[i] = 0;
+}
+
+__global__ void kb_bin_count(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.n) bin_count_atom(i, e.xr, e.ctl, e.grid, e.cell_of, e.cell_count);
+}
+
+// The scans of the build: bin counts -> bin starts (ATOMS 0), the list's row
+// lengths -> row starts, total into flags[2] (ATOMS 1).
+template <int ATOMS>
+__device__ __forceinline__ void redo_scan_arrays(const RedoEntry& e, const int*& in, int*& out,
+                                                 int& n) {
+    in = ATOMS ? e.full_count : e.cell_count;
+    out = ATOMS ? e.nb_start : e.cell_start;
+    n = ATOMS ? e.n : e.ncell;
+}
+
+template <int ATOMS>
+__global__ void kb_scan_tiles(const RedoEntry* T, const int2* map) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    const int* in;
+    int* out;
+    int n;
+    redo_scan_arrays<ATOMS>(e, in, out, n);
+    scan_tile(mb.y, in, out, n, e.scan_tmp);
+}
+
+// one block per entry
+template <int ATOMS>
+__global__ void kb_scan_sums(const RedoEntry* T) {
+    const RedoEntry& e = T[blockIdx.x];
+    const int* in;
+    int* out;
+    int n;
+    redo_scan_arrays<ATOMS>(e, in, out, n);
+    scan_sums(e.scan_tmp, (n + SCAN_TILE - 1) / SCAN_TILE, out + n, ATOMS ? e.flags + 2 : nullptr);
+}
+
+template <int ATOMS>
+__global__ void kb_scan_add(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    const int* in;
+    int* out;
+    int n;
+    redo_scan_arrays<ATOMS>(e, in, out, n);
+    if (i < n) out[i] += e.scan_tmp[i / SCAN_TILE];
+}
+
+__global__ void kb_bin_fill(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.n) bin_fill_atom(i, e.cell_of, e.cell_start, e.cell_fill, e.cell_particles);
+}
+
+__global__ void kb_bin_sort(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.ncell) bin_sort_cell(i, e.cell_start, e.cell_particles);
+}
+
+__global__ void kb_bin_gather(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.n) e.bin_x[i] = e.xr[e.cell_particles[i]];
+}
+
+__global__ void __launch_bounds__(TMPS_BLOCK) kb_list_rows(const RedoEntry* T, const int2* map) {
+    __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
+    REDO_ITEM(map)
+    if (i < e.n)
+        list_rows_item(i, lrow + threadIdx.x, e.xr, e.n, e.ctl, e.grid, e.cell_start,
+                       e.cell_particles, e.bin_x, e.skin, e.max_nb, e.full_count, e.half_count,
+                       e.flags, e.row_tmp);
+}
+
+// every entry's build flags (overflow, long row, slot count) into one array
+__global__ void kb_list_flags(const RedoEntry* T, int entries, int4* out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= entries) return;
+    const int* f = T[q].flags;
+    out[q] = make_int4(f[0], f[1], f[2], 0);
+}
+
+// the CSR rows (from the scratch rows, or by the two-pass fill), and x0
+__global__ void kb_list_finish(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i >= e.n) return;
+    if (e.long_rows)
+        list_fill_atom(i, e.xr, e.ctl, e.grid, e.cell_start, e.cell_particles, e.skin, e.nb_start,
+                       e.nb_j);
+    else
+        list_compact_atom(i, e.n, e.row_tmp, e.nb_start, e.nb_j);
+    e.x0[i] = e.xr[i];
+}
+
+__global__ void kb_hist_restore(const RedoEntry* T, const int2* map) {
+    REDO_ITEM(map)
+    if (i < e.n)
+        hist_restore_atom<float4>(i, e.nb_start, e.nb_j, e.nb_flag, e.nb_hist, e.onb_start,
+                                  e.onb_j, e.onb_flag, e.onb_hist, e.old_n);
 }
 
 }  // namespace
@@ -652,8 +849,7 @@ int ensure_list_capacity(System& s, size_t need) { return grow_list(s, need); }
 // Builds the full list: one pass into per-atom scratch rows, a scan, and a
 // compaction (two passes, count then fill, if some row exceeds the scratch).
 // *total_out receives the slot count (a host synchronisation per rebuild).
-int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
-    // s.flags was cleared by launch_bins (flags[2] receives the slot count)
+int list_reserve_rows(System& s) {
     const size_t rows_need = (size_t)std::max(s.n, 1) * LIST_ROW_CAP;
     if (rows_need > s.row_tmp_cap) {
         HIP_TRY(hipStreamSynchronize(s.stream));
@@ -661,6 +857,13 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
         HIP_TRY(hipMalloc(&s.row_tmp, rows_need * sizeof(int)));
         s.row_tmp_cap = rows_need;
     }
+    return 0;
+}
+
+int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
+    // s.flags was cleared by launch_bins (flags[2] receives the slot count)
+    int rc = list_reserve_rows(s);
+    if (rc) return rc;
     if (s.n > 0) {
         hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                            s.cell_particles, xr, s.bin_x);
@@ -670,7 +873,7 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
                            s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp);
         HIP_TRY(hipGetLastError());
     }
-    int rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
+    rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
     if (rc) return rc;
     int hv[3];
     HIP_TRY(hipMemcpyAsync(hv, s.flags, 3 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
@@ -776,5 +979,56 @@ int launch_set_half_history(System& s, const int* d_partner, const double* d_she
                            s.nb_start, s.nb_j, s.nb_flag, (float4*)s.nb_hist, s.tag, d_half_start,
                            d_partner, d_shear);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+
+#define REDO_LAUNCH(kernel, nb, ...)                                                      \
+    do {                                                                                  \
+        if ((nb) > 0) {                                                                   \
+            hipLaunchKernelGGL(kernel, dim3(nb), dim3(TMPS_BLOCK), 0, st, __VA_ARGS__);   \
+            HIP_TRY(hipGetLastError());                                                   \
+        }                                                                                 \
+    } while (0)
+
+int launch_redo_save(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    REDO_LAUNCH(kb_list_save, M.n_save, T, M.save);
+    return 0;
+}
+
+int launch_redo_pbc(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    REDO_LAUNCH(kb_pbc, M.n_atoms, T, M.atoms);
+    return 0;
+}
+
+int launch_redo_bins(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    REDO_LAUNCH(kb_bins_zero, M.n_cells, T, M.cells);
+    REDO_LAUNCH(kb_bin_count, M.n_atoms, T, M.atoms);
+    REDO_LAUNCH(kb_scan_tiles<0>, M.n_cell_tiles, T, M.cell_tiles);
+    REDO_LAUNCH(kb_scan_sums<0>, M.entries, T);
+    REDO_LAUNCH(kb_scan_add<0>, M.n_cells, T, M.cells);
+    REDO_LAUNCH(kb_bin_fill, M.n_atoms, T, M.atoms);
+    REDO_LAUNCH(kb_bin_sort, M.n_cells, T, M.cells);
+    return 0;
+}
+
+int launch_redo_rows(const RedoEntry* T, const RedoMaps& M, int4* flags_out, hipStream_t st) {
+    REDO_LAUNCH(kb_bin_gather, M.n_atoms, T, M.atoms);
+    REDO_LAUNCH(kb_list_rows, M.n_atoms, T, M.atoms);
+    REDO_LAUNCH(kb_scan_tiles<1>, M.n_atom_tiles, T, M.atom_tiles);
+    REDO_LAUNCH(kb_scan_sums<1>, M.entries, T);
+    REDO_LAUNCH(kb_scan_add<1>, M.n_atoms, T, M.atoms);
+    REDO_LAUNCH(kb_list_flags, (M.entries + TMPS_BLOCK - 1) / TMPS_BLOCK, T, M.entries, flags_out);
+    return 0;
+}
+
+int launch_redo_list(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    REDO_LAUNCH(kb_list_finish, M.n_atoms, T, M.atoms);
+    return 0;
+}
+
+int launch_redo_restore(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    REDO_LAUNCH(kb_hist_restore, M.n_atoms, T, M.atoms);
     return 0;
 }

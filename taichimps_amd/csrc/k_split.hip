@@ -172,9 +172,8 @@ __device__ __forceinline__ float cundall_scale_f(float fv, float g) {
 // ---------------------------------------------------------- conversions
 
 // x (FP64) of every atom from its split encoding, into xr (x, y, z, r).
-__global__ void k_split_to_x64(SplitPtrs P, double4* xr, const Ctl* ctl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+__device__ __forceinline__ void split_to_x64_atom(int i, const SplitPtrs& P, double4* xr,
+                                                  const Ctl* ctl) {
     const DevBox& B = ctl->box;
     const SplitConst& sc = ctl->sc;
     int c[3];
@@ -189,12 +188,16 @@ __global__ void k_split_to_x64(SplitPtrs P, double4* xr, const Ctl* ctl) {
     xr[i] = make_double4(x[0], x[1], x[2], (double)u.w);
 }
 
+__global__ void k_split_to_x64(SplitPtrs P, double4* xr, const Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) split_to_x64_atom(i, P, xr, ctl);
+}
+
 // Split encoding of every atom from xr on grid G; u0 encodes x0 (the
 // positions of the last build) in the same bins, relative to the build box.
-__global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0, const Ctl* ctl,
-                               GridP G, uint32_t* cell) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+__device__ __forceinline__ void x64_to_split_atom(int i, const SplitPtrs& P, const double4* xr,
+                                                  const double4* x0, const Ctl* ctl,
+                                                  const GridP& G, uint32_t* cell) {
     const DevBox& B = ctl->box;
     const double4 p = xr[i], q = x0[i];
     const double xs[3] = {p.x, p.y, p.z}, x0s[3] = {q.x, q.y, q.z};
@@ -217,16 +220,21 @@ __global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0
     P.u0[i] = make_float4(u0[0], u0[1], u0[2], 0.f);
 }
 
+__global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0, const Ctl* ctl,
+                               GridP G, uint32_t* cell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) x64_to_split_atom(i, P, xr, x0, ctl, G, cell);
+}
+
 // Per slot of the neighbor list: the cell of i minus the cell of j, wrapped
 // to the nearest periodic image, 3 bits per axis (offset by 3). Between two
 // builds the split cells do not change, so the one-kernel step reads this
 // instead of gathering and unpacking the neighbour's cell. List partners sit
 // within one cell of each other; a re-encode without a rebuild moves an atom
 // by at most one more, so |difference| <= 3 always fits.
-__global__ void k_slot_codes(int n, const int* nb_start, const int* nb_j, const uint32_t* cell,
-                             const Ctl* ctl, uint16_t* code) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void slot_codes_atom(int i, const int* nb_start, const int* nb_j,
+                                                const uint32_t* cell, const Ctl* ctl,
+                                                uint16_t* code) {
     int ci[3];
     unpack_cell(cell[i], ci[0], ci[1], ci[2]);
     const int per[3] = {ctl->box.per[0], ctl->box.per[1], ctl->box.per[2]};
@@ -243,6 +251,12 @@ __global__ void k_slot_codes(int n, const int* nb_start, const int* nb_j, const 
         }
         code[s] = (uint16_t)c;
     }
+}
+
+__global__ void k_slot_codes(int n, const int* nb_start, const int* nb_j, const uint32_t* cell,
+                             const Ctl* ctl, uint16_t* code) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slot_codes_atom(i, nb_start, nb_j, cell, ctl, code);
 }
 
 // ------------------------------------------------------------ stage kernels
@@ -369,10 +383,8 @@ __global__ void k_ke_leaves_split(SplitPtrs P, double* partial) {
 
 // ------------------------------------------------------------ fused steps
 
-__global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
-    if (ctl->abort) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+__device__ __forceinline__ void step_begin_split_atom(int i, const SplitPtrs& P, Ctl* ctl,
+                                                      float dt, int kick) {
     const int t = ctl->timestep + 1;
     const SplitConst& sc = ctl->sc;
     float4 u = P.up[i];
@@ -401,13 +413,21 @@ __global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
     }
 }
 
+__global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) step_begin_split_atom(i, P, ctl, dt, kick);
+}
+
+// Block `blk` of the force pass (its partial: partial[blk]).
 template <int STYLE, int CUNDALL, int FINAL>
-__global__ void k_step_force_split(SplitPtrs P, Ctl* ctl, PairP pp, float dt, float gl, float ga,
-                                   double* partial) {
+__device__ __forceinline__ void step_force_split_block(const SplitPtrs& P, Ctl* ctl,
+                                                       const PairP& pp, float dt, float gl,
+                                                       float ga, double* partial, int blk) {
     if (ctl->abort) return;
     const int t = ctl->timestep + 1;
     if (ctl->trigger == t) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * TMPS_BLOCK + threadIdx.x;
     double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < P.n) {
         const SplitConst sc = ctl->sc;
@@ -449,7 +469,13 @@ __global__ void k_step_force_split(SplitPtrs P, Ctl* ctl, PairP pp, float dt, fl
         P.vm_out[i] = v;
         P.om_out[i] = w;
     }
-    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+    block_tree_sum<7>(leaf, partial + (size_t)blk * 8);
+}
+
+template <int STYLE, int CUNDALL, int FINAL>
+__global__ void k_step_force_split(SplitPtrs P, Ctl* ctl, PairP pp, float dt, float gl, float ga,
+                                   double* partial) {
+    step_force_split_block<STYLE, CUNDALL, FINAL>(P, ctl, pp, dt, gl, ga, partial, blockIdx.x);
 }
 
 __global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, const float4* wsrc,
@@ -791,6 +817,71 @@ __global__ void k_kick_split(float4* vm, float4* om, const float4* f, const floa
     om[i] = w;
 }
 
+// ------------------------------------------------ batched redo (batch.hip)
+//
+// An ensemble member's redone rebuild step, many members per launch: each
+// kernel runs the per-atom body of the stage kernel the single-system redo
+// launches, over a block map of (entry, local block) pairs.
+
+// The member's buffers with set in_buf read and set out_buf written.
+__device__ __forceinline__ SplitPtrs redo_ptrs(const RedoEntry& e, int in_buf, int out_buf) {
+    SplitPtrs p;
+    p.up = e.up; p.f = e.ff; p.tq = e.tqf; p.u0 = e.u0;
+    p.vm = in_buf ? e.vmf2 : e.vmf;
+    p.om = in_buf ? e.omf2 : e.omf;
+    p.vm_out = out_buf ? e.vmf2 : e.vmf;
+    p.om_out = out_buf ? e.omf2 : e.omf;
+    p.cell = e.cell; p.vir = e.vir;
+    p.nb_start = e.nb_start; p.nb_j = e.nb_j; p.nb_flag = e.nb_flag;
+    p.nb_hist = e.nb_hist;
+    p.n = e.n;
+    return p;
+}
+
+// the step's start in set 0 (no half-kick), then xr from the encoding
+__global__ void kb_redo_begin_split(const RedoEntry* T, const int2* map, float dt) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    const int i = mb.y * TMPS_BLOCK + threadIdx.x;
+    if (i >= e.n) return;
+    const SplitPtrs P = redo_ptrs(e, 0, 0);
+    if (!e.ctl->abort) step_begin_split_atom(i, P, e.ctl, dt, 0);
+    if (e.to_x64) split_to_x64_atom(i, P, e.xr, e.ctl);
+}
+
+__global__ void kb_redo_encode_split(const RedoEntry* T, const int2* map) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    const int i = mb.y * TMPS_BLOCK + threadIdx.x;
+    if (i < e.n) x64_to_split_atom(i, redo_ptrs(e, 0, 0), e.xr, e.x0, e.ctl, e.grid, e.cell);
+}
+
+__global__ void kb_redo_codes_split(const RedoEntry* T, const int2* map) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    const int i = mb.y * TMPS_BLOCK + threadIdx.x;
+    if (e.codes && i < e.n) slot_codes_atom(i, e.nb_start, e.nb_j, e.cell, e.ctl, e.nb_code);
+}
+
+// the force pass of the redone step: reads set 0, writes v / omega to set 1
+template <int STYLE>
+__global__ void kb_redo_force_split(const RedoEntry* T, const int2* map, float dt) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    step_force_split_block<STYLE, 0, 1>(redo_ptrs(e, 0, 1), e.ctl, e.pair, dt, 0.f, 0.f,
+                                        e.partial, mb.y);
+}
+
+__global__ void kb_redo_copy_vw_split(const RedoEntry* T, const int2* map) {
+    const int2 mb = map[blockIdx.x];
+    const RedoEntry& e = T[mb.x];
+    if (e.ctl->abort) return;
+    const int i = mb.y * TMPS_BLOCK + threadIdx.x;
+    if (i >= e.n) return;
+    e.vmf[i] = e.vmf2[i];
+    e.omf[i] = e.omf2[i];
+}
+
 }  // namespace
 
 // ================================================================ launchers
@@ -1087,6 +1178,52 @@ int launch_batch_split_tail(const void* table, const int2* blocks, int nblocks, 
     if (nblocks <= 0) return 0;
     hipLaunchKernelGGL(k_batch_tail_split, dim3(nblocks), dim3(TMPS_BLOCK), 0, st,
                        static_cast<const BatchSplit*>(table), blocks);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------ batched redo (batch.hip)
+
+int launch_redo_begin_split(const RedoEntry* T, const RedoMaps& M, float dt, hipStream_t st) {
+    if (M.n_atoms <= 0) return 0;
+    hipLaunchKernelGGL(kb_redo_begin_split, dim3(M.n_atoms), dim3(TMPS_BLOCK), 0, st, T, M.atoms,
+                       dt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_redo_encode_split(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    if (M.n_atoms <= 0) return 0;
+    hipLaunchKernelGGL(kb_redo_encode_split, dim3(M.n_atoms), dim3(TMPS_BLOCK), 0, st, T, M.atoms);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(kb_redo_codes_split, dim3(M.n_atoms), dim3(TMPS_BLOCK), 0, st, T, M.atoms);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_redo_force_split(const RedoEntry* T, const RedoMaps& M, float dt, hipStream_t st) {
+    if (M.n_style[TMPS_PAIR_HOOKE_HISTORY] > 0)
+        hipLaunchKernelGGL(kb_redo_force_split<TMPS_PAIR_HOOKE_HISTORY>,
+                           dim3(M.n_style[TMPS_PAIR_HOOKE_HISTORY]), dim3(TMPS_BLOCK), 0, st, T,
+                           M.style[TMPS_PAIR_HOOKE_HISTORY], dt);
+    HIP_TRY(hipGetLastError());
+    if (M.n_style[TMPS_PAIR_HERTZ_HISTORY] > 0)
+        hipLaunchKernelGGL(kb_redo_force_split<TMPS_PAIR_HERTZ_HISTORY>,
+                           dim3(M.n_style[TMPS_PAIR_HERTZ_HISTORY]), dim3(TMPS_BLOCK), 0, st, T,
+                           M.style[TMPS_PAIR_HERTZ_HISTORY], dt);
+    HIP_TRY(hipGetLastError());
+    if (M.n_style[TMPS_PAIR_GRANULAR] > 0)
+        hipLaunchKernelGGL(kb_redo_force_split<TMPS_PAIR_GRANULAR>,
+                           dim3(M.n_style[TMPS_PAIR_GRANULAR]), dim3(TMPS_BLOCK), 0, st, T,
+                           M.style[TMPS_PAIR_GRANULAR], dt);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_redo_copy_vw_split(const RedoEntry* T, const RedoMaps& M, hipStream_t st) {
+    if (M.n_atoms <= 0) return 0;
+    hipLaunchKernelGGL(kb_redo_copy_vw_split, dim3(M.n_atoms), dim3(TMPS_BLOCK), 0, st, T,
+                       M.atoms);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -10,6 +10,8 @@
 #include "../../include/tmps.h"
 
 #define TMPS_BLOCK 256
+// items per block of the rebuild's exclusive scans
+#define TMPS_SCAN_TILE (TMPS_BLOCK * 4)
 
 // ---------------------------------------------------------------- device state
 
@@ -96,6 +98,45 @@ struct GridP {
     int g[3];
     double csize[3];
     int ncell;
+};
+
+// One FP32-split ensemble member whose step is redone with a list rebuild,
+// as the batched redo kernels read it (the single-system redo's buffers,
+// grid and decisions). The host fills it in two passes: before the list
+// build, and after the build's slot count is read back (long_rows, codes,
+// step, fast, gi, done, and the list's buffers, which the build can move).
+struct RedoEntry {
+    Ctl* ctl;
+    int n, ncell, old_n, onb_cap;
+    int copy_blocks, save_blocks, max_nb, servo_on;
+    int long_rows, codes, step, done;
+    int to_x64, pad;
+    int fast[3], gi[3];
+    GridP grid;
+    double skin;
+    PairP pair;
+    double* partial;
+    double4 *xr, *vm, *x0, *bin_x;
+    float4 *up, *up2, *vmf, *vmf2, *omf, *omf2, *ff, *tqf, *u0;
+    uint32_t* cell;
+    double* vir;
+    int *cell_of, *cell_count, *cell_start, *cell_fill, *cell_particles, *scan_tmp;
+    int *full_count, *half_count, *flags, *row_tmp;
+    int *nb_start, *nb_j, *onb_start, *onb_j;
+    uint8_t *nb_flag, *nb_flag2, *onb_flag;
+    float4 *nb_hist, *nb_hist2, *onb_hist;
+    uint16_t* nb_code;
+};
+
+// Block maps of a batched redo: (entry, local block) pairs, per kind of
+// item. atoms: nblocks_for(n) per entry; cells: nblocks_for(ncell + 1);
+// cell_tiles / atom_tiles: scan tiles over ncell / n; copy: the entries that
+// resume from buffer set 1, copy_blocks each; save: save_blocks per entry;
+// style: the atoms blocks of the entries with each pair style.
+struct RedoMaps {
+    int entries, max_parts;
+    const int2 *atoms, *cells, *cell_tiles, *atom_tiles, *copy, *save, *style[3];
+    int n_atoms, n_cells, n_cell_tiles, n_atom_tiles, n_copy, n_save, n_style[3];
 };
 
 // ---------------------------------------------------------------- host system
@@ -212,10 +253,18 @@ int clear_forces(System& s);
 int pair_compute(System& s, double dt, int shearupdate, int accumulate);
 int servo_end_of_step(System& s, double dt, double* tensor_out);
 int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double gl, double ga);
+// redo_step's host work for an FP32-split ensemble member whose step the
+// batched redo kernels redo: redo_batch_begin up to the list build (from the
+// control block in s.hctl), redo_batch_list after it, given the build's
+// flags (overflow, long row, slot count).
+int redo_batch_begin(System& s, RedoEntry& e);
+int redo_batch_list(System& s, int4 flags, RedoEntry& e);
 }  // namespace tmps_api
 
 // grows the list buffers to hold `need` slots (k_rebuild.hip)
 int ensure_list_capacity(System& s, size_t need);
+// sizes the single-pass list build's scratch rows for s.n atoms (k_rebuild.hip)
+int list_reserve_rows(System& s);
 // frees the adjoint engine's buffers (adjoint.hip)
 void adjoint_free(System& s);
 
@@ -281,6 +330,29 @@ int launch_batch_split_step(int style, const void* table, const int2* blocks, in
 int launch_batch_split_tail(const void* table, const int2* blocks, int nblocks, hipStream_t st);
 int launch_batch_close(const CloseEntry* table, int ninst, int max_parts, double dt,
                        hipStream_t st);
+// The batched redo of ensemble members' rebuild steps, in redo_step's order:
+// resume (buffer set 1 -> 0 where needed, trigger and abort cleared), the
+// step's start and the FP64 positions, the list save, PBC wrap, bins, list
+// rows with their flags read back (the host then sizes the lists), the CSR
+// rows and x0, the build record, the split encoding, the history restore,
+// the force pass, the step's end, v / omega into set 0, and the rejoin.
+int launch_redo_resume(const RedoEntry* T, const RedoMaps& M, hipStream_t st);      // k_fp64
+int launch_redo_begin_split(const RedoEntry* T, const RedoMaps& M, float dt,
+                            hipStream_t st);                                         // k_split
+int launch_redo_save(const RedoEntry* T, const RedoMaps& M, hipStream_t st);        // k_rebuild
+int launch_redo_pbc(const RedoEntry* T, const RedoMaps& M, hipStream_t st);         // k_rebuild
+int launch_redo_bins(const RedoEntry* T, const RedoMaps& M, hipStream_t st);        // k_rebuild
+int launch_redo_rows(const RedoEntry* T, const RedoMaps& M, int4* flags_out,
+                     hipStream_t st);                                                // k_rebuild
+int launch_redo_list(const RedoEntry* T, const RedoMaps& M, hipStream_t st);        // k_rebuild
+int launch_redo_built(const RedoEntry* T, const RedoMaps& M, hipStream_t st);       // k_fp64
+int launch_redo_encode_split(const RedoEntry* T, const RedoMaps& M, hipStream_t st);  // k_split
+int launch_redo_restore(const RedoEntry* T, const RedoMaps& M, hipStream_t st);     // k_rebuild
+int launch_redo_force_split(const RedoEntry* T, const RedoMaps& M, float dt,
+                            hipStream_t st);                                         // k_split
+int launch_redo_end(const RedoEntry* T, const RedoMaps& M, double dt, hipStream_t st);  // k_fp64
+int launch_redo_copy_vw_split(const RedoEntry* T, const RedoMaps& M, hipStream_t st);  // k_split
+int launch_redo_rejoin(const RedoEntry* T, const RedoMaps& M, hipStream_t st);      // k_fp64
 // the initial_integrate half-kick of a segment's first step, in buffer 0
 int launch_kick_f64(System& s, double dt);
 int launch_kick_split(System& s, double dt);

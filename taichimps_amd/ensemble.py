@@ -4,10 +4,11 @@ No reference counterpart: the reference runs one Simulation at a time. An
 Ensemble takes FP32-split Simulations whose runs the device path covers
 (nve/sphere, a pair style with contact history, optionally deform/pressure;
 no fix damping/cundall) and advances all of them with one launch per step and
-pair style (libtmps_hip.so, tmps_batch_*). Every member's state after
-``Ensemble.run(steps)`` is bitwise the state ``member.run(steps)`` gives it
-alone; members may differ in size, box, pair style, coefficients and servo
-target, but share the time step.
+pair style (libtmps_hip.so, tmps_batch_*); the steps that members redo with a
+list rebuild are redone together, one launch per kernel of the redo for all
+of them. Every member's state after ``Ensemble.run(steps)`` is bitwise the
+state ``member.run(steps)`` gives it alone; members may differ in size, box,
+pair style, coefficients and servo target, but share the time step.
 
     ens = Ensemble([sim_a, sim_b, ...])
     ens.run(500, autotune=True)
@@ -28,7 +29,6 @@ from .EXTRA_FIX.deform_pressure import FixDeformPressure
 
 SEGMENT_LENGTHS = (8, 8, 16, 16, 32, 32)
 BLOCK_ORDERS = ("sequence", "xcd", "sequence", "xcd", "sequence", "xcd")
-REDO_THREADS = (4, 8, 16)
 
 
 class Ensemble:
@@ -114,16 +114,15 @@ class Ensemble:
         """The launch configuration in use and what the autotuner measured."""
         if self._batch is None:
             raise RuntimeError("the ensemble has not run yet")
-        seg, order, rates, redone, threads, redo_ms = self._batch.tuning()
+        seg, order, rates, redone, passes, redo_ms = self._batch.tuning()
         return {
             "segment_length": seg,
             "block_order": "xcd" if order else "sequence",
             "rates_atom_steps_per_s": {f"{SEGMENT_LENGTHS[k]}/{BLOCK_ORDERS[k]}": float(rates[k])
                                        for k in range(len(rates))},
             "redone_steps": redone,
-            "redo_threads": threads,
-            "redo_ms_per_member": {str(t): float(redo_ms[k])
-                                   for k, t in enumerate(REDO_THREADS)},
+            "redo_passes": passes,
+            "redo_ms": redo_ms,
         }
 
     def close(self) -> None:

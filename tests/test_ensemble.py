@@ -74,6 +74,8 @@ def test_members_match_their_single_runs(autotune):
         assert_same(ens_sims[k], solo[k], f"member {k} continued")
     t = ens.tuning()
     assert t["redone_steps"] >= 0
+    if t["redone_steps"] > 0:
+        assert t["redo_passes"] >= 1 and t["redo_ms"] > 0.0, t
     if autotune:
         rates = t["rates_atom_steps_per_s"]
         assert all(v > 0 for v in rates.values()), rates
@@ -103,6 +105,33 @@ def test_zero_steps_is_setup_only():
         s.run(0)
     for k in range(2):
         assert_same(sims[k], solo[k], f"setup member {k}")
+
+
+def test_long_rows_and_forced_rebuilds():
+    """Members whose list rows exceed the single-pass scratch of 32 (a skin of
+    1.2 spacings: about 45 partners per atom), rebuilt every third step
+    without the skin check, beside a member with it: the redone steps take
+    the two-pass list fill, and several members redo in the same pass."""
+    def sims():
+        out = []
+        for side, seed, style in ((8, 201, "hooke/history"), (9, 202, "hertz/history")):
+            w = workloads.lattice(side, side, side, 1.0, seed, "uniform")
+            w["v"] = np.random.default_rng(seed).normal(0.0, 0.5, size=w["v"].shape)
+            out.append(H.device_sim(w, style=style, precision="f32split", servo=True, skin=1.2,
+                                    check=False, every=3, kn=4.0e5, kt=2.0 / 7.0 * 4.0e5,
+                                    xmu=0.5))
+        out.append(member(MEMBERS[0]))
+        return out
+
+    ens_sims, solo = sims(), sims()
+    Ensemble(ens_sims).run(20)
+    for s in solo:
+        s.run(20)
+    assert solo[0].neighbor.build_count >= 6
+    counts = solo[1].atom._native.counters()
+    assert counts["nslots"] > 32 * solo[1].atom.nlocal // 2
+    for k in range(len(solo)):
+        assert_same(ens_sims[k], solo[k], f"member {k}")
 
 
 def test_rejects_what_it_does_not_run():
