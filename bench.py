@@ -391,7 +391,7 @@ def ensemble_c5(steps=500, n_grad=256, every=100):
     ens.run(steps)
     torch.cuda.synchronize()
     t_ens2 = time.perf_counter() - t0
-    redone2 = ens.tuning()["redone_steps"]
+    tuning2 = ens.tuning()
 
     solo = [build_member(w, "f32split") for w in ws]
     setup(solo)
@@ -447,7 +447,9 @@ def ensemble_c5(steps=500, n_grad=256, every=100):
         "one_at_a_time_atom_steps_per_s": natoms * steps / t_solo,
         "speedup_vs_one_at_a_time": t_solo / t_ens,
         "members_bitwise_equal_single_runs": len(ws) - differ,
-        "tuning": tuning, "rerun_redone_steps": redone2, "list_builds": rebuilds,
+        "tuning": tuning, "rerun_redone_steps": tuning2["redone_steps"],
+        "rerun_redo_passes": tuning2["redo_passes"], "rerun_redo_ms": tuning2["redo_ms"],
+        "list_builds": rebuilds,
         "gradients": {"count": len(grads), "steps": steps, "checkpoint_every": every,
                       "seconds": t_grad,
                       "by_size": [grads[q] for q in
@@ -481,6 +483,8 @@ def main(argv=None):
                     help="only run the triaxial block and print it")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the ensemble block and print it")
+    ap.add_argument("--ensemble-grads", type=int, default=256, metavar="N",
+                    help="members whose adjoint gradients the ensemble block computes")
     ap.add_argument("--dims", type=int, nargs=3, default=None,
                     help="lattice dimensions (default 100 100 100)")
     ap.add_argument("--precision", choices=("both", "f64", "f32split"), default="both",
@@ -497,7 +501,7 @@ def main(argv=None):
         print(json.dumps(adjoint_c4()))
         return 0
     if args.ensemble_only:
-        print(json.dumps(ensemble_c5()))
+        print(json.dumps(ensemble_c5(n_grad=args.ensemble_grads)))
         return 0
     if args.triaxial_only:
         print(json.dumps(triaxial_c3()))
@@ -582,7 +586,7 @@ def main(argv=None):
     if not args.no_adjoint:
         out["adjoint"] = adjoint_c4()
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_c5()
+        out["ensemble"] = ensemble_c5(n_grad=args.ensemble_grads)
     if args.triaxial:
         out["triaxial"] = triaxial_c3()
     if cpu is not None:
