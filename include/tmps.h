@@ -177,6 +177,25 @@ int tmps_virial_reduce(tmps_system* s, int keflag, double out6[6]);
 /* compute_ke_kernel, thermo.py:46-60 */
 int tmps_ke(tmps_system* s, double* ke_trans, double* ke_rot);
 
+/* ---- per-contact computes (computes/contact_atom.py, fabric.py,
+ *      stress_atom.py) --------------------------------------------------------
+ * Evaluated on the current positions and neighbor list (FP32-split systems
+ * decode their positions to FP64 first). Sums over atoms or contacts are
+ * pairwise trees, so they are deterministic. */
+/* ComputeContactAtom.compute, contact_atom.py:19-64: per atom, the list
+ * partners overlapping it (|d|^2 < (ri + rj)^2 and |d|^2 > 1e-18), counts
+ * n ints (nullable); total (nullable): their sum, twice the contacts */
+int tmps_contact_atom(tmps_system* s, int* counts, long long* total);
+/* ComputeFabric.compute, fabric.py:26-97: phi = (sum over contacts of
+ * n n^T) / Nc as [xx yy zz xy xz yz] (zeros without contacts), and Nc
+ * (nullable) */
+int tmps_fabric(tmps_system* s, double phi[6], int* ncontacts);
+/* ComputeStressAtom.compute / compute_macro, stress_atom.py:30-109: per-atom
+ * stress -virial (minus m v_a v_b when keflag) into stress (n x 6,
+ * nullable), and its sum over atoms into sum6 (nullable; compute_macro
+ * divides it by the box volume) */
+int tmps_stress_atom(tmps_system* s, int keflag, double* stress, double sum6[6]);
+
 /* ---- Simulation (simulation.py) ------------------------------------------- */
 /* Simulation.run(steps), simulation.py:320-332, with an nve/sphere
  * integrator, the configured pair style, optional damping/cundall

@@ -759,6 +759,112 @@ void virial_tensor(const Oracle& s, int keflag, int mode, double out[6]) {
     for (int k = 0; k < 6; ++k) out[k] = buf[k];
 }
 
+// Pairwise tree over per-atom leaves of NC components (zero-padded to a power
+// of two, adjacent pairs combined level by level), as virial_tensor mode 0.
+void pairwise_tree(std::vector<double>& buf, int n, int nc, double* out) {
+    size_t P = 1;
+    while (P < (size_t)n) P <<= 1;
+    buf.resize(P * nc, 0.0);
+    for (size_t w = 1; w < P; w <<= 1)
+        for (size_t i = 0; i < P; i += 2 * w)
+            for (int k = 0; k < nc; ++k) buf[i * nc + k] = buf[i * nc + k] + buf[(i + w) * nc + k];
+    for (int k = 0; k < nc; ++k) out[k] = buf[k];
+}
+
+// ----------------------------------------------------- per-contact computes
+
+// ComputeContactAtom.compute_kernel, contact_atom.py:19-52: every half-list
+// pair (i, j > i) that overlaps at the current positions (rsq < radsum^2 and
+// rsq > 1e-18) is tallied to both atoms.
+void contact_atom(const Oracle& s, int* count) {
+    for (int i = 0; i < s.n; ++i) count[i] = 0;
+    if (!s.has_built) return;
+    for (int i = 0; i < s.n; ++i) {
+        const double* xi = &s.x[3 * i];
+        const double ri = s.radius[i];
+        for (int k = 0; k < s.num_neighbors[i]; ++k) {
+            const int j = s.neighbors[(size_t)i * s.max_nb + k];
+            double d[3] = {xi[0] - s.x[3 * j], xi[1] - s.x[3 * j + 1], xi[2] - s.x[3 * j + 2]};
+            minimum_image(s, d);
+            const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double radsum = ri + s.radius[j];
+            if (rsq < radsum * radsum && rsq > 1e-18) {
+                ++count[i];
+                ++count[j];
+            }
+        }
+    }
+}
+
+// ComputeFabric.compute_kernel, fabric.py:26-78: the sums of n_a n_b over the
+// overlapping half-list pairs (n = d / |d|, d = x_i - x_j) and their number Nc,
+// then phi = sum * (1 / Nc). The sums are per-atom leaves (ascending j) in a
+// pairwise tree over atoms; the reference adds them with float atomics in an
+// unspecified order.
+void fabric(const Oracle& s, double phi[6], int* ncontacts) {
+    std::vector<double> buf((size_t)std::max(s.n, 1) * 7, 0.0);
+    if (s.has_built) {
+        for (int i = 0; i < s.n; ++i) {
+            const double* xi = &s.x[3 * i];
+            const double ri = s.radius[i];
+            double* leaf = &buf[(size_t)i * 7];
+            for (int k = 0; k < s.num_neighbors[i]; ++k) {
+                const int j = s.neighbors[(size_t)i * s.max_nb + k];
+                double d[3] = {xi[0] - s.x[3 * j], xi[1] - s.x[3 * j + 1],
+                               xi[2] - s.x[3 * j + 2]};
+                minimum_image(s, d);
+                const double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const double radsum = ri + s.radius[j];
+                if (rsq < radsum * radsum && rsq > 1e-18) {
+                    const double r = std::sqrt(rsq);
+                    const double nx = d[0] / r, ny = d[1] / r, nz = d[2] / r;
+                    leaf[0] = leaf[0] + nx * nx;
+                    leaf[1] = leaf[1] + ny * ny;
+                    leaf[2] = leaf[2] + nz * nz;
+                    leaf[3] = leaf[3] + nx * ny;
+                    leaf[4] = leaf[4] + nx * nz;
+                    leaf[5] = leaf[5] + ny * nz;
+                    leaf[6] = leaf[6] + 1.0;
+                }
+            }
+        }
+    }
+    double sum[7];
+    pairwise_tree(buf, std::max(s.n, 1), 7, sum);
+    const int nc = (int)sum[6];
+    *ncontacts = nc;
+    if (nc > 0) {
+        const double inv_nc = 1.0 / (double)nc;
+        for (int k = 0; k < 6; ++k) phi[k] = sum[k] * inv_nc;
+    } else {
+        for (int k = 0; k < 6; ++k) phi[k] = 0.0;
+    }
+}
+
+// ComputeStressAtom.compute_stress_kernel, stress_atom.py:30-52: s = -virial,
+// minus m v_a v_b with keflag; the macro sum of s over atoms is a pairwise
+// tree (the reference adds with float atomics).
+void stress_atom(const Oracle& s, int keflag, double* stress, double sum6[6]) {
+    std::vector<double> buf((size_t)std::max(s.n, 1) * 6, 0.0);
+    for (int i = 0; i < s.n; ++i) {
+        double* o = &buf[(size_t)i * 6];
+        for (int k = 0; k < 6; ++k) o[k] = -s.virial[6 * i + k];
+        if (keflag) {
+            const double m = s.rmass[i];
+            const double* vi = &s.v[3 * i];
+            o[0] = o[0] - m * vi[0] * vi[0];
+            o[1] = o[1] - m * vi[1] * vi[1];
+            o[2] = o[2] - m * vi[2] * vi[2];
+            o[3] = o[3] - m * vi[0] * vi[1];
+            o[4] = o[4] - m * vi[0] * vi[2];
+            o[5] = o[5] - m * vi[1] * vi[2];
+        }
+        if (stress)
+            for (int k = 0; k < 6; ++k) stress[6 * i + k] = o[k];
+    }
+    pairwise_tree(buf, std::max(s.n, 1), 6, sum6);
+}
+
 // ------------------------------------------------------------- the servo
 
 // FixDeformPressure._strain_h_rate, deform_pressure.py:447-472 (x/y/z entries)
@@ -1139,6 +1245,13 @@ void orc_virial(void* h, int keflag, int mode, double* out6) {
 }
 int orc_servo_end_of_step(void* h, double dt) {
     return servo_end_of_step(*static_cast<Oracle*>(h), dt);
+}
+void orc_contact_atom(void* h, int* count) { contact_atom(*static_cast<Oracle*>(h), count); }
+void orc_fabric(void* h, double* phi6, int* ncontacts) {
+    fabric(*static_cast<Oracle*>(h), phi6, ncontacts);
+}
+void orc_stress_atom(void* h, int keflag, double* stress, double* sum6) {
+    stress_atom(*static_cast<Oracle*>(h), keflag, stress, sum6);
 }
 
 int orc_npairs(void* h) { return static_cast<Oracle*>(h)->npairs; }

@@ -108,6 +108,9 @@ def lib():
         L.orc_get_grid.argtypes = [vp, ip, dp, ip]
         L.orc_get_counters.argtypes = [vp, ip]
         L.orc_get_tensor.argtypes = [vp, dp]
+        L.orc_contact_atom.argtypes = [vp, ip]
+        L.orc_fabric.argtypes = [vp, dp, ip]
+        L.orc_stress_atom.argtypes = [vp, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -355,6 +358,27 @@ class OracleSim:
 
     def servo_end_of_step(self, dt) -> int:
         return lib().orc_servo_end_of_step(self._h, float(dt))
+
+    # per-contact computes ----------------------------------------------------
+    def contact_atom(self):
+        """ComputeContactAtom.compute (contact_atom.py:19-64): int32[n]."""
+        out = np.zeros(self.n, dtype=np.int32)
+        lib().orc_contact_atom(self._h, _ip(out))
+        return out
+
+    def fabric(self):
+        """ComputeFabric.compute (fabric.py:26-97): (phi[6], Nc)."""
+        phi = np.zeros(6)
+        nc = np.zeros(1, dtype=np.int32)
+        lib().orc_fabric(self._h, _dp(phi), _ip(nc))
+        return phi, int(nc[0])
+
+    def stress_atom(self, keflag=False):
+        """ComputeStressAtom.compute (stress_atom.py:30-90): (stress[n,6], sum[6])."""
+        st = np.zeros((self.n, 6))
+        tot = np.zeros(6)
+        lib().orc_stress_atom(self._h, 1 if keflag else 0, _dp(st), _dp(tot))
+        return st, tot
 
     # inspection ------------------------------------------------------------
     def pairs(self):

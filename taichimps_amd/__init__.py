@@ -6,7 +6,7 @@ over hand-written HIP kernels in libtmps_hip.so, reached through its C ABI.
 """
 
 from .atom import AtomSystem
-from .computes.thermo import Computes
+from .computes import ComputeContactAtom, ComputeFabric, ComputeStressAtom, Computes
 from .constants import f32split, f64
 from .contact_history import ContactHistory
 from .domain import Domain
@@ -16,7 +16,8 @@ from .neighbor import NeighborList
 from .simulation import Simulation
 
 __all__ = [
-    "AtomSystem", "Computes", "ContactHistory", "Domain", "FixDampingCundall",
+    "AtomSystem", "ComputeContactAtom", "ComputeFabric", "ComputeStressAtom", "Computes",
+    "ContactHistory", "Domain", "FixDampingCundall",
     "FixDeformPressure", "FixNVESphere", "GranHertzHistory", "GranHookeHistory",
     "GranularPair", "NeighborList", "PairGranular", "Simulation", "f32split", "f64",
 ]

@@ -51,6 +51,7 @@ SYMBOLS = (
     "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
     "tmps_set_launch_mode", "tmps_adjoint_forward", "tmps_adjoint_backward",
     "tmps_batch_create", "tmps_batch_destroy", "tmps_batch_run", "tmps_batch_tuning",
+    "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom",
 )
 
 
@@ -147,6 +148,9 @@ def load(path: str | None = None):
         "tmps_servo_setup": (C.c_int, [vp, C.c_int, C.c_double]),
         "tmps_servo_end_of_step": (C.c_int, [vp, C.c_double, dp]),
         "tmps_virial_reduce": (C.c_int, [vp, C.c_int, dp]),
+        "tmps_contact_atom": (C.c_int, [vp, ip, C.POINTER(C.c_longlong)]),
+        "tmps_fabric": (C.c_int, [vp, dp, ip]),
+        "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
         "tmps_ke": (C.c_int, [vp, dp, dp]),
         "tmps_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
         "tmps_get_counters": (C.c_int, [vp, C.POINTER(Counters)]),
@@ -365,6 +369,28 @@ class NativeSystem:
         a, b = np.zeros(1), np.zeros(1)
         check(self._L.tmps_ke(self._h, _dp(a), _dp(b)))
         return float(a[0]), float(b[0])
+
+    def contact_atom(self, n):
+        """Per-atom overlapping partners (int32[n]) and their total."""
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        total = C.c_longlong(0)
+        check(self._L.tmps_contact_atom(self._h, _ip(out), C.byref(total)))
+        return out[:n], int(total.value)
+
+    def fabric(self):
+        """Contact-normal fabric [xx yy zz xy xz yz] and the contact count."""
+        phi = np.zeros(6)
+        nc = np.zeros(1, dtype=np.int32)
+        check(self._L.tmps_fabric(self._h, _dp(phi), _ip(nc)))
+        return phi, int(nc[0])
+
+    def stress_atom(self, n, keflag=False, per_atom=True):
+        """Per-atom stress (n x 6, or None) and its sum over atoms."""
+        st = np.zeros((max(n, 1), 6)) if per_atom else None
+        tot = np.zeros(6)
+        check(self._L.tmps_stress_atom(self._h, 1 if keflag else 0,
+                                       _dp(st) if per_atom else None, _dp(tot)))
+        return (st[:n] if per_atom else None), tot
 
     def run(self, nsteps, dt, run_pre=True, cundall=None):
         gl, ga = (0.0, 0.0) if cundall is None else cundall

@@ -1,5 +1,9 @@
-"""Global thermodynamic computes (reference: computes/)."""
+"""Computes (reference: computes/): global thermodynamic sums and the
+per-contact analysis computes."""
 
+from .contact_atom import ComputeContactAtom
+from .fabric import ComputeFabric
+from .stress_atom import ComputeStressAtom
 from .thermo import BOLTZMANN, Computes
 
-__all__ = ["Computes", "BOLTZMANN"]
+__all__ = ["BOLTZMANN", "ComputeContactAtom", "ComputeFabric", "ComputeStressAtom", "Computes"]

@@ -1148,6 +1148,7 @@ void free_all(System& s) {
     dfree(s.row_tmp);
     dfree(s.partial);
     dfree(s.level);
+    dfree(s.cmp_count); dfree(s.cmp_stress); dfree(s.cmp_sum);
     dfree(s.ctl);
     if (s.hctl) (void)hipHostFree(s.hctl);
     s.hctl = nullptr;
@@ -1607,6 +1608,101 @@ int tmps_ke(tmps_system* h, double* ke_trans, double* ke_rot) {
     if (rc) return rc;
     *ke_trans = s.hctl->vsum[0];
     *ke_rot = s.hctl->vsum[1];
+    return 0;
+}
+
+// ------------------------------------------------------- per-contact computes
+
+// The computes read the FP64 positions in xr (an FP32-split system decodes
+// its positions there first) and the current neighbor list; before the
+// first build every row is empty, as the reference's zeroed num_neighbors.
+static int computes_prepare(System& s) {
+    if (!s.cmp_sum) {
+        int rc = dalloc(&s.cmp_sum, 8);
+        if (rc) return rc;
+    }
+    return split_positions_to_xr(s);
+}
+
+int tmps_contact_atom(tmps_system* h, int* counts, long long* total) {
+    System& s = *sys(h);
+    if (total) *total = 0;
+    if (s.n <= 0) return 0;
+    std::vector<int> tmp;
+    int* out = counts;
+    if (!out) {
+        tmp.assign(s.n, 0);
+        out = tmp.data();
+    }
+    if (!s.has_built) {
+        std::fill(out, out + s.n, 0);
+        return 0;
+    }
+    int rc = computes_prepare(s);
+    if (rc) return rc;
+    if (s.cmp_count_cap < s.n) {
+        dfree(s.cmp_count);
+        rc = dalloc(&s.cmp_count, (size_t)s.n);
+        if (rc) return rc;
+        s.cmp_count_cap = s.n;
+    }
+    rc = launch_contact_atom(s, s.cmp_count);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, s.cmp_count, sizeof(int) * (size_t)s.n, hipMemcpyDeviceToHost,
+                           s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (total) {
+        long long t = 0;
+        for (int i = 0; i < s.n; ++i) t += out[i];
+        *total = t;
+    }
+    return 0;
+}
+
+int tmps_fabric(tmps_system* h, double phi[6], int* ncontacts) {
+    System& s = *sys(h);
+    for (int k = 0; k < 6; ++k) phi[k] = 0.0;
+    if (ncontacts) *ncontacts = 0;
+    if (s.n <= 0 || !s.has_built) return 0;
+    int rc = computes_prepare(s);
+    if (!rc) rc = launch_fabric(s, s.cmp_sum);
+    if (rc) return rc;
+    double sum[8];
+    HIP_TRY(hipMemcpyAsync(sum, s.cmp_sum, sizeof(double) * 7, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    const int nc = (int)sum[6];
+    if (ncontacts) *ncontacts = nc;
+    if (nc > 0) {
+        // fabric.py:70-76: phi = sum * (1 / Nc)
+        const double inv_nc = 1.0 / (double)nc;
+        for (int k = 0; k < 6; ++k) phi[k] = sum[k] * inv_nc;
+    }
+    return 0;
+}
+
+int tmps_stress_atom(tmps_system* h, int keflag, double* stress, double sum6[6]) {
+    System& s = *sys(h);
+    if (sum6)
+        for (int k = 0; k < 6; ++k) sum6[k] = 0.0;
+    if (s.n <= 0) return 0;
+    int rc = computes_prepare(s);
+    if (rc) return rc;
+    if (s.cmp_stress_cap < s.n) {
+        dfree(s.cmp_stress);
+        rc = dalloc(&s.cmp_stress, 6 * (size_t)s.n);
+        if (rc) return rc;
+        s.cmp_stress_cap = s.n;
+    }
+    rc = launch_stress_atom(s, keflag ? 1 : 0, s.cmp_stress, s.cmp_sum);
+    if (rc) return rc;
+    if (stress)
+        HIP_TRY(hipMemcpyAsync(stress, s.cmp_stress, sizeof(double) * 6 * (size_t)s.n,
+                               hipMemcpyDeviceToHost, s.stream));
+    double sum[8];
+    HIP_TRY(hipMemcpyAsync(sum, s.cmp_sum, sizeof(double) * 6, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (sum6)
+        for (int k = 0; k < 6; ++k) sum6[k] = sum[k];
     return 0;
 }
 

@@ -1102,6 +1102,126 @@ __global__ void kb_redo_rejoin(const RedoEntry* T) {
     ctl->bdone = e.done;
 }
 
+// ---------------------------------------------------- per-contact computes
+
+// Domain.minimum_image, orthogonal branch (domain.py:481-486), with the image
+// count taken as min_image_vshift takes it.
+__device__ __forceinline__ void min_image(const ImgP& B, double d[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (B.per[k] == 1) d[k] -= round(B.fast[k] ? d[k] * B.inv[k] : d[k] / B.prd[k]) * B.prd[k];
+}
+
+// Separation of the pair (a, b) = (min(i, j), max(i, j)) at the positions in
+// xr, as the half-list computes take it: d = x_a - x_b, minimum image.
+__device__ __forceinline__ double pair_sep(const ImgP& B, const double4& xa, const double4& xb,
+                                           double d[3]) {
+    d[0] = xa.x - xb.x;
+    d[1] = xa.y - xb.y;
+    d[2] = xa.z - xb.z;
+    min_image(B, d);
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// ComputeContactAtom.compute_kernel, contact_atom.py:19-52: the partners of
+// atom i that overlap it at the current positions (rsq < (ri + rj)^2 and
+// rsq > 1e-18). The full-list row holds each pair from both sides, so the
+// count is the reference's two-sided tally of its half list, without atomics.
+__global__ void k_contact_atom(const double4* xr, const int* nb_start, const int* nb_j,
+                               const Ctl* ctl, int n, int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ImgP B = img_params(ctl);
+    const double4 xi = xr[i];
+    int c = 0;
+    for (int s = nb_start[i], e = nb_start[i + 1]; s < e; ++s) {
+        const int j = nb_j[s];
+        const double4 xj = xr[j];
+        const bool first = i < j;
+        double d[3];
+        const double rsq = pair_sep(B, first ? xi : xj, first ? xj : xi, d);
+        const double radsum = first ? xi.w + xj.w : xj.w + xi.w;
+        if (rsq < radsum * radsum && rsq > 1e-18) ++c;
+    }
+    count[i] = c;
+}
+
+// ComputeFabric.compute_kernel, fabric.py:26-78: per atom i, the sums of
+// n_a n_b over its touching partners j > i (ascending j; n = d / |d|) and
+// their number, as leaves of one pairwise tree per block.
+__global__ void k_fabric_leaves(const double4* xr, const int* nb_start, const int* nb_j,
+                                const Ctl* ctl, int n, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double leaf[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const ImgP B = img_params(ctl);
+        const double4 xi = xr[i];
+        for (int s = nb_start[i], e = nb_start[i + 1]; s < e; ++s) {
+            const int j = nb_j[s];
+            if (j <= i) continue;
+            const double4 xj = xr[j];
+            double d[3];
+            const double rsq = pair_sep(B, xi, xj, d);
+            const double radsum = xi.w + xj.w;
+            if (rsq < radsum * radsum && rsq > 1e-18) {
+                const double r = sqrt(rsq);
+                const double nx = d[0] / r, ny = d[1] / r, nz = d[2] / r;
+                leaf[0] = leaf[0] + nx * nx;
+                leaf[1] = leaf[1] + ny * ny;
+                leaf[2] = leaf[2] + nz * nz;
+                leaf[3] = leaf[3] + nx * ny;
+                leaf[4] = leaf[4] + nx * nz;
+                leaf[5] = leaf[5] + ny * nz;
+                leaf[6] = leaf[6] + 1.0;
+            }
+        }
+    }
+    block_tree_sum<7>(leaf, partial + (size_t)blockIdx.x * 8);
+}
+
+// ComputeStressAtom.compute_stress_kernel, stress_atom.py:30-52: s = -virial
+// (- m v_a v_b with keflag) per atom, and the pairwise-tree sum of s over the
+// atoms (the macro stress before the division by the volume). Velocities come
+// from vm64 (FP64 systems) or vm32 (FP32-split systems); .w is the mass.
+__global__ void k_stress_atom(const double* vir, const double4* vm64, const float4* vm32,
+                              int keflag, int n, double* stress, double* partial) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] = -vir[6 * (size_t)i + k];
+        if (keflag) {
+            double m, vx, vy, vz;
+            if (vm64) {
+                const double4 v = vm64[i];
+                m = v.w; vx = v.x; vy = v.y; vz = v.z;
+            } else {
+                const float4 v = vm32[i];
+                m = v.w; vx = v.x; vy = v.y; vz = v.z;
+            }
+            s[0] = s[0] - m * vx * vx;
+            s[1] = s[1] - m * vy * vy;
+            s[2] = s[2] - m * vz * vz;
+            s[3] = s[3] - m * vx * vy;
+            s[4] = s[4] - m * vx * vz;
+            s[5] = s[5] - m * vy * vz;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stress[6 * (size_t)i + k] = s[k];
+    }
+    block_tree_sum<6>(s, partial + (size_t)blockIdx.x * 8);
+}
+
+// The final pairwise tree over nparts block partials, NC components, into out.
+template <int L, int NC>
+__global__ void __launch_bounds__(FINAL_THREADS) k_final_tree_out(const double* partial,
+                                                                   int nparts, double* out) {
+    double o[NC];
+    final_tree_L<L, NC>(partial, nparts, o);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NC; ++k) out[k] = o[k];
+}
+
 }  // namespace
 
 // ================================================================ launchers
@@ -1435,6 +1555,51 @@ int launch_copy_state(System& s, int honor_abort) {
                            (const int*)s.nb_start, s.ctl, honor_abort);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// ------------------------------------------------ per-contact computes
+
+#define TREE_OUT_CASE(Lv)                                                                     \
+    case Lv:                                                                                  \
+        if (nc == 6)                                                                          \
+            hipLaunchKernelGGL((k_final_tree_out<Lv, 6>), dim3(1), dim3(FINAL_THREADS), 0,     \
+                               s.stream, s.partial, nparts, out);                             \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_final_tree_out<Lv, 7>), dim3(1), dim3(FINAL_THREADS), 0,     \
+                               s.stream, s.partial, nparts, out);                             \
+        break;
+
+static int launch_tree_out(System& s, int nc, int nparts, double* out) {
+    switch (final_leaves(nparts)) {
+        TREE_OUT_CASE(1) TREE_OUT_CASE(2) TREE_OUT_CASE(4) TREE_OUT_CASE(8) TREE_OUT_CASE(16)
+        TREE_OUT_CASE(32) TREE_OUT_CASE(64)
+        default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_contact_atom(System& s, int* d_count) {
+    LAUNCH_N(k_contact_atom, s.n, s.xr, s.nb_start, s.nb_j, s.ctl, s.n, d_count);
+    return 0;
+}
+
+int launch_fabric(System& s, double* d_sum7) {
+    const int nb = std::max(nblocks_for(s.n), 1);
+    hipLaunchKernelGGL(k_fabric_leaves, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.nb_start,
+                       s.nb_j, s.ctl, s.n, s.partial);
+    HIP_TRY(hipGetLastError());
+    return launch_tree_out(s, 7, nb, d_sum7);
+}
+
+int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6) {
+    const int nb = std::max(nblocks_for(s.n), 1);
+    const bool f64 = s.precision == TMPS_FP64;
+    hipLaunchKernelGGL(k_stress_atom, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.vir,
+                       f64 ? s.vm : nullptr, f64 ? nullptr : s.vmf, keflag, s.n, d_stress,
+                       s.partial);
+    HIP_TRY(hipGetLastError());
+    return launch_tree_out(s, 6, nb, d_sum6);
 }
 
 // ------------------------------------------------ batched redo (batch.hip)

@@ -228,6 +228,13 @@ struct System {
     // the adjoint engine's checkpoints and work buffers (adjoint.hip)
     struct AdjContext* adj = nullptr;
 
+    // outputs of the per-contact computes, grown on first use: per-atom
+    // counts, per-atom stress [n][6], and the reduced sums (8 doubles)
+    int* cmp_count = nullptr;
+    double* cmp_stress = nullptr;
+    double* cmp_sum = nullptr;
+    int cmp_count_cap = 0, cmp_stress_cap = 0;
+
     // profiling
     bool profiling = false;
     // one-shot: the next step kernel records its own start / stop time
@@ -377,6 +384,10 @@ int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], in
 int launch_remap_apply_fused(System& s);
 void drop_graphs(System& s);
 int launch_final_tree(System& s, int nc, int nparts);
+// per-contact computes on the FP64 positions in xr (k_fp64.hip)
+int launch_contact_atom(System& s, int* d_count);
+int launch_fabric(System& s, double* d_sum7);
+int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6);
 
 // FP32-split path (k_split.hip)
 int launch_split_to_x64(System& s);
