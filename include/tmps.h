@@ -151,6 +151,14 @@ int tmps_history_restore(tmps_system* s);  /* restore_state, :147-154 */
 /* seed the current list's history in the flat half-list layout
  * (partner tag or -1, shear), as ContactHistory.partner/shear.from_numpy */
 int tmps_history_set_pairs(tmps_system* s, const int* partner, const double* shear, int npairs);
+/* ContactHistory.load_restart, contact_history.py:176-214: seed the saved
+ * history from a restart file, per atom (n == nlocal) a strictly ascending row
+ * of partner indices nbr[start[i] .. start[i+1]) and their shear (3 doubles
+ * each, in the lower-index-first convention). The next save is skipped (the
+ * reference's setup_pending) and the following list build's restore carries
+ * the seeded contacts onto the new list. */
+int tmps_history_seed(tmps_system* s, int n, const int* start, const int* nbr,
+                      const double* shear);
 
 /* ---- pair styles (GRANULAR/base.py:25-38) ------------------------------- */
 int tmps_pair_set(tmps_system* s, const tmps_pair_params* p);

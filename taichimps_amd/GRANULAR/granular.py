@@ -86,6 +86,17 @@ class PairGranular(GranularPair):
         self.uses_history = self.tangential_id in (2, 3)
         if self.tangential_id != 0:
             self._setup_tangential(tangential, tangential_coeffs or [])
+        # what a restart file stores (GranularModel::write_restart): sub-model
+        # names and raw coefficients, NULL as LAMMPS's -1 sentinel
+        self.submodels: dict[str, tuple[str, list[float]]] = {
+            "normal": (normal, [float(c) for c in normal_coeffs]),
+            "damping": (damping, []),
+            "tangential": (tangential, [-1.0 if c is None else float(c)
+                                        for c in (tangential_coeffs or [])]),
+            "rolling": ("none", []),
+            "twisting": ("none", []),
+            "heat": ("none", []),
+        }
 
     def _damping_coefficient(self, normal: str) -> float:
         """The damping sub-model's constant (granular.py:207-238)."""

@@ -51,7 +51,7 @@ SYMBOLS = (
     "tmps_set_counters", "tmps_last_pressure", "tmps_set_profiling", "tmps_kernel_times",
     "tmps_set_launch_mode", "tmps_adjoint_forward", "tmps_adjoint_backward",
     "tmps_batch_create", "tmps_batch_destroy", "tmps_batch_run", "tmps_batch_tuning",
-    "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom",
+    "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
 )
 
 
@@ -149,6 +149,7 @@ def load(path: str | None = None):
         "tmps_servo_end_of_step": (C.c_int, [vp, C.c_double, dp]),
         "tmps_virial_reduce": (C.c_int, [vp, C.c_int, dp]),
         "tmps_contact_atom": (C.c_int, [vp, ip, C.POINTER(C.c_longlong)]),
+        "tmps_history_seed": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "tmps_fabric": (C.c_int, [vp, dp, ip]),
         "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
         "tmps_ke": (C.c_int, [vp, dp, dp]),
@@ -333,6 +334,16 @@ class NativeSystem:
         check(self._L.tmps_history_set_pairs(self._h, _ip(pa), _dp(sh), int(len(pa))))
 
     # stages -----------------------------------------------------------------
+    def history_seed(self, start, nbr, shear):
+        """Seed the saved history: CSR rows of partner indices and their shear."""
+        st = np.ascontiguousarray(start, dtype=np.int32)
+        nb = np.ascontiguousarray(nbr, dtype=np.int32)
+        sh = np.ascontiguousarray(shear, dtype=np.float64).reshape(-1, 3)
+        n = len(st) - 1
+        nb_ = nb if len(nb) else np.zeros(1, dtype=np.int32)
+        sh_ = sh if len(sh) else np.zeros((1, 3))
+        check(self._L.tmps_history_seed(self._h, n, _ip(st), _ip(nb_), _dp(sh_)))
+
     def pair_set(self, p: PairParams):
         check(self._L.tmps_pair_set(self._h, C.byref(p)))
 

@@ -74,6 +74,7 @@ class ContactHistory:
         self.rolling = _PairField(self, "rolling")
         self.setup_pending = False
         self._atom = None
+        self._seed = None  # restart seed waiting for the atoms
 
     def _bind(self, atom) -> None:
         self._atom = atom
@@ -95,6 +96,7 @@ class ContactHistory:
         if atom.nlocal == 0:
             return
         self._bind(atom)
+        self._push_seed(atom)
         atom._native.history_restore()
         self.setup_pending = False
 
@@ -111,5 +113,74 @@ class ContactHistory:
         return out
 
     def load_restart(self, partner_tags, values) -> None:
-        """Restart seeding (contact_history.py:176-214) is outside the device path."""
-        raise NotImplementedError("restart files are not part of the device DEM step")
+        """Seed the saved history from a restart file, keyed by partner tag
+        (contact_history.py:176-214, FixNeighHistory::unpack_restart).
+
+        Entry k of the lists belongs to local atom k (the order the atoms were
+        given to add_particles). Each entry under atom i for partner j is in
+        the i-to-j convention. As in the reference, a pair (i, j > i) takes
+        the value stored under atom i, its lower-index owner. The first list
+        build after this (setup_pending skips the save before it) carries the
+        seeded contacts onto the new list.
+        """
+        n = min(len(partner_tags), self.max_atoms)
+        rows = []
+        for i in range(n):
+            t = np.asarray(partner_tags[i], dtype=np.int64).reshape(-1)
+            m = len(t)
+            if m > self.max_neighbors:
+                raise ValueError(
+                    f"atom {i} has {m} stored contacts but the history only has "
+                    f"{self.max_neighbors} slots per atom")
+            v = np.asarray(values[i], dtype=np.float64).reshape(m, -1)[:, :3]
+            rows.append((t, v))
+        self._seed = rows
+        self.setup_pending = True
+        if self._atom is not None and self._atom.nlocal > 0:
+            self._push_seed(self._atom)
+
+    def _push_seed(self, atom) -> None:
+        """Hand a pending restart seed to the device as per-atom rows of
+        partner indices (both atoms of each pair, the owner's value)."""
+        rows = getattr(self, "_seed", None)
+        if rows is None:
+            return
+        self._seed = None
+        self._bind(atom)
+        n = atom.nlocal
+        tags = atom.tag.to_numpy()[:n].astype(np.int64)
+        order = np.argsort(tags, kind="stable")
+        owner, other, val = [], [], []
+        for i, (t, v) in enumerate(rows[:n]):
+            if len(t) == 0:
+                continue
+            pos = np.minimum(np.searchsorted(tags[order], t), n - 1)
+            found = tags[order][pos] == t
+            j = order[pos]
+            own = found & (j > i)  # pairs whose lower-index atom is i
+            owner.append(np.full(int(own.sum()), i, dtype=np.int64))
+            other.append(j[own])
+            val.append(v[own])
+        if owner:
+            a = np.concatenate(owner)
+            b = np.concatenate(other)
+            sv = np.concatenate(val)
+        else:
+            a = b = np.zeros(0, dtype=np.int64)
+            sv = np.zeros((0, 3))
+        # both rows of each pair hold the lower-index-first value
+        ra, rb = np.concatenate([a, b]), np.concatenate([b, a])
+        rv = np.concatenate([sv, sv])
+        srt = np.lexsort((rb, ra))
+        ra, rb, rv = ra[srt], rb[srt], rv[srt]
+        if len(ra) > 1:
+            dup = (ra[1:] == ra[:-1]) & (rb[1:] == rb[:-1])
+            if np.any(dup):
+                keep = np.concatenate([[True], ~dup])
+                ra, rb, rv = ra[keep], rb[keep], rv[keep]
+        start = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(np.bincount(ra, minlength=n), out=start[1:])
+        if np.any(np.diff(start) > self.max_neighbors):
+            raise ValueError("an atom has more restored contacts than the history has slots "
+                             "per atom")
+        atom._native.history_seed(start, rb, rv)

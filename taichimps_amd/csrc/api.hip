@@ -310,6 +310,12 @@ int history_reserve(System& s) {
 }
 
 int history_save(System& s) {
+    // a seeded history stands in for the save (save_state returns early while
+    // setup_pending, contact_history.py:141-145)
+    if (s.seed_pending) {
+        s.seed_pending = false;
+        return 0;
+    }
     // keep a copy of the current list: the restore merges the new list with it
     const size_t need = (size_t)std::max(s.nslots, 0);
     int rc = history_reserve(s);
@@ -325,6 +331,7 @@ int history_restore(System& s) {
     if (!s.have_old) return 0;
     int rc = launch_hist_restore(s);
     s.have_old = false;
+    s.seed_pending = false;
     return rc;
 }
 
@@ -761,14 +768,21 @@ int redo_batch_begin(System& s, RedoEntry& e) {
     H.abort = 0;
     H.trigger = -1;
     s.current_step = stp;
-    // history_save
+    // history_save (skipped for a seeded history, as history_save skips it)
     const int old_slots = std::max(s.nslots, 0);
-    int rc = history_reserve(s);
-    if (rc) return rc;
-    e.save_blocks = std::min((std::max(s.n + 1, old_slots) + TMPS_BLOCK - 1) / TMPS_BLOCK, 4096);
     e.copy_blocks = std::min(nblocks_for(std::max(s.n, old_slots)), 4096);
-    s.onslots = old_slots;
-    s.onb_n = s.n;
+    int rc = 0;
+    if (s.seed_pending) {
+        s.seed_pending = false;
+        e.save_blocks = 0;
+    } else {
+        rc = history_reserve(s);
+        if (rc) return rc;
+        e.save_blocks =
+            std::min((std::max(s.n + 1, old_slots) + TMPS_BLOCK - 1) / TMPS_BLOCK, 4096);
+        s.onslots = old_slots;
+        s.onb_n = s.n;
+    }
     s.have_old = true;
     // pbc(s, true)
     if (H.box.vremap)
@@ -829,6 +843,7 @@ int redo_batch_list(System& s, int4 flags, RedoEntry& e) {
     if (rc) return rc;
     s.x64_valid = false;  // the split encoding from the new bins
     s.have_old = false;   // history_restore
+    s.seed_pending = false;
     s.rebuilds += 1;
     Ctl& H = *s.hctl;
     const int stp = s.current_step;
@@ -1467,6 +1482,7 @@ int tmps_neigh_get_bins(tmps_system* h, int* cell_of, int* cell_start, int* cell
 int tmps_history_reset(tmps_system* h) {
     System& s = *sys(h);
     s.have_old = false;
+    s.seed_pending = false;
     return launch_hist_clear(s);
 }
 
@@ -1492,6 +1508,67 @@ int tmps_history_set_pairs(tmps_system* h, const int* partner, const double* she
     if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
     dfree(dpa); dfree(dsh); dfree(dst);
     return rc;
+}
+
+int tmps_history_seed(tmps_system* h, int n, const int* start, const int* nbr,
+                      const double* shear) {
+    System& s = *sys(h);
+    if (n != s.n) return tmps_fail(TMPS_ERR_ARG, "a seeded history covers exactly nlocal atoms");
+    if (!start || (start[n] > 0 && (!nbr || !shear)))
+        return tmps_fail(TMPS_ERR_ARG, "null seeded history arrays");
+    if (start[0] != 0) return tmps_fail(TMPS_ERR_ARG, "seeded history rows must start at 0");
+    for (int i = 0; i < n; ++i) {
+        if (start[i + 1] < start[i])
+            return tmps_fail(TMPS_ERR_ARG, "seeded history row offsets must not decrease");
+        if (start[i + 1] - start[i] > s.max_nb)
+            return tmps_fail(TMPS_ERR_ARG, "an atom has more stored contacts than the history "
+                                           "has slots per atom");
+        for (int k = start[i]; k < start[i + 1]; ++k) {
+            if (nbr[k] < 0 || nbr[k] >= n || nbr[k] == i)
+                return tmps_fail(TMPS_ERR_ARG, "seeded history partner out of range");
+            if (k > start[i] && nbr[k] <= nbr[k - 1])
+                return tmps_fail(TMPS_ERR_ARG, "seeded history rows must be strictly ascending");
+        }
+    }
+    const size_t total = (size_t)start[n];
+    const size_t hsz = split(s) ? sizeof(float4) : sizeof(double4);
+    if (total > s.onb_cap || s.onb_j == nullptr) {
+        dfree(s.onb_j);
+        dfree(s.onb_flag);
+        if (s.onb_hist) (void)hipFree(s.onb_hist);
+        s.onb_hist = nullptr;
+        const size_t cap = total + total / 4 + 1024;
+        int rc = dalloc(&s.onb_j, cap);
+        if (!rc) rc = dalloc(&s.onb_flag, cap);
+        if (rc) return rc;
+        HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
+        s.onb_cap = cap;
+    }
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(s.onb_start, start, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    if (total > 0) {
+        HIP_TRY(hipMemcpy(s.onb_j, nbr, sizeof(int) * total, hipMemcpyHostToDevice));
+        std::vector<uint8_t> ones(total, 1);
+        HIP_TRY(hipMemcpy(s.onb_flag, ones.data(), total, hipMemcpyHostToDevice));
+        if (split(s)) {
+            std::vector<float4> hv(total);
+            for (size_t k = 0; k < total; ++k)
+                hv[k] = make_float4((float)shear[3 * k], (float)shear[3 * k + 1],
+                                    (float)shear[3 * k + 2], 0.0f);
+            HIP_TRY(hipMemcpy(s.onb_hist, hv.data(), total * sizeof(float4), hipMemcpyHostToDevice));
+        } else {
+            std::vector<double4> hv(total);
+            for (size_t k = 0; k < total; ++k)
+                hv[k] = make_double4(shear[3 * k], shear[3 * k + 1], shear[3 * k + 2], 0.0);
+            HIP_TRY(hipMemcpy(s.onb_hist, hv.data(), total * sizeof(double4),
+                              hipMemcpyHostToDevice));
+        }
+    }
+    s.onslots = (int)total;
+    s.onb_n = n;
+    s.have_old = true;
+    s.seed_pending = true;
+    return 0;
 }
 
 int tmps_pair_set(tmps_system* h, const tmps_pair_params* p) {

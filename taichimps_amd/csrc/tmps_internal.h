@@ -174,6 +174,9 @@ struct System {
     size_t nb_cap = 0, onb_cap = 0;
     int nslots = 0, onslots = 0, onb_n = 0;
     bool have_old = false;
+    // the o* copy holds a history seeded from a restart file: the next save
+    // is skipped so that the following restore carries it onto the new list
+    bool seed_pending = false;
     bool x64_valid = false;  // FP32-split: xr holds the authoritative positions
 
     // bins

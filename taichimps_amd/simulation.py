@@ -168,6 +168,8 @@ class Simulation:
         self.neighbor._bind(atom)
         self.neighbor._push()
         self.history._bind(atom)
+        self.history._push_seed(atom)
+        builds_before = N.counters()["build_count"]
         self.pair_style._install(atom)
         if deform is None:
             off = _native.ServoParams()
@@ -182,6 +184,9 @@ class Simulation:
             c = N.counters()
             self.timestep = c["timestep"]
             self.setup_done = bool(c["setup_done"])
+            if c["build_count"] > builds_before:
+                # the run's first build carried any restart seed onto its list
+                self.history.setup_pending = False
             self.neighbor._pull()
             for fix in self.fixes:
                 fix.timestep = self.timestep
