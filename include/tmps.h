@@ -178,6 +178,33 @@ int tmps_servo_set(tmps_system* s, const tmps_servo_params* p);
 int tmps_servo_setup(tmps_system* s, int nsteps_total, double dt);
 int tmps_servo_end_of_step(tmps_system* s, double dt, double tensor_out[6]);
 
+/* ---- fix wall/gran (EXTRA_FIX/wall_gran.py) ------------------------------
+ * A flat, axis-aligned wall in contact with the atoms of an FP64 system:
+ * FixWallGran's settings (wall_gran.py:57-111) after its host derivations
+ * (gammat = 0 when dampflag is 0). A wall owns its per-atom shear history
+ * (max_atoms of the system, zeroed at creation). */
+typedef struct {
+    int axis;          /* 0, 1 or 2 */
+    int side;          /* -1: wall below the atoms (normal +axis); +1: above */
+    int use_history;   /* hooke/history (1) or the history-free hooke (0) */
+    int limit_damping;
+    int shear_axis;    /* in-plane wall velocity axis, -1 for none */
+    double kn, kt, gamman, gammat, xmu;
+} tmps_wall_params;
+typedef struct tmps_wall tmps_wall;
+int tmps_wall_create(tmps_system* s, const tmps_wall_params* p, tmps_wall** out);
+int tmps_wall_destroy(tmps_wall* w);  /* also valid after its system is destroyed */
+/* FixWallGran.reset_history, wall_gran.py:113-118 */
+int tmps_wall_reset(tmps_wall* w);
+/* FixWallGran.post_force_kernel, wall_gran.py:120-211: adds the wall's force
+ * and torque to every atom's, given this step's wall position and its normal
+ * and in-plane velocities (post_force, :213-236, computes them) */
+int tmps_wall_post_force(tmps_wall* w, double dt, double coord, double vwall_n, double vwall_s,
+                         int history_update);
+/* FixWallGran.shear / touch: per-atom shear (3n, nullable) and touch flags
+ * (n, nullable) */
+int tmps_wall_get_history(tmps_wall* w, double* shear, int* touch, int n);
+
 /* ---- computes (computes/thermo.py) --------------------------------------- */
 /* compute_virial_kernel, thermo.py:62-109: sum of the per-atom virial
  * (+ m v_a v_b when keflag) as a pairwise tree over atoms */

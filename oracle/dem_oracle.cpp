@@ -73,6 +73,22 @@ typedef struct {
 
 namespace {
 
+// Mirrors FixWallGran's constructor settings (wall_gran.py:57-111).
+extern "C" {
+typedef struct {
+    int axis, side, use_history, limit_damping, shear_axis;
+    double kn, kt, gamman, gammat, xmu;  // gammat already 0 when dampflag is 0
+} orc_wall_params;
+}
+
+// One flat wall: settings, its motion and its per-atom shear history.
+struct OrcWall {
+    orc_wall_params p{};
+    double coord = 0.0, shear_vel = 0.0, amp = 0.0, period = 0.0, elapsed = 0.0;
+    std::vector<double> shear;  // [n][3]
+    std::vector<int> touch;
+};
+
 struct Oracle {
     int n = 0;
     int max_nb = 64;
@@ -121,6 +137,9 @@ struct Oracle {
     int setup_done = 0;
     int rebuilds_in_run = 0;
     double last_tensor[6] = {0, 0, 0, 0, 0, 0};
+
+    // FixWallGran fixes, applied after damping/cundall in post_force
+    std::vector<OrcWall> walls;
 };
 
 // ---------------------------------------------------------------- Domain
@@ -701,6 +720,104 @@ void cundall_post_force(Oracle& s) {
         }
 }
 
+// ------------------------------------------------------------------ walls
+
+// FixWallGran.post_force_kernel, wall_gran.py:120-211: the wall as a partner
+// of zero radius and infinite mass (radsum = Reff = r_i, meff = m_i, moment
+// arm r_i), hooke normal force with mass-velocity damping, and the
+// linear_history_classic tangential force (or the history-free viscous one).
+void wall_kernel(Oracle& s, OrcWall& w, double dt, double coord, double vwall_n,
+                 double vwall_s, int history_update) {
+    const orc_wall_params& P = w.p;
+    const int axis = P.axis;
+    double nv[3] = {0.0, 0.0, 0.0};
+    nv[axis] = P.side == 1 ? -1.0 : 1.0;
+    double vwall[3] = {0.0, 0.0, 0.0};
+    vwall[axis] = vwall_n;
+    if (P.shear_axis >= 0) vwall[P.shear_axis] = vwall_s;
+    for (int i = 0; i < s.n; ++i) {
+        const double ri = s.radius[i];
+        const double xa = s.x[3 * i + axis];
+        const double dist = P.side == -1 ? xa - coord : coord - xa;
+        const double delta = ri - dist;
+        if (delta <= 0.0) {
+            if (P.use_history) {
+                w.touch[i] = 0;
+                for (int k = 0; k < 3; ++k) w.shear[3 * i + k] = 0.0;
+            }
+            continue;
+        }
+        const double meff = s.rmass[i];
+        double vr[3];
+        for (int k = 0; k < 3; ++k) vr[k] = s.v[3 * i + k] - vwall[k];
+        const double vnnr = vr[0] * nv[0] + vr[1] * nv[1] + vr[2] * nv[2];
+        double vt[3], wr[3], vtr[3];
+        for (int k = 0; k < 3; ++k) vt[k] = vr[k] - vnnr * nv[k];
+        for (int k = 0; k < 3; ++k) wr[k] = ri * s.omega[3 * i + k];
+        const double wxn[3] = {wr[1] * nv[2] - wr[2] * nv[1], wr[2] * nv[0] - wr[0] * nv[2],
+                               wr[0] * nv[1] - wr[1] * nv[0]};
+        for (int k = 0; k < 3; ++k) vtr[k] = vt[k] - wxn[k];
+        const double vrel = std::sqrt(vtr[0] * vtr[0] + vtr[1] * vtr[1] + vtr[2] * vtr[2]);
+        const double damp_prefactor = meff * P.gamman;
+        double fntot = P.kn * delta - damp_prefactor * vnnr;
+        if (P.limit_damping) fntot = std::max(fntot, 0.0);
+        const double fscrit = P.xmu * std::fabs(fntot);
+        const double damp_t = meff * P.gammat;
+        double fs[3] = {0.0, 0.0, 0.0};
+        if (P.use_history) {
+            double h[3] = {w.shear[3 * i], w.shear[3 * i + 1], w.shear[3 * i + 2]};
+            if (w.touch[i] == 0) {
+                h[0] = h[1] = h[2] = 0.0;
+                w.touch[i] = 1;
+            }
+            if (history_update)
+                for (int k = 0; k < 3; ++k) h[k] = h[k] + vtr[k] * dt;
+            const double shrmag = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+            if (history_update) {
+                const double hn = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
+                for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nv[k];
+            }
+            double fdamp[3];
+            for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vtr[k];
+            for (int k = 0; k < 3; ++k) fs[k] = -P.kt * h[k] + fdamp[k];
+            const double magfs = std::sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
+            if (magfs > fscrit) {
+                if (shrmag != 0.0) {
+                    const double sc = fscrit / magfs;
+                    for (int k = 0; k < 3; ++k) fs[k] = fs[k] * sc;
+                    for (int k = 0; k < 3; ++k) h[k] = -(fs[k] - fdamp[k]) / P.kt;
+                } else {
+                    fs[0] = fs[1] = fs[2] = 0.0;
+                }
+            }
+            for (int k = 0; k < 3; ++k) w.shear[3 * i + k] = h[k];
+        } else {
+            double ft = 0.0;
+            if (vrel != 0.0) ft = std::min(fscrit, damp_t * vrel) / vrel;
+            for (int k = 0; k < 3; ++k) fs[k] = -ft * vtr[k];
+        }
+        for (int k = 0; k < 3; ++k) s.f[3 * i + k] = s.f[3 * i + k] + (fntot * nv[k] + fs[k]);
+        const double nxf[3] = {nv[1] * fs[2] - nv[2] * fs[1], nv[2] * fs[0] - nv[0] * fs[2],
+                               nv[0] * fs[1] - nv[1] * fs[0]};
+        for (int k = 0; k < 3; ++k) s.torque[3 * i + k] = s.torque[3 * i + k] + -ri * nxf[k];
+    }
+}
+
+// FixWallGran.post_force, wall_gran.py:213-236: the wall's position and
+// normal velocity under `wiggle`, then the contact kernel; the clock
+// advances by dt per call.
+void wall_post_force(Oracle& s, OrcWall& w, double dt, int history_update) {
+    double coord = w.coord, vwall_n = 0.0;
+    if (w.amp != 0.0) {
+        const double omega_w = 2.0 * M_PI / w.period;
+        const double arg = omega_w * w.elapsed;
+        coord += w.amp * (1.0 - std::cos(arg));
+        vwall_n = w.amp * omega_w * std::sin(arg);
+    }
+    wall_kernel(s, w, dt, coord, vwall_n, w.shear_vel, history_update);
+    w.elapsed += dt;
+}
+
 // ----------------------------------------------------------------- virial
 
 // Per-atom leaf of the virial(+kinetic) sum, compute_virial_kernel thermo.py:62-109
@@ -1072,6 +1189,7 @@ int init_simulation(Oracle& s, double dt) {
         pair_compute(s, dt, 0);
     }
     if (s.cundall) cundall_post_force(s);
+    for (OrcWall& w : s.walls) wall_post_force(s, w, dt, 0);
     return 0;
 }
 
@@ -1086,6 +1204,7 @@ int sub_step(Oracle& s, double dt) {
     clear_forces(s);
     if (s.have_pair) pair_compute(s, dt, 1);
     if (s.cundall) cundall_post_force(s);
+    for (OrcWall& w : s.walls) wall_post_force(s, w, dt, 1);
     nve_final(s, dt);
     if (s.servo.enabled) {
         if (servo_end_of_step(s, dt)) return -2;
@@ -1200,6 +1319,7 @@ int orc_run(void* h, int nsteps, double dt, int run_pre) {
     Oracle& s = *static_cast<Oracle*>(h);
     s.rebuilds_in_run = 0;
     if (s.servo.enabled) servo_setup(s, nsteps, dt);
+    for (OrcWall& w : s.walls) w.elapsed = 0.0;  // FixWallGran.setup
     if (run_pre || !s.setup_done) {
         int rc = init_simulation(s, dt);
         if (rc) return rc;
@@ -1245,6 +1365,34 @@ void orc_virial(void* h, int keflag, int mode, double* out6) {
 }
 int orc_servo_end_of_step(void* h, double dt) {
     return servo_end_of_step(*static_cast<Oracle*>(h), dt);
+}
+// FixWallGran: add a wall (its index is returned), read its history, or run
+// one post_force on the current forces.
+int orc_add_wall(void* h, const orc_wall_params* p, double coord, double shear_vel, double amp,
+                 double period) {
+    Oracle& s = *static_cast<Oracle*>(h);
+    OrcWall w;
+    w.p = *p;
+    w.coord = coord;
+    w.shear_vel = shear_vel;
+    w.amp = amp;
+    w.period = period;
+    w.shear.assign((size_t)3 * s.n, 0.0);
+    w.touch.assign(s.n, 0);
+    s.walls.push_back(w);
+    return (int)s.walls.size() - 1;
+}
+void orc_wall_history(void* h, int idx, double* shear, int* touch) {
+    Oracle& s = *static_cast<Oracle*>(h);
+    const OrcWall& w = s.walls[idx];
+    for (int i = 0; i < s.n; ++i) {
+        for (int k = 0; k < 3; ++k) shear[3 * i + k] = w.shear[3 * i + k];
+        touch[i] = w.touch[i];
+    }
+}
+void orc_wall_post_force(void* h, int idx, double dt, int history_update) {
+    Oracle& s = *static_cast<Oracle*>(h);
+    wall_post_force(s, s.walls[idx], dt, history_update);
 }
 void orc_contact_atom(void* h, int* count) { contact_atom(*static_cast<Oracle*>(h), count); }
 void orc_fabric(void* h, double* phi6, int* ncontacts) {

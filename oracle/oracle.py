@@ -50,6 +50,15 @@ class ServoParams(C.Structure):
     ]
 
 
+class WallParams(C.Structure):
+    _fields_ = [
+        ("axis", C.c_int), ("side", C.c_int), ("use_history", C.c_int),
+        ("limit_damping", C.c_int), ("shear_axis", C.c_int),
+        ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+        ("gammat", C.c_double), ("xmu", C.c_double),
+    ]
+
+
 STYLE_CODE = {"none": 0, "erate": 1, "trate": 2, "vel": 3, "wiggle": 4, "final": 5,
               "delta": 6, "scale": 7, "pressure": 8, "pressure/mean": 9}
 COUPLE_CODE = {"none": 0, "xyz": 1, "xy": 2, "yz": 3, "xz": 4}
@@ -111,6 +120,10 @@ def lib():
         L.orc_contact_atom.argtypes = [vp, ip]
         L.orc_fabric.argtypes = [vp, dp, ip]
         L.orc_stress_atom.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_add_wall.argtypes = [vp, C.POINTER(WallParams), C.c_double, C.c_double,
+                                   C.c_double, C.c_double]
+        L.orc_wall_history.argtypes = [vp, C.c_int, dp, ip]
+        L.orc_wall_post_force.argtypes = [vp, C.c_int, C.c_double, C.c_int]
         _lib = L
     return _lib
 
@@ -358,6 +371,30 @@ class OracleSim:
 
     def servo_end_of_step(self, dt) -> int:
         return lib().orc_servo_end_of_step(self._h, float(dt))
+
+    # fix wall/gran ----------------------------------------------------------
+    def add_wall(self, axis, side, coord, kn, gamman, kt, gammat, xmu, dampflag=1,
+                 history=True, limit_damping=False, shear_axis=-1, shear_vel=0.0,
+                 wiggle_amplitude=0.0, wiggle_period=0.0) -> int:
+        """FixWallGran (wall_gran.py:57-111), applied after damping/cundall in
+        every post_force; returns the wall's index."""
+        p = WallParams()
+        p.axis, p.side = int(axis), int(side)
+        p.use_history, p.limit_damping = (1 if history else 0), (1 if limit_damping else 0)
+        p.shear_axis = int(shear_axis)
+        p.kn, p.kt, p.gamman, p.xmu = float(kn), float(kt), float(gamman), float(xmu)
+        p.gammat = 0.0 if int(dampflag) == 0 else float(gammat)
+        return lib().orc_add_wall(self._h, C.byref(p), float(coord), float(shear_vel),
+                                  float(wiggle_amplitude), float(wiggle_period))
+
+    def wall_history(self, idx):
+        sh = np.zeros((self.n, 3))
+        tc = np.zeros(self.n, dtype=np.int32)
+        lib().orc_wall_history(self._h, int(idx), _dp(sh), _ip(tc))
+        return sh, tc
+
+    def wall_post_force(self, idx, dt, history_update=True):
+        lib().orc_wall_post_force(self._h, int(idx), float(dt), 1 if history_update else 0)
 
     # per-contact computes ----------------------------------------------------
     def contact_atom(self):

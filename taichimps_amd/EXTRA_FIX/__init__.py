@@ -4,5 +4,6 @@ from .base import Fix
 from .damping_cundall import FixDampingCundall
 from .deform_pressure import FixDeformPressure
 from .nve_sphere import FixNVESphere
+from .wall_gran import FixWallGran
 
-__all__ = ["Fix", "FixNVESphere", "FixDampingCundall", "FixDeformPressure"]
+__all__ = ["Fix", "FixNVESphere", "FixDampingCundall", "FixDeformPressure", "FixWallGran"]

@@ -52,6 +52,8 @@ SYMBOLS = (
     "tmps_set_launch_mode", "tmps_adjoint_forward", "tmps_adjoint_backward",
     "tmps_batch_create", "tmps_batch_destroy", "tmps_batch_run", "tmps_batch_tuning",
     "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
+    "tmps_wall_create", "tmps_wall_destroy", "tmps_wall_reset", "tmps_wall_post_force",
+    "tmps_wall_get_history",
 )
 
 
@@ -79,6 +81,17 @@ class ServoParams(C.Structure):
         ("flo", C.c_double * 3), ("fhi", C.c_double * 3), ("dlo", C.c_double * 3),
         ("dhi", C.c_double * 3), ("amp", C.c_double * 3), ("tper", C.c_double * 3),
         ("ptarget", C.c_double * 3), ("pgain", C.c_double * 3),
+    ]
+
+
+class WallParams(C.Structure):
+    """tmps_wall_params"""
+
+    _fields_ = [
+        ("axis", C.c_int), ("side", C.c_int), ("use_history", C.c_int),
+        ("limit_damping", C.c_int), ("shear_axis", C.c_int),
+        ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+        ("gammat", C.c_double), ("xmu", C.c_double),
     ]
 
 
@@ -150,6 +163,12 @@ def load(path: str | None = None):
         "tmps_virial_reduce": (C.c_int, [vp, C.c_int, dp]),
         "tmps_contact_atom": (C.c_int, [vp, ip, C.POINTER(C.c_longlong)]),
         "tmps_history_seed": (C.c_int, [vp, C.c_int, ip, ip, dp]),
+        "tmps_wall_create": (C.c_int, [vp, C.POINTER(WallParams), C.POINTER(vp)]),
+        "tmps_wall_destroy": (C.c_int, [vp]),
+        "tmps_wall_reset": (C.c_int, [vp]),
+        "tmps_wall_post_force": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_int]),
+        "tmps_wall_get_history": (C.c_int, [vp, dp, ip, C.c_int]),
         "tmps_fabric": (C.c_int, [vp, dp, ip]),
         "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
         "tmps_ke": (C.c_int, [vp, dp, dp]),
@@ -497,6 +516,40 @@ class NativeBatch:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.tmps_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NativeWall:
+    """A fix wall/gran's device history (tmps_wall_*), bound to one system."""
+
+    def __init__(self, system: "NativeSystem", p: WallParams):
+        self._L = load()
+        self._sys = system  # keeps the system alive while the wall exists
+        self._h = C.c_void_p()
+        check(self._L.tmps_wall_create(system.handle, C.byref(p), C.byref(self._h)))
+
+    def reset(self):
+        check(self._L.tmps_wall_reset(self._h))
+
+    def post_force(self, dt, coord, vwall_n, vwall_s, history_update):
+        check(self._L.tmps_wall_post_force(self._h, float(dt), float(coord), float(vwall_n),
+                                           float(vwall_s), 1 if history_update else 0))
+
+    def history(self, n):
+        sh = np.zeros((max(n, 1), 3))
+        tc = np.zeros(max(n, 1), dtype=np.int32)
+        check(self._L.tmps_wall_get_history(self._h, _dp(sh), _ip(tc), int(n)))
+        return sh[:n], tc[:n]
+
+    def close(self):
+        if self._h:
+            self._L.tmps_wall_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

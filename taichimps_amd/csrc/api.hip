@@ -1688,6 +1688,85 @@ int tmps_ke(tmps_system* h, double* ke_trans, double* ke_rot) {
     return 0;
 }
 
+// ---------------------------------------------------------------- walls
+
+struct tmps_wall {
+    System* sys = nullptr;
+    tmps_wall_params p{};
+    double4* hist = nullptr;  // [max_atoms]: shear xyz, touch flag in w
+    int cap = 0;
+};
+
+int tmps_wall_create(tmps_system* h, const tmps_wall_params* p, tmps_wall** out) {
+    System& s = *sys(h);
+    if (!p || !out) return tmps_fail(TMPS_ERR_ARG, "null wall arguments");
+    *out = nullptr;
+    if (p->axis < 0 || p->axis > 2) return tmps_fail(TMPS_ERR_ARG, "wall_axis must be 0, 1 or 2");
+    if (p->side != -1 && p->side != 1)
+        return tmps_fail(TMPS_ERR_ARG, "wall_side must be -1 (low) or +1 (high)");
+    if (p->shear_axis == p->axis)
+        return tmps_fail(TMPS_ERR_ARG,
+                         "shear must act in the plane of the wall, not along its normal");
+    if (p->shear_axis < -1 || p->shear_axis > 2)
+        return tmps_fail(TMPS_ERR_ARG, "shear axis must be -1, 0, 1 or 2");
+    if (s.precision != TMPS_FP64)
+        return tmps_fail(TMPS_ERR_UNSUPPORTED, "fix wall/gran runs on FP64 systems");
+    tmps_wall* w = new tmps_wall();
+    w->sys = &s;
+    w->p = *p;
+    w->cap = s.max_atoms;
+    int rc = dalloc(&w->hist, (size_t)std::max(s.max_atoms, 1));
+    if (rc) {
+        delete w;
+        return rc;
+    }
+    *out = w;
+    return 0;
+}
+
+int tmps_wall_destroy(tmps_wall* w) {
+    if (!w) return 0;
+    // touches only the wall's own memory: the system may already be gone
+    // (hipFree waits for work that still uses the buffer)
+    dfree(w->hist);
+    delete w;
+    return 0;
+}
+
+int tmps_wall_reset(tmps_wall* w) {
+    if (!w) return tmps_fail(TMPS_ERR_ARG, "null wall");
+    HIP_TRY(hipMemsetAsync(w->hist, 0, sizeof(double4) * (size_t)w->cap, w->sys->stream));
+    return 0;
+}
+
+int tmps_wall_post_force(tmps_wall* w, double dt, double coord, double vwall_n, double vwall_s,
+                         int history_update) {
+    if (!w) return tmps_fail(TMPS_ERR_ARG, "null wall");
+    System& s = *w->sys;
+    if (s.n > w->cap) return tmps_fail(TMPS_ERR_CAPACITY, "more atoms than the wall's history");
+    return launch_wall_gran(s, w->hist, w->p, dt, coord, vwall_n, vwall_s,
+                            history_update ? 1 : 0);
+}
+
+int tmps_wall_get_history(tmps_wall* w, double* shear, int* touch, int n) {
+    if (!w) return tmps_fail(TMPS_ERR_ARG, "null wall");
+    System& s = *w->sys;
+    if (n < 0 || n > w->cap) return tmps_fail(TMPS_ERR_ARG, "history count out of range");
+    std::vector<double4> hv((size_t)std::max(n, 1));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (n > 0)
+        HIP_TRY(hipMemcpy(hv.data(), w->hist, sizeof(double4) * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        if (shear) {
+            shear[3 * i] = hv[i].x;
+            shear[3 * i + 1] = hv[i].y;
+            shear[3 * i + 2] = hv[i].z;
+        }
+        if (touch) touch[i] = hv[i].w != 0.0 ? 1 : 0;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------- per-contact computes
 
 // The computes read the FP64 positions in xr (an FP32-split system decodes

@@ -1212,6 +1212,107 @@ __global__ void k_stress_atom(const double* vir, const double4* vm64, const floa
     block_tree_sum<6>(s, partial + (size_t)blockIdx.x * 8);
 }
 
+// ------------------------------------------------------------------ walls
+
+// FixWallGran.post_force_kernel, wall_gran.py:120-211: the wall as a partner
+// of zero radius and infinite mass (radsum = Reff = r_i, meff = m_i, moment
+// arm r_i); hooke normal force with mass-velocity damping and the
+// linear_history_classic tangential force, or the history-free viscous one.
+// wh[i] holds the shear history (xyz) and the touch flag (w).
+__global__ void k_wall_gran(const double4* xr, const double4* vm, const double4* om, double4* f,
+                            double4* tq, double4* wh, int n, tmps_wall_params P, double dt,
+                            double coord, double vwall_n, double vwall_s, int history_update) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int axis = P.axis;
+    double nv[3] = {0.0, 0.0, 0.0};
+    nv[axis] = P.side == 1 ? -1.0 : 1.0;
+    double vwall[3] = {0.0, 0.0, 0.0};
+    vwall[axis] = vwall_n;
+    if (P.shear_axis >= 0) vwall[P.shear_axis] = vwall_s;
+    const double4 x4 = xr[i];
+    const double ri = x4.w;
+    const double xs[3] = {x4.x, x4.y, x4.z};
+    const double xa = xs[axis];
+    const double dist = P.side == -1 ? xa - coord : coord - xa;
+    const double delta = ri - dist;
+    if (delta <= 0.0) {
+        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const double4 v4 = vm[i], o4 = om[i];
+    const double meff = v4.w;
+    const double v[3] = {v4.x, v4.y, v4.z}, w[3] = {o4.x, o4.y, o4.z};
+    double vr[3], vt[3], wr[3], vtr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vr[k] = v[k] - vwall[k];
+    const double vnnr = vr[0] * nv[0] + vr[1] * nv[1] + vr[2] * nv[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vt[k] = vr[k] - vnnr * nv[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wr[k] = ri * w[k];
+    const double wxn[3] = {wr[1] * nv[2] - wr[2] * nv[1], wr[2] * nv[0] - wr[0] * nv[2],
+                           wr[0] * nv[1] - wr[1] * nv[0]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vtr[k] = vt[k] - wxn[k];
+    const double vrel = sqrt(vtr[0] * vtr[0] + vtr[1] * vtr[1] + vtr[2] * vtr[2]);
+    const double damp_prefactor = meff * P.gamman;
+    double fntot = P.kn * delta - damp_prefactor * vnnr;
+    if (P.limit_damping) fntot = fntot < 0.0 ? 0.0 : fntot;  // std::max(fntot, 0.0)
+    const double fscrit = P.xmu * fabs(fntot);
+    const double damp_t = meff * P.gammat;
+    double fs[3] = {0.0, 0.0, 0.0};
+    if (P.use_history) {
+        const double4 hh = wh[i];
+        double h[3] = {hh.x, hh.y, hh.z};
+        if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
+        if (history_update)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) h[k] = h[k] + vtr[k] * dt;
+        const double shrmag = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        if (history_update) {
+            const double hn = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nv[k];
+        }
+        double fdamp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vtr[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fs[k] = -P.kt * h[k] + fdamp[k];
+        const double magfs = sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
+        if (magfs > fscrit) {
+            if (shrmag != 0.0) {
+                const double sc = fscrit / magfs;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) fs[k] = fs[k] * sc;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) h[k] = -(fs[k] - fdamp[k]) / P.kt;
+            } else {
+                fs[0] = fs[1] = fs[2] = 0.0;
+            }
+        }
+        wh[i] = make_double4(h[0], h[1], h[2], 1.0);
+    } else {
+        double ft = 0.0;
+        if (vrel != 0.0) {
+            const double dv = damp_t * vrel;
+            ft = (dv < fscrit ? dv : fscrit) / vrel;  // std::min(fscrit, dv)
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fs[k] = -ft * vtr[k];
+    }
+    double4 fo = f[i], to = tq[i];
+    fo.x = fo.x + (fntot * nv[0] + fs[0]);
+    fo.y = fo.y + (fntot * nv[1] + fs[1]);
+    fo.z = fo.z + (fntot * nv[2] + fs[2]);
+    to.x = to.x + -ri * (nv[1] * fs[2] - nv[2] * fs[1]);
+    to.y = to.y + -ri * (nv[2] * fs[0] - nv[0] * fs[2]);
+    to.z = to.z + -ri * (nv[0] * fs[1] - nv[1] * fs[0]);
+    f[i] = fo;
+    tq[i] = to;
+}
+
 // The final pairwise tree over nparts block partials, NC components, into out.
 template <int L, int NC>
 __global__ void __launch_bounds__(FINAL_THREADS) k_final_tree_out(const double* partial,
@@ -1600,6 +1701,13 @@ int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6) 
                        s.partial);
     HIP_TRY(hipGetLastError());
     return launch_tree_out(s, 6, nb, d_sum6);
+}
+
+int launch_wall_gran(System& s, double4* wh, const tmps_wall_params& p, double dt, double coord,
+                     double vwall_n, double vwall_s, int history_update) {
+    LAUNCH_N(k_wall_gran, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, coord, vwall_n, vwall_s,
+             history_update);
+    return 0;
 }
 
 // ------------------------------------------------ batched redo (batch.hip)

@@ -391,6 +391,9 @@ int launch_final_tree(System& s, int nc, int nparts);
 int launch_contact_atom(System& s, int* d_count);
 int launch_fabric(System& s, double* d_sum7);
 int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6);
+// fix wall/gran on an FP64 system; wh: per-atom shear history + touch flag
+int launch_wall_gran(System& s, double4* wh, const tmps_wall_params& p, double dt, double coord,
+                     double vwall_n, double vwall_s, int history_update);
 
 // FP32-split path (k_split.hip)
 int launch_split_to_x64(System& s);
