@@ -132,7 +132,8 @@ class ContactHistory:
                 raise ValueError(
                     f"atom {i} has {m} stored contacts but the history only has "
                     f"{self.max_neighbors} slots per atom")
-            v = np.asarray(values[i], dtype=np.float64).reshape(m, -1)[:, :3]
+            v = (np.asarray(values[i], dtype=np.float64).reshape(m, -1)[:, :3] if m
+                 else np.zeros((0, 3)))
             rows.append((t, v))
         self._seed = rows
         self.setup_pending = True

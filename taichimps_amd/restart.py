@@ -510,7 +510,8 @@ def _history_records(history, nlist, tag: np.ndarray):
     m = int(nlist.npairs[None])
     pi = nlist.pair_i.to_numpy()[:m].astype(np.int64)
     partner = history.partner.to_numpy()[:m].astype(np.int64)
-    shear = np.asarray(history.shear.to_numpy()[:m], dtype=np.float64).reshape(m, -1)[:, :3]
+    shear = (np.asarray(history.shear.to_numpy()[:m], dtype=np.float64).reshape(m, -1)[:, :3]
+             if m else np.zeros((0, 3)))
     nc = np.arange(m)
     on = (partner >= 0) & (pi < n)
     pi, partner, shear, nc = pi[on], partner[on], shear[on], nc[on]

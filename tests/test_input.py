@@ -1,0 +1,295 @@
+"""LAMMPS script driver (reference: input.py, data_reader.py, main.py).
+
+CPU: expression evaluation (the reference's own variable test), the
+argument grammar of pair_style / pair_coeff / fix deform / fix wall/gran with
+their NULL shorthands, read_data, and the loud failure of commands outside
+the path. GPU: scripts run through the parser give, bit for bit, the runs the
+same systems give through the Python API (isotropic compression with the
+servo, a restart resumed from a script, spheres on a floor wall).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from taichimps_amd import input as I
+from taichimps_amd.data_reader import read_data
+
+
+def _parser(tmp_path, text=""):
+    p = tmp_path / "in.test"
+    p.write_text(text)
+    return I.LAMMPSInputParser(p)
+
+
+def test_variable_evaluation(tmp_path):
+    """The reference's tests/test_input_parser.py case."""
+    p = _parser(tmp_path, """
+variable Poi equal 0.23
+variable Gshear equal 29.1e9
+variable kn equal 4.0*${Gshear}/(3*(1-${Poi}))
+variable kt equal ${kn}*2*(1-${Poi})/(2-${Poi})
+""")
+    p.execute()
+    assert abs(p.variables["Poi"] - 0.23) < 1e-6
+    assert abs(p.variables["Gshear"] - 29.1e9) < 1e-6
+    kn = 4.0 * 29.1e9 / (3.0 * (1.0 - 0.23))
+    assert abs(p.variables["kn"] - kn) < 1e-3
+    assert abs(p.variables["kt"] - kn * 2 * (1 - 0.23) / (2 - 0.23)) < 1e-3
+
+
+def test_expressions_comments_and_continuations(tmp_path):
+    p = _parser(tmp_path, """
+variable a equal 2^3      # a comment
+variable b equal sqrt(v_a*2)+PI-PI &
+                 +abs(-1)
+variable s string hello
+variable x index 7
+variable c equal ${a}*$x
+""")
+    p.execute()
+    assert p.variables["a"] == 8.0
+    assert p.variables["b"] == pytest.approx(5.0)
+    assert p.variables["s"] == "hello"
+    assert p.variables["c"] == 56.0
+    with pytest.raises(ValueError):
+        p.evaluate_expression("__import__('os')")
+    with pytest.raises(ValueError):
+        p.evaluate_expression("undefined_name + 1")
+
+
+def test_gran_settings_null_shorthands():
+    ev = float
+    s = I.parse_gran_settings("2e5 NULL 50 NULL 0.5 1".split(), ev)
+    assert s["kt"] == 2e5 * 2.0 / 7.0 and s["gammat"] == 25.0 and not s["limit_damping"]
+    s = I.parse_gran_settings("2e5 1e5 50 10 0.5 0 limit_damping".split(), ev)
+    assert s["kt"] == 1e5 and s["dampflag"] == 0 and s["limit_damping"]
+    with pytest.raises(ValueError):
+        I.parse_gran_settings("2e5 1e5 50 10 0.5 0 bogus".split(), ev)
+    with pytest.raises(ValueError):
+        I.parse_gran_settings("2e5 1e5 50".split(), ev)
+
+
+def test_pair_coeff_granular_grammar():
+    ev = float
+    s = I.parse_pair_coeff_granular(
+        "* * hertz/material 1e7 0.5 0.3 tangential mindlin NULL 1.0 0.5 damping "
+        "coeff_restitution".split(), ev)
+    assert s == dict(normal="hertz/material", normal_coeffs=[1e7, 0.5, 0.3],
+                     tangential="mindlin", tangential_coeffs=[None, 1.0, 0.5],
+                     damping="coeff_restitution", limit_damping=False)
+    s = I.parse_pair_coeff_granular("* * hooke 1e5 0.2 limit_damping".split(), ev)
+    assert s["damping"] == "viscoelastic" and s["limit_damping"]
+    with pytest.raises(ValueError):
+        I.parse_pair_coeff_granular("* * dmt 1 2 3 4".split(), ev)
+    with pytest.raises(NotImplementedError):
+        I.parse_pair_coeff_granular("* * hooke 1e5 0.2 rolling sds 1 2 3".split(), ev)
+
+
+def test_deform_grammar():
+    s = I.parse_deform("1 x pressure 1.0 1e-4 y erate -1e-3 z final 0 10 couple none "
+                       "remap x max/rate 0.01".split(), float)
+    assert s["nevery"] == 1 and s["couple"] == "none" and s["remap"] == "x"
+    assert s["max_rate"] == 0.01
+    assert s["axes"]["x"] == {"style": "pressure", "ptarget": 1.0, "pgain": 1e-4}
+    assert s["axes"]["y"] == {"style": "erate", "rate": -1e-3}
+    assert s["axes"]["z"] == {"style": "final", "flo": 0.0, "fhi": 10.0}
+    with pytest.raises(ValueError):
+        I.parse_deform("1 x bogus 1".split(), float)
+    with pytest.raises(NotImplementedError):
+        I.parse_deform("1 xy erate 0.1".split(), float)
+
+
+def test_wall_gran_grammar():
+    lo = (0.0, -1.0, 2.0)
+    w = I.parse_wall_gran("hooke/history 1e5 NULL 10 NULL 0.5 1 zplane 0.5 NULL".split(), float, lo)
+    assert len(w) == 1
+    assert w[0]["wall_axis"] == 2 and w[0]["wall_side"] == -1 and w[0]["wall_coord"] == 0.5
+    assert w[0]["kt"] == 1e5 * 2.0 / 7.0 and w[0]["gammat"] == 5.0 and w[0]["history"]
+    w = I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 xplane -1 4".split(), float, lo)
+    assert [(x["wall_side"], x["wall_coord"]) for x in w] == [(-1, -1.0), (1, 4.0)]
+    assert not w[0]["history"]
+    w = I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 yplane NULL NULL".split(), float, lo)
+    assert w[0]["wall_coord"] == -1.0  # no bound: the box's lower face
+    with pytest.raises(NotImplementedError):
+        I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 zcylinder 3".split(), float, lo)
+
+
+def _write_data(path, w):
+    r = lambda v: repr(float(v))  # noqa: E731  (shortest round-trip text)
+    n = len(w["x"])
+    lines = ["LAMMPS data file", "", f"{n} atoms", "1 atom types", ""]
+    for k, a in enumerate("xyz"):
+        lines.append(f"{r(w['boxlo'][k])} {r(w['boxhi'][k])} {a}lo {a}hi")
+    lines += ["", "Atoms # sphere", ""]
+    for i in range(n):
+        x = w["x"][i]
+        lines.append(f"{int(w['tag'][i])} 1 {r(2.0 * w['radius'][i])} {r(w['density'])} "
+                     f"{r(x[0])} {r(x[1])} {r(x[2])}")
+    lines += ["", "Velocities", ""]
+    for i in range(n):
+        v, o = w["v"][i], w["omega"][i]
+        lines.append(f"{int(w['tag'][i])} {r(v[0])} {r(v[1])} {r(v[2])} "
+                     f"{r(o[0])} {r(o[1])} {r(o[2])}")
+    path.write_text("\n".join(lines) + "\n")
+
+
+def test_read_data_sphere(tmp_path):
+    from taichimps_amd import workloads
+    w = workloads.lattice(3, 3, 3, 1.1, 4, "uniform")
+    w["omega"] = np.random.default_rng(1).normal(size=w["x"].shape)
+    _write_data(tmp_path / "d.data", w)
+    d = read_data(tmp_path / "d.data")
+    assert d.natoms == 27 and d.natom_types == 1
+    assert d.boxlo == list(w["boxlo"]) and d.boxhi == list(w["boxhi"])
+    assert np.array_equal(d.x, w["x"]) and np.array_equal(d.radius, 0.5 * (2.0 * w["radius"]))
+    assert np.array_equal(d.v, w["v"]) and np.array_equal(d.omega, w["omega"])
+    assert np.array_equal(d.tag, w["tag"]) and np.all(d.density == 1.0)
+    with pytest.raises(FileNotFoundError):
+        read_data(tmp_path / "missing.data")
+
+
+def test_commands_outside_the_path_fail_loudly(tmp_path):
+    p = _parser(tmp_path)
+    for cmd in ("dump", "create_atoms", "velocity"):
+        with pytest.raises(NotImplementedError):
+            p.run_command(cmd, ["1", "all"])
+    with pytest.raises(NotImplementedError):
+        p.run_command("pair_style", ["lj/cut", "2.5"])
+    with pytest.raises(ValueError):
+        p.run_command("compute", ["9", "all", "rdf", "100"])
+    with pytest.raises(ValueError):
+        p.run_command("atom_style", ["atomic"])
+
+
+# ------------------------------------------------------------------ GPU
+
+ISO = """
+units           si
+atom_style      sphere
+boundary        p p p
+read_data       lattice.data
+pair_style      gran/hertz/history 2e5 NULL 50 NULL 0.5 1
+pair_coeff      * *
+neighbor        0.1 bin
+neigh_modify    delay 0 every 1 check yes
+timestep        1e-4
+fix             1 all nve/sphere
+fix             2 all deform/pressure 1 x pressure 1.0 1e-4 y pressure 1.0 1e-4 &
+                z pressure 1.0 1e-4 couple xyz
+compute         cn all contact/atom
+compute         cnave all reduce ave c_cn
+compute         p all pressure NULL pair
+thermo          1
+thermo_style    custom step atoms press c_cnave c_p[1]
+run             100
+run             50
+"""
+
+
+def _api_sim(w, pair="hertz/history", deform=True):
+    import simhelpers as H
+    # the NULL shorthands as the script derives them: Kt = Kn * 2 / 7
+    return H.device_sim(w, style=pair, servo=deform, kt=2e5 * 2.0 / 7.0, gammat=0.5 * 50.0)
+
+
+@pytest.mark.gpu
+def test_isotropic_script_equals_the_api_run_bitwise(tmp_path):
+    from taichimps_amd import workloads
+    w = workloads.lattice(8, 8, 8, 1.1, 1001, "uniform")
+    _write_data(tmp_path / "lattice.data", w)
+    (tmp_path / "in.iso").write_text(ISO)
+    p = I.LAMMPSInputParser(tmp_path / "in.iso")
+    p.execute()
+    sim = _api_sim(w)
+    sim.run(100)
+    sim.run(50)
+    n = len(w["x"])
+    for key in ("x", "v", "omega", "f"):
+        assert np.array_equal(getattr(p.atom, key).to_numpy()[:n],
+                              getattr(sim.atom, key).to_numpy()[:n]), key
+    assert np.array_equal(p.domain.boxhi, sim.domain.boxhi)
+    assert p.simulation.timestep == 150 and p.simulation.last_run_device_resident
+    assert len(p.thermo_lines) == 2
+    step, atoms, press, cnave, pxx = p.thermo_lines[-1].split()
+    assert int(step) == 150 and int(atoms) == n
+    assert float(cnave) == pytest.approx(p.compute_value("cnave"))
+
+
+@pytest.mark.gpu
+def test_restart_written_and_read_by_scripts_resumes_bitwise(tmp_path):
+    from taichimps_amd import workloads
+    w = workloads.lattice(6, 6, 6, 0.98, 5, "uniform")
+    _write_data(tmp_path / "lattice.data", w)
+    head = """
+atom_style sphere
+boundary p p p
+read_data lattice.data
+pair_style granular
+pair_coeff * * hertz/material 1e7 0.5 0.3 tangential mindlin NULL 1.0 0.5 damping coeff_restitution
+neighbor 0.1 bin
+timestep 1e-4
+fix 1 all nve/sphere
+"""
+    (tmp_path / "in.cont").write_text(head + "run 80\nrun 70\n")
+    (tmp_path / "in.first").write_text(head + "run 80\nwrite_restart mid.restart\n")
+    (tmp_path / "in.second").write_text("""
+atom_style sphere
+read_restart mid.restart
+neighbor 0.1 bin
+timestep 1e-4
+fix 1 all nve/sphere
+run 70
+""")
+    cont = I.LAMMPSInputParser(tmp_path / "in.cont")
+    cont.execute()
+    I.LAMMPSInputParser(tmp_path / "in.first").execute()
+    second = I.LAMMPSInputParser(tmp_path / "in.second")
+    second.execute()
+    n = len(w["x"])
+    assert second.simulation.timestep == 150
+    for key in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(getattr(cont.atom, key).to_numpy()[:n],
+                              getattr(second.atom, key).to_numpy()[:n]), key
+
+
+@pytest.mark.gpu
+def test_floor_wall_script_equals_the_api_run_bitwise(tmp_path):
+    import taichimps_amd as tm
+    from taichimps_amd import workloads
+    w = workloads.lattice(5, 5, 3, 1.1, 13, "uniform")
+    w["v"][:, 2] -= 2.0
+    _write_data(tmp_path / "lattice.data", w)
+    floor = float(w["boxlo"][2]) + 0.08
+    (tmp_path / "in.wall").write_text(f"""
+atom_style sphere
+boundary p p f
+read_data lattice.data
+pair_style gran/hooke/history 2e5 NULL 50 NULL 0.5 1
+pair_coeff * *
+neighbor 0.1 bin
+timestep 1e-4
+fix 1 all nve/sphere
+fix 2 all wall/gran hooke/history 2e5 NULL 50 NULL 0.5 1 zplane {floor!r} NULL
+run 60
+""")
+    p = I.LAMMPSInputParser(tmp_path / "in.wall")
+    p.execute()
+    n = len(w["x"])
+    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "f"))
+    atom = tm.AtomSystem(max_atoms=n)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+    c = dict(kn=2e5, gamman=50.0, kt=2e5 * 2.0 / 7.0, gammat=25.0, xmu=0.5)
+    sim = tm.Simulation(dom, atom, nl, tm.ContactHistory(n, 64),
+                        tm.GranHookeHistory(dom, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                                            c["xmu"]), dt=1e-4)
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixWallGran(dom, 2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                               c["xmu"], max_atoms=n))
+    sim.run(60)
+    for key in ("x", "v", "omega"):
+        assert np.array_equal(getattr(p.atom, key).to_numpy()[:n],
+                              getattr(atom, key).to_numpy()[:n]), key

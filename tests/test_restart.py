@@ -249,3 +249,20 @@ def test_lost_history_changes_the_trajectory(tmp_path):
     second.run(70)
     n = cont.atom.nlocal
     assert not np.array_equal(cont.atom.x.to_numpy()[:n], second.atom.x.to_numpy()[:n])
+
+
+def test_history_with_no_touching_contacts_round_trips(tmp_path):
+    """Every atom's FixNeighHistory record is [2, 0]: written and read back."""
+    d = R.arrays()
+    for k in ("pair_i", "pair_j", "partner"):
+        d[k] = d[k][:0]
+    d["shear"] = d["shear"][:0]
+    path = tmp_path / "empty"
+    RS.write_restart(path, R.Atom(d), R.Domain(), timestep=1, dt=1e-4, history=R.History(d),
+                     nlist=R.NeighborList(d))
+    data = RS.read_restart(path)
+    assert data.fix_peratom_styles == ["NEIGH_HISTORY"]
+    for rec in data.fix_extra:
+        assert np.array_equal(rec, [2.0, 0.0])
+        t, v = RS.parse_neigh_history(rec)
+        assert t.size == 0 and v.shape == (0, 3)
