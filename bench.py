@@ -462,6 +462,97 @@ def ensemble_c5(steps=500, n_grad=256, every=100):
     }
 
 
+def next_rows(dims=None, reps=20, warm_steps=20):
+    """The widened rows on the 1M-sphere workload (FP64 system after
+    `warm_steps` steps): per-contact computes, a floor wall's contact kernel,
+    restart files, and the script driver on the 8,000-sphere case. API wall
+    times include each call's device-to-host copy and synchronisation; kernel
+    durations come from rocprofv3 (profiles/)."""
+    import tempfile
+
+    import taichimps_amd as tm
+    from taichimps_amd import restart as RS
+    from taichimps_amd import workloads
+    from taichimps_amd.input import LAMMPSInputParser
+    w = workloads.case("isotropic_hertz_1m", dims=tuple(dims) if dims else None)
+    n = len(w["x"])
+    sim = build_sim(w, "f64")
+    sim.run(warm_steps)
+    atom, dom, nl = sim.atom, sim.domain, sim.neighbor
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    ca, fab, sa = tm.ComputeContactAtom(n), tm.ComputeFabric(), tm.ComputeStressAtom(n)
+    out = {"workload": f"{n} spheres, isotropic compression (gran/hertz/history, FP64) after "
+                       f"{warm_steps} steps", "atoms": n, "reps": reps}
+    cnt = ca.compute(atom, dom, nl)
+    phi, nc = fab.compute(atom, dom, nl)
+    out["contacts"] = nc
+    out["mean_coordination"] = float(cnt.mean())
+    out["fabric_trace"] = float(phi[:3].sum())
+    out["contact_atom_call_ms"] = timed(lambda: ca.compute(atom, dom, nl))
+    out["fabric_call_ms"] = timed(lambda: fab.compute(atom, dom, nl))
+    out["stress_atom_call_ms"] = timed(lambda: sa.compute(atom, dom, kinetic=True))
+    out["stress_macro_call_ms"] = timed(lambda: sa.compute_macro(atom, dom, kinetic=True))
+    # a floor under the lowest layer of spheres (centres 0.55 above the box's
+    # lower face): about a tenth of the atoms touch it
+    dom.pull()
+    floor = float(dom.boxlo[2]) + 0.08
+    wall = tm.FixWallGran(dom, 2, -1, floor, 2.0e5, 50.0, 2.0e5 * 2.0 / 7.0, 25.0, 0.5,
+                          max_atoms=n)
+
+    def wall_call():
+        wall.post_force(atom, 1e-4)
+        atom._native.sync()
+    out["wall_post_force_call_ms"] = timed(wall_call)
+    out["wall_touching_atoms"] = int(wall.touch.to_numpy()[:n].sum())
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "bench.restart")
+        t0 = time.perf_counter()
+        RS.write_restart(path, atom, dom, timestep=sim.timestep, dt=sim.dt,
+                         history=sim.history, nlist=nl, pair_style="gran/hertz/history")
+        out["write_restart_s"] = time.perf_counter() - t0
+        out["restart_bytes"] = os.path.getsize(path)
+        t0 = time.perf_counter()
+        data = RS.read_restart(path)
+        out["read_restart_s"] = time.perf_counter() - t0
+        out["restart_atoms_read"] = len(data.tag)
+        # script driver: the 8,000-sphere case as a script vs the same run via the API
+        w8 = workloads.case("isotropic_hooke_8k")
+        lines = ["LAMMPS data", "", f"{len(w8['x'])} atoms", "1 atom types", ""]
+        for k, a in enumerate("xyz"):
+            lines.append(f"{float(w8['boxlo'][k])!r} {float(w8['boxhi'][k])!r} {a}lo {a}hi")
+        lines += ["", "Atoms # sphere", ""]
+        for i in range(len(w8["x"])):
+            x = w8["x"][i]
+            lines.append(f"{i + 1} 1 {float(2 * w8['radius'][i])!r} 1.0 {float(x[0])!r} "
+                         f"{float(x[1])!r} {float(x[2])!r}")
+        lines += ["", "Velocities", ""]
+        for i in range(len(w8["x"])):
+            v = w8["v"][i]
+            lines.append(f"{i + 1} {float(v[0])!r} {float(v[1])!r} {float(v[2])!r} 0 0 0")
+        with open(os.path.join(tmp, "c1.data"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        with open(os.path.join(tmp, "in.c1"), "w") as f:
+            f.write("atom_style sphere\nboundary p p p\nread_data c1.data\n"
+                    "pair_style gran/hooke/history 2e5 NULL 50 NULL 0.5 1\npair_coeff * *\n"
+                    "neighbor 0.1 bin\nneigh_modify delay 0 every 1 check yes\n"
+                    "timestep 1e-4\nfix 1 all nve/sphere\nfix 2 all deform/pressure 1 "
+                    "x pressure 1.0 1e-4 y pressure 1.0 1e-4 z pressure 1.0 1e-4 couple xyz\n"
+                    "run 1000\n")
+        t0 = time.perf_counter()
+        LAMMPSInputParser(os.path.join(tmp, "in.c1")).execute()
+        out["script_c1_1000_steps_s"] = time.perf_counter() - t0
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -491,6 +582,9 @@ def main(argv=None):
                     help="run one precision only (for profiler passes); the JSON line needs both")
     ap.add_argument("--direct", action="store_true",
                     help="launch each step kernel directly instead of replaying captured graphs")
+    ap.add_argument("--next-rows-only", action="store_true",
+                    help="only time the widened rows (contact computes, floor wall, restart "
+                         "files, script driver) on the 1M-sphere workload and print them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (a known-size "
                          "kernel that scales the profiler's HBM byte counters), then exit")
@@ -505,6 +599,9 @@ def main(argv=None):
         return 0
     if args.triaxial_only:
         print(json.dumps(triaxial_c3()))
+        return 0
+    if args.next_rows_only:
+        print(json.dumps(next_rows(dims=args.dims)))
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

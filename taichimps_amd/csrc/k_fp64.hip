@@ -1670,7 +1670,16 @@ int launch_copy_state(System& s, int honor_abort) {
                                s.stream, s.partial, nparts, out);                             \
         break;
 
+// out: 8 doubles. Up to CLOSE_SPAN partials one k_tree_level block (coalesced
+// loads, LDS transpose) reduces all 8 components; above, final_tree_L's
+// kernel. Both are the same pairwise tree, so the bits agree.
 static int launch_tree_out(System& s, int nc, int nparts, double* out) {
+    if (nparts <= CLOSE_SPAN) {
+        hipLaunchKernelGGL(k_tree_level, dim3(1), dim3(CLOSE_THREADS), 0, s.stream, s.partial,
+                           nparts, out);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     switch (final_leaves(nparts)) {
         TREE_OUT_CASE(1) TREE_OUT_CASE(2) TREE_OUT_CASE(4) TREE_OUT_CASE(8) TREE_OUT_CASE(16)
         TREE_OUT_CASE(32) TREE_OUT_CASE(64)
