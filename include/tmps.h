@@ -178,27 +178,40 @@ int tmps_servo_set(tmps_system* s, const tmps_servo_params* p);
 int tmps_servo_setup(tmps_system* s, int nsteps_total, double dt);
 int tmps_servo_end_of_step(tmps_system* s, double dt, double tensor_out[6]);
 
-/* ---- fix wall/gran (EXTRA_FIX/wall_gran.py) ------------------------------
- * A flat, axis-aligned wall in contact with the atoms of an FP64 system:
- * FixWallGran's settings (wall_gran.py:57-111) after its host derivations
- * (gammat = 0 when dampflag is 0). A wall owns its per-atom shear history
- * (max_atoms of the system, zeroed at creation). */
+/* ---- fix wall/gran (EXTRA_FIX/wall_gran.py) and fix wall/gran/region
+ *      (EXTRA_FIX/wall_gran_region.py) ------------------------------------
+ * A wall in contact with the atoms of an FP64 system, as a partner of zero
+ * radius and infinite mass: a flat, axis-aligned plane (FixWallGran's
+ * settings, wall_gran.py:57-111) or the curved side of a cylinder region
+ * (FixWallGranRegion, wall_gran_region.py:45-112), after their host
+ * derivations (gammat = 0 when dampflag is 0). A wall owns its per-atom shear
+ * history (max_atoms of the system, zeroed at creation). */
+#define TMPS_WALL_PLANE 0
+#define TMPS_WALL_CYLINDER 1
 typedef struct {
-    int axis;          /* 0, 1 or 2 */
-    int side;          /* -1: wall below the atoms (normal +axis); +1: above */
-    int use_history;   /* hooke/history (1) or the history-free hooke (0) */
+    int axis;          /* plane normal / cylinder axis: 0, 1 or 2 */
+    int side;          /* plane: -1 wall below the atoms (normal +axis), +1 above;
+                          cylinder: -1 atoms inside, +1 outside */
+    int use_history;   /* the history styles (1) or the history-free ones (0) */
     int limit_damping;
-    int shear_axis;    /* in-plane wall velocity axis, -1 for none */
+    int shear_axis;    /* plane: in-plane wall velocity axis, -1 for none */
     double kn, kt, gamman, gammat, xmu;
+    int geometry;      /* TMPS_WALL_PLANE or TMPS_WALL_CYLINDER */
+    int hertz;         /* cylinder: hertz normal force and viscoelastic damping */
+    int has_bounds;    /* cylinder: only atoms with axis_lo <= x[axis] <= axis_hi */
+    int pad;
+    double c1, c2, radius, axis_lo, axis_hi;  /* cylinder cross-section centre, radius */
 } tmps_wall_params;
 typedef struct tmps_wall tmps_wall;
 int tmps_wall_create(tmps_system* s, const tmps_wall_params* p, tmps_wall** out);
 int tmps_wall_destroy(tmps_wall* w);  /* also valid after its system is destroyed */
 /* FixWallGran.reset_history, wall_gran.py:113-118 */
 int tmps_wall_reset(tmps_wall* w);
-/* FixWallGran.post_force_kernel, wall_gran.py:120-211: adds the wall's force
- * and torque to every atom's, given this step's wall position and its normal
- * and in-plane velocities (post_force, :213-236, computes them) */
+/* FixWallGran.post_force_kernel, wall_gran.py:120-211, or
+ * FixWallGranRegion.post_force_kernel, wall_gran_region.py:114-271: adds the
+ * wall's force and torque to every atom's. A plane takes this step's wall
+ * position and its normal and in-plane velocities (post_force, :213-236,
+ * computes them); a cylinder ignores coord, vwall_n and vwall_s. */
 int tmps_wall_post_force(tmps_wall* w, double dt, double coord, double vwall_n, double vwall_s,
                          int history_update);
 /* FixWallGran.shear / touch: per-atom shear (3n, nullable) and touch flags

@@ -73,11 +73,14 @@ typedef struct {
 
 namespace {
 
-// Mirrors FixWallGran's constructor settings (wall_gran.py:57-111).
+// Mirrors FixWallGran's constructor settings (wall_gran.py:57-111) and, for
+// geometry 1, FixWallGranRegion's cylinder (wall_gran_region.py:45-112).
 extern "C" {
 typedef struct {
     int axis, side, use_history, limit_damping, shear_axis;
     double kn, kt, gamman, gammat, xmu;  // gammat already 0 when dampflag is 0
+    int geometry, hertz, has_bounds, pad;
+    double c1, c2, radius, axis_lo, axis_hi;
 } orc_wall_params;
 }
 
@@ -803,10 +806,125 @@ void wall_kernel(Oracle& s, OrcWall& w, double dt, double coord, double vwall_n,
     }
 }
 
+// FixWallGranRegion.post_force_kernel, wall_gran_region.py:114-271: the
+// curved side of a cylinder region (zero radius, infinite mass); hooke or
+// hertz (contact radius sqrt(r delta) on stiffness and damping).
+void wall_cyl_kernel(Oracle& s, OrcWall& w, double dt, int history_update) {
+    const orc_wall_params& P = w.p;
+    const int axis = P.axis;
+    for (int i = 0; i < s.n; ++i) {
+        const double* pos = &s.x[3 * i];
+        const double r_i = s.radius[i];
+        const double pos_axis = pos[axis];
+        auto reset = [&]() {
+            if (P.use_history) {
+                w.touch[i] = 0;
+                for (int k = 0; k < 3; ++k) w.shear[3 * i + k] = 0.0;
+            }
+        };
+        if (P.has_bounds == 1 && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) {
+            reset();
+            continue;
+        }
+        double d1, d2;
+        if (axis == 0) {
+            d1 = pos[1] - P.c1;
+            d2 = pos[2] - P.c2;
+        } else if (axis == 1) {
+            d1 = pos[0] - P.c1;
+            d2 = pos[2] - P.c2;
+        } else {
+            d1 = pos[0] - P.c1;
+            d2 = pos[1] - P.c2;
+        }
+        const double r_perp = std::sqrt(d1 * d1 + d2 * d2);
+        if (!(r_perp > 1e-14)) {
+            reset();
+            continue;
+        }
+        const double inv_r_perp = 1.0 / r_perp;
+        const double u1 = d1 * inv_r_perp, u2 = d2 * inv_r_perp;
+        const double s1 = P.side == -1 ? -u1 : u1, s2 = P.side == -1 ? -u2 : u2;
+        double nw[3];
+        if (axis == 0) {
+            nw[0] = 0.0; nw[1] = s1; nw[2] = s2;
+        } else if (axis == 1) {
+            nw[0] = s1; nw[1] = 0.0; nw[2] = s2;
+        } else {
+            nw[0] = s1; nw[1] = s2; nw[2] = 0.0;
+        }
+        const double delta =
+            P.side == -1 ? r_i - (P.radius - r_perp) : r_i - (r_perp - P.radius);
+        if (!(delta > 0.0)) {
+            reset();
+            continue;
+        }
+        const double mi = s.rmass[i];
+        const double* vi = &s.v[3 * i];
+        const double* oi = &s.omega[3 * i];
+        const double oxn[3] = {oi[1] * nw[2] - oi[2] * nw[1], oi[2] * nw[0] - oi[0] * nw[2],
+                               oi[0] * nw[1] - oi[1] * nw[0]};
+        double vr[3], vrt[3];
+        for (int k = 0; k < 3; ++k) vr[k] = vi[k] - r_i * oxn[k];
+        const double vn = vr[0] * nw[0] + vr[1] * nw[1] + vr[2] * nw[2];
+        for (int k = 0; k < 3; ++k) vrt[k] = vr[k] - vn * nw[k];
+        const double vrel = std::sqrt(vrt[0] * vrt[0] + vrt[1] * vrt[1] + vrt[2] * vrt[2]);
+        double poly = 1.0;
+        if (P.hertz == 1) poly = std::sqrt(r_i * delta);
+        const double fn_elastic = P.kn * delta * poly;
+        const double damp_prefactor = mi * P.gamman * poly;
+        double fntot = fn_elastic - damp_prefactor * vn;
+        if (P.limit_damping) fntot = std::max(fntot, 0.0);
+        const double fscrit = P.xmu * std::fabs(fntot);
+        const double damp_t = mi * P.gammat * poly;
+        double ft[3] = {0.0, 0.0, 0.0};
+        if (P.use_history) {
+            double h[3] = {w.shear[3 * i], w.shear[3 * i + 1], w.shear[3 * i + 2]};
+            if (w.touch[i] == 0) {
+                h[0] = h[1] = h[2] = 0.0;
+                w.touch[i] = 1;
+            }
+            if (history_update)
+                for (int k = 0; k < 3; ++k) h[k] = h[k] + vrt[k] * dt;
+            const double shrmag = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+            if (history_update) {
+                const double hn = h[0] * nw[0] + h[1] * nw[1] + h[2] * nw[2];
+                for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nw[k];
+            }
+            double fdamp[3];
+            for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vrt[k];
+            for (int k = 0; k < 3; ++k) ft[k] = -P.kt * poly * h[k] + fdamp[k];
+            const double magfs = std::sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+            if (magfs > fscrit) {
+                if (shrmag != 0.0) {
+                    const double sc = fscrit / magfs;
+                    for (int k = 0; k < 3; ++k) ft[k] = ft[k] * sc;
+                    for (int k = 0; k < 3; ++k) h[k] = -(ft[k] - fdamp[k]) / (P.kt * poly);
+                } else {
+                    ft[0] = ft[1] = ft[2] = 0.0;
+                }
+            }
+            for (int k = 0; k < 3; ++k) w.shear[3 * i + k] = h[k];
+        } else {
+            double fsc = 0.0;
+            if (vrel != 0.0) fsc = std::min(fscrit, damp_t * vrel) / vrel;
+            for (int k = 0; k < 3; ++k) ft[k] = -fsc * vrt[k];
+        }
+        for (int k = 0; k < 3; ++k) s.f[3 * i + k] = s.f[3 * i + k] + (fntot * nw[k] + ft[k]);
+        const double nxf[3] = {nw[1] * ft[2] - nw[2] * ft[1], nw[2] * ft[0] - nw[0] * ft[2],
+                               nw[0] * ft[1] - nw[1] * ft[0]};
+        for (int k = 0; k < 3; ++k) s.torque[3 * i + k] = s.torque[3 * i + k] + -r_i * nxf[k];
+    }
+}
+
 // FixWallGran.post_force, wall_gran.py:213-236: the wall's position and
 // normal velocity under `wiggle`, then the contact kernel; the clock
 // advances by dt per call.
 void wall_post_force(Oracle& s, OrcWall& w, double dt, int history_update) {
+    if (w.p.geometry == 1) {  // FixWallGranRegion.post_force, wall_gran_region.py:273-290
+        wall_cyl_kernel(s, w, dt, history_update);
+        return;
+    }
     double coord = w.coord, vwall_n = 0.0;
     if (w.amp != 0.0) {
         const double omega_w = 2.0 * M_PI / w.period;

@@ -56,6 +56,9 @@ class WallParams(C.Structure):
         ("limit_damping", C.c_int), ("shear_axis", C.c_int),
         ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
         ("gammat", C.c_double), ("xmu", C.c_double),
+        ("geometry", C.c_int), ("hertz", C.c_int), ("has_bounds", C.c_int), ("pad", C.c_int),
+        ("c1", C.c_double), ("c2", C.c_double), ("radius", C.c_double),
+        ("axis_lo", C.c_double), ("axis_hi", C.c_double),
     ]
 
 
@@ -386,6 +389,28 @@ class OracleSim:
         p.gammat = 0.0 if int(dampflag) == 0 else float(gammat)
         return lib().orc_add_wall(self._h, C.byref(p), float(coord), float(shear_vel),
                                   float(wiggle_amplitude), float(wiggle_period))
+
+    def add_wall_cylinder(self, axis, c1, c2, radius, side, kn, gamman, kt=None, gammat=None,
+                          xmu=0.0, dampflag=1, fstyle="hooke", history=None,
+                          limit_damping=False, axis_lo=None, axis_hi=None) -> int:
+        """FixWallGranRegion (wall_gran_region.py:45-112): side -1 inside, +1
+        outside; kt NULL = 2/7 kn, gammat NULL = gamman / 2."""
+        p = WallParams()
+        p.geometry, p.axis = 1, int(axis)
+        p.side = -1 if side in ("in", -1) else 1
+        p.c1, p.c2, p.radius = float(c1), float(c2), float(radius)
+        p.has_bounds = 1 if (axis_lo is not None or axis_hi is not None) else 0
+        p.axis_lo = float(axis_lo) if axis_lo is not None else -1e30
+        p.axis_hi = float(axis_hi) if axis_hi is not None else 1e30
+        p.hertz = 1 if "hertz" in fstyle else 0
+        p.kn, p.gamman, p.xmu = float(kn), float(gamman), float(xmu)
+        p.kt = float(kt) if kt is not None else 2.0 / 7.0 * float(kn)
+        gt = float(gammat) if gammat is not None else 0.5 * float(gamman)
+        p.gammat = 0.0 if int(dampflag) == 0 else gt
+        p.use_history = 1 if (fstyle.endswith("history") if history is None else history) else 0
+        p.limit_damping = 1 if limit_damping else 0
+        p.shear_axis = -1
+        return lib().orc_add_wall(self._h, C.byref(p), 0.0, 0.0, 0.0, 0.0)
 
     def wall_history(self, idx):
         sh = np.zeros((self.n, 3))

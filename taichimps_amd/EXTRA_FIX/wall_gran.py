@@ -154,3 +154,93 @@ class FixWallGran(Fix):
     def setup(self, nsteps_total: int = 0, dt: float = 0.0) -> None:
         """Restart the clock the wall motion is referenced to."""
         self.elapsed = 0.0
+
+
+class FixWallGranRegion(FixWallGran):
+    """fix wall/gran/region with a cylinder region (reference:
+    EXTRA_FIX/wall_gran_region.py, LAMMPS fix_wall_gran_region.cpp).
+
+    The curved side of a cylinder along `axis` (0/1/2 or "x"/"y"/"z") with
+    cross-section centre (c1, c2) and radius `radius`; side "in" (or -1) holds
+    the atoms inside, "out" (or +1) outside. `axis_lo` / `axis_hi` bound the
+    atoms considered (they are not end caps). `fstyle` hooke / hertz with or
+    without `/history`; kt NULL = 2/7 kn, gammat NULL = gamman / 2; the hertz
+    styles scale stiffness and damping by the contact radius sqrt(r delta).
+    One shear history per atom (the side is a single surface)."""
+
+    def __init__(
+        self,
+        domain,
+        axis: int | str = 2,
+        c1: float = 0.0,
+        c2: float = 0.0,
+        radius: float = 1.0,
+        side: str | int = "in",
+        axis_lo: float | None = None,
+        axis_hi: float | None = None,
+        fstyle: str = "hooke",
+        kn: float = 1e5,
+        gamman: float = 0.0,
+        kt: float | None = None,
+        gammat: float | None = None,
+        xmu: float = 0.0,
+        dampflag: int = 1,
+        history: bool | None = None,
+        limit_damping: bool = False,
+        max_atoms: int = 100000,
+        float_type: Any = f64,
+    ) -> None:
+        Fix.__init__(self, domain, float_type=float_type)
+        if isinstance(axis, str):
+            axis = {"x": 0, "y": 1, "z": 2}[axis.lower()]
+        self.axis = int(axis)
+        self.c1, self.c2, self.radius = float(c1), float(c2), float(radius)
+        if isinstance(side, str):
+            self.side = -1 if side.lower() in ("in", "inside", "-1") else 1
+        else:
+            self.side = -1 if side < 0 else 1
+        self.axis_lo = float(axis_lo) if axis_lo is not None else -1e30
+        self.axis_hi = float(axis_hi) if axis_hi is not None else 1e30
+        self.has_axis_bounds = 1 if (axis_lo is not None or axis_hi is not None) else 0
+        self.fstyle = fstyle.lower()
+        self.is_hertz = 1 if "hertz" in self.fstyle else 0
+        self.kn = float(kn)
+        self.gamman = float(gamman)
+        self.kt = float(kt) if kt is not None else (2.0 / 7.0 * self.kn)
+        self.dampflag = int(dampflag)
+        gammat_val = float(gammat) if gammat is not None else (0.5 * self.gamman)
+        self.gammat = 0.0 if self.dampflag == 0 else gammat_val
+        self.xmu = float(xmu)
+        self.limit_damping = 1 if limit_damping else 0
+        self.use_history = (1 if (self.fstyle.endswith("history") if history is None
+                                  else history) else 0)
+        # FixWallGran's attributes a cylinder does not use
+        self.wall_axis, self.wall_side, self.wall_coord = self.axis, self.side, 0.0
+        self.shear_axis, self.shear_vel = -1, 0.0
+        self.wiggle_amplitude = self.wiggle_period = 0.0
+        self.max_atoms = int(max_atoms)
+        self.elapsed = 0.0
+        self._wall = None
+        self._wall_sys = None
+        self._wall_cap = 0
+        self.shear = _HistoryView(self, 0)
+        self.touch = _HistoryView(self, 1)
+
+    def _params(self) -> _native.WallParams:
+        p = _native.WallParams()
+        p.geometry = 1
+        p.axis, p.side = self.axis, self.side
+        p.use_history, p.limit_damping = self.use_history, self.limit_damping
+        p.shear_axis = -1
+        p.kn, p.kt, p.gamman, p.gammat, p.xmu = (self.kn, self.kt, self.gamman, self.gammat,
+                                                 self.xmu)
+        p.hertz, p.has_bounds = self.is_hertz, self.has_axis_bounds
+        p.c1, p.c2, p.radius = self.c1, self.c2, self.radius
+        p.axis_lo, p.axis_hi = self.axis_lo, self.axis_hi
+        return p
+
+    def post_force(self, atom, dt: float) -> None:
+        """The cylinder contact on every atom (wall_gran_region.py:273-290)."""
+        if atom.nlocal == 0:
+            return
+        self._bind(atom).post_force(dt, 0.0, 0.0, 0.0, 1 if self.history_update else 0)

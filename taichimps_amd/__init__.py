@@ -10,7 +10,8 @@ from .computes import ComputeContactAtom, ComputeFabric, ComputeStressAtom, Comp
 from .constants import f32split, f64
 from .contact_history import ContactHistory
 from .domain import Domain
-from .EXTRA_FIX import FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran
+from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran,
+                        FixWallGranRegion)
 from .GRANULAR import GranHertzHistory, GranHookeHistory, GranularPair, PairGranular
 from .neighbor import NeighborList
 from .simulation import Simulation
@@ -18,6 +19,6 @@ from .simulation import Simulation
 __all__ = [
     "AtomSystem", "ComputeContactAtom", "ComputeFabric", "ComputeStressAtom", "Computes",
     "ContactHistory", "Domain", "FixDampingCundall",
-    "FixDeformPressure", "FixNVESphere", "FixWallGran", "GranHertzHistory", "GranHookeHistory",
+    "FixDeformPressure", "FixNVESphere", "FixWallGran", "FixWallGranRegion", "GranHertzHistory", "GranHookeHistory",
     "GranularPair", "NeighborList", "PairGranular", "Simulation", "f32split", "f64",
 ]

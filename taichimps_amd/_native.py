@@ -92,6 +92,9 @@ class WallParams(C.Structure):
         ("limit_damping", C.c_int), ("shear_axis", C.c_int),
         ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
         ("gammat", C.c_double), ("xmu", C.c_double),
+        ("geometry", C.c_int), ("hertz", C.c_int), ("has_bounds", C.c_int), ("pad", C.c_int),
+        ("c1", C.c_double), ("c2", C.c_double), ("radius", C.c_double),
+        ("axis_lo", C.c_double), ("axis_hi", C.c_double),
     ]
 
 

@@ -1701,6 +1701,8 @@ int tmps_wall_create(tmps_system* h, const tmps_wall_params* p, tmps_wall** out)
     System& s = *sys(h);
     if (!p || !out) return tmps_fail(TMPS_ERR_ARG, "null wall arguments");
     *out = nullptr;
+    if (p->geometry != TMPS_WALL_PLANE && p->geometry != TMPS_WALL_CYLINDER)
+        return tmps_fail(TMPS_ERR_ARG, "unknown wall geometry");
     if (p->axis < 0 || p->axis > 2) return tmps_fail(TMPS_ERR_ARG, "wall_axis must be 0, 1 or 2");
     if (p->side != -1 && p->side != 1)
         return tmps_fail(TMPS_ERR_ARG, "wall_side must be -1 (low) or +1 (high)");

@@ -1313,6 +1313,117 @@ __global__ void k_wall_gran(const double4* xr, const double4* vm, const double4*
     tq[i] = to;
 }
 
+// FixWallGranRegion.post_force_kernel, wall_gran_region.py:114-271: the
+// curved side of a cylinder region as a partner of zero radius and infinite
+// mass. Hooke (mass_velocity damping) or hertz (viscoelastic: the contact
+// radius sqrt(r delta) scales stiffness and damping), linear_history_classic
+// or history-free tangential force; atoms outside [axis_lo, axis_hi] (when
+// bounded) or on the axis are out of contact.
+__global__ void k_wall_cyl(const double4* xr, const double4* vm, const double4* om, double4* f,
+                           double4* tq, double4* wh, int n, tmps_wall_params P, double dt,
+                           int history_update) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int axis = P.axis;
+    const double4 x4 = xr[i];
+    const double pos[3] = {x4.x, x4.y, x4.z};
+    const double r_i = x4.w;
+    const double pos_axis = pos[axis];
+    if (P.has_bounds && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) {
+        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const double d1 = pos[a1] - P.c1, d2 = pos[a2] - P.c2;
+    const double r_perp = sqrt(d1 * d1 + d2 * d2);
+    if (!(r_perp > 1e-14)) {
+        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const double inv_r_perp = 1.0 / r_perp;
+    const double u1 = d1 * inv_r_perp, u2 = d2 * inv_r_perp;
+    double nw[3] = {0.0, 0.0, 0.0};
+    nw[a1] = P.side == -1 ? -u1 : u1;
+    nw[a2] = P.side == -1 ? -u2 : u2;
+    const double delta = P.side == -1 ? r_i - (P.radius - r_perp) : r_i - (r_perp - P.radius);
+    if (!(delta > 0.0)) {
+        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const double4 v4 = vm[i], o4 = om[i];
+    const double mi = v4.w;
+    const double vi[3] = {v4.x, v4.y, v4.z}, oi[3] = {o4.x, o4.y, o4.z};
+    const double oxn[3] = {oi[1] * nw[2] - oi[2] * nw[1], oi[2] * nw[0] - oi[0] * nw[2],
+                           oi[0] * nw[1] - oi[1] * nw[0]};
+    double vr[3], vrt[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vr[k] = vi[k] - r_i * oxn[k];
+    const double vn = vr[0] * nw[0] + vr[1] * nw[1] + vr[2] * nw[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vrt[k] = vr[k] - vn * nw[k];
+    const double vrel = sqrt(vrt[0] * vrt[0] + vrt[1] * vrt[1] + vrt[2] * vrt[2]);
+    double poly = 1.0;
+    if (P.hertz) poly = sqrt(r_i * delta);
+    const double fn_elastic = P.kn * delta * poly;
+    const double damp_prefactor = mi * P.gamman * poly;
+    double fntot = fn_elastic - damp_prefactor * vn;
+    if (P.limit_damping) fntot = fntot < 0.0 ? 0.0 : fntot;  // std::max(fntot, 0.0)
+    const double fscrit = P.xmu * fabs(fntot);
+    const double damp_t = mi * P.gammat * poly;
+    double ft[3] = {0.0, 0.0, 0.0};
+    if (P.use_history) {
+        const double4 hh = wh[i];
+        double h[3] = {hh.x, hh.y, hh.z};
+        if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
+        if (history_update)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) h[k] = h[k] + vrt[k] * dt;
+        const double shrmag = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        if (history_update) {
+            const double hn = h[0] * nw[0] + h[1] * nw[1] + h[2] * nw[2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nw[k];
+        }
+        double fdamp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vrt[k];
+        const double ktp = -P.kt * poly;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ft[k] = ktp * h[k] + fdamp[k];
+        const double magfs = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+        if (magfs > fscrit) {
+            if (shrmag != 0.0) {
+                const double sc = fscrit / magfs;
+                const double kp = P.kt * poly;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ft[k] = ft[k] * sc;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) h[k] = -(ft[k] - fdamp[k]) / kp;
+            } else {
+                ft[0] = ft[1] = ft[2] = 0.0;
+            }
+        }
+        wh[i] = make_double4(h[0], h[1], h[2], 1.0);
+    } else {
+        double fsc = 0.0;
+        if (vrel != 0.0) {
+            const double dv = damp_t * vrel;
+            fsc = (dv < fscrit ? dv : fscrit) / vrel;  // std::min(fscrit, dv)
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ft[k] = -fsc * vrt[k];
+    }
+    double4 fo = f[i], to = tq[i];
+    fo.x = fo.x + (fntot * nw[0] + ft[0]);
+    fo.y = fo.y + (fntot * nw[1] + ft[1]);
+    fo.z = fo.z + (fntot * nw[2] + ft[2]);
+    to.x = to.x + -r_i * (nw[1] * ft[2] - nw[2] * ft[1]);
+    to.y = to.y + -r_i * (nw[2] * ft[0] - nw[0] * ft[2]);
+    to.z = to.z + -r_i * (nw[0] * ft[1] - nw[1] * ft[0]);
+    f[i] = fo;
+    tq[i] = to;
+}
+
 // The final pairwise tree over nparts block partials, NC components, into out.
 template <int L, int NC>
 __global__ void __launch_bounds__(FINAL_THREADS) k_final_tree_out(const double* partial,
@@ -1714,8 +1825,11 @@ int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6) 
 
 int launch_wall_gran(System& s, double4* wh, const tmps_wall_params& p, double dt, double coord,
                      double vwall_n, double vwall_s, int history_update) {
-    LAUNCH_N(k_wall_gran, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, coord, vwall_n, vwall_s,
-             history_update);
+    if (p.geometry == TMPS_WALL_CYLINDER)
+        LAUNCH_N(k_wall_cyl, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, history_update);
+    else
+        LAUNCH_N(k_wall_gran, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, coord, vwall_n,
+                 vwall_s, history_update);
     return 0;
 }
 

@@ -7,7 +7,7 @@ its fixes allow. The commands of the DEM path are understood:
     units atom_style dimension boundary newton comm_modify atom_modify
     processors log echo thermo_modify       (accepted; settings of no effect)
     variable <name> equal|string|index ...  print "<text>"
-    region <id> block ...   create_box <n> <region>   read_data <file>
+    region <id> block|cylinder ...   create_box <n> <region>   read_data <file>
     read_restart <file>     write_restart <file>
     pair_style gran/hooke/history|gran/hertz/history Kn Kt gn gt xmu damp [limit_damping]
     pair_style granular  +  pair_coeff * * <normal> <coeffs> [tangential ..] [damping ..]
@@ -15,7 +15,8 @@ its fixes allow. The commands of the DEM path are understood:
     timestep <dt>           thermo <N>   thermo_style custom <keywords>
     fix <id> <group> nve/sphere | damping/cundall gl ga |
         deform[/pressure] N <dim> <style> <args> ... [couple|remap|max/rate ...] |
-        wall/gran hooke[/history] Kn Kt gn gt xmu damp xplane|yplane|zplane lo hi
+        wall/gran hooke[/history] Kn Kt gn gt xmu damp xplane|yplane|zplane lo hi |
+        wall/gran[/region] hooke|hertz[/history] ... zcylinder|cylinder ...|region <cyl>
     unfix <id>
     compute <id> <group> contact/atom | pressure <temp|NULL> [pair] | ke |
         erotate/sphere | stress/atom <temp|NULL> | reduce sum|ave c_<id>
@@ -46,7 +47,8 @@ from .constants import f64
 from .contact_history import ContactHistory
 from .data_reader import read_data
 from .domain import Domain
-from .EXTRA_FIX import FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran
+from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran,
+                        FixWallGranRegion)
 from .GRANULAR import GranHertzHistory, GranHookeHistory, PairGranular
 from .neighbor import NeighborList
 from .simulation import Simulation
@@ -210,11 +212,13 @@ def parse_deform(args: list[str], ev) -> dict:
     return out
 
 
-def parse_wall_gran(args: list[str], ev, boxlo) -> list[dict]:
-    """`hooke|hooke/history Kn Kt gn gt xmu dampflag xplane|yplane|zplane lo
-    hi` as FixWallGran keyword arguments, one wall per non-NULL bound (lo:
-    side -1, hi: side +1); Kt NULL = 2/7 Kn, gamma_t NULL = gamma_n / 2, and no
-    bound means a wall at the box's lower face (input.py:1454-1640)."""
+def parse_wall_gran(args: list[str], ev, boxlo, regions=None) -> list[dict]:
+    """`<fstyle> Kn Kt gn gt xmu dampflag` then `xplane|yplane|zplane lo hi`
+    (FixWallGran, one wall per non-NULL bound: lo side -1, hi side +1; no bound
+    means the box's lower face), `zcylinder R [c1 c2]` / `cylinder dim c1 c2 R`
+    or `region <id>` of a cylinder region (FixWallGranRegion, marked with
+    "cylinder": True); Kt NULL = 2/7 Kn, gamma_t NULL = gamma_n / 2
+    (input.py:1454-1640)."""
     i = 0
     fstyle = "hooke"
     params: list[float | None] = []
@@ -224,15 +228,21 @@ def parse_wall_gran(args: list[str], ev, boxlo) -> list[dict]:
         i += 1
         while i < len(args) and args[i].lower() not in ("xplane", "yplane", "zplane",
                                                         "cylinder", "zcylinder", "region"):
-            params.append(None if args[i].upper() == "NULL" else ev(args[i]))
+            if args[i].upper() == "NULL":
+                params.append(None)
+            else:
+                try:
+                    params.append(ev(args[i]))
+                except ValueError:
+                    break  # a sub-model name of the granular style
             i += 1
-    if fstyle not in ("hooke", "hooke/history"):
-        raise NotImplementedError(f"fix wall/gran {fstyle} is not on the device DEM path")
+    if fstyle == "granular":
+        raise NotImplementedError("fix wall/gran granular is not on the device DEM path")
     get = (lambda k, d: params[k] if len(params) > k and params[k] is not None else d)
     kn = get(0, 1e5)
     kt, gamman, gammat = get(1, None), get(2, 0.0), get(3, None)
     xmu, dampflag = get(4, 0.0), int(get(5, 1))
-    axis, bounds = None, [None, None]
+    axis, bounds, cyl = None, [None, None], None
     while i < len(args):
         tok = args[i].lower()
         if tok in ("xplane", "yplane", "zplane"):
@@ -241,10 +251,29 @@ def parse_wall_gran(args: list[str], ev, boxlo) -> list[dict]:
                 if i + 1 + slot < len(args) and args[i + 1 + slot].upper() != "NULL":
                     bounds[slot] = ev(args[i + 1 + slot])
             i += 3
-        elif tok in ("cylinder", "zcylinder", "region"):
-            raise NotImplementedError("region and cylinder walls are not on the device DEM path")
+        elif tok == "zcylinder":
+            cyl = dict(axis=2, radius=ev(args[i + 1]), c1=0.0, c2=0.0, side="in")
+            i += 2
+            if i + 1 < len(args) and not args[i].isalpha():
+                cyl["c1"], cyl["c2"] = ev(args[i]), ev(args[i + 1])
+                i += 2
+        elif tok == "cylinder":
+            cyl = dict(axis="xyz".index(args[i + 1].lower()), c1=ev(args[i + 2]),
+                       c2=ev(args[i + 3]), radius=ev(args[i + 4]), side="in")
+            i += 5
+        elif tok == "region":
+            reg = (regions or {}).get(args[i + 1])
+            if not isinstance(reg, dict):
+                raise NotImplementedError("fix wall/gran/region needs a cylinder region")
+            cyl = dict(reg)
+            i += 2
         else:
             i += 1
+    if cyl is not None:
+        return [dict(cylinder=True, fstyle=fstyle, kn=kn, gamman=gamman, kt=kt, gammat=gammat,
+                     xmu=xmu, dampflag=dampflag, **cyl)]
+    if fstyle not in ("hooke", "hooke/history"):
+        raise NotImplementedError(f"fix wall/gran {fstyle} planes are not on the device DEM path")
     if axis is None:
         axis = 2
     if bounds == [None, None]:
@@ -522,13 +551,26 @@ class LAMMPSInputParser:
             if self.domain is not None:
                 self.domain.set_boundary(self.boundary)
         elif cmd == "region":
-            if args[1] != "block":
+            if args[1] == "block":
+                b = [ev(t) for t in args[2:8]]
+                self.regions[args[0]] = ([b[0], b[2], b[4]], [b[1], b[3], b[5]])
+            elif args[1] == "cylinder":
+                # region <id> cylinder <dim> <c1> <c2> <radius> <lo> <hi> [side in|out]
+                lo_t, hi_t = args[6], args[7]
+                bound = (lambda t: None if t.upper() in ("INF", "-INF", "EDGE") else ev(t))
+                reg = dict(axis="xyz".index(args[2]), c1=ev(args[3]), c2=ev(args[4]),
+                           radius=ev(args[5]), axis_lo=bound(lo_t), axis_hi=bound(hi_t),
+                           side="in")
+                if "side" in args[8:]:
+                    reg["side"] = args[8:][args[8:].index("side") + 1]
+                self.regions[args[0]] = reg
+            else:
                 raise NotImplementedError(f"region style {args[1]} is not supported")
-            b = [ev(t) for t in args[2:8]]
-            self.regions[args[0]] = ([b[0], b[2], b[4]], [b[1], b[3], b[5]])
         elif cmd == "create_box":
-            lo, hi = self.regions[args[1]]
-            self._new_system(lo, hi, 1)
+            reg = self.regions[args[1]]
+            if isinstance(reg, dict):
+                raise NotImplementedError("create_box needs a block region")
+            self._new_system(reg[0], reg[1], 1)
         elif cmd == "read_data":
             d = read_data(self.workdir / args[0])
             if any(d.tilt):
@@ -634,9 +676,14 @@ class LAMMPSInputParser:
             kw = parse_deform(args, ev)
             self._add_fix(fid, FixDeformPressure(self.domain, var_eval=self.evaluate_variable,
                                                  **kw))
-        elif style == "wall/gran":
-            walls = [FixWallGran(self.domain, max_atoms=self.atom.max_atoms, **kw)
-                     for kw in parse_wall_gran(args, ev, self.domain.boxlo)]
+        elif style in ("wall/gran", "wall/gran/region"):
+            walls = []
+            for kw in parse_wall_gran(args, ev, self.domain.boxlo, self.regions):
+                if kw.pop("cylinder", False):
+                    walls.append(FixWallGranRegion(self.domain, max_atoms=self.atom.max_atoms,
+                                                   **kw))
+                else:
+                    walls.append(FixWallGran(self.domain, max_atoms=self.atom.max_atoms, **kw))
             self._add_fix(fid, walls[0] if len(walls) == 1 else walls)
         else:
             raise NotImplementedError(f"fix {style} is not on the device DEM path")

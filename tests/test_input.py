@@ -112,8 +112,17 @@ def test_wall_gran_grammar():
     assert not w[0]["history"]
     w = I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 yplane NULL NULL".split(), float, lo)
     assert w[0]["wall_coord"] == -1.0  # no bound: the box's lower face
+    w = I.parse_wall_gran("hertz/history 1e5 NULL 10 NULL 0.5 1 zcylinder 3 1 2".split(),
+                          float, lo)
+    assert w == [dict(cylinder=True, fstyle="hertz/history", kn=1e5, gamman=10.0, kt=None,
+                      gammat=None, xmu=0.5, dampflag=1, axis=2, radius=3.0, c1=1.0, c2=2.0,
+                      side="in")]
+    reg = {"c": dict(axis=0, c1=1.0, c2=2.0, radius=4.0, axis_lo=0.0, axis_hi=None,
+                     side="out")}
+    w = I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 region c".split(), float, lo, reg)
+    assert w[0]["cylinder"] and w[0]["side"] == "out" and w[0]["axis_lo"] == 0.0
     with pytest.raises(NotImplementedError):
-        I.parse_wall_gran("hooke 1e5 2e4 10 5 0.5 1 zcylinder 3".split(), float, lo)
+        I.parse_wall_gran("granular hooke 1e5 0.2 zplane 0 NULL".split(), float, lo)
 
 
 def _write_data(path, w):
@@ -293,3 +302,49 @@ run 60
     for key in ("x", "v", "omega"):
         assert np.array_equal(getattr(p.atom, key).to_numpy()[:n],
                               getattr(atom, key).to_numpy()[:n]), key
+
+
+@pytest.mark.gpu
+def test_cylinder_region_wall_script_equals_the_api_run_bitwise(tmp_path):
+    import taichimps_amd as tm
+    from taichimps_amd import workloads
+    w = workloads.lattice(4, 4, 4, 1.1, 31, "uniform")
+    w["v"][:, 0] += 1.0
+    _write_data(tmp_path / "lattice.data", w)
+    lo, hi = w["boxlo"], w["boxhi"]
+    cx, cy = float(0.5 * (lo[0] + hi[0])), float(0.5 * (lo[1] + hi[1]))
+    (tmp_path / "in.cyl").write_text(f"""
+atom_style sphere
+boundary f f f
+read_data lattice.data
+pair_style gran/hooke/history 2e5 NULL 50 NULL 0.5 1
+pair_coeff * *
+neighbor 0.1 bin
+timestep 1e-4
+region can cylinder z {cx!r} {cy!r} 2.6 INF INF side in
+fix 1 all nve/sphere
+fix 2 all wall/gran/region hertz/history 2e5 NULL 50 NULL 0.5 1 region can
+run 80
+""")
+    p = I.LAMMPSInputParser(tmp_path / "in.cyl")
+    p.execute()
+    n = len(w["x"])
+    dom = tm.Domain(lo, hi, ("f", "f", "f"))
+    atom = tm.AtomSystem(max_atoms=n)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+    c = dict(kn=2e5, gamman=50.0, kt=2e5 * 2.0 / 7.0, gammat=25.0, xmu=0.5)
+    sim = tm.Simulation(dom, atom, nl, tm.ContactHistory(n, 64),
+                        tm.GranHookeHistory(dom, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                                            c["xmu"]), dt=1e-4)
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixWallGranRegion(dom, axis="z", c1=cx, c2=cy, radius=2.6, side="in",
+                                     fstyle="hertz/history", kn=2e5, gamman=50.0, kt=None,
+                                     gammat=None, xmu=0.5, max_atoms=n))
+    sim.run(80)
+    for key in ("x", "v", "omega"):
+        assert np.array_equal(getattr(p.atom, key).to_numpy()[:n],
+                              getattr(atom, key).to_numpy()[:n]), key
+    wall = p.fixes["2"]
+    assert isinstance(wall, tm.FixWallGranRegion) and wall.touch.to_numpy()[:n].sum() > 0

@@ -239,3 +239,160 @@ def test_spheres_falling_on_a_floor_bitexact_run(history):
         if history and steps == 20:
             assert tc.sum() > 10, "the bottom layer should be on the floor"
     assert not sim.last_run_device_resident
+
+
+# ------------------------------------------------- cylinder region walls (CPU)
+
+def _cyl_one(x, v=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0), r=0.5):
+    o = O.OracleSim(1)
+    rr = np.array([r])
+    o.set_atoms(np.array([x], dtype=float), np.array([v], dtype=float),
+                np.array([omega], dtype=float), rr, O.mass_from_radius(rr, 1.0))
+    o.set_box((-3.0,) * 3, (3.0,) * 3, (0, 0, 0))
+    o.set_neigh(0.1)
+    return o, O.mass_from_radius(rr, 1.0)[0]
+
+
+@pytest.mark.parametrize("side", ["in", "out"])
+@pytest.mark.parametrize("fstyle", ["hooke", "hertz"])
+def test_cylinder_normal_force_closed_form(side, fstyle):
+    """A resting sphere 0.05 into the curved side of a z-cylinder (R = 2):
+    kn delta (times sqrt(r delta) for hertz) along the inward normal."""
+    rp = 1.55 if side == "in" else 2.45
+    c = np.array([0.3, -0.2])
+    x = (c[0] + rp * 0.6, c[1] + rp * 0.8, 0.7)
+    o, m = _cyl_one(x)
+    w = o.add_wall_cylinder(2, c[0], c[1], 2.0, side, KN, GN, KT, GT, MU, fstyle=fstyle)
+    f, t = _force(o, w)
+    delta = 0.05
+    fn = KN * delta * (math.sqrt(0.5 * delta) if fstyle == "hertz" else 1.0)
+    sgn = -1.0 if side == "in" else 1.0
+    np.testing.assert_allclose(f, [sgn * 0.6 * fn, sgn * 0.8 * fn, 0.0], rtol=1e-9, atol=1e-9)
+    assert np.array_equal(t, np.zeros(3))
+
+
+def test_cylinder_tangential_stick_along_the_axis_and_bounds():
+    o, m = _cyl_one((1.55, 0.0, 0.0), v=(0.0, 0.0, 0.2))
+    w = o.add_wall_cylinder(2, 0.0, 0.0, 2.0, "in", KN, GN, KT, GT, MU, fstyle="hertz/history")
+    f, t = _force(o, w)
+    poly = math.sqrt(0.5 * 0.05)
+    fs = -KT * poly * 0.2 * DT - m * GT * poly * 0.2
+    assert f[2] == pytest.approx(fs, rel=1e-12)
+    # n = -x: torque = -r n x fs = -0.5 * (-x) x (fs z) = -0.5 * fs * y
+    np.testing.assert_allclose(t, [0.0, -0.5 * fs, 0.0], rtol=1e-12)
+    sh, tc = o.wall_history(w)
+    assert tc[0] == 1 and sh[0][2] == pytest.approx(0.2 * DT, rel=1e-12)
+    # the same sphere outside [axis_lo, axis_hi] feels nothing and is reset
+    o2, _ = _cyl_one((1.55, 0.0, 0.0), v=(0.0, 0.0, 0.2))
+    w2 = o2.add_wall_cylinder(2, 0.0, 0.0, 2.0, "in", KN, GN, KT, GT, MU,
+                              fstyle="hooke/history", axis_lo=0.5, axis_hi=2.0)
+    f2, _ = _force(o2, w2)
+    assert np.array_equal(f2, np.zeros(3)) and o2.wall_history(w2)[1][0] == 0
+
+
+def test_cylinder_axis_atom_and_separation_feel_nothing():
+    o, _ = _cyl_one((0.0, 0.0, 0.0))
+    w = o.add_wall_cylinder(2, 0.0, 0.0, 0.4, "in", KN, GN, KT, GT, MU)
+    assert np.array_equal(_force(o, w)[0], np.zeros(3))
+    o, _ = _cyl_one((1.0, 0.0, 0.0))
+    w = o.add_wall_cylinder(2, 0.0, 0.0, 2.0, "in", KN, GN, KT, GT, MU)
+    assert np.array_equal(_force(o, w)[0], np.zeros(3))
+
+
+# ------------------------------------------------- cylinder region walls (GPU)
+
+def _cyl_cases():
+    out = []
+    for axis in range(3):
+        for side in ("in", "out"):
+            out.append(dict(axis=axis, side=side, fstyle="hooke/history"))
+    out += [dict(axis=2, side="in", fstyle="hertz/history"),
+            dict(axis=2, side="in", fstyle="hertz"),
+            dict(axis=1, side="out", fstyle="hooke", limit_damping=True),
+            dict(axis=0, side="in", fstyle="hertz/history", axis_lo=1.0, axis_hi=3.0)]
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", _cyl_cases())
+def test_cylinder_wall_bitexact_per_call(case):
+    import taichimps_amd as tm
+    x, r, v, om = _box_system(17)
+    n = len(x)
+    R = 1.5 if case["side"] == "in" else 1.2
+    dom = tm.Domain((0.0, 0.0, 0.0), (4.0, 4.0, 4.0), ("f", "f", "f"))
+    atom = tm.AtomSystem(max_atoms=n)
+    atom.add_particles(x=x, radius=r, density=1.0, v=v, omega=om)
+    atom._bind_domain(dom)
+    kw = {k: case[k] for k in ("limit_damping", "axis_lo", "axis_hi") if k in case}
+    fix = tm.FixWallGranRegion(dom, axis=case["axis"], c1=2.0, c2=2.1, radius=R,
+                               side=case["side"], fstyle=case["fstyle"], kn=KN, gamman=GN,
+                               kt=KT, gammat=GT, xmu=MU, max_atoms=n, **kw)
+    o = O.OracleSim(n)
+    o.set_atoms(x, v, om, r, O.mass_from_radius(r, 1.0))
+    o.set_box((0.0,) * 3, (4.0,) * 3, (0, 0, 0))
+    wi = o.add_wall_cylinder(case["axis"], 2.0, 2.1, R, case["side"], KN, GN, KT, GT, MU,
+                             fstyle=case["fstyle"], **kw)
+    for call in range(4):
+        atom.clear_forces()
+        o.clear_forces()
+        fix.history_update = call != 0
+        fix.post_force(atom, DT)
+        o.wall_post_force(wi, DT, call != 0)
+        a = o.atoms()
+        assert np.array_equal(atom.f.to_numpy()[:n], a["f"]), f"force, call {call}"
+        assert np.array_equal(atom.torque.to_numpy()[:n], a["torque"]), f"torque, call {call}"
+        sh, tc = o.wall_history(wi)
+        assert np.array_equal(fix.shear.to_numpy()[:n], sh)
+        assert np.array_equal(fix.touch.to_numpy()[:n], tc)
+    assert np.count_nonzero(np.any(a["f"] != 0.0, axis=1)) > 5, "the case should hold contacts"
+
+
+@pytest.mark.gpu
+def test_column_in_a_cylinder_on_a_floor_bitexact_run():
+    """A column of spheres inside a z-cylinder container standing on a floor
+    wall, thrown down and sideways: Simulation.run with both walls against
+    the oracle's run with the same walls, bit for bit."""
+    import taichimps_amd as tm
+    from taichimps_amd import workloads
+    w = workloads.lattice(4, 4, 5, 1.1, 29, "uniform")
+    n = len(w["x"])
+    w["v"][:, 2] -= 1.5
+    w["v"][:, 0] += 0.8
+    lo, hi = w["boxlo"], w["boxhi"]
+    cx, cy = 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])
+    R = 2.6
+    floor = float(lo[2]) + 0.08
+    c = dict(kn=2.0e5, gamman=50.0, kt=2.0e5 * 2.0 / 7.0, gammat=25.0, xmu=0.5)
+    dom = tm.Domain(lo, hi, ("f", "f", "f"))
+    atom = tm.AtomSystem(max_atoms=n)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+    sim = tm.Simulation(dom, atom, nl, tm.ContactHistory(n, 64),
+                        tm.GranHookeHistory(dom, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                                            c["xmu"]), dt=DT)
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixWallGran(dom, 2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                               c["xmu"], max_atoms=n))
+    cyl = tm.FixWallGranRegion(dom, axis="z", c1=cx, c2=cy, radius=R, side="in",
+                               fstyle="hooke/history", kn=c["kn"], gamman=c["gamman"],
+                               kt=c["kt"], gammat=c["gammat"], xmu=c["xmu"], max_atoms=n)
+    sim.add_fix(cyl)
+    o = O.from_workload(w)
+    o.set_box(lo, hi, (0, 0, 0))
+    o.set_neigh(0.1)
+    o.set_pair(O.legacy_pair("hooke/history", **c))
+    o.add_wall(2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"], c["xmu"])
+    wc = o.add_wall_cylinder(2, cx, cy, R, "in", c["kn"], c["gamman"], c["kt"], c["gammat"],
+                             c["xmu"], fstyle="hooke/history")
+    for steps in (30, 170):
+        sim.run(steps)
+        assert o.run(steps, DT) == 0
+        a = o.atoms()
+        for key, field in (("x", atom.x), ("v", atom.v), ("omega", atom.omega),
+                           ("f", atom.f), ("torque", atom.torque)):
+            assert np.array_equal(field.to_numpy()[:n], a[key]), f"{key} after {steps}"
+        sh, tc = o.wall_history(wc)
+        assert np.array_equal(cyl.shear.to_numpy()[:n], sh)
+        assert np.array_equal(cyl.touch.to_numpy()[:n], tc)
