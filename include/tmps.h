@@ -180,7 +180,8 @@ int tmps_servo_end_of_step(tmps_system* s, double dt, double tensor_out[6]);
 
 /* ---- fix wall/gran (EXTRA_FIX/wall_gran.py) and fix wall/gran/region
  *      (EXTRA_FIX/wall_gran_region.py) ------------------------------------
- * A wall in contact with the atoms of an FP64 system, as a partner of zero
+ * A wall in contact with the atoms of a system (FP64, or FP32-split with the
+ * contact evaluated in FP64 on decoded positions), as a partner of zero
  * radius and infinite mass: a flat, axis-aligned plane (FixWallGran's
  * settings, wall_gran.py:57-111) or the curved side of a cylinder region
  * (FixWallGranRegion, wall_gran_region.py:45-112), after their host

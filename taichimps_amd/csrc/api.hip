@@ -1711,8 +1711,6 @@ int tmps_wall_create(tmps_system* h, const tmps_wall_params* p, tmps_wall** out)
                          "shear must act in the plane of the wall, not along its normal");
     if (p->shear_axis < -1 || p->shear_axis > 2)
         return tmps_fail(TMPS_ERR_ARG, "shear axis must be -1, 0, 1 or 2");
-    if (s.precision != TMPS_FP64)
-        return tmps_fail(TMPS_ERR_UNSUPPORTED, "fix wall/gran runs on FP64 systems");
     tmps_wall* w = new tmps_wall();
     w->sys = &s;
     w->p = *p;
@@ -1746,6 +1744,9 @@ int tmps_wall_post_force(tmps_wall* w, double dt, double coord, double vwall_n, 
     if (!w) return tmps_fail(TMPS_ERR_ARG, "null wall");
     System& s = *w->sys;
     if (s.n > w->cap) return tmps_fail(TMPS_ERR_CAPACITY, "more atoms than the wall's history");
+    // FP32-split: the wall reads FP64 positions (authority unchanged)
+    int rc = split_positions_to_xr(s);
+    if (rc) return rc;
     return launch_wall_gran(s, w->hist, w->p, dt, coord, vwall_n, vwall_s,
                             history_update ? 1 : 0);
 }

@@ -1219,8 +1219,11 @@ __global__ void k_stress_atom(const double* vir, const double4* vm64, const floa
 // arm r_i); hooke normal force with mass-velocity damping and the
 // linear_history_classic tangential force, or the history-free viscous one.
 // wh[i] holds the shear history (xyz) and the touch flag (w).
-__global__ void k_wall_gran(const double4* xr, const double4* vm, const double4* om, double4* f,
-                            double4* tq, double4* wh, int n, tmps_wall_params P, double dt,
+// V: double4 (FP64 system) or float4 (FP32-split: v, omega, force and torque
+// in FP32, positions decoded to FP64 in xr); the contact is FP64 either way.
+template <typename V>
+__global__ void k_wall_gran(const double4* xr, const V* vm, const V* om, V* f, V* tq,
+                            double4* wh, int n, tmps_wall_params P, double dt,
                             double coord, double vwall_n, double vwall_s, int history_update) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1240,7 +1243,7 @@ __global__ void k_wall_gran(const double4* xr, const double4* vm, const double4*
         if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
     }
-    const double4 v4 = vm[i], o4 = om[i];
+    const V v4 = vm[i], o4 = om[i];
     const double meff = v4.w;
     const double v[3] = {v4.x, v4.y, v4.z}, w[3] = {o4.x, o4.y, o4.z};
     double vr[3], vt[3], wr[3], vtr[3];
@@ -1302,7 +1305,7 @@ __global__ void k_wall_gran(const double4* xr, const double4* vm, const double4*
 #pragma unroll
         for (int k = 0; k < 3; ++k) fs[k] = -ft * vtr[k];
     }
-    double4 fo = f[i], to = tq[i];
+    V fo = f[i], to = tq[i];
     fo.x = fo.x + (fntot * nv[0] + fs[0]);
     fo.y = fo.y + (fntot * nv[1] + fs[1]);
     fo.z = fo.z + (fntot * nv[2] + fs[2]);
@@ -1319,8 +1322,9 @@ __global__ void k_wall_gran(const double4* xr, const double4* vm, const double4*
 // radius sqrt(r delta) scales stiffness and damping), linear_history_classic
 // or history-free tangential force; atoms outside [axis_lo, axis_hi] (when
 // bounded) or on the axis are out of contact.
-__global__ void k_wall_cyl(const double4* xr, const double4* vm, const double4* om, double4* f,
-                           double4* tq, double4* wh, int n, tmps_wall_params P, double dt,
+template <typename V>
+__global__ void k_wall_cyl(const double4* xr, const V* vm, const V* om, V* f, V* tq,
+                           double4* wh, int n, tmps_wall_params P, double dt,
                            int history_update) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1350,7 +1354,7 @@ __global__ void k_wall_cyl(const double4* xr, const double4* vm, const double4* 
         if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
     }
-    const double4 v4 = vm[i], o4 = om[i];
+    const V v4 = vm[i], o4 = om[i];
     const double mi = v4.w;
     const double vi[3] = {v4.x, v4.y, v4.z}, oi[3] = {o4.x, o4.y, o4.z};
     const double oxn[3] = {oi[1] * nw[2] - oi[2] * nw[1], oi[2] * nw[0] - oi[0] * nw[2],
@@ -1413,7 +1417,7 @@ __global__ void k_wall_cyl(const double4* xr, const double4* vm, const double4* 
 #pragma unroll
         for (int k = 0; k < 3; ++k) ft[k] = -fsc * vrt[k];
     }
-    double4 fo = f[i], to = tq[i];
+    V fo = f[i], to = tq[i];
     fo.x = fo.x + (fntot * nw[0] + ft[0]);
     fo.y = fo.y + (fntot * nw[1] + ft[1]);
     fo.z = fo.z + (fntot * nw[2] + ft[2]);
@@ -1825,11 +1829,22 @@ int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6) 
 
 int launch_wall_gran(System& s, double4* wh, const tmps_wall_params& p, double dt, double coord,
                      double vwall_n, double vwall_s, int history_update) {
-    if (p.geometry == TMPS_WALL_CYLINDER)
-        LAUNCH_N(k_wall_cyl, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, history_update);
-    else
-        LAUNCH_N(k_wall_gran, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, coord, vwall_n,
-                 vwall_s, history_update);
+    const bool f64 = s.precision == TMPS_FP64;
+    if (p.geometry == TMPS_WALL_CYLINDER) {
+        if (f64)
+            LAUNCH_N(k_wall_cyl<double4>, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt,
+                     history_update);
+        else
+            LAUNCH_N(k_wall_cyl<float4>, s.n, s.xr, s.vmf, s.omf, s.ff, s.tqf, wh, s.n, p, dt,
+                     history_update);
+    } else {
+        if (f64)
+            LAUNCH_N(k_wall_gran<double4>, s.n, s.xr, s.vm, s.om, s.f, s.tq, wh, s.n, p, dt, coord,
+                     vwall_n, vwall_s, history_update);
+        else
+            LAUNCH_N(k_wall_gran<float4>, s.n, s.xr, s.vmf, s.omf, s.ff, s.tqf, wh, s.n, p, dt,
+                     coord, vwall_n, vwall_s, history_update);
+    }
     return 0;
 }
 

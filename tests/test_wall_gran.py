@@ -396,3 +396,42 @@ def test_column_in_a_cylinder_on_a_floor_bitexact_run():
         sh, tc = o.wall_history(wc)
         assert np.array_equal(cyl.shear.to_numpy()[:n], sh)
         assert np.array_equal(cyl.touch.to_numpy()[:n], tc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("geometry", ["plane", "cylinder"])
+def test_walls_on_fp32_split_systems_within_tolerance(geometry):
+    """FP32-split systems: velocities, spins, forces and torques in FP32, the
+    contact in FP64 from decoded positions. One call against the FP64
+    oracle: forces within 1e-4 of the largest wall force (positions carry the
+    split encoding's FP32 rounding); the same atoms touch."""
+    import taichimps_amd as tm
+    x, r, v, om = _box_system(23)
+    n = len(x)
+    dom = tm.Domain((0.0, 0.0, 0.0), (4.0, 4.0, 4.0), ("f", "f", "f"))
+    atom = tm.AtomSystem(max_atoms=n, float_type="f32split")
+    atom.add_particles(x=x, radius=r, density=1.0, v=v, omega=om)
+    atom._bind_domain(dom)
+    o = O.OracleSim(n)
+    o.set_atoms(x, v, om, r, O.mass_from_radius(r, 1.0))
+    o.set_box((0.0,) * 3, (4.0,) * 3, (0, 0, 0))
+    if geometry == "plane":
+        fix = tm.FixWallGran(dom, 2, -1, 0.25, KN, GN, KT, GT, MU, max_atoms=n)
+        wi = o.add_wall(2, -1, 0.25, KN, GN, KT, GT, MU)
+    else:
+        fix = tm.FixWallGranRegion(dom, axis="z", c1=2.0, c2=2.1, radius=1.5, side="in",
+                                   fstyle="hertz/history", kn=KN, gamman=GN, kt=KT, gammat=GT,
+                                   xmu=MU, max_atoms=n)
+        wi = o.add_wall_cylinder(2, 2.0, 2.1, 1.5, "in", KN, GN, KT, GT, MU,
+                                 fstyle="hertz/history")
+    atom.clear_forces()
+    o.clear_forces()
+    fix.post_force(atom, DT)
+    o.wall_post_force(wi, DT, True)
+    a = o.atoms()
+    scale = np.abs(a["f"]).max()
+    assert scale > 0.0
+    assert np.abs(atom.f.to_numpy()[:n] - a["f"]).max() < 1e-4 * scale
+    assert np.abs(atom.torque.to_numpy()[:n] - a["torque"]).max() < 1e-4 * max(
+        np.abs(a["torque"]).max(), 1e-30)
+    assert np.array_equal(fix.touch.to_numpy()[:n], o.wall_history(wi)[1])
