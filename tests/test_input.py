@@ -172,6 +172,12 @@ def test_commands_outside_the_path_fail_loudly(tmp_path):
         p.run_command("atom_style", ["atomic"])
 
 
+def test_cli_without_a_script_prints_help(capsys):
+    from taichimps_amd import main as M
+    assert M.main([]) == 0
+    assert "LAMMPS" in capsys.readouterr().out
+
+
 # ------------------------------------------------------------------ GPU
 
 ISO = """

@@ -43,7 +43,9 @@ def test_header_declares_the_expected_entry_points():
     for must in ("tmps_create", "tmps_neigh_build", "tmps_pair_compute", "tmps_nve_initial",
                  "tmps_virial_reduce", "tmps_servo_end_of_step", "tmps_run",
                  "tmps_adjoint_forward", "tmps_adjoint_backward", "tmps_batch_create",
-                 "tmps_batch_run", "tmps_batch_tuning", "tmps_batch_destroy"):
+                 "tmps_batch_run", "tmps_batch_tuning", "tmps_batch_destroy",
+                 "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
+                 "tmps_wall_create", "tmps_wall_post_force"):
         assert must in fns
 
 
@@ -82,6 +84,10 @@ int main(void) {
     F(tmps_servo_params, amp); F(tmps_servo_params, pgain);
     printf("tmps_counters sizeof %zu\n", sizeof(tmps_counters));
     F(tmps_counters, npairs); F(tmps_counters, servo_nsteps);
+    printf("tmps_wall_params sizeof %zu\n", sizeof(tmps_wall_params));
+    F(tmps_wall_params, shear_axis); F(tmps_wall_params, kn); F(tmps_wall_params, xmu);
+    F(tmps_wall_params, geometry); F(tmps_wall_params, has_bounds); F(tmps_wall_params, c1);
+    F(tmps_wall_params, axis_hi);
     return 0;
 }
 """
@@ -96,7 +102,7 @@ def test_struct_layouts_match_the_header(tmp_path):
                     str(exe)], check=True)
     lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     mirror = {"tmps_pair_params": _native.PairParams, "tmps_servo_params": _native.ServoParams,
-              "tmps_counters": _native.Counters}
+              "tmps_counters": _native.Counters, "tmps_wall_params": _native.WallParams}
     for line in lines.splitlines():
         struct, member, value = line.split()
         py = mirror[struct]
@@ -110,3 +116,4 @@ def test_oracle_structs_match_the_library_structs():
     from taichimps_amd import _native
     assert [f[0] for f in O.PairParams._fields_] == [f[0] for f in _native.PairParams._fields_]
     assert [f[0] for f in O.ServoParams._fields_] == [f[0] for f in _native.ServoParams._fields_]
+    assert [f[0] for f in O.WallParams._fields_] == [f[0] for f in _native.WallParams._fields_]
