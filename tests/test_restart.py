@@ -266,3 +266,55 @@ def test_history_with_no_touching_contacts_round_trips(tmp_path):
         assert np.array_equal(rec, [2.0, 0.0])
         t, v = RS.parse_neigh_history(rec)
         assert t.size == 0 and v.shape == (0, 3)
+
+
+@pytest.mark.gpu
+def test_fp32_split_resume_tracks_the_uninterrupted_run():
+    """FP32-split systems: the restart stores decoded FP64 positions and the
+    FP32 history widened to FP64; a resumed run re-encodes them, so it follows
+    the uninterrupted run to FP32 rounding (positions within 1e-4 after 70
+    more steps) with the same contacts carried over."""
+    import os
+    import tempfile
+
+    import taichimps_amd as tm
+    import simhelpers as H
+    w = _packing(seed=9)
+    n = len(w["x"])
+
+    def make(history=None, data=None):
+        src = w if data is None else {
+            "x": data.x, "v": data.v, "omega": data.omega, "radius": data.radius,
+            "density": data.density, "tag": data.tag, "boxlo": data.boxlo,
+            "boxhi": data.boxhi}
+        dom = tm.Domain(src["boxlo"], src["boxhi"], ("p", "p", "p"))
+        atom = tm.AtomSystem(max_atoms=n, float_type="f32split")
+        atom.add_particles(x=src["x"], radius=src["radius"], density=src["density"],
+                           v=src["v"], omega=src["omega"], tag=src["tag"])
+        if data is not None:
+            atom.rmass.from_numpy(data.rmass)
+        nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+        sim = tm.Simulation(dom, atom, nl, history or tm.ContactHistory(n, 64),
+                            H.pair_for(tm, dom, "hooke/history"), dt=1e-4)
+        sim.add_fix(tm.FixNVESphere(dom))
+        return sim
+
+    cont = make()
+    cont.run(80)
+    cont.run(70)
+    first = make()
+    first.run(80)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "mid.restart")
+        _write(first, path, "hooke/history")
+        data = RS.read_restart(path)
+    hist = tm.ContactHistory(n, 64)
+    dec = [RS.parse_neigh_history(e) for e in data.fix_extra]
+    hist.load_restart([t for t, _ in dec], [v for _, v in dec])
+    assert sum(len(t) for t, _ in dec) > 20
+    second = make(history=hist, data=data)
+    second.timestep = data.timestep
+    second.run(70)
+    dx = np.abs(cont.atom.x.to_numpy()[:n] - second.atom.x.to_numpy()[:n]).max()
+    assert dx < 1e-4, dx
+    assert cont.timestep == second.timestep
