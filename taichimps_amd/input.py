@@ -18,8 +18,8 @@ its fixes allow. The commands of the DEM path are understood:
         wall/gran hooke[/history] Kn Kt gn gt xmu damp xplane|yplane|zplane lo hi |
         wall/gran[/region] hooke|hertz[/history] ... zcylinder|cylinder ...|region <cyl>
     unfix <id>
-    compute <id> <group> contact/atom | pressure <temp|NULL> [pair] | ke |
-        erotate/sphere | stress/atom <temp|NULL> | reduce sum|ave c_<id>
+    compute <id> <group> contact/atom | pressure <temp|NULL> | temp | ke |
+        erotate/sphere | stress/atom <temp|NULL> | fabric | reduce <mode> c_<id>[i] ...
     run <N>
 
 Expressions are arithmetic (+ - * / ^, parentheses, sqrt exp ln log sin cos
@@ -42,7 +42,7 @@ import numpy as np
 
 from . import restart as RS
 from .atom import AtomSystem
-from .computes import ComputeContactAtom, Computes, ComputeStressAtom
+from .computes import ComputeContactAtom, ComputeFabric, Computes, ComputeStressAtom
 from .constants import f64
 from .contact_history import ContactHistory
 from .data_reader import read_data
@@ -412,26 +412,36 @@ class LAMMPSInputParser:
         style, args = self.computes[cid]
         atom, dom = self.atom, self.domain
         if style == "pressure":
-            kinetic = args[0] != "NULL" and "pair" not in args[1:]
+            # a NULL temperature compute drops the kinetic term (registry.py:95-115)
+            kinetic = args[0].upper() != "NULL"
             dom.pull()
             p = Computes().compute_pressure_tensor(atom, dom, kinetic=kinetic)
             return float(p[:3].mean()) if index == 0 else float(p[index - 1])
+        if style == "temp":
+            c = Computes()
+            return c.temperature(atom) if index == 0 else float(c.kinetic_tensor(atom)[index - 1])
         if style == "ke":
             return Computes().ke_trans(atom)
         if style == "erotate/sphere":
             return Computes().ke_rot(atom)
+        if style == "fabric":
+            if index == 0:
+                return float("nan")  # a vector compute (registry.py:194-203)
+            phi, _ = ComputeFabric().compute(atom, dom, self.simulation.neighbor)
+            return float(phi[index - 1])
         if style == "reduce":
-            mode, src = args[0], args[1]
-            if not src.startswith("c_"):
+            mode, inputs = args[0], args[1:]
+            if mode not in REDUCE_MODES:
+                raise ValueError(f"compute reduce mode {mode!r} is not supported; "
+                                 f"implemented: {REDUCE_MODES}")
+            src = inputs[max(index, 1) - 1]
+            m = re.fullmatch(r"c_([A-Za-z0-9_]+)(?:\[(\d+)\])?", src)
+            if m is None:
                 raise NotImplementedError("compute reduce over atom variables is not supported")
-            vals = self.compute_peratom(src[2:])
-            if mode == "sum":
-                return float(np.sum(vals))
-            if mode == "ave":
-                return float(np.mean(vals)) if len(vals) else 0.0
-            if mode in ("max", "min"):
-                return float(np.max(vals) if mode == "max" else np.min(vals))
-            raise ValueError(f"unsupported compute reduce mode {mode!r}")
+            vals = np.asarray(self.compute_peratom(m.group(1)), dtype=np.float64)
+            if m.group(2) is not None:
+                vals = vals[:, int(m.group(2)) - 1]
+            return _reduce(mode, vals)
         raise ValueError(f"compute {cid!r} ({style}) has no global value")
 
     def compute_peratom(self, cid: str) -> np.ndarray:
@@ -441,7 +451,7 @@ class LAMMPSInputParser:
             return ComputeContactAtom(self.atom.max_atoms).compute(self.atom, self.domain,
                                                                    sim.neighbor)
         if style == "stress/atom":
-            kinetic = args[0] != "NULL" and "pair" not in args[1:]
+            kinetic = args[0].upper() != "NULL"
             return ComputeStressAtom(self.atom.max_atoms).compute(self.atom, self.domain,
                                                                   kinetic=kinetic)
         raise ValueError(f"compute {cid!r} ({style}) has no per-atom values")
@@ -643,9 +653,11 @@ class LAMMPSInputParser:
                     self.simulation.remove_fix(one)
         elif cmd == "compute":
             style = args[2]
-            if style not in ("contact/atom", "pressure", "ke", "erotate/sphere", "reduce",
-                             "stress/atom"):
+            if style not in ("contact/atom", "pressure", "temp", "ke", "erotate/sphere", "reduce",
+                             "stress/atom", "fabric"):
                 raise ValueError(f"Unsupported compute style {style!r}")
+            if style in ("pressure", "stress/atom") and len(args) < 4:
+                raise ValueError(f"compute {style} needs a temperature compute ID or NULL")
             self.computes[args[0]] = (style, args[3:])
         elif cmd == "run":
             sim = self._ensure_simulation()
@@ -721,6 +733,23 @@ class LAMMPSInputParser:
             self.pair_style_name = d.pair_style
             self.pair_style = self._pair_from(d.pair_style, dict(
                 d.pair_settings, limit_damping=bool(d.pair_settings["limit_damping"])))
+
+
+REDUCE_MODES = ("sum", "ave", "min", "max", "sumsq", "avesq", "sumabs", "aveabs", "maxabs")
+
+
+def _reduce(mode: str, v: np.ndarray) -> float:
+    """One compute reduce over per-atom values (registry.py:240-260)."""
+    if v.size == 0:
+        return 0.0
+    op = {"sum": np.sum, "ave": np.mean, "min": np.min, "max": np.max}
+    if mode in op:
+        return float(op[mode](v))
+    if mode in ("sumsq", "avesq"):
+        return float((np.sum if mode == "sumsq" else np.mean)(v ** 2))
+    if mode in ("sumabs", "aveabs"):
+        return float((np.sum if mode == "sumabs" else np.mean)(np.abs(v)))
+    return float(np.max(np.abs(v)))
 
 
 def _strip_comment(line: str) -> str:

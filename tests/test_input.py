@@ -172,6 +172,14 @@ def test_commands_outside_the_path_fail_loudly(tmp_path):
         p.run_command("atom_style", ["atomic"])
 
 
+def test_reduce_modes():
+    v = np.array([1.0, -2.0, 3.0])
+    got = {m: I._reduce(m, v) for m in I.REDUCE_MODES}
+    assert got == dict(sum=2.0, ave=2.0 / 3.0, min=-2.0, max=3.0, sumsq=14.0, avesq=14.0 / 3.0,
+                       sumabs=6.0, aveabs=2.0, maxabs=3.0)
+    assert I._reduce("ave", np.zeros(0)) == 0.0
+
+
 def test_cli_without_a_script_prints_help(capsys):
     from taichimps_amd import main as M
     assert M.main([]) == 0
@@ -196,6 +204,9 @@ fix             2 all deform/pressure 1 x pressure 1.0 1e-4 y pressure 1.0 1e-4 
 compute         cn all contact/atom
 compute         cnave all reduce ave c_cn
 compute         p all pressure NULL pair
+compute         s all stress/atom NULL
+compute         sxx all reduce sum c_s[1]
+compute         fab all fabric
 thermo          1
 thermo_style    custom step atoms press c_cnave c_p[1]
 run             100
@@ -230,6 +241,11 @@ def test_isotropic_script_equals_the_api_run_bitwise(tmp_path):
     step, atoms, press, cnave, pxx = p.thermo_lines[-1].split()
     assert int(step) == 150 and int(atoms) == n
     assert float(cnave) == pytest.approx(p.compute_value("cnave"))
+    # per-atom stress summed by compute reduce is minus the pair pressure times volume
+    vol = float(np.prod(np.asarray(p.domain.boxhi) - np.asarray(p.domain.boxlo)))
+    assert p.compute_value("sxx") == pytest.approx(-p.compute_value("p", 1) * vol, rel=1e-10)
+    fab = [p.compute_value("fab", k) for k in (1, 2, 3)]
+    assert sum(fab) == pytest.approx(1.0, abs=1e-12)
 
 
 @pytest.mark.gpu
