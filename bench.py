@@ -219,14 +219,15 @@ def copy_calibration(mib):
     return 0
 
 
-def cpu_baseline(w, warmup, steps):
-    """The single-threaded CPU oracle on the same workload, with the same run
-    split as the device (a `warmup` run with setup, then `steps` timed)."""
+def cpu_baseline(w, warmup, steps, threaded=False):
+    """The CPU oracle on the same workload, with the same run split as the
+    device (a `warmup` run with setup, then `steps` timed): single-threaded,
+    or its OpenMP build on the host's threads (`threaded`)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import oracle as O
     from taichimps_amd.workloads import ISOTROPIC as C
     O.build()
-    o = O.from_workload(w)
+    o = O.from_workload(w, threaded=threaded)
     o.set_neigh(C["skin"])
     o.set_pair(O.legacy_pair("hertz/history", kn=C["kn"], gamman=C["gamman"], kt=C["kt"],
                              gammat=C["gammat"], xmu=C["xmu"]))
@@ -237,7 +238,8 @@ def cpu_baseline(w, warmup, steps):
     elapsed = time.perf_counter() - t0
     p = o.last_pressure_tensor()
     return {"atom_steps_per_s": len(w["x"]) * steps / elapsed, "elapsed_s": elapsed,
-            "mean_pressure": float(p[:3].mean()), "x": o.atoms()["x"]}
+            "mean_pressure": float(p[:3].mean()), "x": o.atoms()["x"],
+            "threads": O.num_threads(threaded)}
 
 
 def adjoint_c4(steps=2000, every=100, fd_rel=1e-6):
@@ -634,12 +636,19 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu_baseline:
+        # the serial oracle also checks the device's FP64 positions bit for bit
         cb = cpu_baseline(w, args.warmup, args.steps)
-        cpu = {"value": cb["atom_steps_per_s"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (C++ FP64 restatement, 1 thread) on the same {n}-atom "
-                         f"workload: {args.warmup} setup steps untimed, then {args.steps} steps "
-                         f"timed ({cb['elapsed_s']:.1f} s)",
-               "mean_pressure": cb["mean_pressure"]}
+        ct = cpu_baseline(w, args.warmup, args.steps, threaded=True)
+        cpu = {"value": ct["atom_steps_per_s"], "unit": "atom-steps/s", "cores": ct["threads"],
+               "kind": "port",
+               "sample": f"CPU oracle (C++ FP64 restatement; OpenMP build, {ct['threads']} "
+                         f"threads, pair tally with atomics as the reference's kernels) on the "
+                         f"same {n}-atom workload: {args.warmup} setup steps untimed, then "
+                         f"{args.steps} steps timed ({ct['elapsed_s']:.1f} s)",
+               "mean_pressure": ct["mean_pressure"],
+               "single_thread": {"value": cb["atom_steps_per_s"], "cores": 1,
+                                 "elapsed_s": cb["elapsed_s"]},
+               "host_cpus": len(os.sched_getaffinity(0))}
 
     out = {
         "metric": "atom-steps/s @1M Hertz/history: FP32-split vs FP64 plain; % HBM peak",

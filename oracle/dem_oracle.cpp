@@ -27,6 +27,20 @@
 #include <cstring>
 #include <vector>
 
+// liboracle_omp.so (ORC_OMP, -fopenmp) is the same restatement with its
+// per-atom and per-pair loops on host threads and the pair tally's updates of
+// shared atoms made atomic, as the reference's Taichi kernels do: the CPU
+// baseline of the benchmark. Its sums depend on thread timing, so only the
+// serial liboracle.so checks parity.
+#ifdef ORC_OMP
+#include <omp.h>
+#define ORC_PAR _Pragma("omp parallel for schedule(static)")
+#define ORC_ATOMIC _Pragma("omp atomic")
+#else
+#define ORC_PAR
+#define ORC_ATOMIC
+#endif
+
 namespace {
 
 enum { STYLE_HOOKE_HISTORY = 0, STYLE_HERTZ_HISTORY = 1, STYLE_GRANULAR = 2 };
@@ -277,6 +291,9 @@ int build_neighbor_list(Oracle& s) {
     int overflow = 0;
     s.num_neighbors.assign(s.n, 0);
     s.neighbors.assign((size_t)s.n * s.max_nb, 0);
+#ifdef ORC_OMP
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : overflow)
+#endif
     for (int i = 0; i < s.n; ++i) {
         const double* pi = &s.x[3 * i];
         const double ri = s.radius[i];
@@ -307,7 +324,7 @@ int build_neighbor_list(Oracle& s) {
                             double rad_sum = ri + s.radius[j] + s.skin;
                             if (rsq < rad_sum * rad_sum) {
                                 if (count < s.max_nb) row[count++] = j;
-                                else overflow = 1;
+                                else overflow |= 1;
                             }
                         }
                     }
@@ -359,14 +376,17 @@ bool nlist_decide(Oracle& s, int timestep) {
     if (!s.check) return true;
     if (s.n == 0) return false;
     const double trig = 0.25 * s.skin * s.skin;
-    bool need = false;
+    int need = 0;
+#ifdef ORC_OMP
+#pragma omp parallel for schedule(static) reduction(| : need)
+#endif
     for (int i = 0; i < s.n; ++i) {
         double d0 = s.x[3 * i] - s.x0[3 * i];
         double d1 = s.x[3 * i + 1] - s.x0[3 * i + 1];
         double d2 = s.x[3 * i + 2] - s.x0[3 * i + 2];
-        if (d0 * d0 + d1 * d1 + d2 * d2 > trig) need = true;
+        if (d0 * d0 + d1 * d1 + d2 * d2 > trig) need |= 1;
     }
-    return need;
+    return need != 0;
 }
 
 // --------------------------------------------------------- ContactHistory
@@ -430,16 +450,16 @@ inline double norm3(const double a[3]) { return std::sqrt(dot(a, a)); }
 inline void tally(Oracle& s, int i, int j, const double dpos[3], const double F[3],
                   const double ti[3], const double tj[3]) {
     for (int k = 0; k < 3; ++k) {
-        s.f[3 * i + k] += F[k];
-        s.f[3 * j + k] += -F[k];
-        s.torque[3 * i + k] += ti[k];
-        s.torque[3 * j + k] += tj[k];
+        ORC_ATOMIC s.f[3 * i + k] += F[k];
+        ORC_ATOMIC s.f[3 * j + k] += -F[k];
+        ORC_ATOMIC s.torque[3 * i + k] += ti[k];
+        ORC_ATOMIC s.torque[3 * j + k] += tj[k];
     }
     double vir[6] = {0.5 * (dpos[0] * F[0]), 0.5 * (dpos[1] * F[1]), 0.5 * (dpos[2] * F[2]),
                      0.5 * (dpos[0] * F[1]), 0.5 * (dpos[0] * F[2]), 0.5 * (dpos[1] * F[2])};
     for (int k = 0; k < 6; ++k) {
-        s.virial[6 * i + k] += vir[k];
-        s.virial[6 * j + k] += vir[k];
+        ORC_ATOMIC s.virial[6 * i + k] += vir[k];
+        ORC_ATOMIC s.virial[6 * j + k] += vir[k];
     }
 }
 
@@ -447,6 +467,7 @@ inline void tally(Oracle& s, int i, int j, const double dpos[3], const double F[
 // GranHertzHistory.compute_kernel (hertz_history.py:57-171).
 void pair_legacy_history(Oracle& s, double dt, int shearupdate, bool hertz) {
     const orc_pair_params& P = s.pair;
+    ORC_PAR
     for (int nc = 0; nc < s.npairs; ++nc) {
         const int i = s.pair_i[nc], j = s.pair_j[nc];
         const double* xi = &s.x[3 * i];
@@ -561,6 +582,7 @@ void pair_granular(Oracle& s, double dt, int history_update) {
     const orc_pair_params& P = s.pair;
     const bool uses_history =
         P.tangential_model == TANG_LINEAR_HISTORY || P.tangential_model == TANG_MINDLIN;
+    ORC_PAR
     for (int nc = 0; nc < s.npairs; ++nc) {
         const int i = s.pair_i[nc], j = s.pair_j[nc];
         const double* xi = &s.x[3 * i];
@@ -690,6 +712,7 @@ void clear_forces(Oracle& s) {
 // FixNVESphere.initial_integrate_kernel, nve_sphere.py:43-76
 void nve_initial(Oracle& s, double dt) {
     const double dtv = dt, dtf = 0.5 * dt;
+    ORC_PAR
     for (int i = 0; i < s.n; ++i) {
         const double m = s.rmass[i], r = s.radius[i];
         const double inertia = 0.4 * m * r * r;
@@ -702,6 +725,7 @@ void nve_initial(Oracle& s, double dt) {
 // FixNVESphere.final_integrate_kernel, nve_sphere.py:78-96
 void nve_final(Oracle& s, double dt) {
     const double dtf = 0.5 * dt;
+    ORC_PAR
     for (int i = 0; i < s.n; ++i) {
         const double m = s.rmass[i], r = s.radius[i];
         const double inertia = 0.4 * m * r * r;
@@ -987,10 +1011,16 @@ void virial_tensor(const Oracle& s, int keflag, int mode, double out[6]) {
     size_t P = 1;
     while (P < (size_t)n) P <<= 1;
     std::vector<double> buf(P * 6, 0.0);
+    ORC_PAR
     for (int i = 0; i < n; ++i) virial_leaf(s, i, keflag, &buf[(size_t)i * 6]);
-    for (size_t w = 1; w < P; w <<= 1)
-        for (size_t i = 0; i < P; i += 2 * w)
+    for (size_t w = 1; w < P; w <<= 1) {
+        const long long pairs = (long long)(P / (2 * w));
+        ORC_PAR
+        for (long long q = 0; q < pairs; ++q) {
+            const size_t i = (size_t)q * 2 * w;
             for (int k = 0; k < 6; ++k) buf[i * 6 + k] = buf[i * 6 + k] + buf[(i + w) * 6 + k];
+        }
+    }
     for (int k = 0; k < 6; ++k) out[k] = buf[k];
 }
 
@@ -1258,6 +1288,7 @@ int servo_end_of_step(Oracle& s, double dt) {
         const double oh0 = -0.0 / (xprd * yprd);
         const double oh1 = (0.0 * 0.0 - yprd * 0.0) / (xprd * yprd * zprd);
         const double oh2 = -0.0 / (yprd * zprd);
+        ORC_PAR
         for (int i = 0; i < s.n; ++i) {
             double* xi = &s.x[3 * i];
             double d0 = xi[0] - old_lo[0], d1 = xi[1] - old_lo[1], d2 = xi[2] - old_lo[2];
@@ -1511,6 +1542,14 @@ void orc_wall_history(void* h, int idx, double* shear, int* touch) {
 void orc_wall_post_force(void* h, int idx, double dt, int history_update) {
     Oracle& s = *static_cast<Oracle*>(h);
     wall_post_force(s, s.walls[idx], dt, history_update);
+}
+// Host threads the loops use (1 in the serial build).
+int orc_num_threads(void) {
+#ifdef ORC_OMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 void orc_contact_atom(void* h, int* count) { contact_atom(*static_cast<Oracle*>(h), count); }
 void orc_fabric(void* h, double* phi6, int* ncontacts) {

@@ -25,6 +25,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# the threaded build (benchmark CPU baseline only; its sums depend on thread timing)
+_OMP_LIB_PATH = os.path.join(_HERE, "liboracle_omp.so")
 
 
 class PairParams(C.Structure):
@@ -74,15 +76,16 @@ def build() -> str:
     return _LIB_PATH
 
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(threaded: bool = False):
+    """The serial oracle, or with `threaded` its OpenMP build."""
+    if threaded not in _libs:
+        path = _OMP_LIB_PATH if threaded else _LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
         vp = C.c_void_p
@@ -120,6 +123,7 @@ def lib():
         L.orc_get_grid.argtypes = [vp, ip, dp, ip]
         L.orc_get_counters.argtypes = [vp, ip]
         L.orc_get_tensor.argtypes = [vp, dp]
+        L.orc_num_threads.argtypes = []
         L.orc_contact_atom.argtypes = [vp, ip]
         L.orc_fabric.argtypes = [vp, dp, ip]
         L.orc_stress_atom.argtypes = [vp, C.c_int, dp, dp]
@@ -127,8 +131,8 @@ def lib():
                                    C.c_double, C.c_double]
         L.orc_wall_history.argtypes = [vp, C.c_int, dp, ip]
         L.orc_wall_post_force.argtypes = [vp, C.c_int, C.c_double, C.c_int]
-        _lib = L
-    return _lib
+        _libs[threaded] = L
+    return _libs[threaded]
 
 
 def _dp(a):
@@ -264,15 +268,16 @@ def servo_params(axes=None, couple="none", max_rate=0.0, normalize=False, remap=
 class OracleSim:
     """One simulated system held by the C++ oracle."""
 
-    def __init__(self, n: int, max_nb: int = 64):
+    def __init__(self, n: int, max_nb: int = 64, threaded: bool = False):
         self.n = int(n)
         self.max_nb = int(max_nb)
-        self._h = lib().orc_create(self.n, self.max_nb)
+        self._L = lib(threaded)
+        self._h = self._L.orc_create(self.n, self.max_nb)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().orc_destroy(h)
+            self._L.orc_destroy(h)
             self._h = None
 
     # state -----------------------------------------------------------------
@@ -283,97 +288,97 @@ class OracleSim:
         rr, mm = arr(radius, (n,)), arr(rmass, (n,))
         tt = np.ascontiguousarray(
             np.arange(1, n + 1, dtype=np.int32) if tag is None else np.asarray(tag, dtype=np.int32))
-        lib().orc_set_atoms(self._h, _dp(xx), _dp(vv), _dp(oo), _dp(rr), _dp(mm), _ip(tt))
+        self._L.orc_set_atoms(self._h, _dp(xx), _dp(vv), _dp(oo), _dp(rr), _dp(mm), _ip(tt))
 
     def set_forces(self, f, torque):
         ff = np.ascontiguousarray(np.asarray(f, dtype=np.float64))
         tt = np.ascontiguousarray(np.asarray(torque, dtype=np.float64))
-        lib().orc_set_forces(self._h, _dp(ff), _dp(tt))
+        self._L.orc_set_forces(self._h, _dp(ff), _dp(tt))
 
     def atoms(self) -> dict:
         n = self.n
         out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
         out["virial"] = np.zeros((n, 6))
-        lib().orc_get_atoms(self._h, *(_dp(out[k]) for k in ("x", "v", "omega", "f", "torque", "virial")))
+        self._L.orc_get_atoms(self._h, *(_dp(out[k]) for k in ("x", "v", "omega", "f", "torque", "virial")))
         return out
 
     def set_box(self, lo, hi, periodic=(1, 1, 1)):
         lo_ = np.asarray(lo, dtype=np.float64)
         hi_ = np.asarray(hi, dtype=np.float64)
         per = np.asarray(periodic, dtype=np.int32)
-        lib().orc_set_box(self._h, _dp(lo_), _dp(hi_), _ip(per))
+        self._L.orc_set_box(self._h, _dp(lo_), _dp(hi_), _ip(per))
 
     def set_h_rate(self, h_rate, vremap):
         h = np.asarray(h_rate, dtype=np.float64)
-        lib().orc_set_h_rate(self._h, _dp(h), int(vremap))
+        self._L.orc_set_h_rate(self._h, _dp(h), int(vremap))
 
     def box(self):
         lo, hi, hr = np.zeros(3), np.zeros(3), np.zeros(6)
-        lib().orc_get_box(self._h, _dp(lo), _dp(hi), _dp(hr))
+        self._L.orc_get_box(self._h, _dp(lo), _dp(hi), _dp(hr))
         return lo, hi, hr
 
     def set_neigh(self, skin, delay=0, every=1, check=True, max_grid_cells=200000):
-        lib().orc_set_neigh(self._h, float(skin), int(delay), int(every), 1 if check else 0,
+        self._L.orc_set_neigh(self._h, float(skin), int(delay), int(every), 1 if check else 0,
                             int(max_grid_cells))
 
     def set_pair(self, p: PairParams):
         self._pair = p
-        lib().orc_set_pair(self._h, C.byref(p))
+        self._L.orc_set_pair(self._h, C.byref(p))
 
     def set_servo(self, s: ServoParams):
         self._servo = s
-        lib().orc_set_servo(self._h, C.byref(s))
+        self._L.orc_set_servo(self._h, C.byref(s))
 
     def set_cundall(self, gamma_lin, gamma_ang, enabled=True):
-        lib().orc_set_cundall(self._h, 1 if enabled else 0, float(gamma_lin), float(gamma_ang))
+        self._L.orc_set_cundall(self._h, 1 if enabled else 0, float(gamma_lin), float(gamma_ang))
 
     # stages ----------------------------------------------------------------
     def run(self, nsteps, dt, run_pre=True) -> int:
-        return lib().orc_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0)
+        return self._L.orc_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0)
 
     def init_simulation(self, dt) -> int:
-        return lib().orc_init_simulation(self._h, float(dt))
+        return self._L.orc_init_simulation(self._h, float(dt))
 
     def step(self, dt) -> int:
-        return lib().orc_step(self._h, float(dt))
+        return self._L.orc_step(self._h, float(dt))
 
     def servo_setup(self, nsteps_total, dt):
-        lib().orc_servo_setup(self._h, int(nsteps_total), float(dt))
+        self._L.orc_servo_setup(self._h, int(nsteps_total), float(dt))
 
     def nlist_build(self) -> int:
-        return lib().orc_nlist_build(self._h)
+        return self._L.orc_nlist_build(self._h)
 
     def decide(self, timestep) -> bool:
-        return bool(lib().orc_nlist_decide(self._h, int(timestep)))
+        return bool(self._L.orc_nlist_decide(self._h, int(timestep)))
 
     def reneighbor(self, force=False) -> int:
-        return lib().orc_reneighbor(self._h, 1 if force else 0)
+        return self._L.orc_reneighbor(self._h, 1 if force else 0)
 
     def pbc(self):
-        lib().orc_pbc(self._h)
+        self._L.orc_pbc(self._h)
 
     def clear_forces(self):
-        lib().orc_clear_forces(self._h)
+        self._L.orc_clear_forces(self._h)
 
     def pair_compute(self, dt, shearupdate=True):
-        lib().orc_pair_compute(self._h, float(dt), 1 if shearupdate else 0)
+        self._L.orc_pair_compute(self._h, float(dt), 1 if shearupdate else 0)
 
     def nve_initial(self, dt):
-        lib().orc_nve_initial(self._h, float(dt))
+        self._L.orc_nve_initial(self._h, float(dt))
 
     def nve_final(self, dt):
-        lib().orc_nve_final(self._h, float(dt))
+        self._L.orc_nve_final(self._h, float(dt))
 
     def cundall(self):
-        lib().orc_cundall(self._h)
+        self._L.orc_cundall(self._h)
 
     def virial(self, keflag=True, mode=0):
         out = np.zeros(6)
-        lib().orc_virial(self._h, 1 if keflag else 0, int(mode), _dp(out))
+        self._L.orc_virial(self._h, 1 if keflag else 0, int(mode), _dp(out))
         return out
 
     def servo_end_of_step(self, dt) -> int:
-        return lib().orc_servo_end_of_step(self._h, float(dt))
+        return self._L.orc_servo_end_of_step(self._h, float(dt))
 
     # fix wall/gran ----------------------------------------------------------
     def add_wall(self, axis, side, coord, kn, gamman, kt, gammat, xmu, dampflag=1,
@@ -387,7 +392,7 @@ class OracleSim:
         p.shear_axis = int(shear_axis)
         p.kn, p.kt, p.gamman, p.xmu = float(kn), float(kt), float(gamman), float(xmu)
         p.gammat = 0.0 if int(dampflag) == 0 else float(gammat)
-        return lib().orc_add_wall(self._h, C.byref(p), float(coord), float(shear_vel),
+        return self._L.orc_add_wall(self._h, C.byref(p), float(coord), float(shear_vel),
                                   float(wiggle_amplitude), float(wiggle_period))
 
     def add_wall_cylinder(self, axis, c1, c2, radius, side, kn, gamman, kt=None, gammat=None,
@@ -410,73 +415,78 @@ class OracleSim:
         p.use_history = 1 if (fstyle.endswith("history") if history is None else history) else 0
         p.limit_damping = 1 if limit_damping else 0
         p.shear_axis = -1
-        return lib().orc_add_wall(self._h, C.byref(p), 0.0, 0.0, 0.0, 0.0)
+        return self._L.orc_add_wall(self._h, C.byref(p), 0.0, 0.0, 0.0, 0.0)
 
     def wall_history(self, idx):
         sh = np.zeros((self.n, 3))
         tc = np.zeros(self.n, dtype=np.int32)
-        lib().orc_wall_history(self._h, int(idx), _dp(sh), _ip(tc))
+        self._L.orc_wall_history(self._h, int(idx), _dp(sh), _ip(tc))
         return sh, tc
 
     def wall_post_force(self, idx, dt, history_update=True):
-        lib().orc_wall_post_force(self._h, int(idx), float(dt), 1 if history_update else 0)
+        self._L.orc_wall_post_force(self._h, int(idx), float(dt), 1 if history_update else 0)
 
     # per-contact computes ----------------------------------------------------
     def contact_atom(self):
         """ComputeContactAtom.compute (contact_atom.py:19-64): int32[n]."""
         out = np.zeros(self.n, dtype=np.int32)
-        lib().orc_contact_atom(self._h, _ip(out))
+        self._L.orc_contact_atom(self._h, _ip(out))
         return out
 
     def fabric(self):
         """ComputeFabric.compute (fabric.py:26-97): (phi[6], Nc)."""
         phi = np.zeros(6)
         nc = np.zeros(1, dtype=np.int32)
-        lib().orc_fabric(self._h, _dp(phi), _ip(nc))
+        self._L.orc_fabric(self._h, _dp(phi), _ip(nc))
         return phi, int(nc[0])
 
     def stress_atom(self, keflag=False):
         """ComputeStressAtom.compute (stress_atom.py:30-90): (stress[n,6], sum[6])."""
         st = np.zeros((self.n, 6))
         tot = np.zeros(6)
-        lib().orc_stress_atom(self._h, 1 if keflag else 0, _dp(st), _dp(tot))
+        self._L.orc_stress_atom(self._h, 1 if keflag else 0, _dp(st), _dp(tot))
         return st, tot
 
     # inspection ------------------------------------------------------------
     def pairs(self):
-        m = lib().orc_npairs(self._h)
+        m = self._L.orc_npairs(self._h)
         pi, pj, pa = (np.zeros(m, dtype=np.int32) for _ in range(3))
         sh = np.zeros((m, 3))
-        lib().orc_get_pairs(self._h, _ip(pi), _ip(pj), _ip(pa), _dp(sh))
+        self._L.orc_get_pairs(self._h, _ip(pi), _ip(pj), _ip(pa), _dp(sh))
         return pi, pj, pa, sh
 
     def set_history(self, partner, shear):
         pa = np.ascontiguousarray(np.asarray(partner, dtype=np.int32))
         sh = np.ascontiguousarray(np.asarray(shear, dtype=np.float64))
-        lib().orc_set_history(self._h, _ip(pa), _dp(sh))
+        self._L.orc_set_history(self._h, _ip(pa), _dp(sh))
 
     def grid(self):
         g = np.zeros(3, dtype=np.int32)
         cs = np.zeros(3)
         cell_of = np.zeros(self.n, dtype=np.int32)
-        lib().orc_get_grid(self._h, _ip(g), _dp(cs), _ip(cell_of))
+        self._L.orc_get_grid(self._h, _ip(g), _dp(cs), _ip(cell_of))
         return g, cs, cell_of
 
     def counters(self) -> dict:
         c = np.zeros(6, dtype=np.int32)
-        lib().orc_get_counters(self._h, _ip(c))
+        self._L.orc_get_counters(self._h, _ip(c))
         keys = ("timestep", "build_count", "last_build_step", "npairs", "rebuilds_in_run", "has_built")
         return {k: int(v) for k, v in zip(keys, c)}
 
     def last_pressure_tensor(self):
         out = np.zeros(6)
-        lib().orc_get_tensor(self._h, _dp(out))
+        self._L.orc_get_tensor(self._h, _dp(out))
         return out
 
 
-def from_workload(w: dict, max_nb: int = 64) -> OracleSim:
+def num_threads(threaded: bool = True) -> int:
+    """Host threads the (threaded) oracle's loops run on."""
+    return int(lib(threaded).orc_num_threads())
+
+
+def from_workload(w: dict, max_nb: int = 64, threaded: bool = False) -> OracleSim:
     """An OracleSim loaded with a synthetic workload dict (see taichimps_amd.workloads)."""
-    o = OracleSim(len(w["x"]), max_nb=max_nb)
+    o = OracleSim(len(w["x"]), max_nb=max_nb, threaded=threaded)
     o.set_atoms(w["x"], w["v"], w["omega"], w["radius"],
                 mass_from_radius(w["radius"], w["density"]), w["tag"])
     o.set_box(w["boxlo"], w["boxhi"], (1, 1, 1))
