@@ -8,7 +8,7 @@ its fixes allow. The commands of the DEM path are understood:
     processors log echo thermo_modify       (accepted; settings of no effect)
     variable <name> equal|string|index ...  print "<text>"
     region <id> block|cylinder ...   create_box <n> <region>   read_data <file>
-    read_restart <file>     write_restart <file>
+    read_restart <file>     write_restart <file>     restart <N> <file>
     pair_style gran/hooke/history|gran/hertz/history Kn Kt gn gt xmu damp [limit_damping]
     pair_style granular  +  pair_coeff * * <normal> <coeffs> [tangential ..] [damping ..]
     neighbor <skin> bin     neigh_modify delay|every|check ...
@@ -317,6 +317,8 @@ class LAMMPSInputParser:
         self.thermo_keys = ["step", "atoms", "ke", "press"]
         self.thermo_lines: list[str] = []
         self.start_timestep = 0
+        self.restart_every = 0
+        self.restart_file: str | None = None
         self._seed = None
 
     # ------------------------------------------------------------- script
@@ -479,6 +481,9 @@ class LAMMPSInputParser:
             self.simulation = Simulation(self.domain, self.atom, self.neighbor, self.history,
                                          self.pair_style, dt=self.dt)
             self.simulation.timestep = self.start_timestep
+            if self.restart_every and self.restart_file:
+                self.simulation.set_restart(self.restart_every, self.restart_file,
+                                            self._restart_writer)
             for fix in self.fixes.values():
                 for one in fix if isinstance(fix, list) else [fix]:
                     self.simulation.add_fix(one)
@@ -593,13 +598,14 @@ class LAMMPSInputParser:
         elif cmd == "write_restart":
             sim = self.simulation
             step = sim.timestep if sim else self.start_timestep
-            pair = sim.pair_style if sim else self.pair_style
-            RS.write_restart(self.workdir / args[0].replace("*", str(step)), self.atom,
-                             self.domain, timestep=step, dt=self.dt,
-                             history=sim.history if sim else self.history,
-                             pair=pair if isinstance(pair, PairGranular) else None,
-                             pair_style=self.pair_style_name or "granular",
-                             nlist=sim.neighbor if sim else self.neighbor)
+            self._restart_writer(args[0].replace("*", str(step)), step)
+        elif cmd == "restart":
+            # restart <N> <file>: N = 0 turns periodic writing off
+            self.restart_every = int(ev(args[0]))
+            self.restart_file = args[1] if len(args) > 1 else None
+            if self.simulation is not None:
+                self.simulation.set_restart(self.restart_every, self.restart_file,
+                                            self._restart_writer)
         elif cmd == "pair_style":
             self.pair_style_name = args[0]
             if args[0] in ("gran/hooke/history", "gran/hertz/history"):
@@ -670,6 +676,16 @@ class LAMMPSInputParser:
                 print(row, flush=True)
         else:
             raise NotImplementedError(f"command {cmd!r} is not supported")
+
+    def _restart_writer(self, name: str, step: int) -> None:
+        """One restart file (write_restart and `restart N`)."""
+        sim = self.simulation
+        pair = sim.pair_style if sim else self.pair_style
+        RS.write_restart(self.workdir / name, self.atom, self.domain, timestep=step, dt=self.dt,
+                         history=sim.history if sim else self.history,
+                         pair=pair if isinstance(pair, PairGranular) else None,
+                         pair_style=self.pair_style_name or "granular",
+                         nlist=sim.neighbor if sim else self.neighbor)
 
     def _set_pair(self, pair) -> None:
         self.pair_style = pair

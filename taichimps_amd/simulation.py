@@ -56,6 +56,10 @@ class Simulation:
         self.fixes: list[Fix] = []
         self.setup_done = False
         self.run_pre = True
+        # periodic restart output (`restart N file`, simulation.py:74-98)
+        self.restart_every = 0
+        self.restart_file: str | None = None
+        self.restart_writer = None
         self.thermo_freq = thermo_freq
         self.computes = Computes(float_type=float_type)
         self.last_run_device_resident = False
@@ -196,19 +200,61 @@ class Simulation:
                 self.domain.mark_device_authoritative(atom)
         self.last_run_device_resident = True
 
+    def set_restart(self, every: int, path: str | None, writer) -> None:
+        """Periodic restart output, `restart N file` (simulation.py:86-90):
+        writer(path, timestep) at every multiple of `every` during runs."""
+        self.restart_every = int(every)
+        self.restart_file = path
+        self.restart_writer = writer
+
+    def write_restart_now(self) -> None:
+        """One periodic restart file for the current timestep (:92-98)."""
+        if self.restart_writer is None or not self.restart_file:
+            return
+        self.restart_writer(self.restart_file.replace("*", str(self.timestep)), self.timestep)
+
+    def _restart_due(self) -> bool:
+        return bool(self.restart_every and self.restart_writer is not None
+                    and self.timestep % self.restart_every == 0)
+
     def run(self, steps: int) -> None:
         """Run `steps` Verlet steps (simulation.py:320-332)."""
         self.setup_run(steps)
         plan = self._device_plan()
         if plan is not None and self.atom.nlocal > 0:
-            self._run_device(steps, *plan)
+            if self.restart_every and self.restart_writer is not None:
+                self._run_device_chunked(steps, plan)
+            else:
+                self._run_device(steps, *plan)
         else:
             self.last_run_device_resident = False
             if self.run_pre or not self.setup_done:
                 self.init_simulation()
             for _ in range(steps):
                 self.step()
+                if self._restart_due():
+                    self.domain.pull()
+                    self.write_restart_now()
         self.domain.pull()
+
+    def _run_device_chunked(self, steps: int, plan) -> None:
+        """The device run cut at the restart multiples (as run_gpu cuts at
+        I/O intervals, simulation.py:249-292): one fix setup for the whole
+        run, later pieces continuing without Verlet::setup, so the trajectory
+        is the unbroken run's."""
+        saved = self.run_pre
+        remaining = steps
+        try:
+            while remaining > 0:
+                chunk = min(remaining, self.restart_every - self.timestep % self.restart_every)
+                self._run_device(chunk, *plan)
+                self.run_pre = False
+                remaining -= chunk
+                if self._restart_due():
+                    self.domain.pull()
+                    self.write_restart_now()
+        finally:
+            self.run_pre = saved
 
     def run_gpu(self, steps: int) -> None:
         """The reference's batched variant (simulation.py:249-292); identical here."""

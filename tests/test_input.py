@@ -370,3 +370,31 @@ run 80
                               getattr(atom, key).to_numpy()[:n]), key
     wall = p.fixes["2"]
     assert isinstance(wall, tm.FixWallGranRegion) and wall.touch.to_numpy()[:n].sum() > 0
+
+
+@pytest.mark.gpu
+def test_periodic_restart_files_leave_the_run_unchanged(tmp_path):
+    """`restart 30 snap.*.restart` cuts the device run at steps 30, 60, 90
+    (one fix setup, no Verlet::setup between the pieces): the trajectory is
+    the unbroken run's bit for bit, and each file holds its step."""
+    from taichimps_amd import restart as RS
+    from taichimps_amd import workloads
+    w = workloads.lattice(8, 8, 8, 1.1, 1001, "uniform")
+    _write_data(tmp_path / "lattice.data", w)
+    script = ISO.replace("run             100\nrun             50\n",
+                         "restart         30 snap.*.restart\nrun             100\n")
+    assert script != ISO
+    (tmp_path / "in.snap").write_text(script)
+    p = I.LAMMPSInputParser(tmp_path / "in.snap")
+    p.execute()
+    sim = _api_sim(w)
+    sim.run(100)
+    n = len(w["x"])
+    for key in ("x", "v", "omega", "f"):
+        assert np.array_equal(getattr(p.atom, key).to_numpy()[:n],
+                              getattr(sim.atom, key).to_numpy()[:n]), key
+    assert np.array_equal(p.domain.boxhi, sim.domain.boxhi)
+    for step in (30, 60, 90):
+        d = RS.read_restart(tmp_path / f"snap.{step}.restart")
+        assert d.timestep == step and d.natoms == n
+    assert not (tmp_path / "snap.100.restart").exists()
