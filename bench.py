@@ -37,20 +37,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def build_sim(w, precision):
-    import taichimps_amd as tm
-    from taichimps_amd.workloads import ISOTROPIC as C
-    n = len(w["x"])
-    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "p"))
-    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
-    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
-                       omega=w["omega"], tag=w["tag"])
-    nl = tm.NeighborList(dom, skin=C["skin"], max_atoms=n, max_neighbors_per_atom=64)
-    hist = tm.ContactHistory(n, 64)
-    pair = tm.GranHertzHistory(dom, C["kn"], C["gamman"], C["kt"], C["gammat"], C["xmu"])
-    sim = tm.Simulation(dom, atom, nl, hist, pair, dt=C["dt"])
-    sim.add_fix(tm.FixNVESphere(dom))
-    sim.add_fix(tm.FixDeformPressure(dom, p_target=C["p_target"], p_gain=C["p_gain"]))
-    return sim
+    from taichimps_amd import workloads
+    return workloads.build_isotropic(w, precision)
 
 
 # Algorithmic bytes per atom-step of the one-kernel step (own-atom streams,
@@ -71,29 +59,8 @@ def model_bytes(key, n_slot, n_touch):
 
 
 def build_triaxial(w, precision):
-    """The triaxial test: PairGranular hertz/material E = 1e7, damp 0.5,
-    nu 0.3, viscoelastic damping, mindlin (NULL, 1.0, 0.5); damping/cundall
-    0.1 0.1; x and y pressure servo P = 1.0 (gain 1e-4, couple none), z erate
-    -1e-3; dt 1e-4, skin 0.1."""
-    import taichimps_amd as tm
-    n = len(w["x"])
-    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "p"))
-    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
-    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
-                       omega=w["omega"], tag=w["tag"])
-    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
-    hist = tm.ContactHistory(n, 64)
-    pair = tm.PairGranular(dom, normal="hertz/material", normal_coeffs=[1e7, 0.5, 0.3],
-                           tangential="mindlin", tangential_coeffs=[None, 1.0, 0.5],
-                           damping="viscoelastic")
-    sim = tm.Simulation(dom, atom, nl, hist, pair, dt=1e-4)
-    sim.add_fix(tm.FixNVESphere(dom))
-    sim.add_fix(tm.FixDampingCundall(dom, 0.1, 0.1))
-    sim.add_fix(tm.FixDeformPressure(dom, axes={
-        "x": {"style": "pressure", "ptarget": 1.0, "pgain": 1e-4},
-        "y": {"style": "pressure", "ptarget": 1.0, "pgain": 1e-4},
-        "z": {"style": "erate", "rate": -1e-3}}))
-    return sim
+    from taichimps_amd import workloads
+    return workloads.build_triaxial(w, precision)
 
 
 def triaxial_c3(steps=1000, warmup=10):
@@ -334,23 +301,8 @@ def adjoint_c4(steps=2000, every=100, fd_rel=1e-6):
 
 
 def build_member(w, precision):
-    """An ensemble member: its lattice, pair style and coefficients (kt = 2/7 kn,
-    gamman 50, gammat 25), nve/sphere and deform/pressure at its own target."""
-    import taichimps_amd as tm
-    from taichimps_amd.workloads import ISOTROPIC as C
-    n = len(w["x"])
-    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "p"))
-    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
-    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
-                       omega=w["omega"], tag=w["tag"])
-    nl = tm.NeighborList(dom, skin=C["skin"], max_atoms=n, max_neighbors_per_atom=64)
-    hist = tm.ContactHistory(n, 64)
-    cls = tm.GranHookeHistory if w["style"] == "hooke/history" else tm.GranHertzHistory
-    pair = cls(dom, w["kn"], C["gamman"], 2.0 / 7.0 * w["kn"], C["gammat"], w["xmu"])
-    sim = tm.Simulation(dom, atom, nl, hist, pair, dt=C["dt"])
-    sim.add_fix(tm.FixNVESphere(dom))
-    sim.add_fix(tm.FixDeformPressure(dom, p_target=w["p_target"], p_gain=C["p_gain"]))
-    return sim
+    from taichimps_amd import workloads
+    return workloads.build_member(w, precision)
 
 
 def ensemble_c5(steps=500, n_grad=256, every=100):

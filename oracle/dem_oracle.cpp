@@ -241,6 +241,13 @@ void setup_grid(Oracle& s, double max_cutoff) {
         gx = std::max(1, (int)(gx * factor));
         gy = std::max(1, (int)(gy * factor));
         gz = std::max(1, (int)(gz * factor));
+        // deviation from neighbor.py:178-184: the 2-bin collapse of a periodic
+        // axis is applied again after the scaling, which can bring an axis
+        // back to 2 bins (every candidate there would otherwise be listed
+        // twice)
+        if (s.per[0] == 1 && gx == 2) gx = 1;
+        if (s.per[1] == 1 && gy == 2) gy = 1;
+        if (s.per[2] == 1 && gz == 2) gz = 1;
     }
     s.gdim[0] = gx; s.gdim[1] = gy; s.gdim[2] = gz;
     s.csize[0] = 1.0 / gx; s.csize[1] = 1.0 / gy; s.csize[2] = 1.0 / gz;

@@ -98,6 +98,13 @@ void setup_grid(System& s, const DevBox& B, double max_cutoff) {
         gx = std::max(1, (int)(gx * factor));
         gy = std::max(1, (int)(gy * factor));
         gz = std::max(1, (int)(gz * factor));
+        // The scaling can bring a periodic axis back to 2 bins, where the -1
+        // and +1 stencil offsets wrap onto the same bin and every candidate
+        // there would be listed twice. Collapse it again (the reference
+        // collapses only before scaling, neighbor.py:166-178).
+        if (B.per[0] == 1 && gx == 2) gx = 1;
+        if (B.per[1] == 1 && gy == 2) gy = 1;
+        if (B.per[2] == 1 && gz == 2) gz = 1;
     }
     s.grid.g[0] = gx;
     s.grid.g[1] = gy;

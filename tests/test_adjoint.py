@@ -103,11 +103,11 @@ def test_adjoint_matches_finite_differences(style, xmu):
     gc, gr, _ = ADJ.run_backward(sim, MEAN_P)
     assert np.all(np.isfinite(gc)) and np.all(np.isfinite(gr))
     fd = np.array([central(lambda c: device_pmean(build, c, radius, nsteps, every)[0], coeffs, k,
-                           rel=1e-5) for k in range(5)])
+                           rel=1e-6) for k in range(5)])
     assert_gradient(gc, fd, coeffs, 1e-4, f"{style} xmu {xmu} d<P>/d(kn kt gamman gammat xmu)")
     for i in (0, 37, 101, 215):
         fdr = central(lambda r: device_pmean(build, coeffs, r, nsteps, every)[0], radius, i,
-                      rel=1e-5)
+                      rel=1e-6)
         assert_close(gr[i], fdr, 1e-4, f"{style} d<P>/dr[{i}]")
 
 
@@ -124,7 +124,7 @@ def test_adjoint_in_a_colliding_packing():
     assert sim.neighbor.build_count > 2, "the case should rebuild its list"
     gc, gr, _ = ADJ.run_backward(sim, MEAN_P)
     fd = np.array([central(lambda c: device_pmean(build, c, radius, nsteps, every)[0], coeffs, k,
-                           rel=1e-5) for k in range(5)])
+                           rel=1e-6) for k in range(5)])
     assert_gradient(gc, fd, coeffs, 1e-4, "colliding d<P>/d(kn kt gamman gammat xmu)")
 
 
@@ -152,10 +152,10 @@ def test_adjoint_matches_the_oracle_fp64_reference_gradient():
         return float(MEAN_P @ o.last_pressure_tensor())
 
     assert pm(coeffs, radius) == p_dev  # the FP64 forward is bitwise the oracle's
-    fd = np.array([central(lambda c: pm(c, radius), coeffs, k, rel=1e-5) for k in range(5)])
+    fd = np.array([central(lambda c: pm(c, radius), coeffs, k, rel=1e-6) for k in range(5)])
     assert_gradient(gc, fd, coeffs, 1e-4, "oracle d<P>/d(kn kt gamman gammat xmu)")
     for i in (3, 150):
-        fdr = central(lambda r: pm(coeffs, r), radius, i, rel=1e-5)
+        fdr = central(lambda r: pm(coeffs, r), radius, i, rel=1e-6)
         assert_close(gr[i], fdr, 1e-4, f"oracle d<P>/dr[{i}]")
 
 

@@ -34,6 +34,41 @@ def _sha(a):
 
 # ------------------------------------------------------------------ oracle
 
+_FLOOR_SEEDS = (1, 2, 3)
+
+
+def g3_fp64_noise_floor():
+    """Relative mean-pressure difference, per 100-step chunk of G3, between
+    the FP64 oracle run and FP64 runs whose input positions carry U(+-1e-7)
+    noise (about the rounding of a position kept as an FP32 offset within a
+    bin): the worst over three noise seeds, made monotone over the chunks.
+    This is how far two FP64 runs that start one rounding apart drift in this
+    dense Hooke packing."""
+    gold = _json("g3_isotropic_8k.json")
+    ref = np.array([np.mean(r["pressure"][:3]) for r in gold["series"]])
+    w = workloads.case("isotropic_hooke_8k")
+    worst = np.zeros(len(ref))
+    for seed in _FLOOR_SEEDS:
+        wp = dict(w)
+        rng = np.random.default_rng(seed)
+        wp["x"] = w["x"] + rng.uniform(-1e-7, 1e-7, size=w["x"].shape)
+        o = H.oracle_sim(wp, style="hooke/history")
+        for k in range(len(ref)):
+            assert o.run(G.ISO8K_RUN["chunk_steps"], 1e-4, run_pre=k == 0) == 0
+            p = o.last_pressure_tensor()[:3].mean()
+            worst[k] = max(worst[k], abs(p - ref[k]) / abs(ref[k]))
+    return np.maximum.accumulate(worst)
+
+
+def test_g3_fp64_noise_floor_shows_the_packing_is_chaotic():
+    """Inputs one FP32 rounding apart: the FP64 runs agree to 1e-5 after 100
+    steps and disagree by more than 1e-4 after 1,000, the floor the FP32-split
+    G3 check is held to."""
+    floor = g3_fp64_noise_floor()
+    assert floor[0] < 1e-5
+    assert floor[-1] > 1e-4
+
+
 def test_oracle_reproduces_g1_two_sphere():
     gold = _json("g1_two_sphere.json")
     got = G.make_g1()
@@ -162,6 +197,7 @@ def test_device_matches_g3_isotropic_8k(precision):
     gold = _json("g3_isotropic_8k.json")
     w = workloads.case("isotropic_hooke_8k")
     sim = H.device_sim(w, style="hooke/history", precision=precision)
+    floor = None
     for k, ref in enumerate(gold["series"]):
         sim.run_pre = k == 0
         sim.run(G.ISO8K_RUN["chunk_steps"])
@@ -172,13 +208,17 @@ def test_device_matches_g3_isotropic_8k(precision):
             assert np.array_equal(np.array(sim.domain.boxlo), np.asarray(ref["lo"])), k
             assert sim.neighbor.build_count == ref["build_count"], k
         else:
-            # This dense Hooke packing is chaotic at the 1e-7 level: FP64 runs
-            # whose input positions differ by U(+-1e-7) already disagree by up
-            # to 1e-3 in mean pressure over these 1000 steps. FP32-split is
-            # held to 5e-3, the same order as that FP64 floor.
+            # This dense Hooke packing is chaotic at the 1e-7 level (FP64 runs
+            # whose input positions differ by U(+-1e-7) drift apart,
+            # g3_fp64_noise_floor). FP32-split is held to twice that floor,
+            # and never looser than 1e-5, the target after 200 steps.
+            if floor is None:
+                floor = g3_fp64_noise_floor()
             mean, mean_ref = p[:3].mean(), np.mean(ref["pressure"][:3])
-            tol = 1e-5 if k == 0 else 5e-3
-            assert abs(mean - mean_ref) <= tol * abs(mean_ref), (k, mean, mean_ref)
+            tol = max(1e-5, 2.0 * floor[k])
+            err = abs(mean - mean_ref) / abs(mean_ref)
+            print(f"chunk {k}: split rel err {err:.3e}, FP64 floor {floor[k]:.3e}")
+            assert err <= tol, (k, err, floor[k])
     if precision == "f64":
         n = len(w["x"])
         assert _sha(sim.atom.x.to_numpy()[:n]) == gold["x_sha256"]

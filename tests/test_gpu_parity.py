@@ -75,6 +75,36 @@ def test_long_rows_take_the_fill_path_bitexact():
     _assert_state_equal(H.device_state(sim2), H.oracle_state(o2))
 
 
+@pytest.mark.parametrize("n", [200, 1200])
+def test_capped_thin_periodic_box_collapses_after_scaling(n):
+    """A 40 x 40 x 5 periodic box under a 1,000-bin cap scales to a periodic
+    2-bin axis; both the reported grid and the build's own grid collapse it
+    to one bin, so the list has each pair once and equals the oracle's (200
+    atoms: the build's grid is the capped one itself)."""
+    rng = np.random.default_rng(23)
+    hi = np.array([40.0, 40.0, 5.0])
+    x = rng.uniform(0.0, 1.0, size=(n, 3)) * hi
+    r = rng.uniform(0.2, 0.45, size=n)
+    r[0] = 0.45
+    w = dict(x=x, radius=r, density=np.ones(n), v=np.zeros((n, 3)), omega=np.zeros((n, 3)),
+             tag=np.arange(1, n + 1), boxlo=np.zeros(3), boxhi=hi)
+    sim = H.device_sim(w, servo=False, max_nb=128)
+    sim.neighbor.max_grid_cells = 1000
+    sim.neighbor.build(sim.atom)
+    o = H.oracle_sim(w, servo=False, max_nb=128)
+    o.set_neigh(0.1, 0, 1, True, max_grid_cells=1000)
+    assert o.nlist_build() == 0
+    g, _, cell_of = o.grid()
+    assert g.tolist() == [20, 20, 1]
+    assert np.array_equal(sim.neighbor.grid_dim[None], g)
+    assert np.array_equal(sim.neighbor.cell_of.to_numpy()[:n], cell_of)
+    pi, pj, _, _ = o.pairs()
+    dpi, dpj = sim.neighbor.pair_i.to_numpy(), sim.neighbor.pair_j.to_numpy()
+    assert len(set(zip(dpi.tolist(), dpj.tolist()))) == len(dpi)
+    assert np.array_equal(dpi, pi)
+    assert np.array_equal(dpj, pj)
+
+
 def test_capped_grid_bitexact():
     # a box whose uncapped grid exceeds max_grid_cells, as the 1M-atom case does
     w = dense(10, 1.1, seed=5)

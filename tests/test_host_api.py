@@ -153,16 +153,16 @@ def test_workload_generator_is_seeded_and_matches_spec():
 
 
 def test_ensemble_workload_is_seeded_and_matches_spec():
-    """256 members from SeedSequence(1005).spawn(256): N log-uniform in
-    [2048, 65536] (lattice-rounded), kn, xmu and target P in their ranges,
-    hooke and hertz alternating."""
+    """256 members from SeedSequence(1005).spawn(256): N_k log-uniform in
+    [2048, 65536], the first N_k sites of a ceil-sized lattice, kn, xmu and
+    target P in their ranges, hooke and hertz alternating."""
     e1 = workloads.ensemble()
     e2 = workloads.ensemble()
     assert len(e1) == 256
     for a, b in zip(e1, e2):
         assert np.array_equal(a["x"], b["x"]) and a["kn"] == b["kn"]
     ns = np.array([len(w["x"]) for w in e1])
-    assert ns.min() >= 1800 and ns.max() <= 70000
+    assert ns.min() >= 2048 and ns.max() <= 65536
     # log-uniform: about half of the members below the geometric midpoint
     assert 0.35 < np.mean(ns < np.sqrt(2048 * 65536)) < 0.65
     assert all(1e5 <= w["kn"] <= 1e6 and 0.2 <= w["xmu"] <= 0.7
@@ -172,5 +172,13 @@ def test_ensemble_workload_is_seeded_and_matches_spec():
     rng = np.random.default_rng(np.random.SeedSequence(1005).spawn(256)[3])
     target = np.exp(rng.uniform(np.log(2048), np.log(65536)))
     assert e1[3]["kn"] == rng.uniform(1.0e5, 1.0e6)
-    nx = int(round(target ** (1.0 / 3.0)))
-    assert len(e1[3]["x"]) == nx * nx * max(2, int(round(target / (nx * nx))))
+    n_k = int(round(target))
+    assert len(e1[3]["x"]) == n_k
+    nx = int(np.ceil(n_k ** (1.0 / 3.0)))
+    nz = int(np.ceil(n_k / (nx * nx)))
+    assert nx * nx * (nz - 1) < n_k <= nx * nx * nz
+    np.testing.assert_allclose(e1[3]["boxhi"], [nx * 1.1, nx * 1.1, nz * 1.1])
+    # the first N_k sites in lattice order (x slowest, z fastest)
+    full = workloads.lattice(nx, nx, nz, 1.1, 0, "uniform")
+    site = np.round(e1[3]["x"] / 1.1 - 0.5)
+    assert np.array_equal(site, np.round(full["x"][:n_k] / 1.1 - 0.5))

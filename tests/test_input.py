@@ -397,4 +397,20 @@ def test_periodic_restart_files_leave_the_run_unchanged(tmp_path):
     for step in (30, 60, 90):
         d = RS.read_restart(tmp_path / f"snap.{step}.restart")
         assert d.timestep == step and d.natoms == n
+        # the file holds the state of step `step`: an unbroken run of that
+        # many steps, written by write_restart, has the same atoms, box and
+        # contact history records
+        ref_script = ISO.replace("run             100\nrun             50\n",
+                                 f"run             {step}\nwrite_restart   ref.{step}.restart\n")
+        (tmp_path / f"in.ref{step}").write_text(ref_script)
+        I.LAMMPSInputParser(tmp_path / f"in.ref{step}").execute()
+        r = RS.read_restart(tmp_path / f"ref.{step}.restart")
+        assert r.timestep == step
+        for key in ("tag", "x", "v", "omega", "radius", "rmass"):
+            assert np.array_equal(getattr(d, key), getattr(r, key)), (step, key)
+        assert d.boxlo == r.boxlo and d.boxhi == r.boxhi
+        assert len(d.fix_extra) == len(r.fix_extra) == n
+        assert sum(len(e) for e in d.fix_extra) > 0
+        for a, b in zip(d.fix_extra, r.fix_extra):
+            assert np.array_equal(a, b), step
     assert not (tmp_path / "snap.100.restart").exists()

@@ -287,6 +287,28 @@ def test_neighbor_set_equals_brute_force(box, periodic):
     assert np.array_equal(order, np.arange(len(pi)))
 
 
+@pytest.mark.parametrize("n", [1200, 200])
+def test_capped_thin_periodic_box_lists_each_pair_once(n):
+    """40 x 40 x 5 periodic box, cut 1.0 -> 40 x 40 x 5 bins; a 1,000-bin
+    cap scales that by 1/2 to 20 x 20 x 2, a periodic 2-bin axis, which is
+    collapsed to 1 after the scaling: the list still equals the brute-force
+    set with no pair twice."""
+    rng = np.random.default_rng(23)
+    hi = np.array([40.0, 40.0, 5.0])
+    x = rng.uniform(0.0, 1.0, size=(n, 3)) * hi
+    r = rng.uniform(0.2, 0.45, size=n)
+    r[0] = 0.45
+    o = _sim(x, radius=r, lo=np.zeros(3), hi=hi, periodic=(1, 1, 1), skin=0.1, max_nb=128)
+    o.set_neigh(0.1, 0, 1, True, max_grid_cells=1000)
+    assert o.nlist_build() == 0
+    g, _, _ = o.grid()
+    assert g.tolist() == [20, 20, 1]
+    pi, pj, _, _ = o.pairs()
+    got = list(zip(pi.tolist(), pj.tolist()))
+    assert len(set(got)) == len(got)
+    assert set(got) == _brute_pairs(x, r, 0.1, np.zeros(3), hi, (1, 1, 1))
+
+
 @pytest.mark.parametrize("name,dims,expect", [
     ("isotropic_hooke_8k", None, 18),   # 22^3 box, cut 1.2 -> 18 bins per side
     ("adjoint_32k", None, 29),          # 35.2^3 box -> 29 bins per side
