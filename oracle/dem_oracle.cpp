@@ -159,6 +159,72 @@ struct Oracle {
     std::vector<OrcWall> walls;
 };
 
+// ------------------------------------------------------- sin, exp
+// The servo's wiggle and trate styles call sin and exp (deform_pressure.py:
+// 646, 653; math.sin / math.exp on the host path, ti.sin / ti.exp on the
+// device). Host and device libraries may round those differently in the last
+// bit, so the restatement evaluates them by one fixed method that the device
+// build follows too: Cody-Waite reduction (pi/2 in three parts, ln 2 in two,
+// leading parts of 33 and 32 bits) and Taylor polynomials (degree 17/18 on
+// |r| <= pi/4, degree 15 on |r| <= ln(2)/2) in Horner form. Within two ulps
+// of the libm functions (tests/test_oracle_servo.py).
+
+double det_sin_poly(double r) {
+    const double z = r * r;
+    double p = -1.0 / 355687428096000.0;        // -1/17!
+    p = p * z + 1.0 / 1307674368000.0;          //  1/15!
+    p = p * z - 1.0 / 6227020800.0;             // -1/13!
+    p = p * z + 1.0 / 39916800.0;               //  1/11!
+    p = p * z - 1.0 / 362880.0;                 // -1/9!
+    p = p * z + 1.0 / 5040.0;                   //  1/7!
+    p = p * z - 1.0 / 120.0;                    // -1/5!
+    p = p * z + 1.0 / 6.0;                      //  1/3!
+    return r - (r * z) * p;
+}
+
+double det_cos_poly(double r) {
+    const double z = r * r;
+    double p = 1.0 / 6402373705728000.0;        //  1/18!
+    p = p * z - 1.0 / 20922789888000.0;         // -1/16!
+    p = p * z + 1.0 / 87178291200.0;            //  1/14!
+    p = p * z - 1.0 / 479001600.0;              // -1/12!
+    p = p * z + 1.0 / 3628800.0;                //  1/10!
+    p = p * z - 1.0 / 40320.0;                  // -1/8!
+    p = p * z + 1.0 / 720.0;                    //  1/6!
+    p = p * z - 1.0 / 24.0;                     // -1/4!
+    p = p * z + 0.5;
+    return 1.0 - z * p;
+}
+
+double det_sin(double x) {
+    const double pio2_1 = 1.5707963267341256, pio2_2 = 6.077100506303966e-11,
+                 pio2_3 = 2.0222662487959506e-21;
+    const double k = std::rint(x * (2.0 / M_PI));
+    const double r = ((x - k * pio2_1) - k * pio2_2) - k * pio2_3;
+    switch ((long long)k & 3) {
+        case 0: return det_sin_poly(r);
+        case 1: return det_cos_poly(r);
+        case 2: return -det_sin_poly(r);
+        default: return -det_cos_poly(r);
+    }
+}
+
+double det_exp(double x) {
+    const double ln2_1 = 0.6931471806019545, ln2_2 = -4.2009150726810846e-11;
+    const double k = std::rint(x * (1.0 / M_LN2));
+    const double r = (x - k * ln2_1) - k * ln2_2;
+    // 1/n! for n = 15 down to 2, then 1, 1
+    const double c[14] = {1.0 / 1307674368000.0, 1.0 / 87178291200.0, 1.0 / 6227020800.0,
+                          1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
+                          1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0,
+                          1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5};
+    double p = c[0];
+    for (int m = 1; m < 14; ++m) p = p * r + c[m];
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return std::ldexp(p, (int)k);
+}
+
 // ---------------------------------------------------------------- Domain
 
 // Domain.minimum_image, orthogonal branch, domain.py:481-486
@@ -1242,7 +1308,7 @@ int servo_end_of_step(Oracle& s, double dt) {
         switch (S.style[i]) {
             case S_TRATE: {
                 double mid = 0.5 * (ls + hs);
-                double half = 0.5 * (hs - ls) * std::exp(S.rate[i] * elapsed);
+                double half = 0.5 * (hs - ls) * det_exp(S.rate[i] * elapsed);
                 lo_t[i] = mid - half; hi_t[i] = mid + half;
             } break;
             case S_ERATE: {
@@ -1254,7 +1320,7 @@ int servo_end_of_step(Oracle& s, double dt) {
                 lo_t[i] = ls - shift; hi_t[i] = hs + shift;
             } break;
             case S_WIGGLE: {
-                double shift = 0.5 * S.amp[i] * std::sin(2.0 * M_PI * elapsed / S.tper[i]);
+                double shift = 0.5 * S.amp[i] * det_sin(2.0 * M_PI * elapsed / S.tper[i]);
                 lo_t[i] = ls - shift; hi_t[i] = hs + shift;
             } break;
             case S_FINAL:
@@ -1551,6 +1617,9 @@ void orc_wall_post_force(void* h, int idx, double dt, int history_update) {
     wall_post_force(s, s.walls[idx], dt, history_update);
 }
 // Host threads the loops use (1 in the serial build).
+double orc_det_sin(double x) { return det_sin(x); }
+double orc_det_exp(double x) { return det_exp(x); }
+
 int orc_num_threads(void) {
 #ifdef ORC_OMP
     return omp_get_max_threads();

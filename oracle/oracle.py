@@ -131,6 +131,9 @@ def lib(threaded: bool = False):
                                    C.c_double, C.c_double]
         L.orc_wall_history.argtypes = [vp, C.c_int, dp, ip]
         L.orc_wall_post_force.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        for name in ("orc_det_sin", "orc_det_exp"):
+            getattr(L, name).argtypes = [C.c_double]
+            getattr(L, name).restype = C.c_double
         _libs[threaded] = L
     return _libs[threaded]
 
@@ -477,6 +480,16 @@ class OracleSim:
         out = np.zeros(6)
         self._L.orc_get_tensor(self._h, _dp(out))
         return out
+
+
+def det_sin(x: float) -> float:
+    """The servo's sin (wiggle), evaluated by the fixed method the device uses."""
+    return lib().orc_det_sin(float(x))
+
+
+def det_exp(x: float) -> float:
+    """The servo's exp (trate), evaluated by the fixed method the device uses."""
+    return lib().orc_det_exp(float(x))
 
 
 def num_threads(threaded: bool = True) -> int:

@@ -4,6 +4,7 @@
 // follows the same arithmetic as the CPU oracle.
 #pragma once
 
+#include "detmath.h"
 #include "tmps_internal.h"
 
 namespace tmps {
@@ -84,7 +85,7 @@ __device__ inline void servo_update(Ctl* ctl, const double vs[6], double dt, int
         switch (S.style[i]) {
             case 2: {  // trate
                 const double mid = 0.5 * (ls + hs);
-                const double half = 0.5 * (hs - ls) * exp(S.rate[i] * elapsed);
+                const double half = 0.5 * (hs - ls) * det_exp(S.rate[i] * elapsed);
                 lo_t[i] = mid - half; hi_t[i] = mid + half;
             } break;
             case 1: {  // erate
@@ -97,7 +98,7 @@ __device__ inline void servo_update(Ctl* ctl, const double vs[6], double dt, int
             } break;
             case 4: {  // wiggle
                 const double shift =
-                    0.5 * S.amp[i] * sin(2.0 * 3.14159265358979323846 * elapsed / S.tper[i]);
+                    0.5 * S.amp[i] * det_sin(2.0 * 3.14159265358979323846 * elapsed / S.tper[i]);
                 lo_t[i] = ls - shift; hi_t[i] = hs + shift;
             } break;
             case 5:

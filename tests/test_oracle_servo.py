@@ -181,3 +181,29 @@ def test_remap_v_shifts_the_image_partners_velocity():
     np.testing.assert_allclose(out[0][0], [-kn * d, 0.0, 0.0], rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(out[1][0], [-kn * d + 0.5 * m * gn * ldot, 0.0, 0.0], rtol=1e-12,
                                atol=1e-9)
+
+
+def _ulps(a, b):
+    """Distance in units in the last place of b."""
+    return abs(a - b) / math.ulp(b) if b != 0.0 else abs(a) / 5e-324
+
+
+def test_servo_sin_and_exp_are_within_an_ulp_of_libm():
+    """The wiggle and trate styles evaluate sin and exp by one fixed method
+    on host and device (so both round alike); it stays within two ulps of
+    the C library's values, and equals them for most arguments, over the
+    arguments a run meets (2 pi t / period up to thousands of periods,
+    rate * t up to +-50), and is exact at the special points."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-7.0, 7.0, 4000), rng.uniform(-2e4, 2e4, 4000),
+                         np.arange(-64, 65) * (math.pi / 4.0)])
+    u = np.array([_ulps(O.det_sin(x), math.sin(x)) for x in xs if abs(math.sin(x)) > 1e-12])
+    assert u.max() <= 2.0 and np.mean(u == 0.0) > 0.75, (u.max(), np.mean(u == 0.0))
+    es = np.concatenate([rng.uniform(-1.0, 1.0, 4000), rng.uniform(-50.0, 50.0, 4000)])
+    u = np.array([_ulps(O.det_exp(x), math.exp(x)) for x in es])
+    assert u.max() <= 1.0 and np.mean(u == 0.0) > 0.75, (u.max(), np.mean(u == 0.0))
+    assert O.det_sin(0.0) == 0.0 and O.det_exp(0.0) == 1.0
+    # near multiples of pi the absolute error stays at the rounding of the argument
+    for k in range(1, 2000, 37):
+        x = k * math.pi
+        assert abs(O.det_sin(x) - math.sin(x)) < 1e-15 * k
