@@ -7,14 +7,20 @@ nve/sphere + deform/pressure isotropic P = 1.0 (pgain 1e-4, couple xyz,
 remap x). Synthetic inputs from taichimps_amd.workloads.
 
 For each precision (FP64 plain, FP32-split) the script runs `--warmup` steps
-(the run's setup force and first neighbor build included), then times exactly
-`--steps` device-resident steps (Simulation.run with `pre no`) between two
-stream synchronisations, on `--repeats` fresh systems (the median run is
-reported; the runs are bitwise equal). `value` is the FP32-split throughput in
-atom-steps/s; the FP64 figure and their ratio ride along. A further pass of
-50 steps with per-kernel HIP events gives the step kernel's average duration
-for the roofline; the CPU baseline times the single-threaded CPU oracle on the
-same workload. The adjoint (32k spheres), ensemble (256 packings) and, with
+(the run's setup force and first neighbor build included), captures the
+device graphs of the timed run (no step runs), then times exactly `--steps`
+device-resident steps (Simulation.run with `pre no`) with HIP events recorded
+on the system's stream around the run, device synchronised on both sides, on
+`--repeats` fresh systems (the median run is reported; the runs are bitwise
+equal). `value` is the FP32-split throughput in atom-steps/s; the FP64 figure
+and their ratio ride along. A further pass of `--prof-steps` steps with
+per-kernel HIP events gives the step kernel's average duration for the
+roofline. The `c2_200` block runs both precisions 200 steps from the start
+(rebuilds included and counted) and compares their mean pressure with the CPU
+oracle's after those 200 steps. The CPU baseline times the oracle's OpenMP
+build on the host's threads on the same workload and on the 8,000-sphere
+hooke/history case; its serial build checks the FP64 device positions bit for
+bit. The adjoint (32k spheres), ensemble (256 packings) and, with
 `--triaxial`, the 4M-sphere triaxial blocks follow in the same JSON line.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10]
@@ -91,10 +97,30 @@ def triaxial_c3(steps=1000, warmup=10):
     }
 
 
+def timed_run(sim, steps):
+    """Wall time (s) of sim.run(steps) on the device's clock: HIP events
+    recorded on the system's stream (a torch stream handed to the system)
+    before and after the run, the device synchronised on both sides."""
+    import torch
+    N = sim.atom._native
+    stream = torch.cuda.Stream()
+    N.set_stream(stream.cuda_stream)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    N.sync()
+    start.record(stream)
+    sim.run(steps)
+    stop.record(stream)
+    N.sync()
+    torch.cuda.synchronize()
+    N.set_stream(None)
+    return start.elapsed_time(stop) * 1e-3
+
+
 def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder=None, repeats=1):
     """`repeats` fresh systems, each with `warmup` steps then exactly `steps`
-    timed steps; the median timed run is reported (every run does the same
-    work: the trajectories are bitwise equal)."""
+    timed steps (timed_run); the median timed run is reported (every run does
+    the same work: the trajectories are bitwise equal)."""
     times = []
     p_first = None
     for _ in range(max(1, repeats)):
@@ -108,11 +134,8 @@ def run_precision(w, precision, steps, warmup, prof_steps, direct=False, builder
         t_setup = time.perf_counter() - t0
         sim.run_pre = False
         builds0 = sim.neighbor.build_count
-        N.sync()
-        t0 = time.perf_counter()
-        sim.run(steps)
-        N.sync()
-        times.append(time.perf_counter() - t0)
+        sim.prepare_run(steps)
+        times.append(timed_run(sim, steps))
         p_run = N.last_pressure()
         if p_first is None:
             p_first = p_run
@@ -186,7 +209,7 @@ def copy_calibration(mib):
     return 0
 
 
-def cpu_baseline(w, warmup, steps, threaded=False):
+def cpu_baseline(w, warmup, steps, threaded=False, style="hertz/history"):
     """The CPU oracle on the same workload, with the same run split as the
     device (a `warmup` run with setup, then `steps` timed): single-threaded,
     or its OpenMP build on the host's threads (`threaded`)."""
@@ -196,7 +219,7 @@ def cpu_baseline(w, warmup, steps, threaded=False):
     O.build()
     o = O.from_workload(w, threaded=threaded)
     o.set_neigh(C["skin"])
-    o.set_pair(O.legacy_pair("hertz/history", kn=C["kn"], gamman=C["gamman"], kt=C["kt"],
+    o.set_pair(O.legacy_pair(style, kn=C["kn"], gamman=C["gamman"], kt=C["kt"],
                              gammat=C["gammat"], xmu=C["xmu"]))
     o.set_servo(O.servo_params(p_target=C["p_target"], p_gain=C["p_gain"]))
     assert o.run(warmup, C["dt"]) == 0
@@ -207,6 +230,57 @@ def cpu_baseline(w, warmup, steps, threaded=False):
     return {"atom_steps_per_s": len(w["x"]) * steps / elapsed, "elapsed_s": elapsed,
             "mean_pressure": float(p[:3].mean()), "x": o.atoms()["x"],
             "threads": O.num_threads(threaded)}
+
+
+def c2_200(steps=200):
+    """The 1M-sphere isotropic compression from its start: `steps` steps of
+    each precision after the setup force (run with `pre yes`: Verlet::setup
+    and the first neighbor build untimed, in a 0-step run), timed with
+    timed_run, rebuilds counted; the final pressures are compared with the
+    CPU oracle's in the CPU baseline leg."""
+    from taichimps_amd import workloads
+    w = workloads.case("isotropic_hertz_1m")
+    out = {"workload": f"isotropic compression, 1M spheres, {steps} steps from the start"}
+    for prec in ("f32split", "f64"):
+        sim = build_sim(w, prec)
+        sim.run(0)
+        sim.run_pre = False
+        builds0 = sim.neighbor.build_count
+        sim.prepare_run(steps)
+        elapsed = timed_run(sim, steps)
+        p = sim.atom._native.last_pressure()
+        n = sim.atom.nlocal
+        out[prec] = {"atom_steps_per_s": n * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+                     "rebuilds": sim.neighbor.build_count - builds0,
+                     "pressure": p.tolist(), "mean_pressure": float(p[:3].mean())}
+        if prec == "f64":
+            out["f64_x"] = sim.atom.x.to_numpy()[:n]
+        print(f"# c2_200 {prec}: {out[prec]}", file=sys.stderr, flush=True)
+    out["speedup_fp32split_vs_fp64"] = (out["f32split"]["atom_steps_per_s"]
+                                        / out["f64"]["atom_steps_per_s"])
+    return out
+
+
+def cpu_c2_200(steps=200):
+    """The serial CPU oracle's run of the same 200 steps from the start (the
+    checker of the c2_200 block)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from taichimps_amd import workloads
+    from taichimps_amd.workloads import ISOTROPIC as C
+    w = workloads.case("isotropic_hertz_1m")
+    o = O.from_workload(w)
+    o.set_neigh(C["skin"])
+    o.set_pair(O.legacy_pair("hertz/history", kn=C["kn"], gamman=C["gamman"], kt=C["kt"],
+                             gammat=C["gammat"], xmu=C["xmu"]))
+    o.set_servo(O.servo_params(p_target=C["p_target"], p_gain=C["p_gain"]))
+    assert o.run(0, C["dt"]) == 0
+    t0 = time.perf_counter()
+    assert o.run(steps, C["dt"], run_pre=False) == 0
+    elapsed = time.perf_counter() - t0
+    p = o.last_pressure_tensor()
+    return {"mean_pressure": float(p[:3].mean()), "x": o.atoms()["x"], "elapsed_s": elapsed,
+            "build_count": o.counters()["build_count"]}
 
 
 def adjoint_c4(steps=2000, every=100, fd_rel=1e-6):
@@ -513,9 +587,11 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--prof-steps", type=int, default=50)
-    ap.add_argument("--repeats", type=int, default=3,
+    ap.add_argument("--repeats", type=int, default=5,
                     help="fresh systems timed per precision; the median run is reported")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c2-200", action="store_true",
+                    help="skip the 200-steps-from-the-start block")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint calibration block (32k spheres, 2000 steps)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -586,11 +662,16 @@ def main(argv=None):
     step_achieved = step_bytes / (head["ms_per_step"] * 1e-3) / 1e9
     f64_step_bytes = model_bytes(STEP_MODEL["f64"], f64["n_slot"], f64["n_touch"]) * n
 
+    c2 = None if args.no_c2_200 else c2_200()
+
     cpu = None
     if not args.no_cpu_baseline:
         # the serial oracle also checks the device's FP64 positions bit for bit
         cb = cpu_baseline(w, args.warmup, args.steps)
         ct = cpu_baseline(w, args.warmup, args.steps, threaded=True)
+        w1 = workloads.case("isotropic_hooke_8k")
+        c1t = cpu_baseline(w1, 10, 1000, threaded=True, style="hooke/history")
+        c1s = cpu_baseline(w1, 10, 1000, style="hooke/history")
         cpu = {"value": ct["atom_steps_per_s"], "unit": "atom-steps/s", "cores": ct["threads"],
                "kind": "port",
                "sample": f"CPU oracle (C++ FP64 restatement; OpenMP build, {ct['threads']} "
@@ -600,7 +681,23 @@ def main(argv=None):
                "mean_pressure": ct["mean_pressure"],
                "single_thread": {"value": cb["atom_steps_per_s"], "cores": 1,
                                  "elapsed_s": cb["elapsed_s"]},
+               "c1_isotropic_hooke_8k": {
+                   "value": c1t["atom_steps_per_s"], "cores": c1t["threads"],
+                   "single_thread": c1s["atom_steps_per_s"],
+                   "sample": "8,000 spheres, gran/hooke/history, deform/pressure P=1.0: 10 setup "
+                             "steps untimed, then 1,000 steps timed"},
                "host_cpus": len(os.sched_getaffinity(0))}
+        if c2 is not None:
+            oc = cpu_c2_200()
+            for prec in ("f32split", "f64"):
+                c2[prec]["mean_pressure_rel_err_vs_cpu_oracle"] = (
+                    abs(c2[prec]["mean_pressure"] - oc["mean_pressure"]) / abs(oc["mean_pressure"]))
+            c2["cpu_oracle_mean_pressure"] = oc["mean_pressure"]
+            c2["cpu_oracle_rebuilds"] = oc["build_count"] - 1
+            c2["fp64_positions_bitwise_equal_cpu_oracle"] = bool(
+                np.array_equal(c2["f64_x"], oc["x"]))
+    if c2 is not None:
+        c2.pop("f64_x", None)
 
     out = {
         "metric": "atom-steps/s @1M Hertz/history: FP32-split vs FP64 plain; % HBM peak",
@@ -628,6 +725,8 @@ def main(argv=None):
         "mean_pressure_rel_err_vs_fp64": abs(head["mean_pressure"] - f64["mean_pressure"])
         / abs(f64["mean_pressure"]),
         "roofline": {"bound": "hbm", "kernel": "k_step_split (one kernel per step)",
+                     "limiter": "vector ALU issue and dependent neighbour-gather latency, not "
+                                "HBM bandwidth (SQ and TCC counters in profiles/)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
@@ -640,6 +739,7 @@ def main(argv=None):
         "n_slot": head["n_slot"], "n_touch": head["n_touch"], "rebuilds": head["rebuilds"],
         "kernel_ms": head["kernel_ms"],
         "cpu_baseline": cpu,
+        "c2_200": c2,
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_c4()

@@ -253,6 +253,13 @@ int tmps_stress_atom(tmps_system* s, int keflag, double* stress, double sum6[6])
  * host synchronises once per segment (or on a neighbor rebuild). */
 int tmps_run(tmps_system* s, int nsteps, double dt, int run_pre, int cundall, double gamma_lin,
              double gamma_ang);
+/* The segment graphs a tmps_run of `nsteps` from the current step would
+ * launch (with no rebuild request), captured now without running anything,
+ * so that run pays no capture. Same arguments as tmps_run; call it after the
+ * run's setup (a first tmps_run). No reference counterpart: Taichi compiles
+ * its kernels on first use (simulation.py:320-332 runs them). */
+int tmps_prepare_run(tmps_system* s, int nsteps, double dt, int cundall, double gamma_lin,
+                     double gamma_ang);
 int tmps_get_counters(tmps_system* s, tmps_counters* out);
 int tmps_set_counters(tmps_system* s, const tmps_counters* in);
 /* pressure tensor of the last servo update (virial+kinetic over volume) */
@@ -308,6 +315,16 @@ int tmps_batch_destroy(tmps_batch* b);
 int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotune);
 int tmps_batch_tuning(tmps_batch* b, int* seg_len, int* block_order, double rates[6],
                       int* rebuilds, int* redo_passes, double* redo_ms);
+
+/* ---- stream ---------------------------------------------------------------- */
+/* The HIP stream (a hipStream_t) the system's work runs on: a caller's
+ * stream, borrowed (it must outlive its use), or NULL for the system's own
+ * stream. Work already queued on the previous stream completes first. Replaces
+ * the one implicit stream of the reference's Taichi kernels (every kernel of
+ * simulation.py:205-332 runs on it); a caller's stream lets the path join a
+ * caller's ordering (PyTorch's current stream, HIP events around a run). */
+int tmps_set_stream(tmps_system* s, void* stream);
+int tmps_get_stream(tmps_system* s, void** stream);
 
 /* ---- measurement ----------------------------------------------------------- */
 /* How tmps_run issues a segment's kernels: replayed from a captured HIP graph

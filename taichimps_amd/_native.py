@@ -53,7 +53,7 @@ SYMBOLS = (
     "tmps_batch_create", "tmps_batch_destroy", "tmps_batch_run", "tmps_batch_tuning",
     "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
     "tmps_wall_create", "tmps_wall_destroy", "tmps_wall_reset", "tmps_wall_post_force",
-    "tmps_wall_get_history",
+    "tmps_wall_get_history", "tmps_prepare_run", "tmps_set_stream", "tmps_get_stream",
 )
 
 
@@ -176,6 +176,10 @@ def load(path: str | None = None):
         "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
         "tmps_ke": (C.c_int, [vp, dp, dp]),
         "tmps_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "tmps_prepare_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                       C.c_double]),
+        "tmps_set_stream": (C.c_int, [vp, vp]),
+        "tmps_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "tmps_get_counters": (C.c_int, [vp, C.POINTER(Counters)]),
         "tmps_set_counters": (C.c_int, [vp, C.POINTER(Counters)]),
         "tmps_last_pressure": (C.c_int, [vp, dp]),
@@ -430,6 +434,21 @@ class NativeSystem:
         check(self._L.tmps_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0,
                                0 if cundall is None else 1, float(gl), float(ga)))
 
+    def prepare_run(self, nsteps, dt, cundall=None):
+        gl, ga = (0.0, 0.0) if cundall is None else cundall
+        check(self._L.tmps_prepare_run(self._h, int(nsteps), float(dt),
+                                       0 if cundall is None else 1, float(gl), float(ga)))
+
+    def set_stream(self, handle):
+        """Run on a caller's HIP stream (an integer handle, e.g. a torch
+        stream's cuda_stream), or on the system's own with None."""
+        check(self._L.tmps_set_stream(self._h, C.c_void_p(handle) if handle else None))
+
+    def stream(self) -> int:
+        out = C.c_void_p()
+        check(self._L.tmps_get_stream(self._h, C.byref(out)))
+        return int(out.value or 0)
+
     def counters(self) -> dict:
         c = Counters()
         check(self._L.tmps_get_counters(self._h, C.byref(c)))
@@ -438,6 +457,8 @@ class NativeSystem:
     def set_counters(self, **kw):
         c = Counters()
         check(self._L.tmps_get_counters(self._h, C.byref(c)))
+        if all(getattr(c, k) == int(v) for k, v in kw.items()):
+            return  # already so: no device round trip
         for k, v in kw.items():
             setattr(c, k, int(v))
         check(self._L.tmps_set_counters(self._h, C.byref(c)))

@@ -984,6 +984,7 @@ extern "C" {
 int tmps_adjoint_forward(tmps_system* h, int nsteps, int every, double dt, double tensor_out[6],
                          double* coordination) {
     System& s = *reinterpret_cast<System*>(h);
+    s.hctl_fresh = false;
     if (nsteps < 1 || every < 1) return tmps_fail(TMPS_ERR_ARG, "nsteps and every must be >= 1");
     if (s.n <= 0) return tmps_fail(TMPS_ERR_ARG, "the adjoint needs atoms");
     int rc = check_supported(s);
@@ -1031,6 +1032,7 @@ int tmps_adjoint_backward(tmps_system* h, const double dL_dtensor[6], double gra
                           double* grad_radius, double* grad_x, double* grad_v,
                           double* grad_omega) {
     System& s = *reinterpret_cast<System*>(h);
+    s.hctl_fresh = false;
     AdjContext* Ap = s.adj;
     if (!Ap || !Ap->done_forward)
         return tmps_fail(TMPS_ERR_ARG, "tmps_adjoint_backward needs tmps_adjoint_forward first");

@@ -25,7 +25,16 @@
 namespace {
 thread_local std::string g_err;
 
-System* sys(tmps_system* h) { return reinterpret_cast<System*>(h); }
+// An entry point's system. Every entry point but the getters may change the
+// device's control block, so it marks the host mirror stale.
+System* sys(tmps_system* h) {
+    System* s = reinterpret_cast<System*>(h);
+    s->hctl_fresh = false;
+    return s;
+}
+
+// A getter's system: the host mirror stays valid.
+System* sys_view(tmps_system* h) { return reinterpret_cast<System*>(h); }
 }  // namespace
 
 int tmps_fail(int code, const std::string& msg) {
@@ -64,6 +73,15 @@ int ctl_pull(System& s) {
     HIP_TRY(hipStreamSynchronize(s.stream));
     HIP_TRY(hipMemcpy(s.hctl, s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     return 0;
+}
+
+// ctl_pull for the getters: no round trip while the mirror is known fresh
+// (no entry point that could have changed the device's copy ran since).
+int ctl_pull_view(System& s) {
+    if (s.hctl_fresh) return 0;
+    int rc = ctl_pull(s);
+    if (!rc) s.hctl_fresh = true;
+    return rc;
 }
 
 int ctl_push(System& s) {
@@ -624,11 +642,10 @@ int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cu
     return rc;
 }
 
-int launch_segment(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
-                   double ga) {
-    if (s.profiling) return profiled_steps(s, len, kick, last, dt, use_cundall, gl, ga);
-    if (s.launch_mode == TMPS_LAUNCH_DIRECT)
-        return enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga);
+// The graph slot holding the captured segment for these settings, captured
+// now if none does.
+int segment_graph(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
+                  double ga, int* slot_out) {
     uint64_t key = 1469598103934665603ULL;
     key = mix(key, (uint64_t)len);
     key = mix(key, (uint64_t)(kick ? 1 : 0) | (uint64_t)(last ? 2 : 0));
@@ -667,8 +684,29 @@ int launch_segment(System& s, int len, int kick, int last, double dt, int use_cu
         if (e != hipSuccess) return tmps_fail_hip(e, "hipGraphInstantiate", __LINE__);
         s.seg_key[slot] = key;
     }
+    *slot_out = slot;
+    return 0;
+}
+
+int launch_segment(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
+                   double ga) {
+    if (s.profiling) return profiled_steps(s, len, kick, last, dt, use_cundall, gl, ga);
+    if (s.launch_mode == TMPS_LAUNCH_DIRECT)
+        return enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga);
+    int slot = -1;
+    int rc = segment_graph(s, len, kick, last, dt, use_cundall, gl, ga, &slot);
+    if (rc) return rc;
     HIP_TRY(hipGraphLaunch(s.seg_exec[slot], s.stream));
     return 0;
+}
+
+// Segment lengths of a run of nsteps from step t (no rebuild requests): up
+// to TMPS_SEG_LEN steps, even except the last, which ends the run.
+int next_segment_len(int t, int t_end, int* last) {
+    int len = std::min(TMPS_SEG_LEN, t_end - t);
+    *last = t + len == t_end;
+    if (!*last && (len & 1)) --len;
+    return len;
 }
 
 int check_run_error(System& s) {
@@ -922,9 +960,8 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     int kick = 1;
     while (true) {
         while (nin < 2 && t < t_end) {
-            int len = std::min(TMPS_SEG_LEN, t_end - t);
-            const int last = t + len == t_end;
-            if (!last && (len & 1)) --len;
+            int last = 0;
+            const int len = next_segment_len(t, t_end, &last);
             rc = launch_segment(s, len, kick, last, dt, use_cundall, gl, ga);
             kick = 0;
             if (rc) return rc;
@@ -964,8 +1001,31 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     }
     rc = ctl_pull(s);
     if (rc) return rc;
+    s.hctl_fresh = true;
     s.servo_step_count = s.hctl->servo.step_count;
     s.servo_nsteps = s.hctl->servo.nsteps;
+    return 0;
+}
+
+// Captures the segment graphs a run of nsteps (started now, without
+// rebuild requests) launches, without running it: a run's first use of a
+// segment shape otherwise pays the capture and instantiation inside it.
+int prepare_run(System& s, int nsteps, double dt, int use_cundall, double gl, double ga) {
+    if (!s.have_pair) return tmps_fail(TMPS_ERR_ARG, "tmps_prepare_run needs a pair style");
+    if (nsteps < 0) return tmps_fail(TMPS_ERR_ARG, "negative step count");
+    if (nsteps == 0 || s.n == 0 || !s.setup_done || s.launch_mode == TMPS_LAUNCH_DIRECT)
+        return 0;
+    const int t_end = s.timestep + nsteps;
+    int t = s.timestep, kick = 1;
+    while (t < t_end) {
+        int last = 0;
+        const int len = next_segment_len(t, t_end, &last);
+        int slot = -1;
+        const int rc = segment_graph(s, len, kick, last, dt, use_cundall, gl, ga, &slot);
+        if (rc) return rc;
+        kick = 0;
+        t += len;
+    }
     return 0;
 }
 
@@ -1180,7 +1240,8 @@ void free_all(System& s) {
         if (s.seg_event[k]) (void)hipEventDestroy(s.seg_event[k]);
         s.seg_event[k] = nullptr;
     }
-    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.own_stream) (void)hipStreamDestroy(s.own_stream);
+    s.own_stream = nullptr;
     s.stream = nullptr;
 }
 
@@ -1189,7 +1250,8 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     s.max_nb = max_nb;
     s.precision = precision;
     HIP_TRY(hipGetDevice(&s.device));
-    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s.own_stream, hipStreamNonBlocking));
+    s.stream = s.own_stream;
     const size_t na = (size_t)max_atoms;
     int rc = 0;
     rc = rc ? rc : dalloc(&s.xr, na);
@@ -1319,6 +1381,22 @@ int tmps_destroy(tmps_system* h) {
     return 0;
 }
 
+int tmps_set_stream(tmps_system* h, void* stream) {
+    System& s = *sys(h);
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : s.own_stream;
+    if (next == s.stream) return 0;
+    // work already queued on the old stream finishes first
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.stream = next;
+    return 0;
+}
+
+int tmps_get_stream(tmps_system* h, void** stream) {
+    if (!stream) return tmps_fail(TMPS_ERR_ARG, "stream is null");
+    *stream = static_cast<void*>(sys(h)->stream);
+    return 0;
+}
+
 int tmps_set_nlocal(tmps_system* h, int nlocal) {
     System& s = *sys(h);
     if (nlocal < 0 || nlocal > s.max_atoms)
@@ -1368,8 +1446,8 @@ int tmps_set_box(tmps_system* h, const double lo[3], const double hi[3], const i
 }
 
 int tmps_get_box(tmps_system* h, double lo[3], double hi[3], double h_rate[6], int* vremap) {
-    System& s = *sys(h);
-    int rc = ctl_pull(s);
+    System& s = *sys_view(h);
+    int rc = ctl_pull_view(s);
     if (rc) return rc;
     const DevBox& B = s.hctl->box;
     for (int k = 0; k < 3; ++k) {
@@ -1877,9 +1955,14 @@ int tmps_run(tmps_system* h, int nsteps, double dt, int run_pre, int use_cundall
     return run(*sys(h), nsteps, dt, run_pre, use_cundall, gamma_lin, gamma_ang);
 }
 
+int tmps_prepare_run(tmps_system* h, int nsteps, double dt, int use_cundall, double gamma_lin,
+                     double gamma_ang) {
+    return prepare_run(*sys(h), nsteps, dt, use_cundall, gamma_lin, gamma_ang);
+}
+
 int tmps_get_counters(tmps_system* h, tmps_counters* out) {
-    System& s = *sys(h);
-    int rc = ctl_pull(s);
+    System& s = *sys_view(h);
+    int rc = ctl_pull_view(s);
     if (rc) return rc;
     out->timestep = s.timestep;
     out->has_built = s.has_built;
@@ -1913,8 +1996,8 @@ int tmps_set_counters(tmps_system* h, const tmps_counters* in) {
 }
 
 int tmps_last_pressure(tmps_system* h, double out6[6]) {
-    System& s = *sys(h);
-    int rc = ctl_pull(s);
+    System& s = *sys_view(h);
+    int rc = ctl_pull_view(s);
     if (rc) return rc;
     for (int k = 0; k < 6; ++k) out6[k] = s.hctl->tensor[k];
     return 0;

@@ -604,6 +604,7 @@ int tmps_batch_run(tmps_batch* b, int nsteps, double dt, int run_pre, int autotu
     if (!b) return tmps_fail(TMPS_ERR_ARG, "null ensemble");
     if (nsteps < 0) return tmps_fail(TMPS_ERR_ARG, "negative step count");
     Batch& B = *reinterpret_cast<Batch*>(b);
+    for (System* m : B.sys) m->hctl_fresh = false;
     for (size_t k = 0; k < B.sys.size(); ++k)
         if (B.sys[k]->n != B.natoms[k] || !B.sys[k]->have_pair ||
             style_of(*B.sys[k]) != B.styles[k])

@@ -143,7 +143,11 @@ struct RedoMaps {
 
 struct System {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the system's work runs
+    hipStream_t own_stream = nullptr;  // the stream it created (stream, unless a caller's)
+    // the host mirror of the control block equals the device's: set by a
+    // getter's pull or the end of a run, cleared by every other entry point
+    bool hctl_fresh = false;
     int precision = TMPS_FP64;
     int max_atoms = 0, max_nb = 64, n = 0;
 

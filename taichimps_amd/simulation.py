@@ -237,6 +237,17 @@ class Simulation:
                     self.write_restart_now()
         self.domain.pull()
 
+    def prepare_run(self, steps: int) -> None:
+        """Capture the device graphs a following device-resident run(steps)
+        launches, without running anything (no reference counterpart). A no-op
+        before the first run or where the run takes the stage path."""
+        plan = self._device_plan()
+        if plan is None or self.atom.nlocal == 0 or not self.setup_done:
+            return
+        cundall = plan[0]
+        gl_ga = None if cundall is None else (cundall.gamma_lin, cundall.gamma_ang)
+        self.atom._native.prepare_run(steps, self.dt, cundall=gl_ga)
+
     def _run_device_chunked(self, steps: int, plan) -> None:
         """The device run cut at the restart multiples (as run_gpu cuts at
         I/O intervals, simulation.py:249-292): one fix setup for the whole

@@ -464,3 +464,42 @@ def test_chained_runs_match_the_stage_path():
     np_ = a.neighbor.npairs[None]
     assert np.array_equal(a.history.shear.to_numpy()[:np_], b.history.shear.to_numpy()[:np_])
     assert _native is not None
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32split"])
+def test_run_on_a_caller_stream_with_prepared_graphs_is_bitwise_the_plain_run(precision):
+    """tmps_prepare_run captures a run's segment graphs ahead of it and
+    tmps_set_stream moves the system onto a caller's stream (here a torch
+    stream, which HIP events can bracket): the run's results do not change,
+    rebuilds included, and a following run on the system's own stream
+    continues it. The getters after the run (counters, box, pressure) read the
+    run's end state."""
+    torch = pytest.importorskip("torch")
+    w = dense(9, 1.0, seed=61)
+    rng = np.random.default_rng(62)
+    w["v"] = rng.normal(0.0, 1.5, size=w["v"].shape)
+    a = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02)
+    b = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02)
+    for s in (a, b):
+        s.run(3)
+        s.run_pre = False
+    a.run(45)
+    a.run(13)
+    b.prepare_run(45)
+    stream = torch.cuda.Stream()
+    b.atom._native.set_stream(stream.cuda_stream)
+    assert b.atom._native.stream() == stream.cuda_stream
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    b.run(45)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    assert start.elapsed_time(stop) > 0.0
+    b.atom._native.set_stream(None)
+    b.run(13)
+    assert a.neighbor.build_count == b.neighbor.build_count > 3
+    sa, sb = H.device_state(a), H.device_state(b)
+    _assert_state_equal(sa, sb, ("x", "v", "omega", "f", "torque"))
+    assert np.array_equal(sa["boxhi"], sb["boxhi"])
+    assert np.array_equal(a.atom._native.last_pressure(), b.atom._native.last_pressure())
+    assert a.atom._native.counters() == b.atom._native.counters()
