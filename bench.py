@@ -371,6 +371,45 @@ def adjoint_c4(steps=2000, every=100, fd_rel=1e-6):
                    "steps of the same packing they converge at 1e-6",
         "forward_s": t_fwd, "backward_s": t_bwd, "rebuilds": sim.neighbor.build_count,
         "note": "Z has zero gradient almost everywhere (piecewise constant)",
+        "_check_inputs": (w, theta, steps, float(p), g_c, g_r, sim.neighbor.build_count),
+    }
+
+
+def cpu_adjoint_check(block):
+    """The checker of the adjoint block: the tangent-linear restatement
+    (oracle/tlm.cpp) differentiates the same 2,000-step run forward on the
+    host's threads, with the forward's contact onsets, slips and rebuilds and
+    no step size, so a change of the collision sequence cannot enter it. Its
+    FP64 values must be the device forward's bit for bit; its derivatives
+    d<P>/d(kn, kt, gamman, gammat, xmu) and along a random radius direction
+    are compared with the adjoint's."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from taichimps_amd.workloads import ISOTROPIC as C
+    w, theta, steps, p_dev, g_c, g_r, builds = block.pop("_check_inputs")
+    u = np.random.default_rng([1004, 2]).normal(size=len(w["x"]))
+    t0 = time.perf_counter()
+    t, d, tl_builds = O.tlm_tangent(w, theta, steps, dradius=u, style="hertz/history",
+                                    skin=C["skin"], dt=C["dt"], p_target=C["p_target"],
+                                    p_gain=C["p_gain"])
+    elapsed = time.perf_counter() - t0
+    mean_p = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0]) / 3.0
+    tl = mean_p @ d
+    names = ("kn", "kt", "gamman", "gammat", "xmu")
+    comp = {}
+    for k, name in enumerate(names):
+        comp[name] = {"adjoint": float(g_c[k]), "tangent_linear": float(tl[k]),
+                      "rel_err": abs(g_c[k] - tl[k]) / abs(tl[k])}
+    comp["radius_direction"] = {"adjoint": float(u @ g_r), "tangent_linear": float(tl[5]),
+                                "rel_err": abs(u @ g_r - tl[5]) / abs(tl[5])}
+    block["tlm_check"] = {
+        "method": "tangent-linear (forward-mode) run of the same 2000 steps on the CPU "
+                  "(oracle/tlm.cpp), FP64 values bitwise the device forward's",
+        "forward_values_bitwise_equal": bool(float(mean_p @ t) == p_dev and tl_builds == builds),
+        "dP": comp,
+        "max_rel_err": max(c["rel_err"] for c in comp.values()),
+        "threads": int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0))),
+        "elapsed_s": elapsed,
     }
 
 
@@ -622,7 +661,9 @@ def main(argv=None):
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.adjoint_only:
-        print(json.dumps(adjoint_c4()))
+        block = adjoint_c4()
+        cpu_adjoint_check(block)
+        print(json.dumps(block))
         return 0
     if args.ensemble_only:
         print(json.dumps(ensemble_c5(n_grad=args.ensemble_grads)))
@@ -743,6 +784,10 @@ def main(argv=None):
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_c4()
+        if args.no_cpu_baseline:
+            out["adjoint"].pop("_check_inputs")
+        else:
+            cpu_adjoint_check(out["adjoint"])
     if not args.no_ensemble:
         out["ensemble"] = ensemble_c5(n_grad=args.ensemble_grads)
     if args.triaxial:

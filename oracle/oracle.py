@@ -64,6 +64,16 @@ class WallParams(C.Structure):
     ]
 
 
+class TlmParams(C.Structure):
+    """orc_tlm_params (tlm.cpp)"""
+
+    _fields_ = [("hertz", C.c_int), ("limit_damping", C.c_int),
+                ("kn", C.c_double), ("kt", C.c_double), ("gamman", C.c_double),
+                ("gammat", C.c_double), ("xmu", C.c_double),
+                ("skin", C.c_double), ("dt", C.c_double), ("ptarget", C.c_double),
+                ("pgain", C.c_double)]
+
+
 STYLE_CODE = {"none": 0, "erate": 1, "trate": 2, "vel": 3, "wiggle": 4, "final": 5,
               "delta": 6, "scale": 7, "pressure": 8, "pressure/mean": 9}
 COUPLE_CODE = {"none": 0, "xyz": 1, "xy": 2, "yz": 3, "xz": 4}
@@ -131,6 +141,11 @@ def lib(threaded: bool = False):
                                    C.c_double, C.c_double]
         L.orc_wall_history.argtypes = [vp, C.c_int, dp, ip]
         L.orc_wall_post_force.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        tp = C.POINTER(TlmParams)
+        L.orc_tlm_forward.argtypes = [C.c_int, dp, dp, dp, dp, dp, ip, dp, dp, tp, C.c_int, dp,
+                                      dp, dp, ip]
+        L.orc_tlm_tangent.argtypes = [C.c_int, dp, dp, dp, dp, dp, ip, dp, dp, tp, C.c_int, dp,
+                                      dp, dp, ip]
         for name in ("orc_det_sin", "orc_det_exp"):
             getattr(L, name).argtypes = [C.c_double]
             getattr(L, name).restype = C.c_double
@@ -490,6 +505,51 @@ def det_sin(x: float) -> float:
 def det_exp(x: float) -> float:
     """The servo's exp (trate), evaluated by the fixed method the device uses."""
     return lib().orc_det_exp(float(x))
+
+
+# ------------------------------------------------ tangent-linear run (tlm.cpp)
+
+def _tlm_args(w, coeffs, style, skin, dt, p_target, p_gain):
+    n = len(w["x"])
+    arr = lambda a, shape: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+    r = arr(np.broadcast_to(np.asarray(w["radius"], dtype=np.float64), (n,)), (n,))
+    m = arr(mass_from_radius(r, w["density"]), (n,))
+    tag = np.ascontiguousarray(np.asarray(w["tag"], dtype=np.int32))
+    kn, kt, gn, gt, xmu = (float(c) for c in coeffs)
+    p = TlmParams(1 if style == "hertz/history" else 0, 0, kn, kt, gn, gt, xmu, float(skin),
+                  float(dt), float(p_target), float(p_gain))
+    return (n, arr(w["x"], (n, 3)), arr(w["v"], (n, 3)), arr(w["omega"], (n, 3)), r, m, tag,
+            arr(w["boxlo"], (3,)), arr(w["boxhi"], (3,)), p)
+
+
+def tlm_forward(w, coeffs, nsteps, style="hertz/history", skin=0.1, dt=1e-4, p_target=1.0,
+                p_gain=1e-4):
+    """Simulation.run(nsteps) of the isotropic compression by the
+    tangent-linear restatement's FP64 form: (x, v, pressure tensor, builds)."""
+    n, x, v, om, r, m, tag, lo, hi, p = _tlm_args(w, coeffs, style, skin, dt, p_target, p_gain)
+    xo, vo, t = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(6)
+    b = np.zeros(1, dtype=np.int32)
+    rc = lib().orc_tlm_forward(n, _dp(x), _dp(v), _dp(om), _dp(r), _dp(m), _ip(tag), _dp(lo),
+                               _dp(hi), C.byref(p), int(nsteps), _dp(xo), _dp(vo), _dp(t), _ip(b))
+    if rc:
+        raise RuntimeError("the box collapsed")
+    return xo, vo, t, int(b[0])
+
+
+def tlm_tangent(w, coeffs, nsteps, dradius=None, style="hertz/history", skin=0.1, dt=1e-4,
+                p_target=1.0, p_gain=1e-4):
+    """The same run differentiated forward: (pressure tensor [6], d tensor /
+    d(kn, kt, gamman, gammat, xmu, radius direction) [6 x 6], builds)."""
+    n, x, v, om, r, m, tag, lo, hi, p = _tlm_args(w, coeffs, style, skin, dt, p_target, p_gain)
+    t, dt6 = np.zeros(6), np.zeros((6, 6))
+    b = np.zeros(1, dtype=np.int32)
+    dr = None if dradius is None else np.ascontiguousarray(np.asarray(dradius, dtype=np.float64))
+    rc = lib().orc_tlm_tangent(n, _dp(x), _dp(v), _dp(om), _dp(r), _dp(m), _ip(tag), _dp(lo),
+                               _dp(hi), C.byref(p), int(nsteps),
+                               _dp(dr) if dr is not None else None, _dp(t), _dp(dt6), _ip(b))
+    if rc:
+        raise RuntimeError("the box collapsed")
+    return t, dt6, int(b[0])
 
 
 def num_threads(threaded: bool = True) -> int:

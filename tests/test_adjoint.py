@@ -201,3 +201,38 @@ def test_torch_function_chains_the_loss():
     gc, gr, _ = ADJ.run_backward(sim, MEAN_P)
     assert np.allclose(coeffs.grad.numpy(), 2.0 * (p - target) * gc, rtol=1e-12)
     assert np.allclose(radius.grad.numpy(), 2.0 * (p - target) * gr, rtol=1e-12)
+
+
+@pytest.mark.parametrize("style,xmu", [("hertz/history", 0.5), ("hooke/history", 0.5)])
+def test_adjoint_equals_the_tangent_linear_run(style, xmu):
+    """A check the collision sequence cannot spoil: the tangent-linear
+    restatement (oracle/tlm.cpp) differentiates the same discrete run forward,
+    with the forward's branches (contact onsets, slips, rebuilds), and no step
+    size. On the adjoint case's recipe (16^3, 600 steps, rebuilds, contacts
+    forming and breaking) the device forward is bitwise the restatement's FP64
+    run, and the adjoint's d<P>/d(kn, kt, gamman, gammat, xmu) and its
+    derivative along a random radius direction equal the tangent-linear ones
+    to 1e-8 of the largest sensitivity and 1e-6 of each resolvable one."""
+    c = workloads.ISOTROPIC
+    w = workloads.case("adjoint_32k", dims=(16, 16, 16))
+    steps, every = 600, 100
+    build = ADJ.isotropic_builder(w, style=style, skin=c["skin"], dt=c["dt"],
+                                  p_target=c["p_target"], p_gain=c["p_gain"])
+    radius = np.asarray(w["radius"], dtype=np.float64).copy()
+    coeffs = np.array([c["kn"], c["kt"], c["gamman"], c["gammat"], xmu])
+    p_dev, sim, _ = device_pmean(build, coeffs, radius, steps, every)
+    builds = sim.neighbor.build_count
+    gc, gr, _ = ADJ.run_backward(sim, MEAN_P)
+    u = np.random.default_rng(9).normal(size=len(radius))
+    t, d, tl_builds = O.tlm_tangent(w, coeffs, steps, dradius=u, style=style, skin=c["skin"],
+                                    dt=c["dt"], p_target=c["p_target"], p_gain=c["p_gain"])
+    assert float(MEAN_P @ t) == p_dev and tl_builds == builds > 2
+    tl = MEAN_P @ d
+    s_adj = np.append(coeffs * gc, float(u @ gr))
+    s_tl = np.append(coeffs * tl[:5], tl[5])
+    big = np.abs(s_tl).max()
+    print(f"{style}: adjoint {s_adj}, tangent {s_tl}")
+    assert np.abs(s_adj - s_tl).max() <= 1e-8 * big, (s_adj, s_tl)
+    for k in range(6):
+        if abs(s_tl[k]) >= 1e-3 * big:
+            assert abs(s_adj[k] - s_tl[k]) <= 1e-6 * abs(s_tl[k]), (k, s_adj[k], s_tl[k])
