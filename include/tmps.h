@@ -239,6 +239,14 @@ int tmps_contact_atom(tmps_system* s, int* counts, long long* total);
  * n n^T) / Nc as [xx yy zz xy xz yz] (zeros without contacts), and Nc
  * (nullable) */
 int tmps_fabric(tmps_system* s, double phi[6], int* ncontacts);
+/* No counterpart in the reference: tmps_fabric that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same results as
+ * tmps_fabric, same kernels; their scratch (the block partials and the sum)
+ * is a fixed file-scope device array of 256 KiB + 64 B per GPU, held until
+ * the process ends, so it takes at most 1,048,576 atoms (TMPS_ERR_CAPACITY
+ * beyond). Calls that overlap on one GPU share that array and race. */
+int tmps_fabric_noalloc(tmps_system* s, double phi[6], int* ncontacts);
 /* ComputeStressAtom.compute / compute_macro, stress_atom.py:30-109: per-atom
  * stress -virial (minus m v_a v_b when keflag) into stress (n x 6,
  * nullable), and its sum over atoms into sum6 (nullable; compute_macro

@@ -54,6 +54,7 @@ SYMBOLS = (
     "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
     "tmps_wall_create", "tmps_wall_destroy", "tmps_wall_reset", "tmps_wall_post_force",
     "tmps_wall_get_history", "tmps_prepare_run", "tmps_set_stream", "tmps_get_stream",
+    "tmps_fabric_noalloc",
 )
 
 
@@ -173,6 +174,7 @@ def load(path: str | None = None):
                                            C.c_int]),
         "tmps_wall_get_history": (C.c_int, [vp, dp, ip, C.c_int]),
         "tmps_fabric": (C.c_int, [vp, dp, ip]),
+        "tmps_fabric_noalloc": (C.c_int, [vp, dp, ip]),
         "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
         "tmps_ke": (C.c_int, [vp, dp, dp]),
         "tmps_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double]),

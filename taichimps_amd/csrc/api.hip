@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1236,6 +1237,8 @@ void free_all(System& s) {
     s.hctl = nullptr;
     if (s.hstatus) (void)hipHostFree(s.hstatus);
     s.hstatus = nullptr;
+    if (s.hsum) (void)hipHostFree(s.hsum);
+    s.hsum = nullptr;
     for (int k = 0; k < System::kGraphSlots; ++k) {
         if (s.seg_event[k]) (void)hipEventDestroy(s.seg_event[k]);
         s.seg_event[k] = nullptr;
@@ -1294,6 +1297,7 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     rc = rc ? rc : dalloc(&s.ctl, 1);
     if (rc) return rc;
     HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&s.hsum, sizeof(double) * 8, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&s.hstatus, sizeof(int) * 4 * System::kGraphSlots,
                           hipHostMallocDefault));
     for (int k = 0; k < System::kGraphSlots; ++k)
@@ -1920,6 +1924,65 @@ int tmps_fabric(tmps_system* h, double phi[6], int* ncontacts) {
         // fabric.py:70-76: phi = sum * (1 / Nc)
         const double inv_nc = 1.0 / (double)nc;
         for (int k = 0; k < 6; ++k) phi[k] = sum[k] * inv_nc;
+    }
+    return 0;
+}
+
+// ---- fabric without allocation
+
+// The no-allocation fabric's scratch: the block partials (up to CLOSE_SPAN
+// blocks of 8 doubles, i.e. 1,048,576 atoms) and the 8-double sum. One array
+// per GPU, alive until the process ends; every call on a GPU uses it, so
+// calls that overlap on one GPU race.
+#define TMPS_NOALLOC_PARTIALS 4096
+static __device__ __align__(256) double scratch[TMPS_NOALLOC_PARTIALS * 8 + 8];
+
+namespace {
+// scratch's device address, per GPU (the array's name is no device address in
+// host code): taken once per device with that device current
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+int scratch_ptr(int device, double** out) {
+    if (device < 0 || device >= 64) return tmps_fail(TMPS_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != device) HIP_TRY(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return tmps_fail_hip(e, "hipGetSymbolAddress(scratch)", __LINE__);
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<double*>(g_scratch_ptr[device]);
+    return 0;
+}
+}  // namespace
+
+int tmps_fabric_noalloc(tmps_system* h, double phi[6], int* ncontacts) {
+    System& s = *sys(h);
+    for (int k = 0; k < 6; ++k) phi[k] = 0.0;
+    if (ncontacts) *ncontacts = 0;
+    if (s.n <= 0 || !s.has_built) return 0;
+    if (nblocks_for(s.n) > TMPS_NOALLOC_PARTIALS)
+        return tmps_fail(TMPS_ERR_CAPACITY,
+                         "tmps_fabric_noalloc takes at most 1,048,576 atoms (its fixed scratch "
+                         "holds 4,096 block partials); use tmps_fabric");
+    double* sc = nullptr;
+    int rc = scratch_ptr(s.device, &sc);
+    if (!rc) rc = split_positions_to_xr(s);
+    if (!rc) rc = launch_fabric_into(s, sc, sc + TMPS_NOALLOC_PARTIALS * 8);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s.hsum, sc + TMPS_NOALLOC_PARTIALS * 8, sizeof(double) * 7,
+                           hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    const int nc = (int)s.hsum[6];
+    if (ncontacts) *ncontacts = nc;
+    if (nc > 0) {
+        const double inv_nc = 1.0 / (double)nc;
+        for (int k = 0; k < 6; ++k) phi[k] = s.hsum[k] * inv_nc;
     }
     return 0;
 }

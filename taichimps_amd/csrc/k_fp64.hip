@@ -1817,6 +1817,18 @@ int launch_fabric(System& s, double* d_sum7) {
     return launch_tree_out(s, 7, nb, d_sum7);
 }
 
+int launch_fabric_into(System& s, double* partial, double* d_sum7) {
+    const int nb = std::max(nblocks_for(s.n), 1);
+    if (nb > CLOSE_SPAN) return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for one tree block");
+    hipLaunchKernelGGL(k_fabric_leaves, dim3(nb), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.nb_start,
+                       s.nb_j, s.ctl, s.n, partial);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_tree_level, dim3(1), dim3(CLOSE_THREADS), 0, s.stream, partial, nb,
+                       d_sum7);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6) {
     const int nb = std::max(nblocks_for(s.n), 1);
     const bool f64 = s.precision == TMPS_FP64;

@@ -228,6 +228,7 @@ struct System {
     uint64_t seg_key[kGraphSlots] = {0, 0, 0, 0};
     int seg_next = 0;
     int* hstatus = nullptr;  // pinned: [slot][timestep, trigger, abort, error]
+    double* hsum = nullptr;  // pinned: 8 doubles, a reduction result read back
     hipEvent_t seg_event[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
 
     int launch_mode = TMPS_LAUNCH_GRAPH;
@@ -394,6 +395,9 @@ int launch_final_tree(System& s, int nc, int nparts);
 // per-contact computes on the FP64 positions in xr (k_fp64.hip)
 int launch_contact_atom(System& s, int* d_count);
 int launch_fabric(System& s, double* d_sum7);
+// launch_fabric with the block partials in `partial` (nblocks_for(n) x 8
+// doubles, at most CLOSE_SPAN blocks) instead of the system's buffer
+int launch_fabric_into(System& s, double* partial, double* d_sum7);
 int launch_stress_atom(System& s, int keflag, double* d_stress, double* d_sum6);
 // fix wall/gran on an FP64 system; wh: per-atom shear history + touch flag
 int launch_wall_gran(System& s, double4* wh, const tmps_wall_params& p, double dt, double coord,
