@@ -1104,16 +1104,16 @@ int tmps_adjoint_backward(tmps_system* h, const double dL_dtensor[6], double gra
     }
     HIP_TRY(hipStreamSynchronize(s.stream));
     AdjCtl hc;
-    HIP_TRY(hipMemcpy(&hc, A.actl, sizeof(AdjCtl), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(s.stream, &hc, A.actl, sizeof(AdjCtl), hipMemcpyDeviceToHost));
     if (grad_coeffs)
         for (int k = 0; k < 5; ++k) grad_coeffs[k] = hc.grad[k];
-    if (grad_radius) HIP_TRY(hipMemcpy(grad_radius, A.rbar, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad_radius) HIP_TRY(copy_sync(s.stream, grad_radius, A.rbar, n * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<double4> tmp(grad_x || grad_v || grad_omega ? n : 0);
     const double4* srcs[3] = {A.Xb, A.Vb, A.Wb};
     double* dsts[3] = {grad_x, grad_v, grad_omega};
     for (int q = 0; q < 3; ++q) {
         if (!dsts[q]) continue;
-        HIP_TRY(hipMemcpy(tmp.data(), srcs[q], n * sizeof(double4), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, tmp.data(), srcs[q], n * sizeof(double4), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) {
             dsts[q][3 * i + 0] = tmp[i].x;
             dsts[q][3 * i + 1] = tmp[i].y;

@@ -54,13 +54,12 @@ int tmps_fail_hip(hipError_t e, const char* what, int line) {
 namespace tmps_api {
 
 template <typename T>
-int dalloc(T** p, size_t count) {
+int dalloc(hipStream_t st, T** p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
     HIP_TRY(hipMalloc((void**)p, count * sizeof(T)));
-    HIP_TRY(hipMemset(*p, 0, count * sizeof(T)));
-    // the null-stream memset is not ordered with the system's stream
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemsetAsync(*p, 0, count * sizeof(T), st));
+    HIP_TRY(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -72,7 +71,7 @@ void dfree(T*& p) {
 
 int ctl_pull(System& s) {
     HIP_TRY(hipStreamSynchronize(s.stream));
-    HIP_TRY(hipMemcpy(s.hctl, s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(s.stream, s.hctl, s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -86,7 +85,7 @@ int ctl_pull_view(System& s) {
 }
 
 int ctl_push(System& s) {
-    HIP_TRY(hipMemcpy(s.ctl, s.hctl, sizeof(Ctl), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s.stream, s.ctl, s.hctl, sizeof(Ctl), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -142,15 +141,15 @@ int ensure_cells(System& s, int ncell) {
     // slack, so that a box that grows a little under the servo does not
     // reallocate at a rebuild
     int cap = ncell + ncell / 4 + 1;
-    int rc = dalloc(&s.cell_count, cap);
-    if (!rc) rc = dalloc(&s.cell_start, cap);
-    if (!rc) rc = dalloc(&s.cell_fill, cap);
+    int rc = dalloc(s.stream, &s.cell_count, cap);
+    if (!rc) rc = dalloc(s.stream, &s.cell_start, cap);
+    if (!rc) rc = dalloc(s.stream, &s.cell_fill, cap);
     if (rc) return rc;
     s.cell_cap = cap;
     const int need_scan = (std::max(cap, s.max_atoms + 1) + 1023) / 1024 + 64;
     if (need_scan > s.scan_tmp_cap) {
         dfree(s.scan_tmp);
-        rc = dalloc(&s.scan_tmp, need_scan);
+        rc = dalloc(s.stream, &s.scan_tmp, need_scan);
         if (rc) return rc;
         s.scan_tmp_cap = need_scan;
     }
@@ -326,8 +325,8 @@ int history_reserve(System& s) {
         if (s.onb_hist) (void)hipFree(s.onb_hist);
         s.onb_hist = nullptr;
         const size_t cap = need + need / 4 + 1024;
-        int rc = dalloc(&s.onb_j, cap);
-        if (!rc) rc = dalloc(&s.onb_flag, cap);
+        int rc = dalloc(s.stream, &s.onb_j, cap);
+        if (!rc) rc = dalloc(s.stream, &s.onb_flag, cap);
         if (rc) return rc;
         HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
         s.onb_cap = cap;
@@ -622,7 +621,7 @@ int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cu
             s.prof_cnt[0] += 1;
         }
         int aborted = 0;
-        HIP_TRY(hipMemcpy(&aborted, &s.ctl->abort, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, &aborted, &s.ctl->abort, sizeof(int), hipMemcpyDeviceToHost));
         if (len > 0 && !aborted) {
             for (int k = 0; k < len; ++k) {
                 HIP_TRY(hipEventElapsedTime(&ms, se[3 * k], se[3 * k + 1]));
@@ -1039,11 +1038,11 @@ int set_field(System& s, int field, const void* host, int count) {
     HIP_TRY(hipStreamSynchronize(s.stream));
     const double* hd = static_cast<const double*>(host);
     if (field == TMPS_TAG) {
-        HIP_TRY(hipMemcpy(s.tag, host, sizeof(int) * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.tag, host, sizeof(int) * n, hipMemcpyHostToDevice));
         return 0;
     }
     if (field == TMPS_VIRIAL) {
-        HIP_TRY(hipMemcpy(s.vir, host, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.vir, host, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
         return 0;
     }
     if (field == TMPS_X0) return tmps_fail(TMPS_ERR_ARG, "x0 is read-only");
@@ -1060,7 +1059,7 @@ int set_field(System& s, int field, const void* host, int count) {
             case TMPS_RMASS: dst = s.vm; break;
             default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
         }
-        HIP_TRY(hipMemcpy(buf.data(), dst, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, buf.data(), dst, sizeof(double4) * n, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             if (field == TMPS_RADIUS) {
                 buf[i].w = hd[i];
@@ -1074,7 +1073,7 @@ int set_field(System& s, int field, const void* host, int count) {
                 buf[i].z = hd[3 * i + 2];
             }
         }
-        HIP_TRY(hipMemcpy(dst, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, dst, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
         return 0;
     }
     // FP32-split
@@ -1088,7 +1087,7 @@ int set_field(System& s, int field, const void* host, int count) {
             if (rc) return rc;
             HIP_TRY(hipStreamSynchronize(s.stream));
         }
-        HIP_TRY(hipMemcpy(s.xr, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.xr, buf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
         s.x64_valid = true;
         return 0;
     }
@@ -1103,7 +1102,7 @@ int set_field(System& s, int field, const void* host, int count) {
         case TMPS_RMASS: dst = s.vmf; break;
         default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
     }
-    HIP_TRY(hipMemcpy(buf.data(), dst, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(s.stream, buf.data(), dst, sizeof(float4) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) {
         if (field == TMPS_RADIUS) {
             buf[i].w = (float)hd[i];
@@ -1117,24 +1116,24 @@ int set_field(System& s, int field, const void* host, int count) {
             buf[i].z = (float)hd[3 * i + 2];
         }
     }
-    HIP_TRY(hipMemcpy(dst, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s.stream, dst, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     if (field == TMPS_RADIUS) {
         // the FP64 scratch positions carry the radius too
         std::vector<double4> xb(n);
-        HIP_TRY(hipMemcpy(xb.data(), s.xr, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, xb.data(), s.xr, sizeof(double4) * n, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) xb[i].w = hd[i];
-        HIP_TRY(hipMemcpy(s.xr, xb.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.xr, xb.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
     }
     if (field == TMPS_RADIUS || field == TMPS_RMASS) {
         // omega.w = 1 / (0.4 m r^2)
         std::vector<float4> om(n), vm(n);
-        HIP_TRY(hipMemcpy(om.data(), s.omf, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(vm.data(), s.vmf, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, om.data(), s.omf, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, vm.data(), s.vmf, sizeof(float4) * n, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             const double r = s.h_radius[i], m = vm[i].w;
             om[i].w = (m > 0 && r > 0) ? (float)(1.0 / (0.4 * m * r * r)) : 0.f;
         }
-        HIP_TRY(hipMemcpy(s.omf, om.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.omf, om.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -1146,11 +1145,11 @@ int get_field(System& s, int field, void* host, int count) {
     HIP_TRY(hipStreamSynchronize(s.stream));
     double* hd = static_cast<double*>(host);
     if (field == TMPS_TAG) {
-        HIP_TRY(hipMemcpy(host, s.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, host, s.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
         return 0;
     }
     if (field == TMPS_VIRIAL) {
-        HIP_TRY(hipMemcpy(host, s.vir, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, host, s.vir, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
         return 0;
     }
     if (field == TMPS_RADIUS) {
@@ -1175,7 +1174,7 @@ int get_field(System& s, int field, void* host, int count) {
             default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
         }
         std::vector<double4> buf(n);
-        HIP_TRY(hipMemcpy(buf.data(), src, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, buf.data(), src, sizeof(double4) * n, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             if (field == TMPS_RMASS) {
                 hd[i] = buf[i].w;
@@ -1197,7 +1196,7 @@ int get_field(System& s, int field, void* host, int count) {
         default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
     }
     std::vector<float4> buf(n);
-    HIP_TRY(hipMemcpy(buf.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(s.stream, buf.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) {
         if (field == TMPS_RMASS) {
             hd[i] = buf[i].w;
@@ -1257,44 +1256,44 @@ int create(System& s, int max_atoms, int max_nb, int precision) {
     s.stream = s.own_stream;
     const size_t na = (size_t)max_atoms;
     int rc = 0;
-    rc = rc ? rc : dalloc(&s.xr, na);
-    rc = rc ? rc : dalloc(&s.x0, na);
-    rc = rc ? rc : dalloc(&s.vir, 6 * na);
-    rc = rc ? rc : dalloc(&s.tag, na);
+    rc = rc ? rc : dalloc(s.stream, &s.xr, na);
+    rc = rc ? rc : dalloc(s.stream, &s.x0, na);
+    rc = rc ? rc : dalloc(s.stream, &s.vir, 6 * na);
+    rc = rc ? rc : dalloc(s.stream, &s.tag, na);
     if (precision == TMPS_FP64) {
-        rc = rc ? rc : dalloc(&s.vm, na);
-        rc = rc ? rc : dalloc(&s.om, na);
-        rc = rc ? rc : dalloc(&s.vm2, na);
-        rc = rc ? rc : dalloc(&s.om2, na);
-        rc = rc ? rc : dalloc(&s.xr2, na);
-        rc = rc ? rc : dalloc(&s.f, na);
-        rc = rc ? rc : dalloc(&s.tq, na);
+        rc = rc ? rc : dalloc(s.stream, &s.vm, na);
+        rc = rc ? rc : dalloc(s.stream, &s.om, na);
+        rc = rc ? rc : dalloc(s.stream, &s.vm2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.om2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.xr2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.f, na);
+        rc = rc ? rc : dalloc(s.stream, &s.tq, na);
     } else {
-        rc = rc ? rc : dalloc(&s.up, na);
-        rc = rc ? rc : dalloc(&s.vmf, na);
-        rc = rc ? rc : dalloc(&s.omf, na);
-        rc = rc ? rc : dalloc(&s.vmf2, na);
-        rc = rc ? rc : dalloc(&s.omf2, na);
-        rc = rc ? rc : dalloc(&s.up2, na);
-        rc = rc ? rc : dalloc(&s.ff, na);
-        rc = rc ? rc : dalloc(&s.tqf, na);
-        rc = rc ? rc : dalloc(&s.u0, na);
-        rc = rc ? rc : dalloc(&s.cell, na);
+        rc = rc ? rc : dalloc(s.stream, &s.up, na);
+        rc = rc ? rc : dalloc(s.stream, &s.vmf, na);
+        rc = rc ? rc : dalloc(s.stream, &s.omf, na);
+        rc = rc ? rc : dalloc(s.stream, &s.vmf2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.omf2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.up2, na);
+        rc = rc ? rc : dalloc(s.stream, &s.ff, na);
+        rc = rc ? rc : dalloc(s.stream, &s.tqf, na);
+        rc = rc ? rc : dalloc(s.stream, &s.u0, na);
+        rc = rc ? rc : dalloc(s.stream, &s.cell, na);
     }
-    rc = rc ? rc : dalloc(&s.nb_start, na + 1);
-    rc = rc ? rc : dalloc(&s.onb_start, na + 1);
-    rc = rc ? rc : dalloc(&s.cell_of, na);
-    rc = rc ? rc : dalloc(&s.cell_particles, na);
-    rc = rc ? rc : dalloc(&s.bin_x, na);
-    rc = rc ? rc : dalloc(&s.full_count, na + 1);
-    rc = rc ? rc : dalloc(&s.half_count, na + 1);
-    rc = rc ? rc : dalloc(&s.flags, 4);
+    rc = rc ? rc : dalloc(s.stream, &s.nb_start, na + 1);
+    rc = rc ? rc : dalloc(s.stream, &s.onb_start, na + 1);
+    rc = rc ? rc : dalloc(s.stream, &s.cell_of, na);
+    rc = rc ? rc : dalloc(s.stream, &s.cell_particles, na);
+    rc = rc ? rc : dalloc(s.stream, &s.bin_x, na);
+    rc = rc ? rc : dalloc(s.stream, &s.full_count, na + 1);
+    rc = rc ? rc : dalloc(s.stream, &s.half_count, na + 1);
+    rc = rc ? rc : dalloc(s.stream, &s.flags, 4);
     const int nparts = nblocks_for(std::max(max_atoms, 1)) + 1;
-    rc = rc ? rc : dalloc(&s.partial, (size_t)nparts * 8);
+    rc = rc ? rc : dalloc(s.stream, &s.partial, (size_t)nparts * 8);
     s.partial_cap = nparts;
     s.level_cap = nparts / 4096 + 1;
-    rc = rc ? rc : dalloc(&s.level, (size_t)s.level_cap * 8);
-    rc = rc ? rc : dalloc(&s.ctl, 1);
+    rc = rc ? rc : dalloc(s.stream, &s.level, (size_t)s.level_cap * 8);
+    rc = rc ? rc : dalloc(s.stream, &s.ctl, 1);
     if (rc) return rc;
     HIP_TRY(hipHostMalloc((void**)&s.hctl, sizeof(Ctl), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&s.hsum, sizeof(double) * 8, hipHostMallocDefault));
@@ -1510,18 +1509,18 @@ int tmps_neigh_get_pairs(tmps_system* h, int* pair_i, int* pair_j, int* partner,
     const int m = s.npairs;
     int *dpi = nullptr, *dpj = nullptr, *dpa = nullptr, *dst = nullptr;
     double* dsh = nullptr;
-    int rc = dalloc(&dpi, m);
-    if (!rc) rc = dalloc(&dpj, m);
-    if (!rc) rc = dalloc(&dpa, m);
-    if (!rc) rc = dalloc(&dsh, 3 * (size_t)m);
-    if (!rc) rc = dalloc(&dst, s.n + 1);
+    int rc = dalloc(s.stream, &dpi, m);
+    if (!rc) rc = dalloc(s.stream, &dpj, m);
+    if (!rc) rc = dalloc(s.stream, &dpa, m);
+    if (!rc) rc = dalloc(s.stream, &dsh, 3 * (size_t)m);
+    if (!rc) rc = dalloc(s.stream, &dst, s.n + 1);
     if (!rc) rc = launch_half_pairs(s, dpi, dpj, dpa, dsh, dst);
     if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
     if (!rc) {
-        if (pair_i) (void)hipMemcpy(pair_i, dpi, sizeof(int) * m, hipMemcpyDeviceToHost);
-        if (pair_j) (void)hipMemcpy(pair_j, dpj, sizeof(int) * m, hipMemcpyDeviceToHost);
-        if (partner) (void)hipMemcpy(partner, dpa, sizeof(int) * m, hipMemcpyDeviceToHost);
-        if (shear) (void)hipMemcpy(shear, dsh, sizeof(double) * 3 * m, hipMemcpyDeviceToHost);
+        if (pair_i) (void)copy_sync(s.stream, pair_i, dpi, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (pair_j) (void)copy_sync(s.stream, pair_j, dpj, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (partner) (void)copy_sync(s.stream, partner, dpa, sizeof(int) * m, hipMemcpyDeviceToHost);
+        if (shear) (void)copy_sync(s.stream, shear, dsh, sizeof(double) * 3 * m, hipMemcpyDeviceToHost);
     }
     dfree(dpi); dfree(dpj); dfree(dpa); dfree(dsh); dfree(dst);
     return rc;
@@ -1532,10 +1531,10 @@ int tmps_neigh_get_counts(tmps_system* h, int* num_neighbors, int n) {
     if (n > s.n) n = s.n;
     if (n <= 0) return 0;
     int* dst = nullptr;
-    int rc = dalloc(&dst, s.n + 1);
+    int rc = dalloc(s.stream, &dst, s.n + 1);
     if (!rc) rc = launch_half_pairs(s, nullptr, nullptr, nullptr, nullptr, dst);
     if (!rc && hipStreamSynchronize(s.stream) != hipSuccess) rc = TMPS_ERR_HIP;
-    if (!rc && hipMemcpy(num_neighbors, s.half_count, sizeof(int) * n, hipMemcpyDeviceToHost) !=
+    if (!rc && copy_sync(s.stream, num_neighbors, s.half_count, sizeof(int) * n, hipMemcpyDeviceToHost) !=
                    hipSuccess)
         rc = TMPS_ERR_HIP;
     dfree(dst);
@@ -1559,12 +1558,12 @@ int tmps_neigh_get_bins(tmps_system* h, int* cell_of, int* cell_start, int* cell
     HIP_TRY(hipStreamSynchronize(s.stream));
     const GridP& G = ref;
     if (s.n > 0 && cell_of)
-        HIP_TRY(hipMemcpy(cell_of, s.cell_of, sizeof(int) * s.n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, cell_of, s.cell_of, sizeof(int) * s.n, hipMemcpyDeviceToHost));
     if (cell_start)
-        HIP_TRY(hipMemcpy(cell_start, s.cell_start, sizeof(int) * (G.ncell + 1),
+        HIP_TRY(copy_sync(s.stream, cell_start, s.cell_start, sizeof(int) * (G.ncell + 1),
                           hipMemcpyDeviceToHost));
     if (s.n > 0 && cell_particles)
-        HIP_TRY(hipMemcpy(cell_particles, s.cell_particles, sizeof(int) * s.n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, cell_particles, s.cell_particles, sizeof(int) * s.n, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1585,12 +1584,12 @@ int tmps_history_set_pairs(tmps_system* h, const int* partner, const double* she
     if (npairs == 0) return 0;
     int *dpa = nullptr, *dst = nullptr;
     double* dsh = nullptr;
-    int rc = dalloc(&dpa, npairs);
-    if (!rc) rc = dalloc(&dsh, 3 * (size_t)npairs);
-    if (!rc) rc = dalloc(&dst, s.n + 1);
-    if (!rc && hipMemcpy(dpa, partner, sizeof(int) * npairs, hipMemcpyHostToDevice) != hipSuccess)
+    int rc = dalloc(s.stream, &dpa, npairs);
+    if (!rc) rc = dalloc(s.stream, &dsh, 3 * (size_t)npairs);
+    if (!rc) rc = dalloc(s.stream, &dst, s.n + 1);
+    if (!rc && copy_sync(s.stream, dpa, partner, sizeof(int) * npairs, hipMemcpyHostToDevice) != hipSuccess)
         rc = TMPS_ERR_HIP;
-    if (!rc && hipMemcpy(dsh, shear, sizeof(double) * 3 * npairs, hipMemcpyHostToDevice) != hipSuccess)
+    if (!rc && copy_sync(s.stream, dsh, shear, sizeof(double) * 3 * npairs, hipMemcpyHostToDevice) != hipSuccess)
         rc = TMPS_ERR_HIP;
     if (!rc) rc = launch_half_pairs(s, nullptr, nullptr, nullptr, nullptr, dst);
     if (!rc) rc = launch_set_half_history(s, dpa, dsh, dst);
@@ -1627,29 +1626,29 @@ int tmps_history_seed(tmps_system* h, int n, const int* start, const int* nbr,
         if (s.onb_hist) (void)hipFree(s.onb_hist);
         s.onb_hist = nullptr;
         const size_t cap = total + total / 4 + 1024;
-        int rc = dalloc(&s.onb_j, cap);
-        if (!rc) rc = dalloc(&s.onb_flag, cap);
+        int rc = dalloc(s.stream, &s.onb_j, cap);
+        if (!rc) rc = dalloc(s.stream, &s.onb_flag, cap);
         if (rc) return rc;
         HIP_TRY(hipMalloc(&s.onb_hist, cap * hsz));
         s.onb_cap = cap;
     }
     HIP_TRY(hipStreamSynchronize(s.stream));
-    HIP_TRY(hipMemcpy(s.onb_start, start, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(s.stream, s.onb_start, start, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
     if (total > 0) {
-        HIP_TRY(hipMemcpy(s.onb_j, nbr, sizeof(int) * total, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.onb_j, nbr, sizeof(int) * total, hipMemcpyHostToDevice));
         std::vector<uint8_t> ones(total, 1);
-        HIP_TRY(hipMemcpy(s.onb_flag, ones.data(), total, hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(s.stream, s.onb_flag, ones.data(), total, hipMemcpyHostToDevice));
         if (split(s)) {
             std::vector<float4> hv(total);
             for (size_t k = 0; k < total; ++k)
                 hv[k] = make_float4((float)shear[3 * k], (float)shear[3 * k + 1],
                                     (float)shear[3 * k + 2], 0.0f);
-            HIP_TRY(hipMemcpy(s.onb_hist, hv.data(), total * sizeof(float4), hipMemcpyHostToDevice));
+            HIP_TRY(copy_sync(s.stream, s.onb_hist, hv.data(), total * sizeof(float4), hipMemcpyHostToDevice));
         } else {
             std::vector<double4> hv(total);
             for (size_t k = 0; k < total; ++k)
                 hv[k] = make_double4(shear[3 * k], shear[3 * k + 1], shear[3 * k + 2], 0.0);
-            HIP_TRY(hipMemcpy(s.onb_hist, hv.data(), total * sizeof(double4),
+            HIP_TRY(copy_sync(s.stream, s.onb_hist, hv.data(), total * sizeof(double4),
                               hipMemcpyHostToDevice));
         }
     }
@@ -1804,7 +1803,7 @@ int tmps_wall_create(tmps_system* h, const tmps_wall_params* p, tmps_wall** out)
     w->sys = &s;
     w->p = *p;
     w->cap = s.max_atoms;
-    int rc = dalloc(&w->hist, (size_t)std::max(s.max_atoms, 1));
+    int rc = dalloc(s.stream, &w->hist, (size_t)std::max(s.max_atoms, 1));
     if (rc) {
         delete w;
         return rc;
@@ -1847,7 +1846,7 @@ int tmps_wall_get_history(tmps_wall* w, double* shear, int* touch, int n) {
     std::vector<double4> hv((size_t)std::max(n, 1));
     HIP_TRY(hipStreamSynchronize(s.stream));
     if (n > 0)
-        HIP_TRY(hipMemcpy(hv.data(), w->hist, sizeof(double4) * (size_t)n, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(s.stream, hv.data(), w->hist, sizeof(double4) * (size_t)n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) {
         if (shear) {
             shear[3 * i] = hv[i].x;
@@ -1866,7 +1865,7 @@ int tmps_wall_get_history(tmps_wall* w, double* shear, int* touch, int n) {
 // first build every row is empty, as the reference's zeroed num_neighbors.
 static int computes_prepare(System& s) {
     if (!s.cmp_sum) {
-        int rc = dalloc(&s.cmp_sum, 8);
+        int rc = dalloc(s.stream, &s.cmp_sum, 8);
         if (rc) return rc;
     }
     return split_positions_to_xr(s);
@@ -1890,7 +1889,7 @@ int tmps_contact_atom(tmps_system* h, int* counts, long long* total) {
     if (rc) return rc;
     if (s.cmp_count_cap < s.n) {
         dfree(s.cmp_count);
-        rc = dalloc(&s.cmp_count, (size_t)s.n);
+        rc = dalloc(s.stream, &s.cmp_count, (size_t)s.n);
         if (rc) return rc;
         s.cmp_count_cap = s.n;
     }
@@ -1996,7 +1995,7 @@ int tmps_stress_atom(tmps_system* h, int keflag, double* stress, double sum6[6])
     if (rc) return rc;
     if (s.cmp_stress_cap < s.n) {
         dfree(s.cmp_stress);
-        rc = dalloc(&s.cmp_stress, 6 * (size_t)s.n);
+        rc = dalloc(s.stream, &s.cmp_stress, 6 * (size_t)s.n);
         if (rc) return rc;
         s.cmp_stress_cap = s.n;
     }

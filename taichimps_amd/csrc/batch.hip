@@ -202,7 +202,7 @@ int write_close_table(Batch& B) {
         ce[k].natoms = s.n;
         ce[k].pad = 0;
     }
-    HIP_TRY(hipMemcpy(B.d_close, ce.data(), ninst * sizeof(CloseEntry), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(B.stream, B.d_close, ce.data(), ninst * sizeof(CloseEntry), hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -294,6 +294,16 @@ void adjoint_free(System& s);
 int tmps_fail_hip(hipError_t e, const char* what, int line);
 int tmps_fail(int code, const std::string& msg);
 
+// A copy ordered on the given stream and complete on return. Host code never
+// uses the null stream: HIP refuses its synchronous copies and fills
+// (hipErrorStreamCaptureImplicit) while any thread of the process captures a
+// graph, and systems driven from several host threads capture at any time.
+inline hipError_t copy_sync(hipStream_t st, void* dst, const void* src, size_t bytes,
+                            hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+
 inline int nblocks_for(int n) { return (n + TMPS_BLOCK - 1) / TMPS_BLOCK; }
 
 // rebuild (k_rebuild.hip)

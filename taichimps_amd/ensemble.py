@@ -132,19 +132,33 @@ class Ensemble:
 
 
 def gradients(builders: Sequence[Callable], coeffs: Sequence, radii: Sequence, nsteps: int,
-              every: int, dL_dtensor=None) -> list:
+              every: int, dL_dtensor=None, workers: int = 16) -> list:
     """Per-member gradients of the final pressure: for member k, builder k
     (coeffs k [kn, kt, gamman, gammat, xmu], radii k) builds its FP64 twin,
     whose run the adjoint engine reverses. dL_dtensor defaults to the mean
     pressure (1/3 on the diagonal). Returns a list of
-    (final pressure tensor, dL/dcoeffs [5], dL/dradius [n])."""
+    (final pressure tensor, dL/dcoeffs [5], dL/dradius [n]).
+
+    Members are independent systems on streams of their own, so `workers`
+    host threads drive their forward and backward passes at once (the
+    library calls release the interpreter lock) and the GPU runs several
+    members' kernels together; each member's result does not depend on it."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from . import adjoint as ADJ
 
     g = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0]) / 3.0 if dL_dtensor is None else dL_dtensor
-    out = []
-    for build, c, r in zip(builders, coeffs, radii):
+
+    def one(item):
+        build, c, r = item
         sim = build(np.asarray(c, dtype=np.float64), np.asarray(r, dtype=np.float64))
         tensor, _ = ADJ.run_forward(sim, nsteps, every)
         gc, gr, _ = ADJ.run_backward(sim, g)
-        out.append((tensor, gc, gr))
-    return out
+        del sim
+        return tensor, gc, gr
+
+    items = list(zip(builders, coeffs, radii))
+    if workers <= 1 or len(items) <= 1:
+        return [one(it) for it in items]
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(one, items))

@@ -150,3 +150,21 @@ def test_rejects_what_it_does_not_run():
         Ensemble([H.device_sim(w, precision="f32split"), b])
     with pytest.raises(ValueError):
         Ensemble([])
+
+
+def test_member_gradients_do_not_depend_on_the_thread_count():
+    """ensemble.gradients drives members' adjoints on host threads at once;
+    each member's pressure and gradients are bitwise those of a one-thread
+    pass."""
+    from taichimps_amd import adjoint as ADJ
+    from taichimps_amd.ensemble import gradients
+    ws = [workloads.lattice(side, side, side, 1.05, 200 + side, "uniform") for side in (6, 9, 7, 8)]
+    builders = [ADJ.isotropic_builder(w, style="hertz/history" if k % 2 else "hooke/history",
+                                      skin=0.02) for k, w in enumerate(ws)]
+    coeffs = [np.array([2e5 * (1 + 0.1 * k), 2.0 / 7.0 * 2e5, 50.0, 25.0, 0.4]) for k in range(4)]
+    radii = [w["radius"] for w in ws]
+    one = gradients(builders, coeffs, radii, 40, 15, workers=1)
+    many = gradients(builders, coeffs, radii, 40, 15, workers=4)
+    for (t1, c1, r1), (t2, c2, r2) in zip(one, many):
+        assert np.array_equal(t1, t2) and np.array_equal(c1, c2) and np.array_equal(r1, r2)
+        assert np.all(np.isfinite(c1)) and np.abs(c1).max() > 0.0
