@@ -10,8 +10,8 @@ from .computes import ComputeContactAtom, ComputeFabric, ComputeStressAtom, Comp
 from .constants import f32split, f64
 from .contact_history import ContactHistory
 from .domain import Domain
-from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran,
-                        FixWallGranRegion)
+from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixPrint,
+                        FixWallGran, FixWallGranRegion)
 from .GRANULAR import GranHertzHistory, GranHookeHistory, GranularPair, PairGranular
 from .neighbor import NeighborList
 from .simulation import Simulation
@@ -19,6 +19,6 @@ from .simulation import Simulation
 __all__ = [
     "AtomSystem", "ComputeContactAtom", "ComputeFabric", "ComputeStressAtom", "Computes",
     "ContactHistory", "Domain", "FixDampingCundall",
-    "FixDeformPressure", "FixNVESphere", "FixWallGran", "FixWallGranRegion", "GranHertzHistory", "GranHookeHistory",
+    "FixDeformPressure", "FixNVESphere", "FixPrint", "FixWallGran", "FixWallGranRegion", "GranHertzHistory", "GranHookeHistory",
     "GranularPair", "NeighborList", "PairGranular", "Simulation", "f32split", "f64",
 ]

@@ -16,7 +16,8 @@ its fixes allow. The commands of the DEM path are understood:
     fix <id> <group> nve/sphere | damping/cundall gl ga |
         deform[/pressure] N <dim> <style> <args> ... [couple|remap|max/rate ...] |
         wall/gran hooke[/history] Kn Kt gn gt xmu damp xplane|yplane|zplane lo hi |
-        wall/gran[/region] hooke|hertz[/history] ... zcylinder|cylinder ...|region <cyl>
+        wall/gran[/region] hooke|hertz[/history] ... zcylinder|cylinder ...|region <cyl> |
+        print N "<text with ${var}>" [file <path>] [screen yes|no] [title <text>]
     unfix <id>
     compute <id> <group> contact/atom | pressure <temp|NULL> | temp | ke |
         erotate/sphere | stress/atom <temp|NULL> | fabric | reduce <mode> c_<id>[i] ...
@@ -47,8 +48,8 @@ from .constants import f64
 from .contact_history import ContactHistory
 from .data_reader import read_data
 from .domain import Domain
-from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixWallGran,
-                        FixWallGranRegion)
+from .EXTRA_FIX import (FixDampingCundall, FixDeformPressure, FixNVESphere, FixPrint,
+                        FixWallGran, FixWallGranRegion)
 from .GRANULAR import GranHertzHistory, GranHookeHistory, PairGranular
 from .neighbor import NeighborList
 from .simulation import Simulation
@@ -320,6 +321,7 @@ class LAMMPSInputParser:
         self.restart_every = 0
         self.restart_file: str | None = None
         self._seed = None
+        self._resolving: set[str] = set()
 
     # ------------------------------------------------------------- script
     def load_script(self) -> None:
@@ -343,7 +345,10 @@ class LAMMPSInputParser:
         def one(m):
             name = m.group(1) or m.group(2)
             if name not in self.variables and name not in self.variable_exprs:
-                raise ValueError(f"substitution of an undefined variable {name!r}")
+                kw = self._keyword(name)
+                if kw is None:
+                    raise ValueError(f"substitution of an undefined variable {name!r}")
+                return _fmt(kw)
             v = self.evaluate_variable(name)
             return _fmt(v)
         return re.sub(r"\$\{([A-Za-z0-9_]+)\}|\$([A-Za-z0-9_])", one, s)
@@ -355,6 +360,15 @@ class LAMMPSInputParser:
         toks = line.split(None, 1)
         if toks[0] == "print":  # printed text keeps its quotes' content whole
             return ["print", self._subst(toks[1].strip().strip("\"'")) if len(toks) > 1 else ""]
+        head = line.split()
+        if len(head) >= 4 and head[0] == "fix" and head[3] == "print":
+            # fix print's text is substituted when it prints, its other words
+            # now (input.py:788-794)
+            try:
+                words = shlex.split(line, posix=True)
+            except ValueError:
+                words = head
+            return [w if k == 5 else self._subst(w) for k, w in enumerate(words)]
         line = self._subst(line)
         try:
             return shlex.split(line, posix=True)
@@ -385,6 +399,11 @@ class LAMMPSInputParser:
         if isinstance(node, ast.Name):
             if node.id == "PI":
                 return math.pi
+            # a state keyword first, so `variable lx equal lx` reads the box
+            # (the reference refreshes these names before each evaluation)
+            kw = self._keyword(node.id)
+            if kw is not None:
+                return kw
             if node.id in self.variables or node.id in self.variable_exprs:
                 return self.evaluate_variable(node.id)
             raise ValueError(f"unknown name {node.id!r} in an expression")
@@ -400,11 +419,92 @@ class LAMMPSInputParser:
 
     def evaluate_variable(self, name: str) -> float | str:
         """An equal-style variable is evaluated now (its value is dynamic)."""
+        name = name.removeprefix("v_")
         if name in self.variable_exprs:
-            return self.evaluate_expression(self.variable_exprs[name])
+            if name in self._resolving:
+                raise ValueError(f"variable {name!r} refers to itself")
+            self._resolving.add(name)
+            try:
+                return self.evaluate_expression(self.variable_exprs[name])
+            finally:
+                self._resolving.discard(name)
         if name in self.variables:
             return self.variables[name]
+        kw = self._keyword(name)
+        if kw is not None:
+            return kw
         raise ValueError(f"undefined variable {name!r}")
+
+    def _keyword(self, name: str) -> float | None:
+        """The state values an expression may name (input.py:158-198): step,
+        lstep, dt, time, lx ly lz (xlength ...), vol, voidratio, ke (translational
+        plus rotational), ke_trans, ke_rot, atoms, and e once Vs is stored.
+        None for other names."""
+        sim, dom, atom = self.simulation, self.domain, self.atom
+        step = sim.timestep if sim is not None else self.start_timestep
+        if name in ("step", "lstep"):
+            return float(step)
+        if name == "dt":
+            return float(sim.dt if sim is not None else self.dt)
+        if name == "time":
+            return float(step) * float(sim.dt if sim is not None else self.dt)
+        if dom is None:
+            return None
+        axis = {"lx": 0, "ly": 1, "lz": 2, "xlength": 0, "ylength": 1, "zlength": 2}
+        if name in axis:
+            dom.pull()
+            return float(dom.prd[axis[name]])
+        if name == "vol":
+            dom.pull()
+            return float(dom.volume)
+        if name == "e":
+            # with Vs (the solid volume) stored, e is the current box's void
+            # ratio (input.py:216-221)
+            vs = self.variables.get("Vs")
+            if isinstance(vs, (int, float)) and not isinstance(vs, bool) and vs > 0.0:
+                dom.pull()
+                return (float(dom.volume) - float(vs)) / float(vs)
+            return None
+        if atom is None:
+            return None
+        if name == "atoms":
+            return float(atom.nlocal)
+        if name == "voidratio":
+            r = atom.radius.to_numpy()[: atom.nlocal]
+            solid = float(np.sum(4.0 / 3.0 * np.pi * r ** 3))
+            dom.pull()
+            return (float(dom.volume) - solid) / solid if solid > 0.0 else 0.0
+        if name in ("ke", "ke_trans", "ke_rot"):
+            if atom.nlocal == 0:
+                return 0.0
+            kt, kr = Computes()._ke(atom)
+            return float({"ke": kt + kr, "ke_trans": kt, "ke_rot": kr}[name])
+        return None
+
+    def evaluate_string_template(self, template_str: str) -> str:
+        """fix print's text with ${name} and $name replaced (input.py:115-155):
+        an equal-style variable is evaluated now and written with %.15g, and
+        one that will not evaluate is an error naming it; a stored number is
+        written with %.15g, a string as it is, an unknown name as nothing."""
+        def one(m: re.Match) -> str:
+            name = m.group(1)
+            if name in self.variable_exprs:
+                try:
+                    v = self.evaluate_variable(name)
+                except (ValueError, KeyError, ZeroDivisionError, TypeError) as exc:
+                    raise ValueError(f"variable {name!r} could not be evaluated: "
+                                     f"{self.variable_exprs[name]!r} ({exc})") from exc
+                return f"{float(v):.15g}" if not isinstance(v, str) else v
+            kw = self._keyword(name)
+            if kw is not None:
+                return f"{kw:.15g}"
+            v = self.variables.get(name, "")
+            if isinstance(v, (int, float)) and not isinstance(v, bool):
+                return f"{float(v):.15g}"
+            return str(v)
+
+        res = re.sub(r"\$\{([a-zA-Z0-9_]+)\}", one, template_str)
+        return re.sub(r"\$([a-zA-Z0-9_]+)", one, res)
 
     # ------------------------------------------------------------ computes
     def compute_value(self, cid: str, index: int = 0) -> float:
@@ -713,6 +813,26 @@ class LAMMPSInputParser:
                 else:
                     walls.append(FixWallGran(self.domain, max_atoms=self.atom.max_atoms, **kw))
             self._add_fix(fid, walls[0] if len(walls) == 1 else walls)
+        elif style == "print":
+            # fix <id> <group> print <N> <text> [file <path>] [screen yes|no]
+            # [title <text>] (input.py:1170-1204)
+            nevery, text = int(ev(args[0])), args[1]
+            path = title = None
+            screen = False
+            k = 2
+            while k < len(args):
+                if args[k] in ("file", "title", "screen") and k + 1 < len(args):
+                    if args[k] == "file":
+                        path = self.workdir / args[k + 1]
+                    elif args[k] == "title":
+                        title = args[k + 1]
+                    else:
+                        screen = args[k + 1].lower() == "yes"
+                    k += 2
+                else:
+                    k += 1
+            self._add_fix(fid, FixPrint(self.domain, nevery, text, self.evaluate_string_template,
+                                        filepath=path, title=title, screen=screen, fix_id=fid))
         else:
             raise NotImplementedError(f"fix {style} is not on the device DEM path")
 

@@ -156,6 +156,8 @@ class Simulation:
             return None
         cundall = deform = None
         for fix in self.fixes:
+            if getattr(fix, "output_only", False):
+                continue  # fix print: the run is cut at its multiples
             if type(fix) is FixDampingCundall and cundall is None:
                 cundall = fix
             elif type(fix) is FixDeformPressure and deform is None:
@@ -222,7 +224,7 @@ class Simulation:
         self.setup_run(steps)
         plan = self._device_plan()
         if plan is not None and self.atom.nlocal > 0:
-            if self.restart_every and self.restart_writer is not None:
+            if self._intervals():
                 self._run_device_chunked(steps, plan)
             else:
                 self._run_device(steps, *plan)
@@ -248,19 +250,36 @@ class Simulation:
         gl_ga = None if cundall is None else (cundall.gamma_lin, cundall.gamma_ang)
         self.atom._native.prepare_run(steps, self.dt, cundall=gl_ga)
 
+    def _output_fixes(self) -> list:
+        return [f for f in self.fixes if getattr(f, "output_only", False)]
+
+    def _intervals(self) -> list[int]:
+        """The I/O periods a device run is cut at: periodic restarts and
+        output-only fixes (fix print's N), as run_gpu collects them
+        (simulation.py:264-270)."""
+        out = [f.nevery for f in self._output_fixes() if f.nevery > 0]
+        if self.restart_every and self.restart_writer is not None:
+            out.append(self.restart_every)
+        return out
+
     def _run_device_chunked(self, steps: int, plan) -> None:
-        """The device run cut at the restart multiples (as run_gpu cuts at
-        I/O intervals, simulation.py:249-292): one fix setup for the whole
-        run, later pieces continuing without Verlet::setup, so the trajectory
-        is the unbroken run's."""
+        """The device run cut at the I/O multiples (as run_gpu cuts it,
+        simulation.py:249-292): one fix setup for the whole run, later pieces
+        continuing without Verlet::setup, so the trajectory is the unbroken
+        run's. At each cut the output fixes see the steps that passed (their
+        end_of_step comes inside the step, before the restart write)."""
         saved = self.run_pre
         remaining = steps
+        periods = self._intervals()
         try:
             while remaining > 0:
-                chunk = min(remaining, self.restart_every - self.timestep % self.restart_every)
+                chunk = min([remaining] + [p - self.timestep % p for p in periods])
                 self._run_device(chunk, *plan)
                 self.run_pre = False
                 remaining -= chunk
+                for fix in self._output_fixes():
+                    fix.timestep = self.timestep
+                    fix.advance(chunk)
                 if self._restart_due():
                     self.domain.pull()
                     self.write_restart_now()

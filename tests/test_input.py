@@ -186,6 +186,84 @@ def test_cli_without_a_script_prints_help(capsys):
     assert "LAMMPS" in capsys.readouterr().out
 
 
+def _print_parser(tmp_path, text):
+    from taichimps_amd.domain import Domain
+    p = _parser(tmp_path, text)
+    p.domain = Domain([0.0, 0.0, 0.0], [2.0, 3.0, 4.0], ("p", "p", "p"))
+    p.execute()
+    return p
+
+
+def test_fix_print_keeps_its_text_and_reads_its_options(tmp_path):
+    """The text is substituted when it prints, the other words when read
+    (input.py:788-794, 1170-1204); a file gets the default header, a title,
+    or none for a blank title (print.py:39-47)."""
+    p = _print_parser(tmp_path, """
+variable n equal 5
+variable w equal 2*v_n
+fix 1 all print ${n} "step ${step} w $w lx ${lx}" file out1.txt screen no
+fix 2 all print 3 "a ${n}" file out2.txt title "# step a"
+fix 3 all print 7 "b" file out3.txt title ""
+fix 4 all print 2 "c" screen yes
+""")
+    f1, f2, f3, f4 = (p.fixes[k] for k in "1234")
+    assert isinstance(f1, I.FixPrint) and f1.nevery == 5
+    assert f1.template_str == "step ${step} w $w lx ${lx}"
+    assert f1.filepath == tmp_path / "out1.txt" and not f1.screen
+    assert f4.screen and f4.filepath is None
+    for f in (f1, f2, f3):
+        f.close()
+    assert (tmp_path / "out1.txt").read_text() == "# Fix print output for fix 1\n"
+    assert (tmp_path / "out2.txt").read_text() == "# step a\n"
+    assert (tmp_path / "out3.txt").read_text() == ""
+
+
+def test_fix_print_writes_on_absolute_multiples(tmp_path):
+    """A multiple of N on the absolute timestep (print.py:49-66): a run
+    resumed at step 1000003 with N = 4 prints at 1000004, 1000008."""
+    out = []
+    f = I.FixPrint(None, 4, "t", lambda t: out.append(t) or f"row {len(out)}",
+                   filepath=tmp_path / "o.txt", fix_id="p")
+    for step in range(1000003, 1000010):
+        f.timestep = step
+        f.end_of_step(None, 1e-4)
+    # the device run's cut: 3 steps that end on a multiple print once
+    f.timestep = 1000012
+    f.advance(3)
+    f.close()
+    assert f.step_count == 10 and f.lines == ["row 1", "row 2", "row 3"]
+    assert (tmp_path / "o.txt").read_text().splitlines() == [
+        "# Fix print output for fix p", "row 1", "row 2", "row 3"]
+
+
+def test_string_template_substitution(tmp_path):
+    """${name} and $name (several characters, as the reference reads them),
+    equal-style variables evaluated now with %.15g, stored strings as they
+    are, unknown names as nothing, state keywords (input.py:115-198)."""
+    p = _print_parser(tmp_path, """
+variable n equal 1000000
+variable third equal 1/3
+variable tag string hello
+variable s equal 2*step+lx
+variable bad equal 1/0
+variable Vs equal 6.0
+""")
+    p.start_timestep = 7
+    ev = p.evaluate_string_template
+    assert ev("${n} $n ${third}") == "1000000 1000000 0.333333333333333"
+    assert ev("$tag-${tag}") == "hello-hello"
+    assert ev("[${nope}]") == "[]"
+    assert ev("${step} ${lx} ${ly} ${vol} ${s}") == "7 2 3 24 16"
+    assert ev("${time}") == f"{7 * p.dt:.15g}"
+    with pytest.raises(ValueError, match="bad"):
+        ev("${bad}")
+    # the void ratio from the stored solid volume (input.py:216-221)
+    assert ev("${e}") == f"{(24.0 - 6.0) / 6.0:.15g}"
+    # keywords are also names in expressions and in immediate ${} text
+    assert p.evaluate_expression("lx*ly*lz") == 24.0
+    assert p.evaluate_expression("${vol}/2") == 12.0
+
+
 # ------------------------------------------------------------------ GPU
 
 ISO = """
@@ -414,3 +492,100 @@ def test_periodic_restart_files_leave_the_run_unchanged(tmp_path):
         for a, b in zip(d.fix_extra, r.fix_extra):
             assert np.array_equal(a, b), step
     assert not (tmp_path / "snap.100.restart").exists()
+
+
+PRINT = """
+units           si
+atom_style      sphere
+boundary        p p p
+read_data       lattice.data
+pair_style      gran/hertz/history 2e5 NULL 50 NULL 0.5 1
+pair_coeff      * *
+neighbor        0.1 bin
+neigh_modify    delay 0 every 1 check yes
+timestep        1e-4
+fix             1 all nve/sphere
+fix             2 all deform/pressure 1 x pressure 1.0 1e-4 y pressure 1.0 1e-4 &
+                z pressure 1.0 1e-4 couple xyz
+compute         1 all pressure thermo_temp
+compute         cn all contact/atom
+compute         cnave all reduce ave c_cn
+variable        pxx equal c_1[1]
+variable        pyy equal c_1[2]
+variable        pzz equal c_1[3]
+variable        pxy equal c_1[4]
+variable        p equal (v_pxx+v_pyy+v_pzz)/3
+variable        lx equal lx
+variable        cn equal c_cnave
+variable        Vs equal {vs!r}
+variable        e equal (vol-v_Vs)/v_Vs
+variable        KEtra equal ke_trans
+variable        KErot equal ke_rot
+fix             out1 all print 20 "${{step}} ${{pxx}} ${{pyy}} ${{pzz}} ${{pxy}} ${{p}}" &
+                file mean_stress.txt title "# step pxx pyy pzz pxy p"
+fix             out2 all print 25 "${{step}} ${{lx}} ${{ly}} ${{lz}} ${{e}} ${{cn}}" file box.txt
+fix             out3 all print 50 "${{step}} ${{KEtra}} ${{KErot}}" file energy.txt screen no
+run             100
+"""
+
+
+def _rows(path):
+    return [line.split() for line in path.read_text().splitlines() if not line.startswith("#")]
+
+
+@pytest.mark.gpu
+def test_fix_print_output_equals_the_oracle_run(tmp_path):
+    """A script with three fix print outputs (periods 20, 25, 50) runs
+    device-resident, cut at the print steps, and every printed value is the
+    oracle's at that step: pressure tensor, mean pressure, box lengths, void
+    ratio and mean contact number as the same %.15g text (the FP64 path is
+    bitwise), kinetic energies within 1e-12 (the oracle's are numpy sums, the
+    device's a pairwise tree)."""
+    import simhelpers as H
+    from taichimps_amd import workloads
+    w = workloads.lattice(8, 8, 8, 1.1, 1001, "uniform")
+    _write_data(tmp_path / "lattice.data", w)
+    vs = float(np.sum(4.0 / 3.0 * np.pi * w["radius"] ** 3))
+    (tmp_path / "in.print").write_text(PRINT.format(vs=vs))
+    p = I.LAMMPSInputParser(tmp_path / "in.print")
+    p.execute()
+    assert p.simulation.timestep == 100 and p.simulation.last_run_device_resident
+
+    o = H.oracle_sim(w, style="hertz/history", kt=2e5 * 2.0 / 7.0, gammat=0.5 * 50.0)
+    dt = 1e-4
+    o.servo_setup(100, dt)
+    assert o.init_simulation(dt) == 0
+    m = 4.0 / 3.0 * np.pi * w["radius"] ** 3 * w["density"]
+    want = {"mean_stress.txt": [], "box.txt": [], "energy.txt": []}
+    g = (lambda v: f"{v:.15g}")
+    for step in range(1, 101):
+        assert o.step(dt) == 0
+        lo, hi, _ = o.box()
+        prd = hi - lo
+        vol = float(prd[0] * prd[1] * prd[2])
+        if step % 20 == 0:
+            t = o.virial(keflag=True) / vol
+            want["mean_stress.txt"].append([str(step)] + [g(v) for v in t[:4]]
+                                           + [g((t[0] + t[1] + t[2]) / 3)])
+        if step % 25 == 0:
+            cn = float(np.mean(o.contact_atom().astype(np.float64)))
+            want["box.txt"].append([str(step)] + [g(v) for v in prd]
+                                   + [g((vol - vs) / vs), g(cn)])
+        if step % 50 == 0:
+            a = o.atoms()
+            ket = 0.5 * np.sum(m * np.sum(a["v"] ** 2, axis=1))
+            ker = 0.5 * np.sum(0.4 * m * w["radius"] ** 2 * np.sum(a["omega"] ** 2, axis=1))
+            want["energy.txt"].append([step, ket, ker])
+    assert (tmp_path / "mean_stress.txt").read_text().startswith("# step pxx pyy pzz pxy p\n")
+    assert (tmp_path / "box.txt").read_text().startswith("# Fix print output for fix out2\n")
+    assert _rows(tmp_path / "mean_stress.txt") == want["mean_stress.txt"]
+    assert _rows(tmp_path / "box.txt") == want["box.txt"]
+    got = _rows(tmp_path / "energy.txt")
+    assert [int(r[0]) for r in got] == [50, 100]
+    for r, (step, ket, ker) in zip(got, want["energy.txt"]):
+        assert float(r[1]) == pytest.approx(ket, rel=1e-12)
+        assert float(r[2]) == pytest.approx(ker, rel=1e-12)
+    # the cuts leave the trajectory the oracle's
+    dev, orc = H.device_state(p.simulation), H.oracle_state(o)
+    for key in ("x", "v", "omega", "boxlo", "boxhi"):
+        assert np.array_equal(dev[key], orc[key]), key
