@@ -306,36 +306,50 @@ __device__ __forceinline__ void block_tree_sum(double leaf[NC], double* out) {
     __syncthreads();
 }
 
-// The same sums as block_tree_sum, with each wavefront's levels formed in LDS
-// instead of by DPP moves: for many components this issues far fewer vector
-// instructions. Entry p of a level is the sum of entries 2p and 2p + 1 of the
-// level before (written in place: a round of the loop covers whole
-// components, and a wavefront's reads precede its writes), so every addition
-// is the one the pairwise tree makes.
-template <int NC>
-__device__ __forceinline__ void block_tree_sum_lds(double leaf[NC], double* out) {
-    __shared__ double buf[TMPS_BLOCK / 64][NC * 64];
-    __shared__ double wsum[TMPS_BLOCK / 64][NC];
+// Lane k < 8 of the calling wavefront receives component k of the pairwise
+// tree over its 64 lanes of x (8 components a lane), formed in registers:
+// three reduce-scatter levels (lanes at distance 1, 2, 4 each add the half
+// of their components they keep, so every lane ends with one component
+// summed over its aligned group of 8 lanes), then butterflies at distances
+// 8, 16, 32. Every addition joins two aligned sibling subtrees and
+// floating-point addition is commutative, so each node of the pairwise tree
+// gets the same bits as in a tree formed level by level. All 64 lanes must
+// call it.
+__device__ __forceinline__ double wave_sum8(const double x[8]) {
+    const int lane = threadIdx.x & 63;
+    const bool u1 = (lane & 1) != 0, u2 = (lane & 2) != 0, u4 = (lane & 4) != 0;
+    double a4[4], a2[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double send = u1 ? x[j] : x[4 + j];
+        const double keep = u1 ? x[4 + j] : x[j];
+        a4[j] = keep + __shfl_xor(send, 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double send = u2 ? a4[j] : a4[2 + j];
+        const double keep = u2 ? a4[2 + j] : a4[j];
+        a2[j] = keep + __shfl_xor(send, 2);
+    }
+    double a1 = (u4 ? a2[1] : a2[0]) + __shfl_xor(u4 ? a2[0] : a2[1], 4);
+    a1 = a1 + __shfl_xor(a1, 8);
+    a1 = a1 + __shfl_xor(a1, 16);
+    a1 = a1 + __shfl_xor(a1, 32);
+    // lane l holds component 4(l & 1) + (l & 2) + ((l >> 2) & 1) of its group
+    const int k = lane & 7;
+    return __shfl(a1, ((k >> 2) & 1) | (k & 2) | ((k & 1) << 2));
+}
+
+// The same tree over a block's 256 threads: thread k < 8 receives component
+// k, the wavefronts' subtrees joined as (w0 + w1) + (w2 + w3).
+__device__ __forceinline__ double block_sum8(const double leaf[8]) {
+    __shared__ double ws[TMPS_BLOCK / 64][8];
+    const double a = wave_sum8(leaf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* b = buf[wave];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) b[k * 64 + lane] = leaf[k];
-#pragma unroll
-    for (int size = 32; size >= 1; size >>= 1) {
-        for (int q = lane; q < NC * size; q += 64) {
-            const int k = q / size, p = q - k * size;
-            const double x = b[k * 64 + 2 * p], y = b[k * 64 + 2 * p + 1];
-            b[k * 64 + p] = x + y;
-        }
-    }
-    if (lane < NC) wsum[wave][lane] = b[lane * 64];
+    if (lane < 8) ws[wave][lane] = a;
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-            out[k] = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
-    }
-    __syncthreads();
+    const int k = threadIdx.x & 7;
+    return (ws[0][k] + ws[1][k]) + (ws[2][k] + ws[3][k]);
 }
 
 }  // namespace tmps

@@ -373,13 +373,6 @@ inline int final_leaves(int nparts, int threads = FINAL_THREADS) {
 // Partials one close block reduces: CLOSE_BATCH rounds of CLOSE_THREADS.
 #define CLOSE_SPAN (CLOSE_THREADS * CLOSE_BATCH)
 
-// Orders LDS accesses of one wavefront: its lanes' earlier writes are visible
-// to its later reads.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // The tree of final_tree_L over 8 components for up to CLOSE_SPAN partials.
 // Wavefront w owns partials [w*64L, (w+1)*64L) and loads them in L coalesced
@@ -505,13 +498,11 @@ __global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const doub
     step_end_block<L>(ctl, partial, nparts, dt, advance, apply_servo, natoms);
 }
 
-// End of a fused step, in one block: the pairwise tree over the step pass's
-// block partials (publish_block), the rebuild request they carry, and the
-// servo. The control block is copied into LDS and written back whole, both
-// as one parallel round trip, so the serial servo arithmetic in between waits
-// on no global memory. A step whose atoms asked for a rebuild records the
-// trigger and the abort that skips the rest of its segment; an aborted
-// segment is left untouched.
+// End of a fused step, in one block, for passes that publish their block
+// partials (publish_block): the pairwise tree over them, then close_update.
+// The control block is copied into LDS and written back whole, both as one
+// parallel round trip, so the serial servo arithmetic in between waits on no
+// global memory. An aborted segment is left untouched.
 template <int L>
 __device__ __forceinline__ void step_close_block(Ctl* ctl, const double* partial, int nparts,
                                                  double dt, int servo_on, int natoms,
@@ -527,37 +518,7 @@ __device__ __forceinline__ void step_close_block(Ctl* ctl, const double* partial
     for (int k = threadIdx.x; k < NW; k += CLOSE_THREADS) lds[k] = src[k];
     double out[8];
     close_tree<L>(partial, nparts, out);  // ends with a barrier: cs is complete
-    if (threadIdx.x == 0) {
-        int wb = 0;
-        if (!cs.abort) {
-            wb = 1;
-            const int t = cs.timestep + 1;
-            if (out[7] != 0.0) {
-                // the skin check asked for a rebuild: the step's outputs are
-                // discarded and the host redoes it after the build
-                cs.trigger = t;
-                cs.abort = 1;
-            } else {
-                for (int k = 0; k < 6; ++k) cs.vsum[k] = out[k];
-                cs.ntouch = (int)out[6];
-                if (servo_on) {
-                    servo_update(&cs, out, dt, natoms);
-                    if (!cs.abort) update_split_consts(&cs, 0);
-                }
-                if (!cs.abort) {
-                    cs.timestep = t;
-                    if (member) {
-                        // an ensemble member: the next step reads the other
-                        // set; its run ends with step bend
-                        cs.bpar ^= 1;
-                        cs.bkick = 0;
-                        if (t == cs.bend) cs.bdone = 1;
-                    }
-                }
-            }
-        }
-        writeback = wb;
-    }
+    if (threadIdx.x == 0) writeback = close_update(cs, out, dt, servo_on, natoms, member) ? 1 : 0;
     __syncthreads();
     if (writeback) {
         unsigned* dst = reinterpret_cast<unsigned*>(ctl);
