@@ -277,7 +277,9 @@ int neigh_record(System& s, int total, int fast[3], int gi[3]) {
 
 // box_known: the host mirror of the control block is current (the caller
 // pulled it and nothing since changed the box), so no round trip is needed.
-int neigh_build(System& s, bool box_known = false) {
+// wrapped: the PBC wrap ran just before (reneighbor), so every position lies
+// inside the box on its periodic axes.
+int neigh_build(System& s, bool box_known = false, bool wrapped = false) {
     if (s.n <= 0) return 0;
     int rc = 0;
     if (split(s)) {
@@ -294,7 +296,7 @@ int neigh_build(System& s, bool box_known = false) {
     rc = launch_bins(s, s.xr);
     if (rc) return rc;
     int overflow = 0, total = 0;
-    rc = launch_list(s, s.xr, &overflow, &total);
+    rc = launch_list(s, s.xr, &overflow, &total, wrapped);
     if (rc) return rc;
     if (overflow) return list_overflow_error(s);
     rc = launch_hist_clear(s);
@@ -435,7 +437,7 @@ int reneighbor(System& s, bool force, int* rebuilt) {
         if (rc) return rc;
     }
     rc = pbc(s);
-    if (!rc) rc = neigh_build(s);
+    if (!rc) rc = neigh_build(s, false, true);
     if (!rc) rc = history_restore(s);
     if (rc) return rc;
     *rebuilt = 1;
@@ -751,7 +753,7 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     s.current_step = stp;
     rc = history_save(s);
     if (!rc) rc = pbc(s, true);
-    if (!rc) rc = neigh_build(s, true);
+    if (!rc) rc = neigh_build(s, true, true);
     if (!rc) rc = history_restore(s);
     if (!rc) rc = ensure_split(s);
     if (rc) return rc;

@@ -20,14 +20,6 @@
 
 namespace tmps {
 
-// Orders LDS accesses of one wavefront: its lanes' earlier writes are visible
-// to its later reads.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // MI355X hands workgroup w to XCD (w mod 8). Blocks of consecutive atoms share
 // neighbours (and so L2 lines), so each XCD is given one contiguous range of
 // blocks: a bijection of [0, nb) onto itself.

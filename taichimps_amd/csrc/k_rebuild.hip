@@ -335,15 +335,51 @@ __global__ void k_bin_gather(int n, const int* particles, const double4* xr, dou
     if (p < n) bx[p] = xr[particles[p]];
 }
 
-// One bin-order range [a, b) of candidates for atom i: the cutoff test of
-// k_list (same expressions, so the same accepted set), each accepted j
-// inserted in ascending order into the thread's LDS row (column of a
-// [LIST_ROW_CAP][TMPS_BLOCK] array). Candidates are read four at a time.
+// Candidate j at pj for atom i at pi: the cutoff test of k_list_fill, each
+// accepted j inserted in ascending order into the thread's LDS row (column
+// of a [LIST_ROW_CAP][TMPS_BLOCK] array). `img` selects the axes that take
+// the nearest image (bit k: axis k); the others use the plain difference.
+// On a periodic axis of at least 3 bins, a candidate in a bin reached
+// without wrapping differs from i by less than two bin widths: if that is
+// under half the box, the nearest image is the plain difference itself
+// (round gives 0, and d - prd * 0 is d); if not, both it and the nearest
+// image exceed a bin width, so the axis alone rules the pair out either way.
+// Only wrapped bins, and periodic axes of fewer than 3 bins, need the image,
+// and the accepted set is the one the image on every periodic axis gives.
+// That needs every position inside the box on its periodic axes (a build
+// after the PBC wrap); other builds take the image on every periodic axis.
+__device__ __forceinline__ void list_test(int i, int j, const double4& pi, const double4& pj,
+                                          int img, const double prd[3], const double inv[3],
+                                          const bool fast[3], double skin, int* row, int& nfull,
+                                          int& nhalf) {
+    if (j == i) return;
+    double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
+    if (img & 1) d0 = list_image(d0, prd[0], inv[0], fast[0]);
+    if (img & 2) d1 = list_image(d1, prd[1], inv[1], fast[1]);
+    if (img & 4) d2 = list_image(d2, prd[2], inv[2], fast[2]);
+    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    const double rad_sum = pi.w + pj.w + skin;
+    if (rsq < rad_sum * rad_sum) {
+        if (nfull < LIST_ROW_CAP) {
+            int q2 = nfull - 1;
+            while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
+                row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
+                --q2;
+            }
+            row[(q2 + 1) * TMPS_BLOCK] = j;
+        }
+        ++nfull;
+        if (j > i) ++nhalf;
+    }
+}
+
+// One bin-order range [a, b) of candidates for atom i, read four at a time,
+// each tested by list_test with image mask `img`.
 __device__ __forceinline__ void list_range(int a, int b, int i, const double4& pi,
-                                           const int* particles, const double4* bx,
-                                           const int per[3], const double prd[3],
-                                           const double inv[3], const bool fast[3], double skin,
-                                           int* row, int& nfull, int& nhalf) {
+                                           const int* particles, const double4* bx, int img,
+                                           const double prd[3], const double inv[3],
+                                           const bool fast[3], double skin, int* row, int& nfull,
+                                           int& nhalf) {
     for (int p0 = a; p0 < b; p0 += 4) {
         int jq[4];
         double4 xq[4];
@@ -354,29 +390,9 @@ __device__ __forceinline__ void list_range(int a, int b, int i, const double4& p
             xq[q] = bx[p];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = jq[q];
-            if (p0 + q >= b || j == i) continue;
-            const double4 pj = xq[q];
-            double d0 = pi.x - pj.x, d1 = pi.y - pj.y, d2 = pi.z - pj.z;
-            if (per[0] == 1) d0 = list_image(d0, prd[0], inv[0], fast[0]);
-            if (per[1] == 1) d1 = list_image(d1, prd[1], inv[1], fast[1]);
-            if (per[2] == 1) d2 = list_image(d2, prd[2], inv[2], fast[2]);
-            const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
-            const double rad_sum = pi.w + pj.w + skin;
-            if (rsq < rad_sum * rad_sum) {
-                if (nfull < LIST_ROW_CAP) {
-                    int q2 = nfull - 1;
-                    while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
-                        row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
-                        --q2;
-                    }
-                    row[(q2 + 1) * TMPS_BLOCK] = j;
-                }
-                ++nfull;
-                if (j > i) ++nhalf;
-            }
-        }
+        for (int q = 0; q < 4; ++q)
+            if (p0 + q < b)
+                list_test(i, jq[q], pi, xq[q], img, prd, inv, fast, skin, row, nfull, nhalf);
     }
 }
 
@@ -399,7 +415,7 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
                                                const int* cell_start, const int* particles,
                                                const double4* bx, double skin, int max_nb,
                                                int* full_count, int* half_count, int* flags,
-                                               int* rows) {
+                                               int* rows, int wrapped) {
     const int i = particles[p];
     const DevBox& B = ctl->box;
     const double4 pi = bx[p];  // = xr[i]
@@ -423,24 +439,48 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
         x0 = max(x0, 0);
         x1 = min(x1, gx - 1);
     }
-    int nfull = 0, nhalf = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        if (per[2] == 1 && gz == 1 && dz != 0) continue;
-        int nz = c[2] + dz;
-        if (per[2] == 1) nz = nz < 0 ? nz + gz : (nz >= gz ? nz - gz : nz);
-        if (nz < 0 || nz >= gz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            if (per[1] == 1 && gy == 1 && dy != 0) continue;
-            int ny = c[1] + dy;
-            if (per[1] == 1) ny = ny < 0 ? ny + gy : (ny >= gy ? ny - gy : ny);
-            if (ny < 0 || ny >= gy) continue;
-            const int rb = ny * gx + nz * gxy;
-            list_range(cell_start[rb + x0], cell_start[rb + x1 + 1], i, pi, particles, bx, per,
-                       prd, inv, fast, skin, row, nfull, nhalf);
-            if (w >= 0)
-                list_range(cell_start[rb + w], cell_start[rb + w + 1], i, pi, particles, bx, per,
-                           prd, inv, fast, skin, row, nfull, nhalf);
+    // per axis: whether an unwrapped bin still needs the nearest image (every
+    // periodic axis where the positions may lie outside the box)
+    const int img_base = (per[0] == 1 && (gx < 3 || !wrapped) ? 1 : 0) |
+                         (per[1] == 1 && (gy < 3 || !wrapped) ? 2 : 0) |
+                         (per[2] == 1 && (gz < 3 || !wrapped) ? 4 : 0);
+    // The 9 (dy, dz) rows in the stencil's order: the bin-row base of each
+    // (-1: no such row) and which axes its bins reached by wrapping, then the
+    // main x-piece bounds of all of them, requested before any candidate is
+    // read, so a row costs one memory round trip instead of two.
+    int rbase[9], rimg[9], ra[9], rbnd[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int dz = r / 3 - 1, dy = r % 3 - 1;
+        rimg[r] = img_base;
+        bool skip = (per[2] == 1 && gz == 1 && dz != 0) || (per[1] == 1 && gy == 1 && dy != 0);
+        int nz = c[2] + dz, ny = c[1] + dy;
+        if (per[2] == 1 && (nz < 0 || nz >= gz)) {
+            nz = nz < 0 ? nz + gz : nz - gz;
+            rimg[r] |= 4;
         }
+        if (per[1] == 1 && (ny < 0 || ny >= gy)) {
+            ny = ny < 0 ? ny + gy : ny - gy;
+            rimg[r] |= 2;
+        }
+        if (nz < 0 || nz >= gz || ny < 0 || ny >= gy) skip = true;
+        rbase[r] = skip ? -1 : ny * gx + nz * gxy;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        ra[r] = rbase[r] < 0 ? 0 : cell_start[rbase[r] + x0];
+        rbnd[r] = rbase[r] < 0 ? 0 : cell_start[rbase[r] + x1 + 1];
+    }
+    int nfull = 0, nhalf = 0;
+    const int ximg = per[0] == 1 ? 1 : 0;
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        if (rbase[r] < 0) continue;
+        list_range(ra[r], rbnd[r], i, pi, particles, bx, rimg[r], prd, inv, fast, skin, row,
+                   nfull, nhalf);
+        if (w >= 0)
+            list_range(cell_start[rbase[r] + w], cell_start[rbase[r] + w + 1], i, pi, particles,
+                       bx, rimg[r] | ximg, prd, inv, fast, skin, row, nfull, nhalf);
     }
     full_count[i] = nfull;
     half_count[i] = nhalf;
@@ -456,12 +496,13 @@ __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int
                                                           GridP G, const int* cell_start,
                                                           const int* particles, const double4* bx,
                                                           double skin, int max_nb, int* full_count,
-                                                          int* half_count, int* flags, int* rows) {
+                                                          int* half_count, int* flags, int* rows,
+                                                          int wrapped) {
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n)
         list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, G, cell_start, particles, bx, skin,
-                       max_nb, full_count, half_count, flags, rows);
+                       max_nb, full_count, half_count, flags, rows, wrapped);
 }
 
 // Rows from the slot-major scratch of k_list_rows into the CSR list.
@@ -692,9 +733,10 @@ __global__ void __launch_bounds__(TMPS_BLOCK) kb_list_rows(const RedoEntry* T, c
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
     REDO_ITEM(map)
     if (i < e.n)
+        // the batched redo wraps the positions (kb_pbc) before its build
         list_rows_item(i, lrow + threadIdx.x, e.xr, e.n, e.ctl, e.grid, e.cell_start,
                        e.cell_particles, e.bin_x, e.skin, e.max_nb, e.full_count, e.half_count,
-                       e.flags, e.row_tmp);
+                       e.flags, e.row_tmp, 1);
 }
 
 // every entry's build flags (overflow, long row, slot count) into one array
@@ -860,7 +902,7 @@ int list_reserve_rows(System& s) {
     return 0;
 }
 
-int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out) {
+int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out, bool wrapped) {
     // s.flags was cleared by launch_bins (flags[2] receives the slot count)
     int rc = list_reserve_rows(s);
     if (rc) return rc;
@@ -870,7 +912,8 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out)
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
                            s.ctl, s.grid, s.cell_start, s.cell_particles, s.bin_x, s.skin,
-                           s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp);
+                           s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp,
+                           wrapped ? 1 : 0);
         HIP_TRY(hipGetLastError());
     }
     rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);

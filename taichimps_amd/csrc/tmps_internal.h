@@ -309,7 +309,9 @@ inline int nblocks_for(int n) { return (n + TMPS_BLOCK - 1) / TMPS_BLOCK; }
 // rebuild (k_rebuild.hip)
 int launch_pbc(System& s);
 int launch_bins(System& s, const double4* xr);
-int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out);
+// wrapped: every position lies inside the box on its periodic axes (the PBC
+// wrap ran since the positions last changed)
+int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out, bool wrapped);
 int launch_hist_restore(System& s);
 // the current list, touch flags and history into the o* copy (save_state)
 int launch_list_save(System& s);
@@ -431,6 +433,14 @@ int launch_clear_split(System& s);
 int launch_split_consts(System& s, int set_build);
 int launch_split_fields_to_f64(System& s, int field, double* d_out);
 int launch_split_fields_from_f64(System& s, int field, const double* d_in);
+
+// Orders LDS accesses of one wavefront: its lanes' earlier writes are visible
+// to its later reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Per-step constants of the split encoding from the current box; with
 // set_build the current box also becomes the reference of the skin check.
