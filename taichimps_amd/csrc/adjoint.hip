@@ -151,6 +151,13 @@ struct AdjContext {
     std::vector<uint8_t*> flb, fla;
     std::vector<double4*> sold;
     double4* tmpx = nullptr;
+    // the steps at which the forward pass built its list: a replay, which
+    // repeats the forward run bit for bit, builds at exactly these
+    std::vector<int> build_steps;
+    // each replayed step's control block after its servo (device), read back
+    // once per segment
+    Ctl* ctl_rec = nullptr;
+    std::vector<Ctl> ctl_host;
     // adjoint state
     double4 *Xb = nullptr, *Vb = nullptr, *Wb = nullptr, *Fb = nullptr, *Tb = nullptr;
     double4 *xhb = nullptr, *vhb = nullptr, *whb = nullptr, *Fcb = nullptr, *Tcb = nullptr;
@@ -796,29 +803,41 @@ int replay_segment(System& s, AdjContext& A, const Checkpoint& cp, int K) {
     A.fla.assign(K, nullptr);
     A.sold.assign(K, nullptr);
     if (K > A.rec_steps_cap) {
-        dfree(A.XH); dfree(A.VH); dfree(A.WH); dfree(A.FT); dfree(A.TT);
+        dfree(A.XH); dfree(A.VH); dfree(A.WH); dfree(A.FT); dfree(A.TT); dfree(A.ctl_rec);
         rc = dmalloc(&A.XH, (size_t)K * n);
         if (!rc) rc = dmalloc(&A.VH, (size_t)K * n);
         if (!rc) rc = dmalloc(&A.WH, (size_t)K * n);
         if (!rc) rc = dmalloc(&A.FT, (size_t)K * n);
         if (!rc) rc = dmalloc(&A.TT, (size_t)K * n);
+        if (!rc) rc = dmalloc(&A.ctl_rec, (size_t)K);
         if (rc) return rc;
         A.rec_steps_cap = K;
     }
     const double dt = A.dt;
+    const bool servo = s.have_servo && s.servo.enabled;
+    // The steps run on the device back to back: the list is rebuilt at the
+    // steps the forward pass rebuilt at (no displacement check, whose result
+    // is known), and each step's control block after its servo is recorded
+    // on the device; the host reads the records once, after the segment, for
+    // the boxes the reverse steps need. Only a rebuild waits for the device
+    // (its build reads the box and the slot count).
     for (int k = 0; k < K; ++k) {
         StepRec& R = A.steps[k];
-        const Ctl before = *s.hctl;
-        box_of(before, R.box.lo, R.box.prd);
-        for (int q = 0; q < 3; ++q) R.box.per[q] = before.box.per[q];
+        for (int q = 0; q < 3; ++q) R.box.per[q] = cp.ctl.box.per[q];
         s.timestep += 1;
         rc = launch_nve_initial_f64(s, dt);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(A.tmpx, s.xr, n * sizeof(double4), hipMemcpyDeviceToDevice,
                                s.stream));
-        int rebuilt = 0;
-        rc = tmps_api::reneighbor(s, false, &rebuilt);
-        if (rc) return rc;
+        s.current_step = s.timestep;
+        const int rebuilt =
+            std::binary_search(A.build_steps.begin(), A.build_steps.end(), s.timestep) ? 1 : 0;
+        if (rebuilt) {
+            rc = tmps_api::reneighbor_now(s);
+            if (rc) return rc;
+            // the build read the box of the step into the host mirror
+            box_of(*s.hctl, R.box.lo, R.box.prd);
+        }
         R.list = (int)A.lists.size() - 1;
         R.rebuilt = rebuilt;
         if (rebuilt) {
@@ -861,17 +880,34 @@ int replay_segment(System& s, AdjContext& A, const Checkpoint& cp, int K) {
         A.fla[k] = arena_copy(A.rec, s.nb_flag, (size_t)s.nslots, s.stream, &rc);
         if (rc) return rc;
         rc = launch_nve_final_f64(s, dt);
-        if (!rc) rc = tmps_api::servo_end_of_step(s, dt, nullptr);
+        if (!rc) rc = tmps_api::servo_end_of_step_launch(s, dt);
         if (rc) return rc;
-        const Ctl& after = *s.hctl;
+        HIP_TRY(hipMemcpyAsync(A.ctl_rec + k, s.ctl, sizeof(Ctl), hipMemcpyDeviceToDevice,
+                               s.stream));
+    }
+    A.ctl_host.resize((size_t)K);
+    HIP_TRY(copy_sync(s.stream, A.ctl_host.data(), A.ctl_rec, (size_t)K * sizeof(Ctl),
+                      hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k) {
+        StepRec& R = A.steps[k];
+        const Ctl& before = k == 0 ? cp.ctl : A.ctl_host[(size_t)k - 1];
+        const Ctl& after = A.ctl_host[(size_t)k];
+        if (after.error)
+            return tmps_fail(after.error, "adjoint replay: fix deform/pressure drove a box "
+                                          "dimension to non-positive");
+        box_of(before, R.box.lo, R.box.prd);
         box_of(after, R.box.lo_n, R.box.prd_n);
         for (int q = 0; q < 6; ++q) R.box.tensor[q] = after.tensor[q];
-        if (!(s.have_servo && s.servo.enabled)) {
+        if (!servo) {
             // no servo: the box stays and nothing reads the tensor
             box_of(before, R.box.lo_n, R.box.prd_n);
             for (int q = 0; q < 6; ++q) R.box.tensor[q] = 0.0;
         }
     }
+    // the host mirror follows the device to the segment's end
+    *s.hctl = A.ctl_host[(size_t)K - 1];
+    s.servo_step_count = s.hctl->servo.step_count;
+    s.servo_nsteps = s.hctl->servo.nsteps;
     return 0;
 }
 
@@ -974,7 +1010,7 @@ void adjoint_free(System& s) {
     dfree(A->Xb); dfree(A->Vb); dfree(A->Wb); dfree(A->Fb); dfree(A->Tb);
     dfree(A->xhb); dfree(A->vhb); dfree(A->whb); dfree(A->Fcb); dfree(A->Tcb);
     dfree(A->mbar); dfree(A->rbar); dfree(A->Sb); dfree(A->Sbo); dfree(A->actl);
-    dfree(A->part); dfree(A->tmpx);
+    dfree(A->part); dfree(A->tmpx); dfree(A->ctl_rec);
     delete A;
     s.adj = nullptr;
 }
@@ -1009,14 +1045,17 @@ int tmps_adjoint_forward(tmps_system* h, int nsteps, int every, double dt, doubl
                               (size_t)s.nslots, s.stream, &rc);
     if (!rc) rc = tmps_api::pair_compute(s, dt, 0, 1);
     if (rc) return rc;
-    for (int t0 = 0; t0 < nsteps; t0 += every) {
+    s.build_log.clear();
+    s.log_builds = true;
+    for (int t0 = 0; t0 < nsteps && !rc; t0 += every) {
         Checkpoint* c = new Checkpoint();
         A.cps.push_back(c);
         rc = save_checkpoint(s, *c);
-        if (rc) return rc;
-        rc = tmps_api::run(s, std::min(every, nsteps - t0), dt, 0, 0, 0.0, 0.0);
-        if (rc) return rc;
+        if (!rc) rc = tmps_api::run(s, std::min(every, nsteps - t0), dt, 0, 0, 0.0, 0.0);
     }
+    s.log_builds = false;
+    A.build_steps = s.build_log;
+    if (rc) return rc;
     Checkpoint* end = new Checkpoint();
     A.cps.push_back(end);
     rc = save_checkpoint(s, *end);

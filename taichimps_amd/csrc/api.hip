@@ -250,6 +250,7 @@ int neigh_record(System& s, int total, int fast[3], int gi[3]) {
     s.has_built = 1;
     s.build_count += 1;
     s.last_build_step = s.current_step;
+    if (s.log_builds) s.build_log.push_back(s.current_step);
     // the copy of the list the next rebuild saves for the history remap,
     // allocated with the list (not inside a later rebuild)
     if (!s.onb_j && s.nb_cap > 0) {
@@ -444,6 +445,16 @@ int reneighbor(System& s, bool force, int* rebuilt) {
     return 0;
 }
 
+int reneighbor_now(System& s) {
+    s.current_step = s.timestep;
+    int rc = 0;
+    if (s.has_built) rc = history_save(s);
+    if (!rc) rc = pbc(s);
+    if (!rc) rc = neigh_build(s, false, true);
+    if (!rc) rc = history_restore(s);
+    return rc;
+}
+
 int clear_forces(System& s) { return split(s) ? launch_clear_split(s) : launch_clear_f64(s); }
 
 int pair_compute(System& s, double dt, int shearupdate, int accumulate) {
@@ -479,7 +490,7 @@ void strain_h_rate(const tmps_servo_params& S, const DevServo& SV, double dt, do
     }
 }
 
-int servo_end_of_step(System& s, double dt, double* tensor_out) {
+int servo_end_of_step_launch(System& s, double dt) {
     if (!s.have_servo || !s.servo.enabled) return 0;
     int rc = ensure_split(s);
     if (rc) return rc;
@@ -492,7 +503,12 @@ int servo_end_of_step(System& s, double dt, double* tensor_out) {
         rc = launch_remap_apply(s);
         if (rc) return rc;
     }
-    rc = launch_remap_clear(s);
+    return launch_remap_clear(s);
+}
+
+int servo_end_of_step(System& s, double dt, double* tensor_out) {
+    if (!s.have_servo || !s.servo.enabled) return 0;
+    int rc = servo_end_of_step_launch(s, dt);
     if (rc) return rc;
     rc = ctl_pull(s);
     if (rc) return rc;

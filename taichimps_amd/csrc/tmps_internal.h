@@ -235,6 +235,10 @@ struct System {
 
     // the adjoint engine's checkpoints and work buffers (adjoint.hip)
     struct AdjContext* adj = nullptr;
+    // steps at which the list was built while log_builds is set (the adjoint
+    // forward pass logs them for its replays)
+    std::vector<int> build_log;
+    bool log_builds = false;
 
     // outputs of the per-contact computes, grown on first use: per-atom
     // counts, per-atom stress [n][6], and the reduced sums (8 doubles)
@@ -264,9 +268,14 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
               bool pull_after = true);
 int init_simulation(System& s, double dt, int use_cundall, double gl, double ga);
 int reneighbor(System& s, bool force, int* rebuilt);
+// reneighbor's build without its decision: a rebuild at step s.timestep
+int reneighbor_now(System& s);
 int clear_forces(System& s);
 int pair_compute(System& s, double dt, int shearupdate, int accumulate);
 int servo_end_of_step(System& s, double dt, double* tensor_out);
+// servo_end_of_step's kernels only: the host mirror is not read back and a
+// collapsed box is left in the control block's error field
+int servo_end_of_step_launch(System& s, double dt);
 int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double gl, double ga);
 // redo_step's host work for an FP32-split ensemble member whose step the
 // batched redo kernels redo: redo_batch_begin up to the list build (from the
