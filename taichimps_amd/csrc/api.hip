@@ -34,7 +34,10 @@ System* sys(tmps_system* h) {
     return s;
 }
 
-// A getter's system: the host mirror stays valid.
+// The system of an entry point that enqueues no work that changes the
+// device's control block (getters, host-only settings), or that keeps track
+// of the host mirror itself (ctl_edit / ctl_commit, run): the mirror's
+// freshness stays as it is.
 System* sys_view(tmps_system* h) { return reinterpret_cast<System*>(h); }
 }  // namespace
 
@@ -87,6 +90,26 @@ int ctl_pull_view(System& s) {
 int ctl_push(System& s) {
     HIP_TRY(copy_sync(s.stream, s.ctl, s.hctl, sizeof(Ctl), hipMemcpyHostToDevice));
     return 0;
+}
+
+// An edit of the control block without a round trip while the host mirror is
+// fresh: ctl_edit brings the mirror up to date only when enqueued work may
+// have changed the device's copy; ctl_commit stores the edited mirror into
+// the device's copy in stream order (a kernel argument, no synchronisation),
+// after which the mirror is still the device's state at the end of the
+// enqueued work.
+int ctl_edit(System& s) {
+    const int rc = ctl_pull_view(s);
+    if (!rc) std::memcpy(&s.edit_base, s.hctl, sizeof(Ctl));
+    return rc;
+}
+
+// (an edit that changed nothing stores nothing)
+int ctl_commit(System& s) {
+    if (std::memcmp(&s.edit_base, s.hctl, sizeof(Ctl)) == 0) return 0;
+    const int rc = launch_ctl_store(s);
+    if (rc) s.hctl_fresh = false;
+    return rc;
 }
 
 bool split(const System& s) { return s.precision == TMPS_FP32_SPLIT; }
@@ -200,6 +223,7 @@ int ensure_split(System& s) {
     rc = launch_x64_to_split(s);
     if (rc) return rc;
     s.x64_valid = false;
+    s.hctl_fresh = false;  // the split constants were set on the device
     return 0;
 }
 
@@ -936,6 +960,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     s.rebuilds = 0;
     int rc = 0;
     if (run_pre || !s.setup_done) {
+        s.hctl_fresh = false;  // Verlet::setup changes the device's control block
         rc = init_simulation(s, dt, use_cundall, gl, ga);
         if (rc) return rc;
     }
@@ -945,7 +970,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     }
     rc = ensure_split(s);
     if (rc) return rc;
-    rc = ctl_pull(s);
+    rc = ctl_edit(s);
     if (rc) return rc;
     Ctl& H = *s.hctl;
     H.timestep = s.timestep;
@@ -958,7 +983,10 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     H.check = s.check;
     H.trigger_sq = 0.25 * s.skin * s.skin;
     H.remap.step = -1;
-    rc = ctl_push(s);
+    rc = ctl_commit(s);
+    // from here the device's copy runs ahead of the mirror until the end of
+    // the run brings it back
+    s.hctl_fresh = false;
     if (rc) return rc;
     if (split(s)) {
         rc = launch_split_consts(s, 0);
@@ -970,12 +998,13 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     // pinned memory, so a rebuild request costs at most one segment of
     // skipped kernels.
     struct Seg {
-        int start, len, slot;
+        int start, len, slot, last;
     };
     Seg inflight[2];
     int nin = 0, next_slot = 0;
     int t = s.timestep;
     int kick = 1;
+    bool mirror_back = false;  // the last segment's copy of the control block arrived
     while (true) {
         while (nin < 2 && t < t_end) {
             int last = 0;
@@ -987,8 +1016,15 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
             next_slot = (next_slot + 1) % System::kGraphSlots;
             HIP_TRY(hipMemcpyAsync(s.hstatus + 4 * slot, &s.ctl->timestep, 4 * sizeof(int),
                                    hipMemcpyDeviceToHost, s.stream));
+            // the run's last segment also brings the whole control block
+            // back into the (pinned) mirror, so the end of the run needs no
+            // further round trip; the host touches the mirror only after
+            // this segment's event or a stream synchronisation
+            if (last)
+                HIP_TRY(hipMemcpyAsync(s.hctl, s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                                       s.stream));
             HIP_TRY(hipEventRecord(s.seg_event[slot], s.stream));
-            inflight[nin++] = Seg{t, len, slot};
+            inflight[nin++] = Seg{t, len, slot, last};
             t += len;
         }
         if (nin == 0) break;
@@ -1001,6 +1037,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         if (!aborted && !err) {
             s.timestep = done;
             if (s.has_built) s.current_step = s.timestep;
+            mirror_back = sg.last != 0;
             continue;
         }
         // skip whatever is still queued; it sees the abort and does nothing
@@ -1016,9 +1053,12 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         if (rc) return rc;
         t = s.timestep;
         kick = 1;  // the redone step ended with final_integrate
+        mirror_back = false;
     }
-    rc = ctl_pull(s);
-    if (rc) return rc;
+    if (!mirror_back) {
+        rc = ctl_pull(s);
+        if (rc) return rc;
+    }
     s.hctl_fresh = true;
     s.servo_step_count = s.hctl->servo.step_count;
     s.servo_nsteps = s.hctl->servo.nsteps;
@@ -1403,7 +1443,7 @@ int tmps_destroy(tmps_system* h) {
 }
 
 int tmps_set_stream(tmps_system* h, void* stream) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     hipStream_t next = stream ? static_cast<hipStream_t>(stream) : s.own_stream;
     if (next == s.stream) return 0;
     // work already queued on the old stream finishes first
@@ -1439,7 +1479,7 @@ int tmps_get_field(tmps_system* h, int field, void* host, int count) {
 }
 
 int tmps_sync(tmps_system* h) {
-    HIP_TRY(hipStreamSynchronize(sys(h)->stream));
+    HIP_TRY(hipStreamSynchronize(sys_view(h)->stream));
     return 0;
 }
 
@@ -1487,7 +1527,7 @@ int tmps_clear_forces(tmps_system* h) { return clear_forces(*sys(h)); }
 
 int tmps_neigh_config(tmps_system* h, double skin, int delay, int every, int check,
                       int max_grid_cells, int max_neighbors_per_atom) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     if (every < 1) return tmps_fail(TMPS_ERR_ARG, "neigh_modify every must be >= 1");
     if (max_grid_cells < 1) return tmps_fail(TMPS_ERR_ARG, "max_grid_cells must be >= 1");
     if (max_neighbors_per_atom < 1)
@@ -1678,7 +1718,7 @@ int tmps_history_seed(tmps_system* h, int n, const int* start, const int* nbr,
 }
 
 int tmps_pair_set(tmps_system* h, const tmps_pair_params* p) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     if (!p) return tmps_fail(TMPS_ERR_ARG, "null pair params");
     if (p->style < 0 || p->style > 2) return tmps_fail(TMPS_ERR_ARG, "unknown pair style");
     PairP& q = s.pair;
@@ -1717,7 +1757,7 @@ int tmps_cundall(tmps_system* h, double gamma_lin, double gamma_ang) {
 }
 
 int tmps_servo_set(tmps_system* h, const tmps_servo_params* p) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     if (!p) return tmps_fail(TMPS_ERR_ARG, "null servo params");
     for (int i = 0; i < 3; ++i)
         if (p->style[i] < 0 || p->style[i] > 9)
@@ -1730,19 +1770,22 @@ int tmps_servo_set(tmps_system* h, const tmps_servo_params* p) {
                 return tmps_fail(TMPS_ERR_UNSUPPORTED,
                                  "FP32-split supports a deforming box only with remap x");
     }
-    s.servo = *p;
-    if (s.servo.nevery < 1) s.servo.nevery = 1;
+    tmps_servo_params q = *p;
+    if (q.nevery < 1) q.nevery = 1;
+    // a captured segment bakes in only whether the servo runs
+    const bool was_on = s.have_servo && s.servo.enabled;
+    s.servo = q;
     s.have_servo = true;
-    int rc = ctl_pull(s);
+    if (was_on != (s.servo.enabled != 0)) invalidate_graphs(s);
+    int rc = ctl_edit(s);
     if (rc) return rc;
     s.hctl->servo.p = s.servo;
-    invalidate_graphs(s);
-    return ctl_push(s);
+    return ctl_commit(s);
 }
 
 int tmps_servo_setup(tmps_system* h, int nsteps_total, double dt) {
-    System& s = *sys(h);
-    int rc = ctl_pull(s);
+    System& s = *sys_view(h);
+    int rc = ctl_edit(s);
     if (rc) return rc;
     DevServo& SV = s.hctl->servo;
     SV.p = s.servo;
@@ -1758,7 +1801,7 @@ int tmps_servo_setup(tmps_system* h, int nsteps_total, double dt) {
     strain_h_rate(s.servo, SV, dt, hr);
     for (int k = 0; k < 6; ++k) B.h_rate[k] = hr[k];
     B.vremap = s.servo.remap == 1 ? 1 : 0;
-    return ctl_push(s);
+    return ctl_commit(s);
 }
 
 int tmps_servo_end_of_step(tmps_system* h, double dt, double tensor_out[6]) {
@@ -2032,12 +2075,12 @@ int tmps_stress_atom(tmps_system* h, int keflag, double* stress, double sum6[6])
 
 int tmps_run(tmps_system* h, int nsteps, double dt, int run_pre, int use_cundall, double gamma_lin,
              double gamma_ang) {
-    return run(*sys(h), nsteps, dt, run_pre, use_cundall, gamma_lin, gamma_ang);
+    return run(*sys_view(h), nsteps, dt, run_pre, use_cundall, gamma_lin, gamma_ang);
 }
 
 int tmps_prepare_run(tmps_system* h, int nsteps, double dt, int use_cundall, double gamma_lin,
                      double gamma_ang) {
-    return prepare_run(*sys(h), nsteps, dt, use_cundall, gamma_lin, gamma_ang);
+    return prepare_run(*sys_view(h), nsteps, dt, use_cundall, gamma_lin, gamma_ang);
 }
 
 int tmps_get_counters(tmps_system* h, tmps_counters* out) {
@@ -2060,19 +2103,19 @@ int tmps_get_counters(tmps_system* h, tmps_counters* out) {
 }
 
 int tmps_set_counters(tmps_system* h, const tmps_counters* in) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     s.timestep = in->timestep;
     s.has_built = in->has_built;
     s.build_count = in->build_count;
     s.last_build_step = in->last_build_step;
     s.current_step = in->current_step;
     s.setup_done = in->setup_done;
-    int rc = ctl_pull(s);
+    int rc = ctl_edit(s);
     if (rc) return rc;
     s.hctl->servo.step_count = in->servo_step_count;
     s.hctl->servo.nsteps = in->servo_nsteps;
     s.hctl->last_build_step = s.last_build_step;
-    return ctl_push(s);
+    return ctl_commit(s);
 }
 
 int tmps_last_pressure(tmps_system* h, double out6[6]) {
@@ -2084,7 +2127,7 @@ int tmps_last_pressure(tmps_system* h, double out6[6]) {
 }
 
 int tmps_set_profiling(tmps_system* h, int enabled) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     s.profiling = enabled != 0;
     for (int c = 0; c < TMPS_PROF_CLASSES; ++c) {
         s.prof_ms[c] = 0.0;
@@ -2094,7 +2137,7 @@ int tmps_set_profiling(tmps_system* h, int enabled) {
 }
 
 int tmps_set_launch_mode(tmps_system* h, int mode) {
-    System& s = *sys(h);
+    System& s = *sys_view(h);
     if (mode != TMPS_LAUNCH_GRAPH && mode != TMPS_LAUNCH_DIRECT)
         return tmps_fail(TMPS_ERR_ARG, "launch mode must be TMPS_LAUNCH_GRAPH or TMPS_LAUNCH_DIRECT");
     s.launch_mode = mode;

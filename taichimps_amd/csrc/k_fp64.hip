@@ -581,6 +581,18 @@ __global__ void k_ctl_rebuilt(Ctl* ctl, int step, int split) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl_rebuilt(ctl, step, split);
 }
 
+// The host's edited mirror of the control block, stored into the device's
+// copy in stream order. The value travels as the kernel's argument, so the
+// host may edit its mirror again as soon as the launch returns.
+static_assert(sizeof(Ctl) % 4 == 0 && sizeof(Ctl) <= 3072,
+              "the control block travels as one kernel argument");
+__global__ void k_ctl_store(Ctl* ctl, Ctl value) {
+    constexpr int NW = (int)(sizeof(Ctl) / 4);
+    const unsigned* src = reinterpret_cast<const unsigned*>(&value);
+    unsigned* dst = reinterpret_cast<unsigned*>(ctl);
+    for (int k = threadIdx.x; k < NW; k += blockDim.x) dst[k] = src[k];
+}
+
 // A step cut short by a rebuild request is about to be redone.
 __global__ void k_ctl_resume(Ctl* ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1628,6 +1640,12 @@ int launch_remap_clear(System& s, int honor_abort) {
 int launch_ctl_rebuilt(System& s, int step) {
     hipLaunchKernelGGL(k_ctl_rebuilt, dim3(1), dim3(64), 0, s.stream, s.ctl, step,
                        s.precision == TMPS_FP32_SPLIT ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ctl_store(System& s) {
+    hipLaunchKernelGGL(k_ctl_store, dim3(1), dim3(256), 0, s.stream, s.ctl, *s.hctl);
     HIP_TRY(hipGetLastError());
     return 0;
 }

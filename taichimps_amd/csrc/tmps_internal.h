@@ -198,6 +198,7 @@ struct System {
 
     Ctl* ctl = nullptr;   // device control block
     Ctl* hctl = nullptr;  // pinned host mirror
+    Ctl edit_base{};      // the mirror as a ctl_edit found it
 
     // host mirrors
     std::vector<double> h_radius;
@@ -407,6 +408,7 @@ int launch_remap_clear(System& s, int honor_abort = 0);
 int launch_ctl_rebuilt(System& s, int step);
 // clears trigger and abort before a redo
 int launch_ctl_resume(System& s);
+int launch_ctl_store(System& s);
 // records a list build (last_build_step, fast_img; FP32-split: sc.gi and the
 // split constants with the current box as the skin check's reference)
 int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int split);
