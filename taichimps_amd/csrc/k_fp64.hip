@@ -375,69 +375,63 @@ inline int final_leaves(int nparts, int threads = FINAL_THREADS) {
 
 
 // The tree of final_tree_L over 8 components for up to CLOSE_SPAN partials.
-// Wavefront w owns partials [w*64L, (w+1)*64L) and loads them in L coalesced
-// rounds (lane l: partial w*64L + 64r + l, one 64-byte row); per component an
-// LDS transpose hands lane l the L consecutive partials [w*64L + L*l, +L),
-// which it sums pairwise in registers, then one wavefront butterfly and the
-// pairwise sum over the wavefronts. Thread, wavefront and block ranges are
-// aligned power-of-two subtrees, so this is final_tree_L's tree. Thread 0
-// receives the sums.
+// Thread t owns component t & 7 of the chunk t >> 3 of 8L consecutive
+// partials, which it loads (eight threads read one partial's 64-byte row) and
+// sums pairwise in registers; chunks are then joined across lanes 8, 16 and
+// 32 apart and the wavefronts' sums pairwise through LDS. Chunks, wavefront
+// ranges and the block's range are aligned power-of-two subtrees of the
+// 512L-leaf range (zero leaves past nparts), so this is final_tree_L's tree.
+// Every thread receives the sums.
+template <int L>
+__device__ __forceinline__ void close_tree_load(const double* partial, int nparts,
+                                                double (&v)[8 * L]) {
+    static_assert(L <= CLOSE_BATCH, "a close block reduces at most CLOSE_SPAN partials");
+    constexpr int S = 8 * L;
+    const int t = threadIdx.x, k = t & 7, c = t >> 3;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int p = c * S + q;
+        v[q] = p < nparts ? partial[(size_t)p * 8 + k] : 0.0;
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void close_tree_sum(double (&v)[8 * L], double out[8]) {
+    constexpr int S = 8 * L;
+    constexpr int NWAVE = CLOSE_THREADS / 64;
+    __shared__ double ws[NWAVE][8];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int w = 1; w < S; w <<= 1)
+#pragma unroll
+        for (int q = 0; q < S; q += 2 * w) v[q] = v[q] + v[q + w];
+    double a = v[0];
+    a = a + __shfl_xor(a, 8);
+    a = a + __shfl_xor(a, 16);
+    a = a + __shfl_xor(a, 32);
+    if (lane < 8) ws[wave][lane] = a;
+    __syncthreads();
+    if (t < 8) {
+        double u[NWAVE];
+#pragma unroll
+        for (int q = 0; q < NWAVE; ++q) u[q] = ws[q][t];
+#pragma unroll
+        for (int w = 1; w < NWAVE; w <<= 1)
+#pragma unroll
+            for (int q = 0; q < NWAVE; q += 2 * w) u[q] = u[q] + u[q + w];
+        ws[0][t] = u[0];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) out[q] = ws[0][q];
+    __syncthreads();
+}
+
 template <int L>
 __device__ void close_tree(const double* partial, int nparts, double out[8]) {
-    static_assert(L <= CLOSE_BATCH, "a close block reduces at most CLOSE_SPAN partials");
-    constexpr int NWAVE = CLOSE_THREADS / 64;
-    __shared__ double xp[NWAVE][64 * L];
-    __shared__ double ws[8][NWAVE];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double4* rows = reinterpret_cast<const double4*>(partial);
-    double e[L][8];
-#pragma unroll
-    for (int r = 0; r < L; ++r) {
-        const int idx = wave * 64 * L + 64 * r + lane;
-        double4 lo = make_double4(0.0, 0.0, 0.0, 0.0), hi = lo;
-        if (idx < nparts) {
-            lo = rows[2 * (size_t)idx];
-            hi = rows[2 * (size_t)idx + 1];
-        }
-        e[r][0] = lo.x; e[r][1] = lo.y; e[r][2] = lo.z; e[r][3] = lo.w;
-        e[r][4] = hi.x; e[r][5] = hi.y; e[r][6] = hi.z; e[r][7] = hi.w;
-    }
-    double* t = xp[wave];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        double v[L];
-        if (L == 1) {
-            v[0] = e[0][k];
-        } else {
-#pragma unroll
-            for (int r = 0; r < L; ++r) t[64 * r + lane] = e[r][k];
-            wave_lds_sync();
-#pragma unroll
-            for (int q = 0; q < L; ++q) v[q] = t[L * lane + q];
-            wave_lds_sync();
-        }
-#pragma unroll
-        for (int w = 1; w < L; w <<= 1)
-#pragma unroll
-            for (int q = 0; q < L; q += 2 * w) v[q] = v[q] + v[q + w];
-        const double sum = wave_tree_sum(v[0]);
-        if (lane == 0) ws[k][wave] = sum;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            double u[NWAVE];
-#pragma unroll
-            for (int q = 0; q < NWAVE; ++q) u[q] = ws[k][q];
-#pragma unroll
-            for (int w = 1; w < NWAVE; w <<= 1)
-#pragma unroll
-                for (int q = 0; q < NWAVE; q += 2 * w) u[q] = u[q] + u[q + w];
-            out[k] = u[0];
-        }
-    }
-    __syncthreads();
+    double v[8 * L];
+    close_tree_load<L>(partial, nparts, v);
+    close_tree_sum<L>(v, out);
 }
 
 // Above CLOSE_SPAN block partials, one block per aligned group of CLOSE_SPAN
@@ -512,12 +506,17 @@ __device__ __forceinline__ void step_close_block(Ctl* ctl, const double* partial
     __shared__ Ctl cs;
     __shared__ int writeback;
     // a segment cut short by a rebuild request skips its remaining closes
+    static_assert(NW <= CLOSE_THREADS, "one control-block word per thread");
     if (ctl->abort) return;
     const unsigned* src = reinterpret_cast<const unsigned*>(ctl);
     unsigned* lds = reinterpret_cast<unsigned*>(&cs);
-    for (int k = threadIdx.x; k < NW; k += CLOSE_THREADS) lds[k] = src[k];
+    // the control block's words and the partials are requested together
+    const unsigned word = threadIdx.x < NW ? src[threadIdx.x] : 0u;
+    double v[8 * L];
+    close_tree_load<L>(partial, nparts, v);
+    if (threadIdx.x < NW) lds[threadIdx.x] = word;
     double out[8];
-    close_tree<L>(partial, nparts, out);  // ends with a barrier: cs is complete
+    close_tree_sum<L>(v, out);  // its barriers complete cs
     if (threadIdx.x == 0) writeback = close_update(cs, out, dt, servo_on, natoms, member) ? 1 : 0;
     __syncthreads();
     if (writeback) {
