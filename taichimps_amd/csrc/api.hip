@@ -758,6 +758,10 @@ int check_run_error(System& s) {
     s.hctl->abort = 0;
     int rc = ctl_push(s);
     if (rc) return rc;
+    // a list build decided on the device records its errors here too
+    if (e == TMPS_ERR_OVERFLOW) return list_overflow_error(s);
+    if (e == TMPS_ERR_UNSUPPORTED)
+        return tmps_fail(e, "FP32-split encoding holds at most 2047 x 2047 x 1023 bins");
     return tmps_fail(e,
                      "fix deform/pressure drove a box dimension to non-positive; check the gain, "
                      "max/rate or strain rate");
@@ -773,8 +777,6 @@ int check_run_error(System& s) {
 // results in buffer 0 and f / torque stored for the next step's half-kick.
 int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga,
               bool pull_after) {
-    const int nparts = nblocks_for(std::max(s.n, 1));
-    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
     const int stp = s.hctl->timestep + 1;
     int rc = 0;
     if (par == 1) {
@@ -793,7 +795,17 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     s.current_step = stp;
     rc = history_save(s);
     if (!rc) rc = pbc(s, true);
-    if (!rc) rc = neigh_build(s, true, true);
+    if (rc) return rc;
+    return redo_tail(s, stp, dt, use_cundall, gl, ga, pull_after);
+}
+
+// redo_step from the list build on: positions wrapped in buffer set 0 (and,
+// FP32-split, decoded into xr), the old list saved, the host mirror current.
+int redo_tail(System& s, int stp, double dt, int use_cundall, double gl, double ga,
+              bool pull_after) {
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    int rc = neigh_build(s, true, true);
     if (!rc) rc = history_restore(s);
     if (!rc) rc = ensure_split(s);
     if (rc) return rc;
@@ -823,6 +835,155 @@ int redo_step(System& s, int par, double dt, int use_cundall, double gl, double 
     s.timestep = s.hctl->timestep;
     s.current_step = s.timestep;
     return 0;
+}
+
+// What a run's rebuild redo decided on the device needs of the host, set up
+// before the run's first segment: bin arrays for the largest grid the list
+// build may choose, the row scratch, and a saved-list copy as large as the
+// list's buffers.
+int dev_redo_prepare(System& s, DevRedo& R) {
+    R.ok = false;
+    if (s.n <= 0 || (split(s) && s.hctl->box.vremap)) return 0;
+    R.max_cutoff = 2.0 * max_radius(s);
+    R.max_cells = std::max(s.max_grid_cells, (int)std::min<long long>(4LL * s.n, 1LL << 26));
+    int rc = ensure_cells(s, R.max_cells);
+    if (!rc) rc = list_reserve_rows(s);
+    if (!rc && s.nb_cap > 0 && (s.onb_j == nullptr || s.onb_cap < s.nb_cap)) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        dfree(s.onb_j);
+        dfree(s.onb_flag);
+        if (s.onb_hist) (void)hipFree(s.onb_hist);
+        s.onb_hist = nullptr;
+        const size_t hsz = split(s) ? sizeof(float4) : sizeof(double4);
+        rc = dalloc(s.stream, &s.onb_j, s.nb_cap);
+        if (!rc) rc = dalloc(s.stream, &s.onb_flag, s.nb_cap);
+        if (rc) return rc;
+        HIP_TRY(hipMalloc(&s.onb_hist, s.nb_cap * hsz));
+        s.onb_cap = s.nb_cap;
+    }
+    if (rc) return rc;
+    R.ok = s.nb_cap > 0 && s.onb_cap >= s.nb_cap && s.cell_cap >= R.max_cells + 1;
+    return 0;
+}
+
+// redo_step with the list build decided on the device: everything is
+// enqueued behind the aborted segment (and the one queued after it, whose
+// kernels see the abort and return), with no host round trip. The host's
+// record of the build is written now; should the device find the build
+// beyond its buffers, the run's next status says so and dev_redo_recover
+// takes the record back and redoes the build on the host.
+int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall, double gl,
+             double ga) {
+    const int nparts = nblocks_for(std::max(s.n, 1));
+    const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
+    int rc = 0;
+    R.stp = stp;
+    R.build_count = s.build_count;
+    R.last_build_step = s.last_build_step;
+    R.has_built = s.has_built;
+    R.seed_pending = s.seed_pending;
+    R.have_old = s.have_old;
+    if (par == 1) {
+        rc = launch_copy_state(s, 0);
+        if (rc) return rc;
+    }
+    rc = launch_ctl_resume(s);
+    if (rc) return rc;
+    rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
+    if (rc) return rc;
+    s.current_step = stp;
+    // history_save: the list (its slot count read on the device) into the
+    // copy, which is as large as the list's buffers
+    if (s.seed_pending) {
+        s.seed_pending = false;
+    } else {
+        rc = launch_list_save(s);
+        if (rc) return rc;
+        s.onb_n = s.n;
+        s.have_old = true;
+    }
+    // pbc
+    if (split(s) && !s.x64_valid) {
+        rc = launch_split_to_x64(s);
+        if (rc) return rc;
+    }
+    s.x64_valid = true;
+    rc = launch_pbc(s);
+    if (!rc) rc = launch_build_dev(s, R.max_cutoff, R.max_cells);
+    if (!rc) rc = launch_ctl_built_dev(s, stp);
+    if (!rc && split(s)) rc = launch_x64_to_split_dev(s);
+    if (!rc) rc = launch_hist_restore(s);
+    if (rc) return rc;
+    s.x64_valid = false;
+    s.have_old = false;
+    s.seed_pending = false;
+    rc = launch_ctl_rebuilt(s, stp);
+    if (rc) return rc;
+    rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
+                  : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
+    if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
+    if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
+    if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
+    if (!rc) rc = launch_remap_clear(s, 1);
+    if (rc) return rc;
+    // neigh_record's bookkeeping (the slot count and grid come back with the
+    // run's last control block)
+    s.has_built = 1;
+    s.build_count += 1;
+    s.last_build_step = stp;
+    if (s.log_builds) s.build_log.push_back(stp);
+    s.rebuilds += 1;
+    s.timestep = stp;
+    R.built = true;
+    R.pending = true;
+    return 0;
+}
+
+// The device found the redo's list build beyond its buffers (TMPS_NEED_HOST
+// in the control block, pulled): the host record goes back to the redo's
+// start, and the build and the rest of the step are redone on the host from
+// the wrapped positions and the saved list the device left.
+int dev_redo_recover(System& s, DevRedo& R, double dt, int use_cundall, double gl, double ga) {
+    s.build_count = R.build_count;
+    s.last_build_step = R.last_build_step;
+    s.has_built = R.has_built;
+    if (s.log_builds && !s.build_log.empty() && s.build_log.back() == R.stp) s.build_log.pop_back();
+    s.rebuilds -= 1;
+    s.timestep = R.stp - 1;
+    s.current_step = R.stp;
+    s.x64_valid = true;
+    s.have_old = R.seed_pending ? R.have_old : true;
+    s.seed_pending = false;
+    s.nslots = s.hctl->nslots;
+    Ctl& H = *s.hctl;
+    H.error = 0;
+    H.abort = 0;
+    H.trigger = -1;
+    int rc = ctl_push(s);
+    if (rc) return rc;
+    R.pending = false;
+    return redo_tail(s, R.stp, dt, use_cundall, gl, ga, true);
+}
+
+// After a run whose rebuilds were decided on the device (host mirror
+// current): the host's slot count and grids from the last build.
+void dev_redo_finish(System& s, DevRedo& R) {
+    if (!R.built) return;
+    const Ctl& H = *s.hctl;
+    s.nslots = H.nslots;
+    s.npairs = H.nslots / 2;
+    s.grid = H.lgrid;
+    DevBox B = H.box;
+    for (int k = 0; k < 3; ++k) {
+        B.lo[k] = H.lo_build[k];
+        B.prd[k] = H.prd_build[k];
+        B.hi[k] = B.lo[k] + B.prd[k];
+    }
+    const GridP keep = s.grid;
+    setup_grid(s, B, R.max_cutoff);
+    s.ref_grid = s.grid;
+    s.grid = keep;
+    R.built = false;
 }
 
 // ------------------------------------------------ batched redo (batch.hip)
@@ -983,6 +1144,9 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     H.check = s.check;
     H.trigger_sq = 0.25 * s.skin * s.skin;
     H.remap.step = -1;
+    DevRedo R;
+    rc = dev_redo_prepare(s, R);
+    if (rc) return rc;
     rc = ctl_commit(s);
     // from here the device's copy runs ahead of the mirror until the end of
     // the run brings it back
@@ -1038,6 +1202,23 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
             s.timestep = done;
             if (s.has_built) s.current_step = s.timestep;
             mirror_back = sg.last != 0;
+            R.pending = false;
+            continue;
+        }
+        if (R.ok && aborted && !err && st4[1] == done + 1) {
+            // the skin check of step done + 1 asked for a rebuild: the step
+            // is redone with the list build decided on the device, queued
+            // behind the segment that follows (its kernels see the abort and
+            // return), and the run goes on from that step with no wait
+            nin = 0;
+            s.timestep = done;
+            if (s.has_built) s.current_step = done;
+            const int stp = done + 1;
+            rc = redo_dev(s, R, ((stp - 1) - sg.start) % 2, stp, dt, use_cundall, gl, ga);
+            if (rc) return rc;
+            t = stp;
+            kick = 1;
+            mirror_back = false;
             continue;
         }
         // skip whatever is still queued; it sees the abort and does nothing
@@ -1045,11 +1226,25 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         nin = 0;
         rc = ctl_pull(s);
         if (rc) return rc;
+        if (s.hctl->error == TMPS_NEED_HOST && R.pending) {
+            rc = dev_redo_recover(s, R, dt, use_cundall, gl, ga);
+            if (rc) return rc;
+            R.built = false;  // the host built the list last
+            rc = dev_redo_prepare(s, R);  // its buffers may have grown
+            if (rc) return rc;
+            t = s.timestep;
+            kick = 1;
+            mirror_back = false;
+            continue;
+        }
         rc = check_run_error(s);
         if (rc) return rc;
         s.timestep = s.hctl->timestep;
         const int stp = s.hctl->timestep + 1;
         rc = redo_step(s, ((stp - 1) - sg.start) % 2, dt, use_cundall, gl, ga);
+        if (rc) return rc;
+        R.built = false;
+        rc = dev_redo_prepare(s, R);
         if (rc) return rc;
         t = s.timestep;
         kick = 1;  // the redone step ended with final_integrate
@@ -1059,6 +1254,15 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         rc = ctl_pull(s);
         if (rc) return rc;
     }
+    if (s.hctl->error == TMPS_NEED_HOST && R.pending) {
+        // the run's last step was a redo the device could not build
+        rc = dev_redo_recover(s, R, dt, use_cundall, gl, ga);
+        if (rc) return rc;
+        R.built = false;
+    } else if (s.hctl->error) {
+        return check_run_error(s);
+    }
+    dev_redo_finish(s, R);
     s.hctl_fresh = true;
     s.servo_step_count = s.hctl->servo.step_count;
     s.servo_nsteps = s.hctl->servo.nsteps;

@@ -576,8 +576,10 @@ __device__ __forceinline__ void ctl_rebuilt(Ctl* ctl, int step, int split) {
     if (split) update_split_consts(ctl, 0);
 }
 
+// (a build that recorded an error keeps its abort: the rest of the redo
+// then leaves the step's state for the host)
 __global__ void k_ctl_rebuilt(Ctl* ctl, int step, int split) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctl_rebuilt(ctl, step, split);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->error) ctl_rebuilt(ctl, step, split);
 }
 
 // The host's edited mirror of the control block, stored into the device's
@@ -619,6 +621,21 @@ __device__ __forceinline__ void ctl_built(Ctl* ctl, int step, int3 fast, int3 gi
 
 __global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl_built(ctl, step, fast, gi, split);
+}
+
+// ctl_built for a list build decided on the device, from its grid; the
+// build box is recorded in both precisions (the host derives the reported
+// grid from it after the run).
+__global__ void k_ctl_built_dev(Ctl* ctl, int step, int split) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->error) return;
+    const GridP& G = ctl->lgrid;
+    ctl_built(ctl, step, make_int3(G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3),
+              make_int3(G.g[0], G.g[1], G.g[2]), split);
+    if (!split)
+        for (int k = 0; k < 3; ++k) {
+            ctl->lo_build[k] = ctl->box.lo[k];
+            ctl->prd_build[k] = ctl->box.prd[k];
+        }
 }
 
 // ---------------------------------------------------- fused step kernels
@@ -1651,6 +1668,13 @@ int launch_ctl_store(System& s) {
 
 int launch_ctl_resume(System& s) {
     hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, s.stream, s.ctl);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_ctl_built_dev(System& s, int step) {
+    hipLaunchKernelGGL(k_ctl_built_dev, dim3(1), dim3(64), 0, s.stream, s.ctl, step,
+                       s.precision == TMPS_FP32_SPLIT ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -113,10 +113,12 @@ __device__ __forceinline__ void bin_count_atom(int i, const double4* xr, const C
     atomicAdd(&count[idx], 1);
 }
 
-__global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, int* cell_of,
-                            int* count) {
+// Gd: the grid of a build decided on the device (ctl->lgrid), else G
+__global__ void k_bin_count(const double4* xr, int n, const Ctl* ctl, GridP G, const GridP* Gd,
+                            int* cell_of, int* count) {
+    if (Gd && ctl->error) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) bin_count_atom(i, xr, ctl, G, cell_of, count);
+    if (i < n) bin_count_atom(i, xr, ctl, Gd ? *Gd : G, cell_of, count);
 }
 
 // build_grid scatter phase, neighbor.py:276-282
@@ -320,11 +322,15 @@ __device__ __forceinline__ void list_fill_atom(int i, const double4* xr, const C
     }
 }
 
-__global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G,
-                            const int* cell_start, const int* particles, double skin,
-                            const int* nb_start, int* nb_j) {
+// Gd / flags: a build decided on the device, which launches this kernel
+// whatever the row lengths: it writes only where a row exceeded the scratch
+// (flags[1]) and the list fits its buffers (no error recorded)
+__global__ void k_list_fill(const double4* xr, int n, const Ctl* ctl, GridP G, const GridP* Gd,
+                            const int* flags, const int* cell_start, const int* particles,
+                            double skin, const int* nb_start, int* nb_j) {
+    if (flags && (!flags[1] || ctl->error)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) list_fill_atom(i, xr, ctl, G, cell_start, particles, skin, nb_start, nb_j);
+    if (i < n) list_fill_atom(i, xr, ctl, Gd ? *Gd : G, cell_start, particles, skin, nb_start, nb_j);
 }
 
 // Positions in bin order (after k_bin_sort): bx[p] = xr[particles[p]], so the
@@ -493,16 +499,18 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
 }
 
 __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
-                                                          GridP G, const int* cell_start,
+                                                          GridP G, const GridP* Gd,
+                                                          const int* cell_start,
                                                           const int* particles, const double4* bx,
                                                           double skin, int max_nb, int* full_count,
                                                           int* half_count, int* flags, int* rows,
                                                           int wrapped) {
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
+    if (Gd && ctl->error) return;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n)
-        list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, G, cell_start, particles, bx, skin,
-                       max_nb, full_count, half_count, flags, rows, wrapped);
+        list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, Gd ? *Gd : G, cell_start, particles,
+                       bx, skin, max_nb, full_count, half_count, flags, rows, wrapped);
 }
 
 // Rows from the slot-major scratch of k_list_rows into the CSR list.
@@ -512,7 +520,12 @@ __device__ __forceinline__ void list_compact_atom(int i, int n, const int* rows,
     for (int k = 0; k < m; ++k) nb_j[a + k] = rows[(size_t)k * n + i];
 }
 
-__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j) {
+// flags: a build decided on the device, which launches this kernel whatever
+// the row lengths: nothing to do where a row exceeded the scratch (k_list_fill
+// writes the list then)
+__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j,
+                               const int* flags) {
+    if (flags && flags[1]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) list_compact_atom(i, n, rows, nb_start, nb_j);
 }
@@ -770,6 +783,142 @@ __global__ void kb_hist_restore(const RedoEntry* T, const int2* map) {
 
 // ================================================================ launchers
 
+// ------------------------------------- builds decided on the device
+//
+// The list build of a device-resident run's rebuild redo reads everything it
+// needs from the control block and the list's buffers, so the host enqueues
+// it without a round trip: the grid from the box of the step, the cell
+// counts and the slot count stay on the device. Where the device cannot
+// serve the build (a grid beyond the bin arrays, a list beyond its buffers)
+// the control block records TMPS_NEED_HOST and the abort, the new list is
+// left empty so that every later kernel of the redo stays in bounds, and the
+// host redoes the build once it reads the run's status.
+
+// NeighborList.setup_grid (neighbor.py:143-189) for the list's own grid, on
+// the device from the control block's box: the host's setup_grid arithmetic
+// in the same order (this unit rounds as written too).
+__global__ void k_list_grid(Ctl* ctl, double max_cutoff, double skin, int max_cells, int cell_cap,
+                            int split) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->error) return;
+    const DevBox& B = ctl->box;
+    double cut = max_cutoff + skin;
+    if (cut <= 0) cut = 1.0;
+    const double xprd = B.prd[0], yprd = B.prd[1], zprd = B.prd[2];
+    const double vol = xprd * yprd * zprd;
+    const double c0 = yprd * zprd - 0.0 * 0.0;
+    const double ca = zprd * xprd - 0.0 * 0.0;
+    const double c2 = xprd * yprd - 0.0 * 0.0;
+    const double w0 = vol / sqrt(c0 * c0);
+    const double w1 = vol / sqrt(ca * ca);
+    const double w2 = vol / sqrt(c2 * c2);
+    int gx = max(1, (int)floor(w0 / cut));
+    int gy = max(1, (int)floor(w1 / cut));
+    int gz = max(1, (int)floor(w2 / cut));
+    if (B.per[0] == 1 && gx == 2) gx = 1;
+    if (B.per[1] == 1 && gy == 2) gy = 1;
+    if (B.per[2] == 1 && gz == 2) gz = 1;
+    const long long total = (long long)gx * gy * gz;
+    if (total > max_cells) {
+        const double factor = pow((double)max_cells / (double)total, 1.0 / 3.0);
+        gx = max(1, (int)(gx * factor));
+        gy = max(1, (int)(gy * factor));
+        gz = max(1, (int)(gz * factor));
+        if (B.per[0] == 1 && gx == 2) gx = 1;
+        if (B.per[1] == 1 && gy == 2) gy = 1;
+        if (B.per[2] == 1 && gz == 2) gz = 1;
+    }
+    if (split && (gx > 2047 || gy > 2047 || gz > 1023)) {
+        ctl->error = TMPS_ERR_UNSUPPORTED;
+        ctl->abort = 1;
+        return;
+    }
+    if ((long long)gx * gy * gz + 1 > cell_cap) {
+        ctl->error = TMPS_NEED_HOST;
+        ctl->abort = 1;
+        return;
+    }
+    GridP G;
+    G.g[0] = gx;
+    G.g[1] = gy;
+    G.g[2] = gz;
+    G.csize[0] = 1.0 / gx;
+    G.csize[1] = 1.0 / gy;
+    G.csize[2] = 1.0 / gz;
+    G.ncell = gx * gy * gz;
+    ctl->lgrid = G;
+}
+
+// Bin counts and fill cursors over the device grid's cells (and one past),
+// and the list build's flags.
+__global__ void k_zero_cells_dev(const Ctl* ctl, int* count, int* fill, int* flags) {
+    if (ctl->error) return;
+    const int m = ctl->lgrid.ncell + 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        count[i] = 0;
+        fill[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
+}
+
+// The exclusive scan of the bin counts over the device grid's cells: tiles
+// launched for the bin arrays' capacity, those past the cell count idle.
+__global__ void k_scan_tiles_dev(const Ctl* ctl, const int* in, int* out, int* tile_sums) {
+    if (ctl->error) return;
+    const int m = ctl->lgrid.ncell;
+    if (blockIdx.x * SCAN_TILE >= m) {
+        if (threadIdx.x == 0) tile_sums[blockIdx.x] = 0;
+        return;
+    }
+    scan_tile(blockIdx.x, in, out, m, tile_sums);
+}
+
+__global__ void k_scan_sums_dev(const Ctl* ctl, int* tile_sums, int ntiles, int* out) {
+    if (ctl->error) return;
+    scan_sums(tile_sums, ntiles, out + ctl->lgrid.ncell, nullptr);
+}
+
+__global__ void k_scan_add_dev(const Ctl* ctl, int* out, const int* tile_sums) {
+    if (ctl->error) return;
+    const int m = ctl->lgrid.ncell;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        out[i] += tile_sums[i / SCAN_TILE];
+}
+
+__global__ void k_bin_fill_dev(const Ctl* ctl, int n, const int* cell_of, const int* start,
+                               int* fill, int* particles) {
+    if (ctl->error) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bin_fill_atom(i, cell_of, start, fill, particles);
+}
+
+__global__ void k_bin_sort_dev(const Ctl* ctl, const int* start, int* particles) {
+    if (ctl->error) return;
+    const int m = ctl->lgrid.ncell;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x)
+        bin_sort_cell(c, start, particles);
+}
+
+// After the rows' scan: NeighborList.build's overflow (neighbor.py:426-431)
+// and the list's buffers checked; on success the slot count goes into the
+// control block, on failure the list is emptied and the error recorded. The
+// decision is the same in every thread: a thread that reads the error
+// another one recorded decides to empty the list as that one did.
+__global__ void k_list_check(Ctl* ctl, const int* flags, int* nb_start, int n, long long cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int err = ctl->error;
+    if (!err && flags[0]) err = TMPS_ERR_OVERFLOW;
+    if (!err && (long long)flags[2] > cap) err = TMPS_NEED_HOST;
+    if (err) {
+        if (i <= n) nb_start[i] = 0;
+        if (i == 0) {
+            ctl->error = err;
+            ctl->abort = 1;
+        }
+    } else if (i == 0) {
+        ctl->nslots = flags[2];
+    }
+}
+
 static inline dim3 grid_for(int n) { return dim3(nblocks_for(n > 0 ? n : 1)); }
 
 int launch_pbc(System& s) {
@@ -846,7 +995,7 @@ int launch_bins(System& s, const double4* xr) {
     HIP_TRY(hipGetLastError());
     if (s.n > 0) {
         hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
-                           s.grid, s.cell_of, s.cell_count);
+                           s.grid, nullptr, s.cell_of, s.cell_count);
         HIP_TRY(hipGetLastError());
     }
     int rc = launch_scan(s, s.cell_count, s.cell_start, ncell, nullptr);
@@ -911,7 +1060,7 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out,
                            s.cell_particles, xr, s.bin_x);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
-                           s.ctl, s.grid, s.cell_start, s.cell_particles, s.bin_x, s.skin,
+                           s.ctl, s.grid, nullptr, s.cell_start, s.cell_particles, s.bin_x, s.skin,
                            s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp,
                            wrapped ? 1 : 0);
         HIP_TRY(hipGetLastError());
@@ -929,16 +1078,75 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out,
     if (s.n > 0) {
         if (!hv[1]) {
             hipLaunchKernelGGL(k_list_compact, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
-                               (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j);
+                               (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j, nullptr);
         } else {
             hipLaunchKernelGGL(k_list_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
-                               s.ctl, s.grid, s.cell_start, s.cell_particles, s.skin,
-                               (const int*)s.nb_start, s.nb_j);
+                               s.ctl, s.grid, nullptr, nullptr, s.cell_start, s.cell_particles,
+                               s.skin, (const int*)s.nb_start, s.nb_j);
         }
         HIP_TRY(hipGetLastError());
     }
     s.nslots = hv[2];
     return 0;
+}
+
+// The list build of a rebuild redo decided on the device (see the kernels'
+// comment): grid, bins, rows, the CSR list and x0, with no host round trip.
+// max_cutoff: twice the largest radius; max_cells: the list grid's cap. The
+// bin arrays must hold max_cells + 1 cells, the row scratch n rows.
+int launch_build_dev(System& s, double max_cutoff, int max_cells) {
+    if (s.n <= 0) return 0;
+    const int split = s.precision == TMPS_FP32_SPLIT ? 1 : 0;
+    const GridP* Gd = &s.ctl->lgrid;
+    const GridP none{};
+    hipLaunchKernelGGL(k_list_grid, dim3(1), dim3(64), 0, s.stream, s.ctl, max_cutoff, s.skin,
+                       max_cells, s.cell_cap, split);
+    HIP_TRY(hipGetLastError());
+    const int cell_blocks = std::min(nblocks_for(std::max(s.cell_cap, 1)), 2048);
+    hipLaunchKernelGGL(k_zero_cells_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
+                       s.cell_count, s.cell_fill, s.flags);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n, s.ctl,
+                       none, Gd, s.cell_of, s.cell_count);
+    HIP_TRY(hipGetLastError());
+    const int ntiles = (std::max(s.cell_cap, 1) + SCAN_TILE - 1) / SCAN_TILE;
+    if (s.scan_tmp_cap < ntiles + 1) return tmps_fail(TMPS_ERR_ARG, "scan workspace too small");
+    hipLaunchKernelGGL(k_scan_tiles_dev, dim3(ntiles), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
+                       s.cell_count, s.cell_start, s.scan_tmp);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.scan_tmp,
+                       ntiles, s.cell_start);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_add_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
+                       s.cell_start, s.scan_tmp);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_fill_dev, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.n,
+                       s.cell_of, s.cell_start, s.cell_fill, s.cell_particles);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_sort_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
+                       s.cell_start, s.cell_particles);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                       s.cell_particles, s.xr, s.bin_x);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n,
+                       s.ctl, none, Gd, s.cell_start, s.cell_particles, s.bin_x, s.skin, s.max_nb,
+                       s.full_count, s.half_count, s.flags, s.row_tmp, 1);
+    HIP_TRY(hipGetLastError());
+    int rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_list_check, grid_for(s.n + 1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
+                       (const int*)s.flags, s.nb_start, s.n, (long long)s.nb_cap);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_list_compact, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
+                       (const int*)s.row_tmp, (const int*)s.nb_start, s.nb_j,
+                       (const int*)s.flags);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_list_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n,
+                       s.ctl, none, Gd, (const int*)s.flags, s.cell_start, s.cell_particles,
+                       s.skin, (const int*)s.nb_start, s.nb_j);
+    HIP_TRY(hipGetLastError());
+    return launch_store_x0(s);
 }
 
 int launch_list_save(System& s) {

@@ -220,10 +220,12 @@ __device__ __forceinline__ void x64_to_split_atom(int i, const SplitPtrs& P, con
     P.u0[i] = make_float4(u0[0], u0[1], u0[2], 0.f);
 }
 
+// Gd: the grid of a build decided on the device (ctl->lgrid), else G
 __global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0, const Ctl* ctl,
-                               GridP G, uint32_t* cell) {
+                               GridP G, const GridP* Gd, uint32_t* cell) {
+    if (Gd && ctl->error) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.n) x64_to_split_atom(i, P, xr, x0, ctl, G, cell);
+    if (i < P.n) x64_to_split_atom(i, P, xr, x0, ctl, Gd ? *Gd : G, cell);
 }
 
 // Per slot of the neighbor list: the cell of i minus the cell of j, wrapped
@@ -899,10 +901,19 @@ int launch_split_to_x64(System& s) {
 }
 
 int launch_x64_to_split(System& s) {
-    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, s.cell);
+    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, nullptr, s.cell);
     // the cells may have changed: refresh the list's per-slot cell offsets
     if (s.nslots > 0 && s.nb_code)
         LAUNCH_S(k_slot_codes, s.n, s.n, s.nb_start, s.nb_j, s.cell, s.ctl, s.nb_code);
+    return 0;
+}
+
+// The split encoding after a list build decided on the device: its grid is
+// the control block's, and the slot offsets follow the rows the build wrote
+// (an emptied list has none).
+int launch_x64_to_split_dev(System& s) {
+    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, &s.ctl->lgrid, s.cell);
+    if (s.nb_code) LAUNCH_S(k_slot_codes, s.n, s.n, s.nb_start, s.nb_j, s.cell, s.ctl, s.nb_code);
     return 0;
 }
 

@@ -52,6 +52,12 @@ struct SplitConst {
     int pad;
 };
 
+struct GridP {
+    int g[3];
+    double csize[3];
+    int ncell;
+};
+
 struct Ctl {
     DevBox box;
     DevServo servo;
@@ -77,7 +83,17 @@ struct Ctl {
     // step reads, the step that ends its run, a pending segment-start
     // half-kick, and whether its run is complete
     int bpar, bend, bkick, bdone;
+    // a list build decided on the device (a run's rebuild redo): its bins
+    // and slot count
+    GridP lgrid;
+    int nslots;
+    int lpad;
 };
+
+// The control block's error code of a device-decided rebuild that needs the
+// host (a list beyond the capacity of its buffers): the run redoes the
+// step's list build on the host and continues. Never returned to a caller.
+#define TMPS_NEED_HOST -100
 
 // One ensemble member as the batched close kernel sees it.
 struct CloseEntry {
@@ -92,12 +108,6 @@ struct PairP {
     double kn, kt, gamman, gammat, xmu;
     int normal_model, damping_model, tangential_model, uses_history;
     double g_kn, g_damp, g_kt, g_xt, g_mu;
-};
-
-struct GridP {
-    int g[3];
-    double csize[3];
-    int ncell;
 };
 
 // One FP32-split ensemble member whose step is redone with a list rebuild,
@@ -267,6 +277,19 @@ int ensure_split(System& s);
 // is done; without it the host mirror is current only in the box and step
 int redo_step(System& s, int par, double dt, int use_cundall, double gl, double ga,
               bool pull_after = true);
+int redo_tail(System& s, int stp, double dt, int use_cundall, double gl, double ga,
+              bool pull_after);
+// A run's rebuild redo with the list build decided on the device (api.hip).
+struct DevRedo {
+    bool ok = false;       // the system's buffers allow it this run
+    bool built = false;    // some redo of this run built a list on the device
+    bool pending = false;  // the last such redo's outcome is not yet read
+    double max_cutoff = 0.0;
+    int max_cells = 0;
+    // the host record before the last redo, taken back should it fail
+    int stp = 0, build_count = 0, last_build_step = 0, has_built = 0;
+    bool seed_pending = false, have_old = false;
+};
 int init_simulation(System& s, double dt, int use_cundall, double gl, double ga);
 int reneighbor(System& s, bool force, int* rebuilt);
 // reneighbor's build without its decision: a rebuild at step s.timestep
@@ -412,6 +435,13 @@ int launch_ctl_store(System& s);
 // records a list build (last_build_step, fast_img; FP32-split: sc.gi and the
 // split constants with the current box as the skin check's reference)
 int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], int split);
+// the same for a list build decided on the device (its grid from the control
+// block; nothing on a recorded error)
+int launch_ctl_built_dev(System& s, int step);
+// a list build decided on the device (k_rebuild.hip)
+int launch_build_dev(System& s, double max_cutoff, int max_cells);
+// the split encoding after it (k_split.hip)
+int launch_x64_to_split_dev(System& s);
 int launch_remap_apply_fused(System& s);
 void drop_graphs(System& s);
 int launch_final_tree(System& s, int nc, int nparts);
