@@ -17,10 +17,10 @@
 
 #include "tmps_internal.h"
 
-// steps per captured graph segment of a device-resident run
-// Steps per captured segment. A rebuild request wastes on average half a
-// segment plus the one queued behind it (their kernels exit at once), while
-// each segment costs one status read; 16 balances the two at ~70 us/step.
+// The longest captured segment of a device-resident run's steps (a run's
+// last segment may take up to twice as many). A rebuild request wastes on
+// average half a segment plus the one queued behind it (their kernels exit
+// at once), while each segment costs one status read.
 #define TMPS_SEG_LEN 16
 
 namespace {
@@ -742,10 +742,36 @@ int launch_segment(System& s, int len, int kick, int last, double dt, int use_cu
     return 0;
 }
 
+// The segment length policy: a power of two from 4 to TMPS_SEG_LEN, at most
+// a quarter of the typical number of steps between list rebuilds (an
+// exponential mean of the gaps, raised again by a long stretch without one).
+void seg_policy(System& s) {
+    int m = 4;
+    while (m < TMPS_SEG_LEN && 2.0 * m <= 0.5 * s.rebuild_gap) m *= 2;
+    s.seg_max = m;
+}
+
+void note_rebuild(System& s, int step) {
+    if (s.last_rebuild_step >= 0 && step > s.last_rebuild_step)
+        s.rebuild_gap = 0.5 * s.rebuild_gap + 0.5 * (double)(step - s.last_rebuild_step);
+    s.last_rebuild_step = step;
+    seg_policy(s);
+}
+
+void note_quiet(System& s, int step) {
+    if (s.last_rebuild_step >= 0 && step - s.last_rebuild_step > s.rebuild_gap) {
+        s.rebuild_gap = (double)(step - s.last_rebuild_step);
+        seg_policy(s);
+    }
+}
+
 // Segment lengths of a run of nsteps from step t (no rebuild requests): up
-// to TMPS_SEG_LEN steps, even except the last, which ends the run.
-int next_segment_len(int t, int t_end, int* last) {
-    int len = std::min(TMPS_SEG_LEN, t_end - t);
+// to seg_max steps, even except the last, which ends the run.
+// With rare rebuilds, a rest of at most twice TMPS_SEG_LEN steps goes in
+// one segment.
+int next_segment_len(int t, int t_end, int seg_max, int* last) {
+    int len = std::min(seg_max, t_end - t);
+    if (seg_max == TMPS_SEG_LEN && t_end - t <= 2 * TMPS_SEG_LEN) len = t_end - t;
     *last = t + len == t_end;
     if (!*last && (len & 1)) --len;
     return len;
@@ -1172,7 +1198,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     while (true) {
         while (nin < 2 && t < t_end) {
             int last = 0;
-            const int len = next_segment_len(t, t_end, &last);
+            const int len = next_segment_len(t, t_end, s.seg_max, &last);
             rc = launch_segment(s, len, kick, last, dt, use_cundall, gl, ga);
             kick = 0;
             if (rc) return rc;
@@ -1203,6 +1229,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
             if (s.has_built) s.current_step = s.timestep;
             mirror_back = sg.last != 0;
             R.pending = false;
+            note_quiet(s, done);
             continue;
         }
         if (R.ok && aborted && !err && st4[1] == done + 1) {
@@ -1216,6 +1243,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
             const int stp = done + 1;
             rc = redo_dev(s, R, ((stp - 1) - sg.start) % 2, stp, dt, use_cundall, gl, ga);
             if (rc) return rc;
+            note_rebuild(s, stp);
             t = stp;
             kick = 1;
             mirror_back = false;
@@ -1243,6 +1271,7 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
         const int stp = s.hctl->timestep + 1;
         rc = redo_step(s, ((stp - 1) - sg.start) % 2, dt, use_cundall, gl, ga);
         if (rc) return rc;
+        note_rebuild(s, stp);
         R.built = false;
         rc = dev_redo_prepare(s, R);
         if (rc) return rc;
@@ -1281,7 +1310,7 @@ int prepare_run(System& s, int nsteps, double dt, int use_cundall, double gl, do
     int t = s.timestep, kick = 1;
     while (t < t_end) {
         int last = 0;
-        const int len = next_segment_len(t, t_end, &last);
+        const int len = next_segment_len(t, t_end, s.seg_max, &last);
         int slot = -1;
         const int rc = segment_graph(s, len, kick, last, dt, use_cundall, gl, ga, &slot);
         if (rc) return rc;

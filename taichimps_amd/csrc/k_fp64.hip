@@ -485,11 +485,32 @@ __device__ __forceinline__ void step_end_block(Ctl* ctl, const double* partial, 
     }
 }
 
+// step_end_block on the close's tree (the same tree): one block of
+// CLOSE_THREADS over at most CLOSE_SPAN partials (the launcher reduces
+// aligned groups first above that).
 template <int L>
-__global__ void __launch_bounds__(FINAL_THREADS) k_step_end(Ctl* ctl, const double* partial,
+__global__ void __launch_bounds__(CLOSE_THREADS) k_step_end(Ctl* ctl, const double* partial,
                                                              int nparts, double dt, int advance,
                                                              int apply_servo, int natoms) {
-    step_end_block<L>(ctl, partial, nparts, dt, advance, apply_servo, natoms);
+    if (advance) {
+        if (ctl->abort) return;
+        const int t = ctl->timestep + 1;
+        if (ctl->trigger == t) {
+            if (threadIdx.x == 0) ctl->abort = 1;
+            return;
+        }
+    }
+    double out[8];
+    close_tree<L>(partial, nparts, out);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; ++k) ctl->vsum[k] = out[k];
+        ctl->ntouch = (int)out[6];
+        if (apply_servo) {
+            servo_update(ctl, out, dt, natoms);
+            if (!ctl->abort) update_split_consts(ctl, 0);
+        }
+        if (advance && !ctl->abort) ctl->timestep = ctl->timestep + 1;
+    }
 }
 
 // End of a fused step, in one block, for passes that publish their block
@@ -1571,16 +1592,27 @@ int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, dou
     }
 }
 
-#define END_CASE(Lv)                                                                         \
-    case Lv:                                                                                 \
-        hipLaunchKernelGGL((k_step_end<Lv>), dim3(1), dim3(FINAL_THREADS), 0, s.stream, s.ctl, \
-                           s.partial, nparts, dt, advance, apply_servo, s.n);                \
+#define END_CASE(Lv)                                                                          \
+    case Lv:                                                                                  \
+        hipLaunchKernelGGL((k_step_end<Lv>), dim3(1), dim3(CLOSE_THREADS), 0, s.stream, s.ctl, \
+                           src, nparts, dt, advance, apply_servo, s.n);                       \
         break;
 
 int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo) {
     if (nparts < 1) nparts = 1;
-    switch (final_leaves(nparts)) {
-        END_CASE(1) END_CASE(2) END_CASE(4) END_CASE(8) END_CASE(16) END_CASE(32) END_CASE(64)
+    const double* src = s.partial;
+    if (nparts > CLOSE_SPAN) {
+        const int groups = (nparts + CLOSE_SPAN - 1) / CLOSE_SPAN;
+        if (groups > CLOSE_SPAN || groups > s.level_cap)
+            return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
+        hipLaunchKernelGGL(k_tree_level, dim3(groups), dim3(CLOSE_THREADS), 0, s.stream, s.partial,
+                           nparts, s.level);
+        HIP_TRY(hipGetLastError());
+        src = s.level;
+        nparts = groups;
+    }
+    switch (final_leaves(nparts, CLOSE_THREADS)) {
+        END_CASE(1) END_CASE(2) END_CASE(4) END_CASE(8)
         default: return tmps_fail(TMPS_ERR_CAPACITY, "too many atoms for the reduction tree");
     }
     HIP_TRY(hipGetLastError());

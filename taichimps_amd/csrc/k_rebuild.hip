@@ -874,7 +874,8 @@ __global__ void k_scan_tiles_dev(const Ctl* ctl, const int* in, int* out, int* t
 
 __global__ void k_scan_sums_dev(const Ctl* ctl, int* tile_sums, int ntiles, int* out) {
     if (ctl->error) return;
-    scan_sums(tile_sums, ntiles, out + ctl->lgrid.ncell, nullptr);
+    const int m = ctl->lgrid.ncell;
+    scan_sums(tile_sums, min(ntiles, (m + SCAN_TILE - 1) / SCAN_TILE), out + m, nullptr);
 }
 
 __global__ void k_scan_add_dev(const Ctl* ctl, int* out, const int* tile_sums) {

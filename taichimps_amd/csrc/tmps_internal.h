@@ -234,13 +234,20 @@ struct System {
     int servo_step_count = 0, servo_nsteps = 0;
 
     // captured step segments, keyed by their launch configuration
-    static const int kGraphSlots = 4;
-    hipGraphExec_t seg_exec[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
-    uint64_t seg_key[kGraphSlots] = {0, 0, 0, 0};
+    static const int kGraphSlots = 12;
+    hipGraphExec_t seg_exec[kGraphSlots] = {};
+    uint64_t seg_key[kGraphSlots] = {};
     int seg_next = 0;
     int* hstatus = nullptr;  // pinned: [slot][timestep, trigger, abort, error]
     double* hsum = nullptr;  // pinned: 8 doubles, a reduction result read back
-    hipEvent_t seg_event[kGraphSlots] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t seg_event[kGraphSlots] = {};
+    // the longest segment a run launches, from the steps between its recent
+    // list rebuilds: a rebuild request leaves the rest of its segment and
+    // the one queued behind it to return at once, so frequent rebuilds call
+    // for short segments (and rare ones for long)
+    int seg_max = 16;
+    int last_rebuild_step = -1;
+    double rebuild_gap = 64.0;
 
     int launch_mode = TMPS_LAUNCH_GRAPH;
 
