@@ -874,6 +874,7 @@ int dev_redo_prepare(System& s, DevRedo& R) {
     R.max_cells = std::max(s.max_grid_cells, (int)std::min<long long>(4LL * s.n, 1LL << 26));
     int rc = ensure_cells(s, R.max_cells);
     if (!rc) rc = list_reserve_rows(s);
+    if (!rc) rc = bins_reserve(s);
     if (!rc && s.nb_cap > 0 && (s.onb_j == nullptr || s.onb_cap < s.nb_cap)) {
         HIP_TRY(hipStreamSynchronize(s.stream));
         dfree(s.onb_j);
@@ -1515,6 +1516,8 @@ void free_all(System& s) {
     if (s.onb_hist) (void)hipFree(s.onb_hist);
     s.nb_hist = s.onb_hist = nullptr;
     dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
+    dfree(s.rs_k1); dfree(s.rs_k2); dfree(s.rs_v1); dfree(s.rs_v2); dfree(s.rs_hist);
+    dfree(s.rs_offs);
     dfree(s.cell_particles);
     dfree(s.bin_x);
     dfree(s.full_count); dfree(s.half_count); dfree(s.scan_tmp); dfree(s.flags);

@@ -920,6 +920,132 @@ __global__ void k_list_check(Ctl* ctl, const int* flags, int* nb_start, int n, l
     }
 }
 
+// ------------------------------------------------ bins by a radix sort
+//
+// NeighborList.build_grid's counting sort (neighbor.py:247-282) as a stable
+// least-significant-digit radix sort of the atoms' cells, 8 bits a pass:
+// per tile of RS_TILE atoms a digit histogram in LDS, an exclusive scan over
+// (digit, tile), and a stable scatter whose ranks come from wavefront ballots.
+// Stability keeps each bin's atoms in ascending index, the order the atomic
+// counting sort reaches with its per-bin insertion sort, so the bins are the
+// same; no global atomics are needed. Gd: a build decided on the device
+// (its grid in the control block, nothing done on a recorded error).
+
+#define RS_DIGITS 256
+#define RS_ROUNDS 16
+#define RS_TILE (TMPS_BLOCK * RS_ROUNDS)
+static_assert(TMPS_BLOCK == RS_DIGITS, "one thread per digit of a tile's histogram");
+
+// The cell of every atom (the sort key, into keys) and each tile's
+// histogram of the first digit, digit-major: hist[d * ntiles + tile].
+__global__ void k_rs_keys(const double4* xr, int n, const Ctl* ctl, GridP G, const GridP* Gd,
+                          int* keys, int* hist, int ntiles) {
+    if (Gd && ctl->error) return;
+    __shared__ int h[RS_DIGITS];
+    const GridP g = Gd ? *Gd : G;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int i = base + r * TMPS_BLOCK + threadIdx.x;
+        if (i < n) {
+            int c[3];
+            cell_coord(ctl->box, g, xr[i], c);
+            const int key = c[0] + c[1] * g.g[0] + c[2] * g.g[0] * g.g[1];
+            keys[i] = key;
+            atomicAdd(&h[key & (RS_DIGITS - 1)], 1);
+        }
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Each tile's histogram of the digit at `shift`.
+__global__ void k_rs_hist(const Ctl* ctl, const GridP* Gd, const int* keys, int n, int shift,
+                          int* hist, int ntiles) {
+    if (Gd && ctl->error) return;
+    __shared__ int h[RS_DIGITS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int i = base + r * TMPS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_DIGITS - 1)], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// The stable scatter of one pass. A tile goes in RS_ROUNDS rounds of
+// TMPS_BLOCK consecutive keys; in a round each lane finds the lanes of its
+// wavefront holding the same digit (eight ballots), its rank among them, and
+// the counts of the earlier wavefronts, so keys of one digit keep their order
+// within the round, rounds follow each other, and offs (the scanned
+// histograms) orders the tiles. vin null: the values are the keys' indices.
+__global__ void k_rs_scatter(const Ctl* ctl, const GridP* Gd, const int* kin, const int* vin,
+                             int* kout, int* vout, int n, int shift, const int* offs,
+                             int ntiles) {
+    if (Gd && ctl->error) return;
+    constexpr int NW = TMPS_BLOCK / 64;
+    __shared__ int base[RS_DIGITS];
+    __shared__ int wcnt[NW][RS_DIGITS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    base[threadIdx.x] = offs[threadIdx.x * ntiles + blockIdx.x];
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const int i = blockIdx.x * RS_TILE + r * TMPS_BLOCK + threadIdx.x;
+        const bool valid = i < n;
+        int key = 0, val = 0, d = 0;
+        if (valid) {
+            key = kin[i];
+            val = vin ? vin[i] : i;
+            d = (key >> shift) & (RS_DIGITS - 1);
+        }
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = ((d >> b) & 1) != 0;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = base[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) add += wcnt[w][threadIdx.x];
+        base[threadIdx.x] += add;
+    }
+}
+
+// Bin starts from the sorted keys: the atoms of bin c sit at [start[c],
+// start[c + 1]); empty bins take the next bin's start.
+__global__ void k_rs_starts(const Ctl* ctl, GridP G, const GridP* Gd, const int* skeys, int n,
+                            int* start) {
+    if (Gd && ctl->error) return;
+    const int ncell = Gd ? Gd->ncell : G.ncell;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n; p += gridDim.x * blockDim.x) {
+        const int k = p < n ? skeys[p] : ncell;
+        const int kp = p > 0 ? skeys[p - 1] : -1;
+        for (int c = kp + 1; c <= k; ++c) start[c] = p;
+    }
+}
+
+__global__ void k_zero_flags(const Ctl* ctl, int guard, int* flags) {
+    if (guard && ctl->error) return;
+    if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+}
+
 static inline dim3 grid_for(int n) { return dim3(nblocks_for(n > 0 ? n : 1)); }
 
 int launch_pbc(System& s) {
@@ -988,30 +1114,102 @@ int launch_scan(System& s, const int* in, int* out, int n, int* total_copy) {
     return 0;
 }
 
+int launch_bins_radix(System& s, const double4* xr, const GridP& G, const GridP* Gd,
+                      int max_ncell);
+
 int launch_bins(System& s, const double4* xr) {
-    const int ncell = s.grid.ncell;
-    // bin counts and fill cursors, and the list build's flags
-    hipLaunchKernelGGL(k_zero_int3, grid_for(ncell + 1), dim3(TMPS_BLOCK), 0, s.stream, s.cell_count,
-                       s.cell_fill, ncell + 1, s.flags, 4);
+    // the list build's flags
+    hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s.stream, s.ctl, 0, s.flags);
     HIP_TRY(hipGetLastError());
-    if (s.n > 0) {
-        hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl,
-                           s.grid, nullptr, s.cell_of, s.cell_count);
-        HIP_TRY(hipGetLastError());
+    return launch_bins_radix(s, xr, s.grid, nullptr, s.grid.ncell);
+}
+
+
+// The radix binning (k_rs_*): cell_of holds each atom's cell, cell_start /
+// cell_particles the bins. max_ncell bounds the grid's cells (the passes
+// cover its bits); G / Gd as for k_bin_count.
+static int rs_reserve(System& s, int ntiles) {
+    const size_t nh = (size_t)RS_DIGITS * ntiles + 1;
+    const size_t na = (size_t)std::max(s.max_atoms, 1);
+    if (!s.rs_k1 || s.rs_na < na) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        for (int** p : {&s.rs_k1, &s.rs_k2, &s.rs_v1, &s.rs_v2}) {
+            if (*p) HIP_TRY(hipFree(*p));
+            HIP_TRY(hipMalloc(p, na * sizeof(int)));
+        }
+        s.rs_na = na;
     }
-    int rc = launch_scan(s, s.cell_count, s.cell_start, ncell, nullptr);
-    if (rc) return rc;
-    if (s.n > 0) {
-        hipLaunchKernelGGL(k_bin_fill, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n, s.cell_of,
-                           s.cell_start, s.cell_fill, s.cell_particles);
-        HIP_TRY(hipGetLastError());
+    if (!s.rs_hist || s.rs_nh < nh) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        for (int** p : {&s.rs_hist, &s.rs_offs}) {
+            if (*p) HIP_TRY(hipFree(*p));
+            HIP_TRY(hipMalloc(p, nh * sizeof(int)));
+        }
+        s.rs_nh = nh;
     }
-    hipLaunchKernelGGL(k_bin_sort, grid_for(ncell), dim3(TMPS_BLOCK), 0, s.stream, ncell,
-                       s.cell_start, s.cell_particles);
-    HIP_TRY(hipGetLastError());
+    const int need_scan = (int)((nh + SCAN_TILE - 1) / SCAN_TILE) + 1;
+    if (s.scan_tmp_cap < need_scan) {
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        if (s.scan_tmp) HIP_TRY(hipFree(s.scan_tmp));
+        HIP_TRY(hipMalloc(&s.scan_tmp, (size_t)need_scan * sizeof(int)));
+        s.scan_tmp_cap = need_scan;
+    }
     return 0;
 }
 
+int bins_reserve(System& s) {
+    return s.n > 0 ? rs_reserve(s, (s.n + RS_TILE - 1) / RS_TILE) : 0;
+}
+
+int launch_bins_radix(System& s, const double4* xr, const GridP& G, const GridP* Gd,
+                      int max_ncell) {
+    if (s.n <= 0) return 0;
+    const int ntiles = (s.n + RS_TILE - 1) / RS_TILE;
+    int rc = rs_reserve(s, ntiles);
+    if (rc) return rc;
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < (long long)max_ncell) ++bits;
+    const int passes = (bits + 7) / 8;
+    const int nh = RS_DIGITS * ntiles;
+    const int scan_tiles = (nh + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_rs_keys, dim3(ntiles), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n, s.ctl, G,
+                       Gd, s.cell_of, s.rs_hist, ntiles);
+    HIP_TRY(hipGetLastError());
+    // pass p reads (kin, vin) and writes (kout, vout); the last writes the bins
+    const int* kin = s.cell_of;
+    const int* vin = nullptr;
+    int* kbuf[2] = {s.rs_k2, s.rs_k1};
+    int* vbuf[2] = {s.rs_v1, s.rs_v2};
+    for (int p = 0; p < passes; ++p) {
+        if (p > 0) {
+            hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, Gd,
+                               kin, s.n, 8 * p, s.rs_hist, ntiles);
+            HIP_TRY(hipGetLastError());
+        }
+        // exclusive scan of the (digit, tile) counts, several blocks always
+        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(TMPS_BLOCK), 0, s.stream,
+                           (const int*)s.rs_hist, s.rs_offs, nh, s.scan_tmp);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.scan_tmp,
+                           scan_tiles, s.rs_offs + nh, (int*)nullptr);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_add, grid_for(nh), dim3(TMPS_BLOCK), 0, s.stream, s.rs_offs, nh,
+                           (const int*)s.scan_tmp);
+        HIP_TRY(hipGetLastError());
+        int* kout = kbuf[p & 1];
+        int* vout = p == passes - 1 ? s.cell_particles : vbuf[p & 1];
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, Gd,
+                           kin, vin, kout, vout, s.n, 8 * p, (const int*)s.rs_offs, ntiles);
+        HIP_TRY(hipGetLastError());
+        kin = kout;
+        vin = vout;
+    }
+    const int sblocks = std::min(nblocks_for(s.n + 1), 2048);
+    hipLaunchKernelGGL(k_rs_starts, dim3(sblocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, G, Gd,
+                       kin, s.n, s.cell_start);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 static int grow_list(System& s, size_t need) {
     if (need <= s.nb_cap) return 0;
@@ -1103,30 +1301,10 @@ int launch_build_dev(System& s, double max_cutoff, int max_cells) {
     hipLaunchKernelGGL(k_list_grid, dim3(1), dim3(64), 0, s.stream, s.ctl, max_cutoff, s.skin,
                        max_cells, s.cell_cap, split);
     HIP_TRY(hipGetLastError());
-    const int cell_blocks = std::min(nblocks_for(std::max(s.cell_cap, 1)), 2048);
-    hipLaunchKernelGGL(k_zero_cells_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
-                       s.cell_count, s.cell_fill, s.flags);
+    hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s.stream, s.ctl, 1, s.flags);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_bin_count, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n, s.ctl,
-                       none, Gd, s.cell_of, s.cell_count);
-    HIP_TRY(hipGetLastError());
-    const int ntiles = (std::max(s.cell_cap, 1) + SCAN_TILE - 1) / SCAN_TILE;
-    if (s.scan_tmp_cap < ntiles + 1) return tmps_fail(TMPS_ERR_ARG, "scan workspace too small");
-    hipLaunchKernelGGL(k_scan_tiles_dev, dim3(ntiles), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
-                       s.cell_count, s.cell_start, s.scan_tmp);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.scan_tmp,
-                       ntiles, s.cell_start);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_add_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
-                       s.cell_start, s.scan_tmp);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_bin_fill_dev, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.ctl, s.n,
-                       s.cell_of, s.cell_start, s.cell_fill, s.cell_particles);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_bin_sort_dev, dim3(cell_blocks), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
-                       s.cell_start, s.cell_particles);
-    HIP_TRY(hipGetLastError());
+    int rc = launch_bins_radix(s, s.xr, none, Gd, max_cells);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                        s.cell_particles, s.xr, s.bin_x);
     HIP_TRY(hipGetLastError());
@@ -1134,7 +1312,7 @@ int launch_build_dev(System& s, double max_cutoff, int max_cells) {
                        s.ctl, none, Gd, s.cell_start, s.cell_particles, s.bin_x, s.skin, s.max_nb,
                        s.full_count, s.half_count, s.flags, s.row_tmp, 1);
     HIP_TRY(hipGetLastError());
-    int rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
+    rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
     if (rc) return rc;
     hipLaunchKernelGGL(k_list_check, grid_for(s.n + 1), dim3(TMPS_BLOCK), 0, s.stream, s.ctl,
                        (const int*)s.flags, s.nb_start, s.n, (long long)s.nb_cap);

@@ -197,6 +197,10 @@ struct System {
     int *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr,
         *cell_particles = nullptr;
     int cell_cap = 0;
+    // the radix binning's keys, values and (digit, tile) counts
+    int *rs_k1 = nullptr, *rs_k2 = nullptr, *rs_v1 = nullptr, *rs_v2 = nullptr;
+    int *rs_hist = nullptr, *rs_offs = nullptr;
+    size_t rs_na = 0, rs_nh = 0;
     int *full_count = nullptr, *half_count = nullptr, *scan_tmp = nullptr, *flags = nullptr;
     int scan_tmp_cap = 0;
 
@@ -445,6 +449,8 @@ int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], in
 // the same for a list build decided on the device (its grid from the control
 // block; nothing on a recorded error)
 int launch_ctl_built_dev(System& s, int step);
+// the binning's work buffers for the current atom count (k_rebuild.hip)
+int bins_reserve(System& s);
 // a list build decided on the device (k_rebuild.hip)
 int launch_build_dev(System& s, double max_cutoff, int max_cells);
 // the split encoding after it (k_split.hip)
