@@ -218,6 +218,16 @@ int tmps_wall_post_force(tmps_wall* w, double dt, double coord, double vwall_n, 
 /* FixWallGran.shear / touch: per-atom shear (3n, nullable) and touch flags
  * (n, nullable) */
 int tmps_wall_get_history(tmps_wall* w, double* shear, int* touch, int n);
+/* The walls a device-resident run (tmps_run) applies inside its one-kernel
+ * step, as Simulation.run_gpu runs the fixes' post_force between the pair
+ * force and final_integrate (simulation.py:236-247; wall_gran.py:213-236):
+ * at most 2, before_cundall[k] set when wall k comes before fix
+ * damping/cundall among the fixes, coord[k] / vwall_s[k] a plane's position
+ * and in-plane velocity for the whole run (a wiggling plane is not fused),
+ * dt the run's timestep. Setup calls them with history_update off. The walls
+ * must outlive the setting; n = 0 clears it. */
+int tmps_run_set_walls(tmps_system* s, int n, tmps_wall* const* walls, const int* before_cundall,
+                       const double* coord, const double* vwall_s, double dt);
 
 /* ---- computes (computes/thermo.py) --------------------------------------- */
 /* compute_virial_kernel, thermo.py:62-109: sum of the per-atom virial

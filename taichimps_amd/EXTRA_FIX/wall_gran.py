@@ -155,6 +155,18 @@ class FixWallGran(Fix):
         """Restart the clock the wall motion is referenced to."""
         self.elapsed = 0.0
 
+    def _fused(self, atom):
+        """(device wall, position, in-plane velocity) for a device-resident
+        run's step, or None for a wiggling plane (its position follows the
+        clock on the host)."""
+        if self.wiggle_amplitude != 0.0:
+            return None
+        return self._bind(atom), self.wall_coord, self.shear_vel
+
+    def _advance(self, calls: int, dt: float) -> None:
+        """The clock after `calls` post_force calls made on the device."""
+        self.elapsed += calls * dt
+
 
 class FixWallGranRegion(FixWallGran):
     """fix wall/gran/region with a cylinder region (reference:

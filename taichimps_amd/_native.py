@@ -54,7 +54,7 @@ SYMBOLS = (
     "tmps_contact_atom", "tmps_fabric", "tmps_stress_atom", "tmps_history_seed",
     "tmps_wall_create", "tmps_wall_destroy", "tmps_wall_reset", "tmps_wall_post_force",
     "tmps_wall_get_history", "tmps_prepare_run", "tmps_set_stream", "tmps_get_stream",
-    "tmps_fabric_noalloc",
+    "tmps_fabric_noalloc", "tmps_run_set_walls",
 )
 
 
@@ -173,6 +173,7 @@ def load(path: str | None = None):
         "tmps_wall_post_force": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_int]),
         "tmps_wall_get_history": (C.c_int, [vp, dp, ip, C.c_int]),
+        "tmps_run_set_walls": (C.c_int, [vp, C.c_int, C.POINTER(vp), ip, dp, dp, C.c_double]),
         "tmps_fabric": (C.c_int, [vp, dp, ip]),
         "tmps_fabric_noalloc": (C.c_int, [vp, dp, ip]),
         "tmps_stress_atom": (C.c_int, [vp, C.c_int, dp, dp]),
@@ -435,6 +436,16 @@ class NativeSystem:
         gl, ga = (0.0, 0.0) if cundall is None else cundall
         check(self._L.tmps_run(self._h, int(nsteps), float(dt), 1 if run_pre else 0,
                                0 if cundall is None else 1, float(gl), float(ga)))
+
+    def set_run_walls(self, walls, dt):
+        """The walls of the next device-resident runs: (NativeWall,
+        before_cundall, coord, vwall_s) in the fixes' order; [] clears them."""
+        n = len(walls)
+        hs = (C.c_void_p * max(n, 1))(*[w[0]._h.value for w in walls])
+        bc = np.array([1 if w[1] else 0 for w in walls] or [0], dtype=np.int32)
+        co = np.array([float(w[2]) for w in walls] or [0.0])
+        vs = np.array([float(w[3]) for w in walls] or [0.0])
+        check(self._L.tmps_run_set_walls(self._h, n, hs, _ip(bc), _dp(co), _dp(vs), float(dt)))
 
     def prepare_run(self, nsteps, dt, cundall=None):
         gl, ga = (0.0, 0.0) if cundall is None else cundall

@@ -41,9 +41,11 @@ def run_forward(sim, nsteps: int, every: int):
     if plan is None:
         raise NotImplementedError("the adjoint needs fix nve/sphere, a pair style, a neighbor "
                                   "list and contact history, with device-resident fixes only")
-    cundall, deform = plan
+    cundall, deform, walls = plan
     if cundall is not None:
         raise NotImplementedError("the adjoint does not cover fix damping/cundall")
+    if walls:
+        raise NotImplementedError("the adjoint does not cover wall fixes")
     if not isinstance(deform, FixDeformPressure):
         raise NotImplementedError("the adjoint differentiates the pressure of fix deform/pressure")
     sim.setup_run(nsteps)

@@ -558,6 +558,19 @@ int servo_end_of_step(System& s, double dt, double* tensor_out) {
 }
 
 // Verlet::setup as Simulation.init_simulation does it, simulation.py:170-203
+// A fused run's walls that come before (pass 0) or after (pass 1)
+// damping/cundall, as post_force calls them in setup (history_update off).
+static int setup_walls(System& s, int pass, double dt) {
+    for (int q = 0; q < s.nrun_walls; ++q) {
+        const WallSpec& w = s.run_spec[q];
+        if ((w.before_cundall ? 0 : 1) != pass) continue;
+        int rc = split_positions_to_xr(s);
+        if (!rc) rc = launch_wall_gran(s, s.run_wh[q][0], w.p, dt, w.coord, w.vwall_n, w.vwall_s, 0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 int init_simulation(System& s, double dt, int use_cundall, double gl, double ga) {
     s.setup_done = 1;
     int rc = clear_forces(s);
@@ -568,8 +581,11 @@ int init_simulation(System& s, double dt, int use_cundall, double gl, double ga)
     s.rebuilds += rebuilt;
     rc = pair_compute(s, dt, 0, 1);
     if (rc) return rc;
+    rc = setup_walls(s, 0, dt);
+    if (rc) return rc;
     if (use_cundall) rc = cundall(s, gl, ga);
-    return rc;
+    if (rc) return rc;
+    return setup_walls(s, 1, dt);
 }
 
 uint64_t mix(uint64_t h, uint64_t v) {
@@ -700,6 +716,9 @@ int segment_graph(System& s, int len, int kick, int last, double dt, int use_cun
     key = mix(key, (uint64_t)(s.have_servo && s.servo.enabled));
     key = mix(key, (uint64_t)(uintptr_t)s.nb_j);
     key = mix(key, (uint64_t)(uintptr_t)s.nb_hist);
+    key = mix(key, (uint64_t)s.nrun_walls);
+    for (int q = 0; q < s.nrun_walls; ++q)
+        key = mix(key, (uint64_t)(uintptr_t)s.run_wh[q][0]);
     int slot = -1;
     for (int k = 0; k < System::kGraphSlots; ++k)
         if (s.seg_exec[k] && s.seg_key[k] == key) slot = k;
@@ -1517,6 +1536,7 @@ void free_all(System& s) {
     s.nb_hist = s.onb_hist = nullptr;
     dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
     dfree(s.rs_k1); dfree(s.rs_k2); dfree(s.rs_v1); dfree(s.rs_v2); dfree(s.rs_hist);
+    dfree(s.d_run_spec);
     dfree(s.rs_offs);
     dfree(s.cell_particles);
     dfree(s.bin_x);
@@ -2079,6 +2099,9 @@ struct tmps_wall {
     System* sys = nullptr;
     tmps_wall_params p{};
     double4* hist = nullptr;  // [max_atoms]: shear xyz, touch flag in w
+    // the history of a fused run's second buffer set (a step reads one set
+    // and writes the other); a run leaves its results in hist
+    double4* hist2 = nullptr;
     int cap = 0;
 };
 
@@ -2101,7 +2124,9 @@ int tmps_wall_create(tmps_system* h, const tmps_wall_params* p, tmps_wall** out)
     w->p = *p;
     w->cap = s.max_atoms;
     int rc = dalloc(s.stream, &w->hist, (size_t)std::max(s.max_atoms, 1));
+    if (!rc) rc = dalloc(s.stream, &w->hist2, (size_t)std::max(s.max_atoms, 1));
     if (rc) {
+        dfree(w->hist);
         delete w;
         return rc;
     }
@@ -2114,6 +2139,7 @@ int tmps_wall_destroy(tmps_wall* w) {
     // touches only the wall's own memory: the system may already be gone
     // (hipFree waits for work that still uses the buffer)
     dfree(w->hist);
+    dfree(w->hist2);
     delete w;
     return 0;
 }
@@ -2121,6 +2147,38 @@ int tmps_wall_destroy(tmps_wall* w) {
 int tmps_wall_reset(tmps_wall* w) {
     if (!w) return tmps_fail(TMPS_ERR_ARG, "null wall");
     HIP_TRY(hipMemsetAsync(w->hist, 0, sizeof(double4) * (size_t)w->cap, w->sys->stream));
+    HIP_TRY(hipMemsetAsync(w->hist2, 0, sizeof(double4) * (size_t)w->cap, w->sys->stream));
+    return 0;
+}
+
+int tmps_run_set_walls(tmps_system* h, int n, tmps_wall* const* walls, const int* before_cundall,
+                       const double* coord, const double* vwall_s, double dt) {
+    System& s = *sys_view(h);
+    if (n < 0 || n > TMPS_RUN_WALLS)
+        return tmps_fail(TMPS_ERR_UNSUPPORTED, "a device-resident run applies at most 2 walls");
+    if (n > 0 && (!walls || !before_cundall || !coord || !vwall_s))
+        return tmps_fail(TMPS_ERR_ARG, "null wall arguments");
+    WallSpec spec[TMPS_RUN_WALLS] = {};
+    for (int q = 0; q < n; ++q) {
+        const tmps_wall* w = walls[q];
+        if (!w || w->sys != &s) return tmps_fail(TMPS_ERR_ARG, "a wall of another system");
+        if (s.n > w->cap) return tmps_fail(TMPS_ERR_CAPACITY, "more atoms than the wall's history");
+        spec[q].p = w->p;
+        spec[q].coord = coord[q];
+        spec[q].vwall_n = 0.0;
+        spec[q].vwall_s = vwall_s[q];
+        spec[q].dt = dt;
+        spec[q].before_cundall = before_cundall[q] ? 1 : 0;
+    }
+    if (n > 0 && !s.d_run_spec) HIP_TRY(hipMalloc(&s.d_run_spec, sizeof(WallSpec) * TMPS_RUN_WALLS));
+    if (n > 0 && std::memcmp(spec, s.run_spec, sizeof(spec)) != 0)
+        HIP_TRY(copy_sync(s.stream, s.d_run_spec, spec, sizeof(spec), hipMemcpyHostToDevice));
+    std::memcpy(s.run_spec, spec, sizeof(spec));
+    s.nrun_walls = n;
+    for (int q = 0; q < TMPS_RUN_WALLS; ++q) {
+        s.run_wh[q][0] = q < n ? walls[q]->hist : nullptr;
+        s.run_wh[q][1] = q < n ? walls[q]->hist2 : nullptr;
+    }
     return 0;
 }
 

@@ -11,6 +11,7 @@
 #include "fused.h"
 #include "servo.h"
 #include "tmps_internal.h"
+#include "wall.h"
 
 using namespace tmps;
 
@@ -31,6 +32,7 @@ struct F64Ptrs {
     uint8_t* nb_flag;
     double4* nb_hist;
     int n;
+    WallRefs W;  // the fused run's walls (nwall 0 elsewhere)
 };
 
 F64Ptrs ptrs(System& s, int in_buf = 0, int out_buf = 0) {
@@ -43,7 +45,20 @@ F64Ptrs ptrs(System& s, int in_buf = 0, int out_buf = 0) {
     p.nb_start = s.nb_start; p.nb_j = s.nb_j; p.nb_flag = s.nb_flag;
     p.nb_hist = static_cast<double4*>(s.nb_hist);
     p.n = s.n;
+    p.W = WallRefs{};
     return p;
+}
+
+// A fused step's walls before (pass 0) or after (pass 1) damping/cundall on
+// atom i at its drifted position, with its half-step v and omega
+// (FixWallGran.post_force between the pair force and final_integrate,
+// simulation.py:236-247).
+__device__ __forceinline__ void f64_walls(const WallRefs& W, int pass, const double4& x,
+                                          const double4& v, const double4& w, int i,
+                                          double F[3], double T[3]) {
+    const double xs[3] = {x.x, x.y, x.z};
+    const double vd[3] = {v.x, v.y, v.z}, wd[3] = {w.x, w.y, w.z};
+    walls_apply<double>(W, pass, xs, x.w, v.w, vd, wd, W.spec[0].dt, i, F, T);
 }
 
 // What the contact gather needs of the box.
@@ -711,7 +726,7 @@ __global__ void k_step_begin_f64(F64Ptrs P, Ctl* ctl, double dt, int kick) {
 // Step t after the drift: contact forces, damping/cundall, final_integrate,
 // the block's share of the virial(+kinetic) tree, and either the next step's
 // half-kick (mid-segment) or the stored f/torque/virial (segment end).
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 __global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, double gl, double ga,
                                  double* partial) {
     if (ctl->abort) return;
@@ -728,6 +743,7 @@ __global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, doubl
         double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int nt = 0;
         gather_f64<STYLE>(i, xi, v, w, P, B, pp, dt, 1, F, T, V, nt);
+        if (WALL) f64_walls(P.W, 0, xi, v, w, i, F, T);
         if (CUNDALL) {
             F[0] *= cundall_scale(F[0] * v.x, gl);
             T[0] *= cundall_scale(T[0] * w.x, ga);
@@ -736,6 +752,7 @@ __global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, doubl
             F[2] *= cundall_scale(F[2] * v.z, gl);
             T[2] *= cundall_scale(T[2] * w.z, ga);
         }
+        if (WALL) f64_walls(P.W, 1, xi, v, w, i, F, T);
         const double dtf = 0.5 * dt;
         const double m = v.w, r = xi.w;
         const double inertia = 0.4 * m * r * r;
@@ -801,6 +818,7 @@ struct Fused64 {
     double* vir;
     const int *nb_start, *nb_j;
     int n;
+    WallRefs W;  // the run's walls (nwall 0 without)
 };
 
 // Where atom j is at step t: step t-1's pending remap, then the drift with
@@ -820,7 +838,7 @@ __device__ __forceinline__ double4 step_position(const RemapRec& R, bool remap, 
 // half-kick (or, at a segment's end, the stored f/torque/virial), and the
 // block's tree partial (publish_block); k_step_close ends the step. Nothing of
 // set `in` is written.
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_F64))) k_step_f64(Fused64 P, Ctl* ctl, PairP pp, double* partial,
                                                           double dt, double gl, double ga) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -957,6 +975,7 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
             for (int c = 0; c < 6; ++c) V[c] = V[c] + o.vir[c];
             ++nt;
         }
+        if (WALL) f64_walls(P.W, 0, xi, v, w, i, F, T);
         if (CUNDALL) {
             F[0] *= cundall_scale(F[0] * v.x, gl);
             T[0] *= cundall_scale(T[0] * w.x, ga);
@@ -965,6 +984,7 @@ __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu
             F[2] *= cundall_scale(F[2] * v.z, gl);
             T[2] *= cundall_scale(T[2] * w.z, ga);
         }
+        if (WALL) f64_walls(P.W, 1, xi, v, w, i, F, T);
         const double dtf = 0.5 * dt;
         const double m = v.w, r = xi.w;
         const double inertia = 0.4 * m * r * r;
@@ -1055,6 +1075,14 @@ __global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1,
     if (honor_abort && ctl->abort) return;
     copy_state_items<T4>(blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, x0, v0, w0,
                          x1, v1, w1, n, fl0, fl1, h0, h1, nb_start);
+}
+
+// The fused run's wall histories, set 1 -> set 0 (with k_copy_state).
+__global__ void k_copy_walls(WallRefs W, int n, const Ctl* ctl, int honor_abort) {
+    if (honor_abort && ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < W.nwall; ++q) W.h_out[q][i] = W.h_in[q][i];
 }
 
 // ------------------------------------------------ batched redo (batch.hip)
@@ -1224,218 +1252,55 @@ __global__ void k_stress_atom(const double* vir, const double4* vm64, const floa
 
 // ------------------------------------------------------------------ walls
 
-// FixWallGran.post_force_kernel, wall_gran.py:120-211: the wall as a partner
-// of zero radius and infinite mass (radsum = Reff = r_i, meff = m_i, moment
-// arm r_i); hooke normal force with mass-velocity damping and the
-// linear_history_classic tangential force, or the history-free viscous one.
-// wh[i] holds the shear history (xyz) and the touch flag (w).
+// FixWallGran.post_force_kernel, wall_gran.py:120-211 (wall_plane_contact),
+// and FixWallGranRegion.post_force_kernel, wall_gran_region.py:114-271
+// (wall_cyl_contact), one thread per atom: the wall's force and torque added
+// to the atom's, the shear history (xyz) and touch flag (w) in wh[i].
 // V: double4 (FP64 system) or float4 (FP32-split: v, omega, force and torque
 // in FP32, positions decoded to FP64 in xr); the contact is FP64 either way.
+template <typename V>
+__device__ __forceinline__ void wall_stage_add(V* f, V* tq, int i, const double dF[3],
+                                               const double dT[3]) {
+    V fo = f[i], to = tq[i];
+    fo.x = fo.x + dF[0];
+    fo.y = fo.y + dF[1];
+    fo.z = fo.z + dF[2];
+    to.x = to.x + dT[0];
+    to.y = to.y + dT[1];
+    to.z = to.z + dT[2];
+    f[i] = fo;
+    tq[i] = to;
+}
+
 template <typename V>
 __global__ void k_wall_gran(const double4* xr, const V* vm, const V* om, V* f, V* tq,
                             double4* wh, int n, tmps_wall_params P, double dt,
                             double coord, double vwall_n, double vwall_s, int history_update) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int axis = P.axis;
-    double nv[3] = {0.0, 0.0, 0.0};
-    nv[axis] = P.side == 1 ? -1.0 : 1.0;
-    double vwall[3] = {0.0, 0.0, 0.0};
-    vwall[axis] = vwall_n;
-    if (P.shear_axis >= 0) vwall[P.shear_axis] = vwall_s;
     const double4 x4 = xr[i];
-    const double ri = x4.w;
     const double xs[3] = {x4.x, x4.y, x4.z};
-    const double xa = xs[axis];
-    const double dist = P.side == -1 ? xa - coord : coord - xa;
-    const double delta = ri - dist;
-    if (delta <= 0.0) {
-        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return;
-    }
     const V v4 = vm[i], o4 = om[i];
-    const double meff = v4.w;
     const double v[3] = {v4.x, v4.y, v4.z}, w[3] = {o4.x, o4.y, o4.z};
-    double vr[3], vt[3], wr[3], vtr[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vr[k] = v[k] - vwall[k];
-    const double vnnr = vr[0] * nv[0] + vr[1] * nv[1] + vr[2] * nv[2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vt[k] = vr[k] - vnnr * nv[k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) wr[k] = ri * w[k];
-    const double wxn[3] = {wr[1] * nv[2] - wr[2] * nv[1], wr[2] * nv[0] - wr[0] * nv[2],
-                           wr[0] * nv[1] - wr[1] * nv[0]};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vtr[k] = vt[k] - wxn[k];
-    const double vrel = sqrt(vtr[0] * vtr[0] + vtr[1] * vtr[1] + vtr[2] * vtr[2]);
-    const double damp_prefactor = meff * P.gamman;
-    double fntot = P.kn * delta - damp_prefactor * vnnr;
-    if (P.limit_damping) fntot = fntot < 0.0 ? 0.0 : fntot;  // std::max(fntot, 0.0)
-    const double fscrit = P.xmu * fabs(fntot);
-    const double damp_t = meff * P.gammat;
-    double fs[3] = {0.0, 0.0, 0.0};
-    if (P.use_history) {
-        const double4 hh = wh[i];
-        double h[3] = {hh.x, hh.y, hh.z};
-        if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
-        if (history_update)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) h[k] = h[k] + vtr[k] * dt;
-        const double shrmag = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
-        if (history_update) {
-            const double hn = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nv[k];
-        }
-        double fdamp[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vtr[k];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fs[k] = -P.kt * h[k] + fdamp[k];
-        const double magfs = sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
-        if (magfs > fscrit) {
-            if (shrmag != 0.0) {
-                const double sc = fscrit / magfs;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) fs[k] = fs[k] * sc;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) h[k] = -(fs[k] - fdamp[k]) / P.kt;
-            } else {
-                fs[0] = fs[1] = fs[2] = 0.0;
-            }
-        }
-        wh[i] = make_double4(h[0], h[1], h[2], 1.0);
-    } else {
-        double ft = 0.0;
-        if (vrel != 0.0) {
-            const double dv = damp_t * vrel;
-            ft = (dv < fscrit ? dv : fscrit) / vrel;  // std::min(fscrit, dv)
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fs[k] = -ft * vtr[k];
-    }
-    V fo = f[i], to = tq[i];
-    fo.x = fo.x + (fntot * nv[0] + fs[0]);
-    fo.y = fo.y + (fntot * nv[1] + fs[1]);
-    fo.z = fo.z + (fntot * nv[2] + fs[2]);
-    to.x = to.x + -ri * (nv[1] * fs[2] - nv[2] * fs[1]);
-    to.y = to.y + -ri * (nv[2] * fs[0] - nv[0] * fs[2]);
-    to.z = to.z + -ri * (nv[0] * fs[1] - nv[1] * fs[0]);
-    f[i] = fo;
-    tq[i] = to;
+    double dF[3], dT[3];
+    if (wall_plane_contact(P, coord, vwall_n, vwall_s, xs, x4.w, v4.w, v, w, dt, history_update,
+                           wh, wh, i, dF, dT))
+        wall_stage_add(f, tq, i, dF, dT);
 }
 
-// FixWallGranRegion.post_force_kernel, wall_gran_region.py:114-271: the
-// curved side of a cylinder region as a partner of zero radius and infinite
-// mass. Hooke (mass_velocity damping) or hertz (viscoelastic: the contact
-// radius sqrt(r delta) scales stiffness and damping), linear_history_classic
-// or history-free tangential force; atoms outside [axis_lo, axis_hi] (when
-// bounded) or on the axis are out of contact.
 template <typename V>
 __global__ void k_wall_cyl(const double4* xr, const V* vm, const V* om, V* f, V* tq,
                            double4* wh, int n, tmps_wall_params P, double dt,
                            int history_update) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int axis = P.axis;
     const double4 x4 = xr[i];
     const double pos[3] = {x4.x, x4.y, x4.z};
-    const double r_i = x4.w;
-    const double pos_axis = pos[axis];
-    if (P.has_bounds && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) {
-        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return;
-    }
-    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-    const double d1 = pos[a1] - P.c1, d2 = pos[a2] - P.c2;
-    const double r_perp = sqrt(d1 * d1 + d2 * d2);
-    if (!(r_perp > 1e-14)) {
-        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return;
-    }
-    const double inv_r_perp = 1.0 / r_perp;
-    const double u1 = d1 * inv_r_perp, u2 = d2 * inv_r_perp;
-    double nw[3] = {0.0, 0.0, 0.0};
-    nw[a1] = P.side == -1 ? -u1 : u1;
-    nw[a2] = P.side == -1 ? -u2 : u2;
-    const double delta = P.side == -1 ? r_i - (P.radius - r_perp) : r_i - (r_perp - P.radius);
-    if (!(delta > 0.0)) {
-        if (P.use_history) wh[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return;
-    }
     const V v4 = vm[i], o4 = om[i];
-    const double mi = v4.w;
     const double vi[3] = {v4.x, v4.y, v4.z}, oi[3] = {o4.x, o4.y, o4.z};
-    const double oxn[3] = {oi[1] * nw[2] - oi[2] * nw[1], oi[2] * nw[0] - oi[0] * nw[2],
-                           oi[0] * nw[1] - oi[1] * nw[0]};
-    double vr[3], vrt[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vr[k] = vi[k] - r_i * oxn[k];
-    const double vn = vr[0] * nw[0] + vr[1] * nw[1] + vr[2] * nw[2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vrt[k] = vr[k] - vn * nw[k];
-    const double vrel = sqrt(vrt[0] * vrt[0] + vrt[1] * vrt[1] + vrt[2] * vrt[2]);
-    double poly = 1.0;
-    if (P.hertz) poly = sqrt(r_i * delta);
-    const double fn_elastic = P.kn * delta * poly;
-    const double damp_prefactor = mi * P.gamman * poly;
-    double fntot = fn_elastic - damp_prefactor * vn;
-    if (P.limit_damping) fntot = fntot < 0.0 ? 0.0 : fntot;  // std::max(fntot, 0.0)
-    const double fscrit = P.xmu * fabs(fntot);
-    const double damp_t = mi * P.gammat * poly;
-    double ft[3] = {0.0, 0.0, 0.0};
-    if (P.use_history) {
-        const double4 hh = wh[i];
-        double h[3] = {hh.x, hh.y, hh.z};
-        if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
-        if (history_update)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) h[k] = h[k] + vrt[k] * dt;
-        const double shrmag = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
-        if (history_update) {
-            const double hn = h[0] * nw[0] + h[1] * nw[1] + h[2] * nw[2];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) h[k] = h[k] - hn * nw[k];
-        }
-        double fdamp[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) fdamp[k] = -damp_t * vrt[k];
-        const double ktp = -P.kt * poly;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) ft[k] = ktp * h[k] + fdamp[k];
-        const double magfs = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
-        if (magfs > fscrit) {
-            if (shrmag != 0.0) {
-                const double sc = fscrit / magfs;
-                const double kp = P.kt * poly;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) ft[k] = ft[k] * sc;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) h[k] = -(ft[k] - fdamp[k]) / kp;
-            } else {
-                ft[0] = ft[1] = ft[2] = 0.0;
-            }
-        }
-        wh[i] = make_double4(h[0], h[1], h[2], 1.0);
-    } else {
-        double fsc = 0.0;
-        if (vrel != 0.0) {
-            const double dv = damp_t * vrel;
-            fsc = (dv < fscrit ? dv : fscrit) / vrel;  // std::min(fscrit, dv)
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) ft[k] = -fsc * vrt[k];
-    }
-    V fo = f[i], to = tq[i];
-    fo.x = fo.x + (fntot * nw[0] + ft[0]);
-    fo.y = fo.y + (fntot * nw[1] + ft[1]);
-    fo.z = fo.z + (fntot * nw[2] + ft[2]);
-    to.x = to.x + -r_i * (nw[1] * ft[2] - nw[2] * ft[1]);
-    to.y = to.y + -r_i * (nw[2] * ft[0] - nw[0] * ft[2]);
-    to.z = to.z + -r_i * (nw[0] * ft[1] - nw[1] * ft[0]);
-    f[i] = fo;
-    tq[i] = to;
+    double dF[3], dT[3];
+    if (wall_cyl_contact(P, pos, x4.w, v4.w, vi, oi, dt, history_update, wh, wh, i, dF, dT))
+        wall_stage_add(f, tq, i, dF, dT);
 }
 
 // The final pairwise tree over nparts block partials, NC components, into out.
@@ -1551,28 +1416,40 @@ int launch_copy_vw(System& s, int honor_abort) {
     return 0;
 }
 
-template <int STYLE>
-static int launch_force_style(System& s, double dt, int final_mode, int cundall, double gl,
-                              double ga, int in_buf, int out_buf) {
+template <int STYLE, int WALL>
+static int launch_force_walls(System& s, const F64Ptrs& P, double dt, int final_mode, int cundall,
+                              double gl, double ga) {
     const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
-    const F64Ptrs P = ptrs(s, in_buf, out_buf);
     if (final_mode) {
         if (cundall)
-            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 1>), g, b, 0, s.stream, P, s.ctl,
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 1, WALL>), g, b, 0, s.stream, P, s.ctl,
                                s.pair, dt, gl, ga, s.partial);
         else
-            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 1>), g, b, 0, s.stream, P, s.ctl,
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 1, WALL>), g, b, 0, s.stream, P, s.ctl,
                                s.pair, dt, gl, ga, s.partial);
     } else {
         if (cundall)
-            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 0>), g, b, 0, s.stream, P, s.ctl,
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 0, WALL>), g, b, 0, s.stream, P, s.ctl,
                                s.pair, dt, gl, ga, s.partial);
         else
-            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 0>), g, b, 0, s.stream, P, s.ctl,
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 0, WALL>), g, b, 0, s.stream, P, s.ctl,
                                s.pair, dt, gl, ga, s.partial);
     }
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// (a fused run's walls, when it has any, in place on set 0: the redo of a
+// step runs its force pass there)
+template <int STYLE>
+static int launch_force_style(System& s, double dt, int final_mode, int cundall, double gl,
+                              double ga, int in_buf, int out_buf) {
+    F64Ptrs P = ptrs(s, in_buf, out_buf);
+    if (s.nrun_walls > 0) {
+        P.W = run_walls(s, 0, 0);
+        return launch_force_walls<STYLE, 1>(s, P, dt, final_mode, cundall, gl, ga);
+    }
+    return launch_force_walls<STYLE, 0>(s, P, dt, final_mode, cundall, gl, ga);
 }
 
 int launch_step_force_f64(System& s, double dt, int final_mode, int cundall, double gl,
@@ -1721,31 +1598,38 @@ int launch_ctl_built(System& s, int step, const int fast[3], const int gi[3], in
 
 // ------------------------------------------------ one kernel per step
 
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 static int fused_f64_go(System& s, const Fused64& P, double dt, double gl, double ga) {
     const int nb = nblocks_for(s.n);
     if (s.kstart || s.kstop) {
-        hipExtLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
-                              s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, s.partial, dt, gl,
-                              ga);
+        hipExtLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL, WALL>), dim3(nb),
+                              dim3(TMPS_BLOCK), 0, s.stream, s.kstart, s.kstop, 0, P, s.ctl,
+                              s.pair, s.partial, dt, gl, ga);
         s.kstart = s.kstop = nullptr;
     } else {
-        hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
-                           s.stream, P, s.ctl, s.pair, s.partial, dt, gl, ga);
+        hipLaunchKernelGGL((k_step_f64<STYLE, CUNDALL, FINAL, WALL>), dim3(nb), dim3(TMPS_BLOCK),
+                           0, s.stream, P, s.ctl, s.pair, s.partial, dt, gl, ga);
     }
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
+template <int STYLE, int WALL>
+static int fused_f64_walls(System& s, const Fused64& P, double dt, int fin, int cundall, double gl,
+                           double ga) {
+    if (fin) {
+        if (cundall) return fused_f64_go<STYLE, 1, 1, WALL>(s, P, dt, gl, ga);
+        return fused_f64_go<STYLE, 0, 1, WALL>(s, P, dt, gl, ga);
+    }
+    if (cundall) return fused_f64_go<STYLE, 1, 0, WALL>(s, P, dt, gl, ga);
+    return fused_f64_go<STYLE, 0, 0, WALL>(s, P, dt, gl, ga);
+}
+
 template <int STYLE>
 static int fused_f64_style(System& s, const Fused64& P, double dt, int fin, int cundall, double gl,
                            double ga) {
-    if (fin) {
-        if (cundall) return fused_f64_go<STYLE, 1, 1>(s, P, dt, gl, ga);
-        return fused_f64_go<STYLE, 0, 1>(s, P, dt, gl, ga);
-    }
-    if (cundall) return fused_f64_go<STYLE, 1, 0>(s, P, dt, gl, ga);
-    return fused_f64_go<STYLE, 0, 0>(s, P, dt, gl, ga);
+    if (P.W.nwall > 0) return fused_f64_walls<STYLE, 1>(s, P, dt, fin, cundall, gl, ga);
+    return fused_f64_walls<STYLE, 0>(s, P, dt, fin, cundall, gl, ga);
 }
 
 int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, double gl,
@@ -1772,6 +1656,7 @@ int launch_fused_f64(System& s, int in_buf, double dt, int fin, int cundall, dou
     P.nb_start = s.nb_start;
     P.nb_j = s.nb_j;
     P.n = s.n;
+    P.W = run_walls(s, in_buf, ob);
     switch (s.pair.style) {
         case TMPS_PAIR_HOOKE_HISTORY:
             return fused_f64_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga);
@@ -1804,6 +1689,8 @@ int launch_copy_state(System& s, int honor_abort) {
                            static_cast<float4*>(s.nb_hist), static_cast<const float4*>(s.nb_hist2),
                            (const int*)s.nb_start, s.ctl, honor_abort);
     HIP_TRY(hipGetLastError());
+    if (s.nrun_walls > 0)
+        LAUNCH_N(k_copy_walls, s.n, run_walls(s, 1, 0), s.n, s.ctl, honor_abort);
     return 0;
 }
 

@@ -14,6 +14,7 @@
 #include "contact.h"
 #include "fused.h"
 #include "tmps_internal.h"
+#include "wall.h"
 
 using namespace tmps;
 
@@ -35,6 +36,7 @@ struct SplitPtrs {
     uint8_t* nb_flag;
     float4* nb_hist;
     int n;
+    WallRefs W;  // the fused run's walls (nwall 0 elsewhere)
 };
 
 SplitPtrs sptrs(System& s, int in_buf = 0, int out_buf = 0) {
@@ -48,6 +50,7 @@ SplitPtrs sptrs(System& s, int in_buf = 0, int out_buf = 0) {
     p.nb_start = s.nb_start; p.nb_j = s.nb_j; p.nb_flag = s.nb_flag;
     p.nb_hist = static_cast<float4*>(s.nb_hist);
     p.n = s.n;
+    p.W = WallRefs{};
     return p;
 }
 
@@ -172,20 +175,37 @@ __device__ __forceinline__ float cundall_scale_f(float fv, float g) {
 // ---------------------------------------------------------- conversions
 
 // x (FP64) of every atom from its split encoding, into xr (x, y, z, r).
-__device__ __forceinline__ void split_to_x64_atom(int i, const SplitPtrs& P, double4* xr,
-                                                  const Ctl* ctl) {
-    const DevBox& B = ctl->box;
-    const SplitConst& sc = ctl->sc;
+// The FP64 position of an encoded one (cell, offset u) in the current box.
+__device__ __forceinline__ void split_position(const DevBox& B, const SplitConst& sc,
+                                               uint32_t cell, const float4& u, double x[3]) {
     int c[3];
-    unpack_cell(P.cell[i], c[0], c[1], c[2]);
-    const float4 u = P.up[i];
+    unpack_cell(cell, c[0], c[1], c[2]);
     const float uu[3] = {u.x, u.y, u.z};
-    double x[3];
     for (int k = 0; k < 3; ++k) {
         const double cw = B.prd[k] / (double)sc.gi[k];
         x[k] = B.lo[k] + cw * ((double)c[k] + 0.5 + (double)uu[k]);
     }
+}
+
+__device__ __forceinline__ void split_to_x64_atom(int i, const SplitPtrs& P, double4* xr,
+                                                  const Ctl* ctl) {
+    const float4 u = P.up[i];
+    double x[3];
+    split_position(ctl->box, ctl->sc, P.cell[i], u, x);
     xr[i] = make_double4(x[0], x[1], x[2], (double)u.w);
+}
+
+// A fused step's walls before (pass 0) or after (pass 1) damping/cundall on
+// atom i, at its drifted position (cell ci, offset u), with its half-step v
+// and omega (FixWallGran.post_force between the pair force and
+// final_integrate, simulation.py:236-247).
+__device__ __forceinline__ void split_walls(const WallRefs& W, int pass, const Ctl* ctl,
+                                            uint32_t ci, const float4& u, const float4& v,
+                                            const float4& w, int i, float F[3], float T[3]) {
+    double xs[3];
+    split_position(ctl->box, ctl->sc, ci, u, xs);
+    const double vd[3] = {v.x, v.y, v.z}, wd[3] = {w.x, w.y, w.z};
+    walls_apply<float>(W, pass, xs, (double)u.w, (double)v.w, vd, wd, W.spec[0].dt, i, F, T);
 }
 
 __global__ void k_split_to_x64(SplitPtrs P, double4* xr, const Ctl* ctl) {
@@ -422,7 +442,7 @@ __global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
 }
 
 // Block `blk` of the force pass (its partial: partial[blk]).
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL = 0>
 __device__ __forceinline__ void step_force_split_block(const SplitPtrs& P, Ctl* ctl,
                                                        const PairP& pp, float dt, float gl,
                                                        float ga, double* partial, int blk) {
@@ -441,11 +461,13 @@ __device__ __forceinline__ void step_force_split_block(const SplitPtrs& P, Ctl* 
         double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int nt = 0;
         gather_split<STYLE>(i, P.cell[i], ui, v, w, P, sc, per, pp, dt, 1, F, T, V, nt);
+        if (WALL) split_walls(P.W, 0, ctl, P.cell[i], ui, v, w, i, F, T);
         if (CUNDALL) {
             F[0] *= cundall_scale_f(F[0] * v.x, gl); T[0] *= cundall_scale_f(T[0] * w.x, ga);
             F[1] *= cundall_scale_f(F[1] * v.y, gl); T[1] *= cundall_scale_f(T[1] * w.y, ga);
             F[2] *= cundall_scale_f(F[2] * v.z, gl); T[2] *= cundall_scale_f(T[2] * w.z, ga);
         }
+        if (WALL) split_walls(P.W, 1, ctl, P.cell[i], ui, v, w, i, F, T);
         const float dtf = 0.5f * dt;
         const float im = 1.0f / v.w;
         const float am[3] = {F[0] * im, F[1] * im, F[2] * im};
@@ -474,10 +496,11 @@ __device__ __forceinline__ void step_force_split_block(const SplitPtrs& P, Ctl* 
     block_tree_sum<7>(leaf, partial + (size_t)blk * 8);
 }
 
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 __global__ void k_step_force_split(SplitPtrs P, Ctl* ctl, PairP pp, float dt, float gl, float ga,
                                    double* partial) {
-    step_force_split_block<STYLE, CUNDALL, FINAL>(P, ctl, pp, dt, gl, ga, partial, blockIdx.x);
+    step_force_split_block<STYLE, CUNDALL, FINAL, WALL>(P, ctl, pp, dt, gl, ga, partial,
+                                                        blockIdx.x);
 }
 
 __global__ void k_copy_vw_split(float4* vdst, float4* wdst, const float4* vsrc, const float4* wsrc,
@@ -524,6 +547,7 @@ struct FusedS {
     const int *nb_start, *nb_j;
     const uint16_t* nb_code;
     int n;
+    WallRefs W;  // the run's walls (nwall 0 without)
 };
 
 // Where atom j is at step t in bin units: under remap x the offsets need no
@@ -540,7 +564,7 @@ __device__ __forceinline__ float4 step_offset(const float dtc[3], float4 u, cons
 // runs the servo. `fin`: the run's last step, which stores f / torque /
 // virial instead of applying the next step's half-kick. Shared by the
 // single-system kernel and the batched one (k_batch.hip).
-template <int STYLE, int CUNDALL>
+template <int STYLE, int CUNDALL, int WALL = 0>
 __device__ __forceinline__ void split_step_block(const FusedS& P, Ctl* ctl, const PairP& pp,
                                                  double* partial, float dt, float gl, float ga,
                                                  int lb, bool fin) {
@@ -693,11 +717,13 @@ __device__ __forceinline__ void split_step_block(const FusedS& P, Ctl* ctl, cons
             for (int c = 0; c < 6; ++c) V[c] += (double)o.vir[c];
             ++nt;
         }
+        if (WALL) split_walls(P.W, 0, ctl, ci, ui, v, w, i, F, T);
         if (CUNDALL) {
             F[0] *= cundall_scale_f(F[0] * v.x, gl); T[0] *= cundall_scale_f(T[0] * w.x, ga);
             F[1] *= cundall_scale_f(F[1] * v.y, gl); T[1] *= cundall_scale_f(T[1] * w.y, ga);
             F[2] *= cundall_scale_f(F[2] * v.z, gl); T[2] *= cundall_scale_f(T[2] * w.z, ga);
         }
+        if (WALL) split_walls(P.W, 1, ctl, ci, ui, v, w, i, F, T);
         const float dtf = 0.5f * dt;
         const float im = 1.0f / v.w;
         const float am[3] = {F[0] * im, F[1] * im, F[2] * im};
@@ -727,11 +753,11 @@ __device__ __forceinline__ void split_step_block(const FusedS& P, Ctl* ctl, cons
     publish_block(leaf, trig, lb, partial);
 }
 
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 __global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
                                                             float dt, float gl, float ga) {
-    split_step_block<STYLE, CUNDALL>(P, ctl, pp, partial, dt, gl, ga,
-                                     xcd_block(blockIdx.x, gridDim.x), FINAL != 0);
+    split_step_block<STYLE, CUNDALL, WALL>(P, ctl, pp, partial, dt, gl, ga,
+                                           xcd_block(blockIdx.x, gridDim.x), FINAL != 0);
 }
 
 // ------------------------------------------------ ensemble (batch.hip)
@@ -997,29 +1023,41 @@ int launch_copy_vw_split(System& s, int honor_abort) {
     return 0;
 }
 
-template <int STYLE>
-static int launch_force_split_style(System& s, double dt, int final_mode, int cundall, double gl,
-                                    double ga, int in_buf, int out_buf) {
+template <int STYLE, int WALL>
+static int launch_force_split_walls(System& s, const SplitPtrs& P, double dt, int final_mode,
+                                    int cundall, double gl, double ga) {
     const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
-    const SplitPtrs P = sptrs(s, in_buf, out_buf);
     const float fdt = (float)dt, fgl = (float)gl, fga = (float)ga;
     if (final_mode) {
         if (cundall)
-            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 1>), g, b, 0, s.stream, P, s.ctl,
-                               s.pair, fdt, fgl, fga, s.partial);
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 1, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
         else
-            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 1>), g, b, 0, s.stream, P, s.ctl,
-                               s.pair, fdt, fgl, fga, s.partial);
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 1, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
     } else {
         if (cundall)
-            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 0>), g, b, 0, s.stream, P, s.ctl,
-                               s.pair, fdt, fgl, fga, s.partial);
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 0, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
         else
-            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 0>), g, b, 0, s.stream, P, s.ctl,
-                               s.pair, fdt, fgl, fga, s.partial);
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 0, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
     }
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// (a fused run's walls, when it has any, in place on set 0: the redo of a
+// step runs its force pass there)
+template <int STYLE>
+static int launch_force_split_style(System& s, double dt, int final_mode, int cundall, double gl,
+                                    double ga, int in_buf, int out_buf) {
+    SplitPtrs P = sptrs(s, in_buf, out_buf);
+    if (s.nrun_walls > 0) {
+        P.W = run_walls(s, 0, 0);
+        return launch_force_split_walls<STYLE, 1>(s, P, dt, final_mode, cundall, gl, ga);
+    }
+    return launch_force_split_walls<STYLE, 0>(s, P, dt, final_mode, cundall, gl, ga);
 }
 
 int launch_step_force_split(System& s, double dt, int final_mode, int cundall, double gl,
@@ -1041,31 +1079,39 @@ int launch_step_force_split(System& s, double dt, int final_mode, int cundall, d
 
 // ------------------------------------------------ one kernel per step
 
-template <int STYLE, int CUNDALL, int FINAL>
+template <int STYLE, int CUNDALL, int FINAL, int WALL>
 static int fused_split_go(System& s, const FusedS& P, double dt, double gl, double ga) {
     const int nb = nblocks_for(s.n);
     if (s.kstart || s.kstop) {
-        hipExtLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
-                              s.stream, s.kstart, s.kstop, 0, P, s.ctl, s.pair, s.partial,
-                              (float)dt, (float)gl, (float)ga);
+        hipExtLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL, WALL>), dim3(nb),
+                              dim3(TMPS_BLOCK), 0, s.stream, s.kstart, s.kstop, 0, P, s.ctl,
+                              s.pair, s.partial, (float)dt, (float)gl, (float)ga);
         s.kstart = s.kstop = nullptr;
     } else {
-        hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL>), dim3(nb), dim3(TMPS_BLOCK), 0,
-                           s.stream, P, s.ctl, s.pair, s.partial, (float)dt, (float)gl, (float)ga);
+        hipLaunchKernelGGL((k_step_split<STYLE, CUNDALL, FINAL, WALL>), dim3(nb),
+                           dim3(TMPS_BLOCK), 0, s.stream, P, s.ctl, s.pair, s.partial, (float)dt,
+                           (float)gl, (float)ga);
     }
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
+template <int STYLE, int WALL>
+static int fused_split_walls(System& s, const FusedS& P, double dt, int fin, int cundall,
+                             double gl, double ga) {
+    if (fin) {
+        if (cundall) return fused_split_go<STYLE, 1, 1, WALL>(s, P, dt, gl, ga);
+        return fused_split_go<STYLE, 0, 1, WALL>(s, P, dt, gl, ga);
+    }
+    if (cundall) return fused_split_go<STYLE, 1, 0, WALL>(s, P, dt, gl, ga);
+    return fused_split_go<STYLE, 0, 0, WALL>(s, P, dt, gl, ga);
+}
+
 template <int STYLE>
 static int fused_split_style(System& s, const FusedS& P, double dt, int fin, int cundall,
                              double gl, double ga) {
-    if (fin) {
-        if (cundall) return fused_split_go<STYLE, 1, 1>(s, P, dt, gl, ga);
-        return fused_split_go<STYLE, 0, 1>(s, P, dt, gl, ga);
-    }
-    if (cundall) return fused_split_go<STYLE, 1, 0>(s, P, dt, gl, ga);
-    return fused_split_go<STYLE, 0, 0>(s, P, dt, gl, ga);
+    if (P.W.nwall > 0) return fused_split_walls<STYLE, 1>(s, P, dt, fin, cundall, gl, ga);
+    return fused_split_walls<STYLE, 0>(s, P, dt, fin, cundall, gl, ga);
 }
 
 int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, double gl,
@@ -1092,6 +1138,7 @@ int launch_fused_split(System& s, int in_buf, double dt, int fin, int cundall, d
     P.nb_j = s.nb_j;
     P.nb_code = s.nb_code;
     P.n = s.n;
+    P.W = run_walls(s, in_buf, ob);
     switch (s.pair.style) {
         case TMPS_PAIR_HOOKE_HISTORY:
             return fused_split_style<TMPS_PAIR_HOOKE_HISTORY>(s, P, dt, fin, cundall, gl, ga);
@@ -1133,6 +1180,7 @@ static FusedS fused_split_ptrs(System& s, int in_buf) {
     P.nb_j = s.nb_j;
     P.nb_code = s.nb_code;
     P.n = s.n;
+    P.W = WallRefs{};  // ensemble members have no walls
     return P;
 }
 

@@ -110,6 +110,29 @@ struct PairP {
     double g_kn, g_damp, g_kt, g_xt, g_mu;
 };
 
+// One wall as a step applies it: its settings, the plane's position and
+// velocities (fixed for a fused run; a cylinder ignores them), and whether it
+// comes before fix damping/cundall among the fixes.
+struct WallSpec {
+    tmps_wall_params p;
+    double coord, vwall_n, vwall_s;
+    double dt;  // the run's timestep (the history advances by v dt in FP64)
+    int before_cundall;
+    int pad;
+};
+
+// The walls of a fused step, at most TMPS_RUN_WALLS: settings in device
+// memory, and per wall the shear history the step reads and the one it
+// writes (the two buffer sets of the step, or the same buffer in place).
+#define TMPS_RUN_WALLS 2
+struct WallRefs {
+    const WallSpec* spec;
+    const double4* h_in[TMPS_RUN_WALLS];
+    double4* h_out[TMPS_RUN_WALLS];
+    int nwall;
+    int pad;
+};
+
 // One FP32-split ensemble member whose step is redone with a list rebuild,
 // as the batched redo kernels read it (the single-system redo's buffers,
 // grid and decisions). The host fills it in two passes: before the list
@@ -255,6 +278,14 @@ struct System {
 
     int launch_mode = TMPS_LAUNCH_GRAPH;
 
+    // walls a device-resident run applies in its fused step (tmps_run_set_walls):
+    // their settings (host copy and device copy) and per wall the shear
+    // history of buffer set 0 (the wall's own) and of set 1
+    int nrun_walls = 0;
+    WallSpec run_spec[TMPS_RUN_WALLS] = {};
+    WallSpec* d_run_spec = nullptr;
+    double4* run_wh[TMPS_RUN_WALLS][2] = {};
+
     // the adjoint engine's checkpoints and work buffers (adjoint.hip)
     struct AdjContext* adj = nullptr;
     // steps at which the list was built while log_builds is set (the adjoint
@@ -349,6 +380,19 @@ inline hipError_t copy_sync(hipStream_t st, void* dst, const void* src, size_t b
 }
 
 inline int nblocks_for(int n) { return (n + TMPS_BLOCK - 1) / TMPS_BLOCK; }
+
+// The fused run's walls as a step that reads buffer set `in` and writes set
+// `out` sees them (in == out: in place).
+inline WallRefs run_walls(const System& s, int in, int out) {
+    WallRefs W{};
+    W.spec = s.d_run_spec;
+    W.nwall = s.nrun_walls;
+    for (int q = 0; q < s.nrun_walls; ++q) {
+        W.h_in[q] = s.run_wh[q][in];
+        W.h_out[q] = s.run_wh[q][out];
+    }
+    return W;
+}
 
 // rebuild (k_rebuild.hip)
 int launch_pbc(System& s);

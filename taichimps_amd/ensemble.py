@@ -48,6 +48,8 @@ class Ensemble:
                                           "fixes only")
             if plan[0] is not None:
                 raise NotImplementedError("ensembles do not run fix damping/cundall")
+            if plan[2]:
+                raise NotImplementedError("ensembles do not run wall fixes")
             if s.atom.nlocal < 1:
                 raise ValueError("every ensemble member needs atoms")
         dts = {float(s.dt) for s in sims}
@@ -66,7 +68,7 @@ class Ensemble:
         sim.neighbor._push()
         sim.history._bind(atom)
         sim.pair_style._install(atom)
-        _, deform = sim._device_plan()
+        _, deform, _ = sim._device_plan()
         if deform is None:
             off = _native.ServoParams()
             off.nevery = 1

@@ -21,6 +21,7 @@ from .EXTRA_FIX.base import Fix
 from .EXTRA_FIX.damping_cundall import FixDampingCundall
 from .EXTRA_FIX.deform_pressure import FixDeformPressure
 from .EXTRA_FIX.nve_sphere import FixNVESphere
+from .EXTRA_FIX.wall_gran import FixWallGran
 from .GRANULAR.base import GranularPair
 from .neighbor import NeighborList
 
@@ -149,12 +150,16 @@ class Simulation:
 
     # --------------------------------------------------------------- runs
     def _device_plan(self):
-        """(cundall, deform) when the run can stay on the device, else None."""
+        """(cundall, deform, walls) when the run can stay on the device, else
+        None. walls: up to two non-wiggling wall fixes, each with whether it
+        comes before damping/cundall among the fixes (their post_force order,
+        simulation.py:236-247)."""
         if not isinstance(self.integrator, FixNVESphere):
             return None
         if self.pair_style is None or self.neighbor is None or self.history is None:
             return None
         cundall = deform = None
+        walls = []
         for fix in self.fixes:
             if getattr(fix, "output_only", False):
                 continue  # fix print: the run is cut at its multiples
@@ -162,13 +167,24 @@ class Simulation:
                 cundall = fix
             elif type(fix) is FixDeformPressure and deform is None:
                 deform = fix
+            elif (isinstance(fix, FixWallGran) and fix.wiggle_amplitude == 0.0
+                  and len(walls) < 2):
+                walls.append((fix, cundall is None))
             else:
                 return None
             if fix.force_reneighbor:
                 return None
-        return cundall, deform
+        return cundall, deform, walls
 
-    def _run_device(self, steps: int, cundall, deform) -> None:
+    def _set_walls(self, walls) -> None:
+        atom = self.atom
+        spec = []
+        for fix, before in walls:
+            w, coord, vs = fix._fused(atom)
+            spec.append((w, before, coord, vs))
+        atom._native.set_run_walls(spec, self.dt)
+
+    def _run_device(self, steps: int, cundall, deform, walls) -> None:
         atom = self.atom
         N = atom._native
         self.neighbor._bind(atom)
@@ -184,9 +200,13 @@ class Simulation:
         N.set_counters(timestep=self.timestep, setup_done=self.setup_done)
         run_pre = self.run_pre or not self.setup_done
         gl_ga = None if cundall is None else (cundall.gamma_lin, cundall.gamma_ang)
+        self._set_walls(walls)
         try:
             N.run(steps, self.dt, run_pre=run_pre, cundall=gl_ga)
+            for fix, _ in walls:
+                fix._advance(steps + (1 if run_pre else 0), self.dt)
         finally:
+            N.set_run_walls([], self.dt)
             c = N.counters()
             self.timestep = c["timestep"]
             self.setup_done = bool(c["setup_done"])
@@ -246,9 +266,13 @@ class Simulation:
         plan = self._device_plan()
         if plan is None or self.atom.nlocal == 0 or not self.setup_done:
             return
-        cundall = plan[0]
+        cundall, _, walls = plan
         gl_ga = None if cundall is None else (cundall.gamma_lin, cundall.gamma_ang)
-        self.atom._native.prepare_run(steps, self.dt, cundall=gl_ga)
+        self._set_walls(walls)
+        try:
+            self.atom._native.prepare_run(steps, self.dt, cundall=gl_ga)
+        finally:
+            self.atom._native.set_run_walls([], self.dt)
 
     def _output_fixes(self) -> list:
         return [f for f in self.fixes if getattr(f, "output_only", False)]

@@ -238,7 +238,8 @@ def test_spheres_falling_on_a_floor_bitexact_run(history):
             assert np.array_equal(wall.touch.to_numpy()[:n], tc)
         if history and steps == 20:
             assert tc.sum() > 10, "the bottom layer should be on the floor"
-    assert not sim.last_run_device_resident
+        # the wall is applied inside the device-resident step
+        assert sim.last_run_device_resident
 
 
 # ------------------------------------------------- cylinder region walls (CPU)
@@ -396,6 +397,119 @@ def test_column_in_a_cylinder_on_a_floor_bitexact_run():
         sh, tc = o.wall_history(wc)
         assert np.array_equal(cyl.shear.to_numpy()[:n], sh)
         assert np.array_equal(cyl.touch.to_numpy()[:n], tc)
+        assert sim.last_run_device_resident
+
+
+def _floor_sim(float_type="f64", order=("cundall", "wall"), seed=31, history=True):
+    """Spheres thrown onto a floor wall (x/y periodic), nve/sphere, optional
+    damping/cundall before or after the wall among the fixes."""
+    import taichimps_amd as tm
+    from taichimps_amd import workloads
+    w = workloads.lattice(6, 5, 4, 1.1, seed, "uniform")
+    n = len(w["x"])
+    w["v"][:, 2] -= 2.0
+    w["v"][:, 0] += 0.5
+    lo, hi = w["boxlo"], w["boxhi"]
+    floor = float(lo[2]) + 0.08
+    c = dict(kn=2.0e5, gamman=50.0, kt=2.0 / 7.0 * 2.0e5, gammat=25.0, xmu=0.5)
+    dom = tm.Domain(lo, hi, ("p", "p", "f"))
+    atom = tm.AtomSystem(max_atoms=n, float_type=float_type)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=w["v"],
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=0.1, max_atoms=n, max_neighbors_per_atom=64)
+    pair = tm.GranHookeHistory(dom, c["kn"], c["gamman"], c["kt"], c["gammat"], c["xmu"])
+    sim = tm.Simulation(dom, atom, nl, tm.ContactHistory(n, 64), pair, dt=DT)
+    sim.add_fix(tm.FixNVESphere(dom))
+    wall = tm.FixWallGran(dom, 2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"],
+                          c["xmu"], history=history, shear_axis=0, shear_vel=0.3, max_atoms=n)
+    for name in order:
+        sim.add_fix(wall if name == "wall" else tm.FixDampingCundall(dom, 0.1, 0.1))
+    return sim, wall, w, c, floor
+
+
+@pytest.mark.gpu
+def test_sheared_floor_with_cundall_device_run_bitexact():
+    """A sheared floor wall after damping/cundall among the fixes (the
+    oracle's post_force order), FP64, through two device-resident runs with
+    list rebuilds: positions, velocities, spins, forces, torques and the wall
+    history bit for bit with the oracle."""
+    sim, wall, w, c, floor = _floor_sim()
+    n = len(w["x"])
+    o = O.from_workload(w)
+    o.set_box(w["boxlo"], w["boxhi"], (1, 1, 0))
+    o.set_neigh(0.1)
+    o.set_pair(O.legacy_pair("hooke/history", **c))
+    o.set_cundall(0.1, 0.1)
+    wi = o.add_wall(2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"], c["xmu"],
+                    shear_axis=0, shear_vel=0.3)
+    touched = 0
+    for steps in (6, 34, 160):
+        sim.run(steps)
+        assert sim.last_run_device_resident
+        assert o.run(steps, DT) == 0
+        a = o.atoms()
+        for key, field in (("x", sim.atom.x), ("v", sim.atom.v), ("omega", sim.atom.omega),
+                           ("f", sim.atom.f), ("torque", sim.atom.torque)):
+            assert np.array_equal(field.to_numpy()[:n], a[key]), f"{key} after {steps}"
+        sh, tc = o.wall_history(wi)
+        assert np.array_equal(wall.shear.to_numpy()[:n], sh)
+        assert np.array_equal(wall.touch.to_numpy()[:n], tc)
+        touched = max(touched, int(tc.sum()))
+    assert touched > 5, "the bottom layer should reach the floor"
+    assert sim.neighbor.build_count >= 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [("wall", "cundall"), ("cundall", "wall")])
+def test_wall_order_in_device_run_equals_stage_steps(order):
+    """The wall before or after damping/cundall: the device-resident run
+    equals the reference's stage sequence (Simulation.run with pre yes:
+    init_simulation, then step() per step, post_force in the fixes' order)
+    bit for bit in FP64, over two runs."""
+    sim, wall, w, _, _ = _floor_sim(order=order)
+    ref, rwall, _, _, _ = _floor_sim(order=order)
+    n = len(w["x"])
+    touched = 0
+    for k, steps in enumerate((6, 114)):
+        sim.run(steps)
+        assert sim.last_run_device_resident
+        ref.setup_run(steps)
+        ref.init_simulation()
+        for _ in range(steps):
+            ref.step()
+        for key in ("x", "v", "omega", "f", "torque"):
+            got = getattr(sim.atom, key).to_numpy()[:n]
+            want = getattr(ref.atom, key).to_numpy()[:n]
+            assert np.array_equal(got, want), f"{key} after {steps}"
+        assert np.array_equal(wall.shear.to_numpy()[:n], rwall.shear.to_numpy()[:n])
+        touched = max(touched, int(rwall.touch.to_numpy()[:n].sum()))
+    assert touched > 5, "the bottom layer should reach the floor"
+
+
+@pytest.mark.gpu
+def test_fp32_split_floor_device_run_tracks_fp64_oracle():
+    """FP32-split, the floor wall inside the device-resident step: positions
+    within 1e-4 and velocities within 1e-3 of the largest speed of the FP64
+    oracle after 150 steps; the same atoms rest on the floor."""
+    sim, wall, w, c, floor = _floor_sim(float_type="f32split")
+    n = len(w["x"])
+    o = O.from_workload(w)
+    o.set_box(w["boxlo"], w["boxhi"], (1, 1, 0))
+    o.set_neigh(0.1)
+    o.set_pair(O.legacy_pair("hooke/history", **c))
+    o.set_cundall(0.1, 0.1)
+    wi = o.add_wall(2, -1, floor, c["kn"], c["gamman"], c["kt"], c["gammat"], c["xmu"],
+                    shear_axis=0, shear_vel=0.3)
+    sim.run(150)
+    assert sim.last_run_device_resident
+    assert o.run(150, DT) == 0
+    a = o.atoms()
+    x = sim.atom.x.to_numpy()[:n]
+    v = sim.atom.v.to_numpy()[:n]
+    assert np.abs(x - a["x"]).max() < 1e-4
+    assert np.abs(v - a["v"]).max() < 1e-3 * np.abs(a["v"]).max()
+    _, tc = o.wall_history(wi)
+    assert np.array_equal(wall.touch.to_numpy()[:n], tc)
 
 
 @pytest.mark.gpu
