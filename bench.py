@@ -98,6 +98,68 @@ def triaxial_c3(steps=1000, warmup=10):
     }
 
 
+def wall_run(steps=100, warmup=10):
+    """Walls inside the device-resident step: the 1M-sphere lattice of the
+    headline workload with z bounded by a sheared floor wall and a ceiling
+    wall on the box faces (x/y periodic; about 970 spheres touch each wall at
+    the start), gran/hertz/history, nve/sphere and damping/cundall 0.1, the
+    spheres thrown down at 0.5, FP32-split and FP64. The same run without the
+    walls (z periodic) is timed beside it."""
+    from taichimps_amd import workloads
+    w = workloads.case("isotropic_hertz_1m")
+    n = len(w["x"])
+    out = {"workload": f"{n} spheres, z between a sheared floor and a ceiling wall, "
+                       f"gran/hertz/history, nve/sphere, damping/cundall 0.1, {steps} steps "
+                       f"after {warmup}", "atoms": n}
+    for prec in ("f32split", "f64"):
+        row = {}
+        for walls in (True, False):
+            sim = wall_sim(w, prec, walls)
+            sim.run(warmup)
+            sim.run_pre = False
+            sim.prepare_run(steps)
+            b0 = sim.atom._native.counters()["build_count"]
+            el = timed_run(sim, steps)
+            key = "walls" if walls else "no_walls"
+            row[key] = {"ms_per_step": el / steps * 1e3, "atom_steps_per_s": n * steps / el,
+                        "device_resident": bool(sim.last_run_device_resident),
+                        "list_builds": sim.atom._native.counters()["build_count"] - b0}
+            if walls:
+                row[key]["floor_touching_atoms"] = int(
+                    sim.fixes[-2].touch.to_numpy()[:n].sum())
+        row["wall_cost_ms_per_step"] = row["walls"]["ms_per_step"] - row["no_walls"]["ms_per_step"]
+        out[prec] = row
+        print(f"# walls {prec}: {row}", file=sys.stderr, flush=True)
+    return out
+
+
+def wall_sim(w, precision, walls):
+    """The wall run's system (see wall_run); walls=False keeps z periodic."""
+    import taichimps_amd as tm
+    from taichimps_amd import workloads
+    c = workloads.ISOTROPIC
+    n = len(w["x"])
+    v = w["v"].copy()
+    v[:, 2] -= 0.5
+    dom = tm.Domain(w["boxlo"], w["boxhi"], ("p", "p", "f" if walls else "p"))
+    atom = tm.AtomSystem(max_atoms=n, float_type=precision)
+    atom.add_particles(x=w["x"], radius=w["radius"], density=w["density"], v=v,
+                       omega=w["omega"], tag=w["tag"])
+    nl = tm.NeighborList(dom, skin=c["skin"], max_atoms=n, max_neighbors_per_atom=64)
+    pair = tm.GranHertzHistory(dom, c["kn"], c["gamman"], c["kt"], c["gammat"], c["xmu"])
+    sim = tm.Simulation(dom, atom, nl, tm.ContactHistory(n, 64), pair, dt=c["dt"])
+    sim.add_fix(tm.FixNVESphere(dom))
+    sim.add_fix(tm.FixDampingCundall(dom, 0.1, 0.1))
+    if walls:
+        lo, hi = float(w["boxlo"][2]), float(w["boxhi"][2])
+        sim.add_fix(tm.FixWallGran(dom, 2, -1, lo, c["kn"], c["gamman"], c["kt"],
+                                   c["gammat"], c["xmu"], shear_axis=0, shear_vel=0.2,
+                                   max_atoms=n))
+        sim.add_fix(tm.FixWallGran(dom, 2, 1, hi, c["kn"], c["gamman"], c["kt"],
+                                   c["gammat"], c["xmu"], max_atoms=n))
+    return sim
+
+
 def timed_run(sim, steps):
     """Wall time (s) of sim.run(steps) on the device's clock: HIP events
     recorded on the system's stream (a torch stream handed to the system)
@@ -646,6 +708,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c2-200", action="store_true",
                     help="skip the 200-steps-from-the-start block")
+    ap.add_argument("--no-walls", action="store_true",
+                    help="skip the 1M-sphere run with walls inside the fused step")
+    ap.add_argument("--walls-only", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint calibration block (32k spheres, 2000 steps)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -685,6 +750,9 @@ def main(argv=None):
         return 0
     if args.triaxial_only:
         print(json.dumps(triaxial_c3()))
+        return 0
+    if args.walls_only:
+        print(json.dumps(wall_run()))
         return 0
     if args.next_rows_only:
         print(json.dumps(next_rows(dims=args.dims)))
@@ -797,6 +865,8 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "c2_200": c2,
     }
+    if not args.no_walls:
+        out["walls"] = wall_run()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_c4()
         if args.no_cpu_baseline:

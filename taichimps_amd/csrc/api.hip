@@ -719,6 +719,7 @@ int segment_graph(System& s, int len, int kick, int last, double dt, int use_cun
     key = mix(key, (uint64_t)s.nrun_walls);
     for (int q = 0; q < s.nrun_walls; ++q)
         key = mix(key, (uint64_t)(uintptr_t)s.run_wh[q][0]);
+    if (s.nrun_walls > 0) key = mix(key, (uint64_t)(uintptr_t)s.run_wnz);
     int slot = -1;
     for (int k = 0; k < System::kGraphSlots; ++k)
         if (s.seg_exec[k] && s.seg_key[k] == key) slot = k;
@@ -1177,6 +1178,8 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
     }
     rc = ensure_split(s);
     if (rc) return rc;
+    rc = launch_walls_nz(s);
+    if (rc) return rc;
     rc = ctl_edit(s);
     if (rc) return rc;
     Ctl& H = *s.hctl;
@@ -1537,6 +1540,7 @@ void free_all(System& s) {
     dfree(s.cell_of); dfree(s.cell_count); dfree(s.cell_start); dfree(s.cell_fill);
     dfree(s.rs_k1); dfree(s.rs_k2); dfree(s.rs_v1); dfree(s.rs_v2); dfree(s.rs_hist);
     dfree(s.d_run_spec);
+    dfree(s.run_wnz);
     dfree(s.rs_offs);
     dfree(s.cell_particles);
     dfree(s.bin_x);
@@ -2171,6 +2175,7 @@ int tmps_run_set_walls(tmps_system* h, int n, tmps_wall* const* walls, const int
         spec[q].before_cundall = before_cundall[q] ? 1 : 0;
     }
     if (n > 0 && !s.d_run_spec) HIP_TRY(hipMalloc(&s.d_run_spec, sizeof(WallSpec) * TMPS_RUN_WALLS));
+    if (n > 0 && !s.run_wnz) HIP_TRY(hipMalloc(&s.run_wnz, (size_t)std::max(s.max_atoms, 1)));
     if (n > 0 && std::memcmp(spec, s.run_spec, sizeof(spec)) != 0)
         HIP_TRY(copy_sync(s.stream, s.d_run_spec, spec, sizeof(spec), hipMemcpyHostToDevice));
     std::memcpy(s.run_spec, spec, sizeof(spec));

@@ -17,6 +17,12 @@
 #ifndef TMPS_STEP_WAVES_SPLIT
 #define TMPS_STEP_WAVES_SPLIT 5
 #endif
+// The FP32-split pass with walls holds the FP64 wall contact beside its own
+// state: at 5 waves it spills (scratch stores on every atom's path), at 4 it
+// does not.
+#ifndef TMPS_STEP_WAVES_SPLIT_WALL
+#define TMPS_STEP_WAVES_SPLIT_WALL 4
+#endif
 
 namespace tmps {
 

@@ -1077,13 +1077,48 @@ __global__ void k_copy_state(T4* x0, T4* v0, T4* w0, const T4* x1, const T4* v1,
                          x1, v1, w1, n, fl0, fl1, h0, h1, nb_start);
 }
 
-// The fused run's wall histories, set 1 -> set 0 (with k_copy_state).
+// The fused run's wall histories, set W.in -> set W.out (with k_copy_state):
+// entries flagged possibly nonzero are copied, an out entry whose in entry
+// is zero is cleared where it may be nonzero, the rest are left as they are
+// (zero on both sides).
 __global__ void k_copy_walls(WallRefs W, int n, const Ctl* ctl, int honor_abort) {
     if (honor_abort && ctl->abort) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int q = 0; q < W.nwall; ++q) W.h_out[q][i] = W.h_in[q][i];
+    const unsigned nz0 = W.nz[i];
+    unsigned nz = nz0;
+    for (int q = 0; q < W.nwall; ++q) {
+        const unsigned bin = wall_nz_bit(q, W.in), bout = wall_nz_bit(q, W.out);
+        if (nz & bin) {
+            W.h_out[q][i] = W.h_in[q][i];
+            nz |= bout;
+        } else if (nz & bout) {
+            W.h_out[q][i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            nz &= ~bout;
+        }
+    }
+    if (nz != nz0) W.nz[i] = (uint8_t)nz;
 }
+
+// WallRefs::nz from the walls' two history sets, at the start of a fused run
+// (other code writes the histories densely and keeps no flags).
+__global__ void k_walls_nz(WallRefs W, const double4* h0_0, const double4* h0_1,
+                           const double4* h1_0, const double4* h1_1, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4* h[TMPS_RUN_WALLS][2] = {{h0_0, h0_1}, {h1_0, h1_1}};
+    unsigned nz = 0;
+    for (int q = 0; q < W.nwall; ++q)
+        for (int b = 0; b < 2; ++b) {
+            const double4 e = h[q][b][i];
+            // bitwise: a negative zero counts as nonzero (a set bit may cover a zero)
+            const unsigned long long a = __double_as_longlong(e.x) | __double_as_longlong(e.y) |
+                                         __double_as_longlong(e.z) | __double_as_longlong(e.w);
+            if (a) nz |= wall_nz_bit(q, b);
+        }
+    W.nz[i] = (uint8_t)nz;
+}
+
 
 // ------------------------------------------------ batched redo (batch.hip)
 //
@@ -1283,9 +1318,11 @@ __global__ void k_wall_gran(const double4* xr, const V* vm, const V* om, V* f, V
     const V v4 = vm[i], o4 = om[i];
     const double v[3] = {v4.x, v4.y, v4.z}, w[3] = {o4.x, o4.y, o4.z};
     double dF[3], dT[3];
-    if (wall_plane_contact(P, coord, vwall_n, vwall_s, xs, x4.w, v4.w, v, w, dt, history_update,
-                           wh, wh, i, dF, dT))
-        wall_stage_add(f, tq, i, dF, dT);
+    double4 hn;
+    const bool touched = wall_plane_contact(P, coord, vwall_n, vwall_s, xs, x4.w, v4.w, v, w, dt,
+                                            history_update, wh + i, hn, dF, dT);
+    if (P.use_history) wh[i] = hn;
+    if (touched) wall_stage_add(f, tq, i, dF, dT);
 }
 
 template <typename V>
@@ -1299,8 +1336,11 @@ __global__ void k_wall_cyl(const double4* xr, const V* vm, const V* om, V* f, V*
     const V v4 = vm[i], o4 = om[i];
     const double vi[3] = {v4.x, v4.y, v4.z}, oi[3] = {o4.x, o4.y, o4.z};
     double dF[3], dT[3];
-    if (wall_cyl_contact(P, pos, x4.w, v4.w, vi, oi, dt, history_update, wh, wh, i, dF, dT))
-        wall_stage_add(f, tq, i, dF, dT);
+    double4 hn;
+    const bool touched =
+        wall_cyl_contact(P, pos, x4.w, v4.w, vi, oi, dt, history_update, wh + i, hn, dF, dT);
+    if (P.use_history) wh[i] = hn;
+    if (touched) wall_stage_add(f, tq, i, dF, dT);
 }
 
 // The final pairwise tree over nparts block partials, NC components, into out.
@@ -1691,6 +1731,15 @@ int launch_copy_state(System& s, int honor_abort) {
     HIP_TRY(hipGetLastError());
     if (s.nrun_walls > 0)
         LAUNCH_N(k_copy_walls, s.n, run_walls(s, 1, 0), s.n, s.ctl, honor_abort);
+    return 0;
+}
+
+int launch_walls_nz(System& s) {
+    if (s.nrun_walls <= 0 || s.n <= 0) return 0;
+    static_assert(TMPS_RUN_WALLS == 2, "k_walls_nz takes two walls' histories");
+    const WallRefs W = run_walls(s, 0, 1);
+    LAUNCH_N(k_walls_nz, s.n, W, s.run_wh[0][0], s.run_wh[0][1], s.run_wh[1][0], s.run_wh[1][1],
+             s.n);
     return 0;
 }
 

@@ -754,7 +754,7 @@ __device__ __forceinline__ void split_step_block(const FusedS& P, Ctl* ctl, cons
 }
 
 template <int STYLE, int CUNDALL, int FINAL, int WALL>
-__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
+__global__ void __launch_bounds__(TMPS_BLOCK) __attribute__((amdgpu_waves_per_eu(WALL ? TMPS_STEP_WAVES_SPLIT_WALL : TMPS_STEP_WAVES_SPLIT))) k_step_split(FusedS P, Ctl* ctl, PairP pp, double* partial,
                                                             float dt, float gl, float ga) {
     split_step_block<STYLE, CUNDALL, WALL>(P, ctl, pp, partial, dt, gl, ga,
                                            xcd_block(blockIdx.x, gridDim.x), FINAL != 0);

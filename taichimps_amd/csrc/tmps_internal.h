@@ -124,14 +124,21 @@ struct WallSpec {
 // The walls of a fused step, at most TMPS_RUN_WALLS: settings in device
 // memory, and per wall the shear history the step reads and the one it
 // writes (the two buffer sets of the step, or the same buffer in place).
+// nz[i] holds, per wall q and buffer set b, bit wall_nz_bit(q, b): clear only
+// where that set's entry of atom i is zero (a set bit may cover a zero).
 #define TMPS_RUN_WALLS 2
 struct WallRefs {
     const WallSpec* spec;
     const double4* h_in[TMPS_RUN_WALLS];
     double4* h_out[TMPS_RUN_WALLS];
+    uint8_t* nz;
     int nwall;
-    int pad;
+    int in, out;  // the buffer sets h_in and h_out belong to
 };
+
+__host__ __device__ constexpr unsigned wall_nz_bit(int wall, int set) {
+    return 1u << (2 * wall + set);
+}
 
 // One FP32-split ensemble member whose step is redone with a list rebuild,
 // as the batched redo kernels read it (the single-system redo's buffers,
@@ -285,6 +292,7 @@ struct System {
     WallSpec run_spec[TMPS_RUN_WALLS] = {};
     WallSpec* d_run_spec = nullptr;
     double4* run_wh[TMPS_RUN_WALLS][2] = {};
+    uint8_t* run_wnz = nullptr;  // [max_atoms]: WallRefs::nz of the run's walls
 
     // the adjoint engine's checkpoints and work buffers (adjoint.hip)
     struct AdjContext* adj = nullptr;
@@ -387,6 +395,9 @@ inline WallRefs run_walls(const System& s, int in, int out) {
     WallRefs W{};
     W.spec = s.d_run_spec;
     W.nwall = s.nrun_walls;
+    W.nz = s.run_wnz;
+    W.in = in;
+    W.out = out;
     for (int q = 0; q < s.nrun_walls; ++q) {
         W.h_in[q] = s.run_wh[q][in];
         W.h_out[q] = s.run_wh[q][out];
@@ -474,6 +485,8 @@ int launch_kick_split(System& s, double dt);
 // buffer set 1 -> buffer set 0 (positions, v, omega, touch flags, history);
 // skipped on abort when honor_abort
 int launch_copy_state(System& s, int honor_abort);
+// WallRefs::nz of the run's walls from their histories (a run's start)
+int launch_walls_nz(System& s);
 
 // shared (k_fp64.hip)
 int launch_step_end(System& s, double dt, int nparts, int advance, int apply_servo);

@@ -14,17 +14,19 @@ namespace tmps {
 // FixWallGran.post_force_kernel, wall_gran.py:120-211: hooke normal force with
 // mass-velocity damping and the linear_history_classic tangential force, or
 // the history-free viscous one. dF / dT receive the force and torque added to
-// the atom (zero out of contact); with history, h_out[i] receives the shear
-// (xyz) and the touch flag (w).
+// the atom (zero out of contact); with history, the atom's shear (xyz) and
+// touch flag (w) are read through h_prev (null: a zero history, read only in
+// contact) and the new ones returned in h_new (zero out of contact).
 __device__ inline bool wall_plane_contact(const tmps_wall_params& P, double coord,
                                           double vwall_n, double vwall_s, const double xs[3],
                                           double ri, double meff, const double v[3],
                                           const double w[3], double dt, int history_update,
-                                          const double4* h_in, double4* h_out, int i,
-                                          double dF[3], double dT[3]) {
+                                          const double4* h_prev, double4& h_new, double dF[3],
+                                          double dT[3]) {
 #pragma clang fp contract(off)
     dF[0] = dF[1] = dF[2] = 0.0;
     dT[0] = dT[1] = dT[2] = 0.0;
+    h_new = make_double4(0.0, 0.0, 0.0, 0.0);
     const int axis = P.axis;
     double nv[3] = {0.0, 0.0, 0.0};
     nv[axis] = P.side == 1 ? -1.0 : 1.0;
@@ -34,10 +36,7 @@ __device__ inline bool wall_plane_contact(const tmps_wall_params& P, double coor
     const double xa = xs[axis];
     const double dist = P.side == -1 ? xa - coord : coord - xa;
     const double delta = ri - dist;
-    if (delta <= 0.0) {
-        if (P.use_history) h_out[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return false;
-    }
+    if (delta <= 0.0) return false;
     double vr[3], vt[3], wr[3], vtr[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) vr[k] = v[k] - vwall[k];
@@ -58,7 +57,7 @@ __device__ inline bool wall_plane_contact(const tmps_wall_params& P, double coor
     const double damp_t = meff * P.gammat;
     double fs[3] = {0.0, 0.0, 0.0};
     if (P.use_history) {
-        const double4 hh = h_in[i];
+        const double4 hh = h_prev ? *h_prev : make_double4(0.0, 0.0, 0.0, 0.0);
         double h[3] = {hh.x, hh.y, hh.z};
         if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
         if (history_update)
@@ -87,7 +86,7 @@ __device__ inline bool wall_plane_contact(const tmps_wall_params& P, double coor
                 fs[0] = fs[1] = fs[2] = 0.0;
             }
         }
-        h_out[i] = make_double4(h[0], h[1], h[2], 1.0);
+        h_new = make_double4(h[0], h[1], h[2], 1.0);
     } else {
         double ft = 0.0;
         if (vrel != 0.0) {
@@ -111,37 +110,30 @@ __device__ inline bool wall_plane_contact(const tmps_wall_params& P, double coor
 // (viscoelastic: the contact radius sqrt(r delta) scales stiffness and
 // damping), linear_history_classic or history-free tangential force; atoms
 // outside [axis_lo, axis_hi] (when bounded) or on the axis are out of contact.
+// The history is read and returned as wall_plane_contact's.
 __device__ inline bool wall_cyl_contact(const tmps_wall_params& P, const double pos[3],
                                         double r_i, double mi, const double vi[3],
                                         const double oi[3], double dt, int history_update,
-                                        const double4* h_in, double4* h_out, int i,
-                                        double dF[3], double dT[3]) {
+                                        const double4* h_prev, double4& h_new, double dF[3],
+                                        double dT[3]) {
 #pragma clang fp contract(off)
     dF[0] = dF[1] = dF[2] = 0.0;
     dT[0] = dT[1] = dT[2] = 0.0;
+    h_new = make_double4(0.0, 0.0, 0.0, 0.0);
     const int axis = P.axis;
     const double pos_axis = pos[axis];
-    if (P.has_bounds && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) {
-        if (P.use_history) h_out[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return false;
-    }
+    if (P.has_bounds && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) return false;
     const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
     const double d1 = pos[a1] - P.c1, d2 = pos[a2] - P.c2;
     const double r_perp = sqrt(d1 * d1 + d2 * d2);
-    if (!(r_perp > 1e-14)) {
-        if (P.use_history) h_out[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return false;
-    }
+    if (!(r_perp > 1e-14)) return false;
     const double inv_r_perp = 1.0 / r_perp;
     const double u1 = d1 * inv_r_perp, u2 = d2 * inv_r_perp;
     double nw[3] = {0.0, 0.0, 0.0};
     nw[a1] = P.side == -1 ? -u1 : u1;
     nw[a2] = P.side == -1 ? -u2 : u2;
     const double delta = P.side == -1 ? r_i - (P.radius - r_perp) : r_i - (r_perp - P.radius);
-    if (!(delta > 0.0)) {
-        if (P.use_history) h_out[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return false;
-    }
+    if (!(delta > 0.0)) return false;
     const double oxn[3] = {oi[1] * nw[2] - oi[2] * nw[1], oi[2] * nw[0] - oi[0] * nw[2],
                            oi[0] * nw[1] - oi[1] * nw[0]};
     double vr[3], vrt[3];
@@ -161,7 +153,7 @@ __device__ inline bool wall_cyl_contact(const tmps_wall_params& P, const double 
     const double damp_t = mi * P.gammat * poly;
     double ft[3] = {0.0, 0.0, 0.0};
     if (P.use_history) {
-        const double4 hh = h_in[i];
+        const double4 hh = h_prev ? *h_prev : make_double4(0.0, 0.0, 0.0, 0.0);
         double h[3] = {hh.x, hh.y, hh.z};
         if (hh.w == 0.0) h[0] = h[1] = h[2] = 0.0;
         if (history_update)
@@ -192,7 +184,7 @@ __device__ inline bool wall_cyl_contact(const tmps_wall_params& P, const double 
                 ft[0] = ft[1] = ft[2] = 0.0;
             }
         }
-        h_out[i] = make_double4(h[0], h[1], h[2], 1.0);
+        h_new = make_double4(h[0], h[1], h[2], 1.0);
     } else {
         double fsc = 0.0;
         if (vrel != 0.0) {
@@ -211,39 +203,82 @@ __device__ inline bool wall_cyl_contact(const tmps_wall_params& P, const double 
     return true;
 }
 
-// One wall of a step on atom i (geometry from its settings).
+// One wall of a step on one atom (geometry from its settings).
 __device__ inline bool wall_contact(const WallSpec& S, const double xs[3], double ri,
                                     double meff, const double v[3], const double w[3], double dt,
-                                    int history_update, const double4* h_in, double4* h_out,
-                                    int i, double dF[3], double dT[3]) {
+                                    int history_update, const double4* h_prev, double4& h_new,
+                                    double dF[3], double dT[3]) {
     if (S.p.geometry == TMPS_WALL_CYLINDER)
-        return wall_cyl_contact(S.p, xs, ri, meff, v, w, dt, history_update, h_in, h_out, i, dF,
+        return wall_cyl_contact(S.p, xs, ri, meff, v, w, dt, history_update, h_prev, h_new, dF,
                                 dT);
     return wall_plane_contact(S.p, S.coord, S.vwall_n, S.vwall_s, xs, ri, meff, v, w, dt,
-                              history_update, h_in, h_out, i, dF, dT);
+                              history_update, h_prev, h_new, dF, dT);
+}
+
+// Whether the atom touches the wall: the contact functions' own test, in the
+// same arithmetic, without the force. A step runs it on every atom and the
+// contact (whose FP64 temporaries would otherwise be live in every atom's
+// path) only on those it admits.
+__device__ inline bool wall_touches(const WallSpec& S, const double xs[3], double ri) {
+#pragma clang fp contract(off)
+    const tmps_wall_params& P = S.p;
+    if (P.geometry != TMPS_WALL_CYLINDER) {
+        const double xa = xs[P.axis];
+        const double dist = P.side == -1 ? xa - S.coord : S.coord - xa;
+        return !(ri - dist <= 0.0);
+    }
+    const double pos_axis = xs[P.axis];
+    if (P.has_bounds && (pos_axis < P.axis_lo || pos_axis > P.axis_hi)) return false;
+    const int a1 = P.axis == 0 ? 1 : 0, a2 = P.axis == 2 ? 1 : 2;
+    const double d1 = xs[a1] - P.c1, d2 = xs[a2] - P.c2;
+    const double r_perp = sqrt(d1 * d1 + d2 * d2);
+    if (!(r_perp > 1e-14)) return false;
+    const double delta = P.side == -1 ? ri - (P.radius - r_perp) : ri - (r_perp - P.radius);
+    return delta > 0.0;
 }
 
 // Adds a fused step's walls that come before (pass 0) or after (pass 1)
 // damping/cundall to the atom's force and torque, F / T in FP64 or FP32 (an
 // FP32 sum takes the FP64 increment and rounds once, as the stage kernels'
-// f[i] + increment does).
+// f[i] + increment does). The histories are dense (a zero entry for an atom
+// out of contact), but only the entries flagged possibly nonzero in W.nz are
+// read or rewritten: an atom out of contact whose out-set entry is already
+// zero costs one byte read, not a 32-byte store per wall.
 template <typename R>
 __device__ inline void walls_apply(const WallRefs& W, int pass, const double xs[3], double ri,
                                    double meff, const double v[3], const double w[3], double dt,
                                    int i, R F[3], R T[3]) {
 #pragma clang fp contract(off)
+    if (W.nwall == 0) return;
+    const unsigned nz0 = W.nz[i];
+    unsigned nz = nz0;
     for (int q = 0; q < W.nwall; ++q) {
         const WallSpec& S = W.spec[q];
         if ((S.before_cundall ? 0 : 1) != pass) continue;
+        const unsigned bin = wall_nz_bit(q, W.in), bout = wall_nz_bit(q, W.out);
         double dF[3], dT[3];
-        if (!wall_contact(S, xs, ri, meff, v, w, dt, 1, W.h_in[q], W.h_out[q], i, dF, dT))
-            continue;
+        double4 hn = make_double4(0.0, 0.0, 0.0, 0.0);
+        const bool touched =
+            wall_touches(S, xs, ri) &&
+            wall_contact(S, xs, ri, meff, v, w, dt, 1, (nz & bin) ? W.h_in[q] + i : nullptr, hn,
+                         dF, dT);
+        if (S.p.use_history) {
+            if (touched) {
+                W.h_out[q][i] = hn;
+                nz |= bout;
+            } else if (nz & bout) {
+                W.h_out[q][i] = make_double4(0.0, 0.0, 0.0, 0.0);
+                nz &= ~bout;
+            }
+        }
+        if (!touched) continue;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             F[k] = (R)((double)F[k] + dF[k]);
             T[k] = (R)((double)T[k] + dT[k]);
         }
     }
+    if (nz != nz0) W.nz[i] = (uint8_t)nz;
 }
 
 }  // namespace tmps
