@@ -700,6 +700,40 @@ int profiled_steps(System& s, int len, int kick, int last, double dt, int use_cu
     return rc;
 }
 
+// The graph slot holding the work `enqueue` puts on the system's stream for
+// `key`, captured now if no slot holds it.
+template <class Enqueue>
+int graph_slot(System& s, uint64_t key, Enqueue&& enqueue, int* slot_out) {
+    int slot = -1;
+    for (int k = 0; k < System::kGraphSlots; ++k)
+        if (s.seg_exec[k] && s.seg_key[k] == key) slot = k;
+    if (slot < 0) {
+        slot = s.seg_next;
+        s.seg_next = (s.seg_next + 1) % System::kGraphSlots;
+        if (s.seg_exec[slot]) {
+            // the slot's graph may still be in flight
+            HIP_TRY(hipStreamSynchronize(s.stream));
+            (void)hipGraphExecDestroy(s.seg_exec[slot]);
+            s.seg_exec[slot] = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s.stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return tmps_fail_hip(e, "hipStreamEndCapture", __LINE__);
+        e = hipGraphInstantiate(&s.seg_exec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return tmps_fail_hip(e, "hipGraphInstantiate", __LINE__);
+        s.seg_key[slot] = key;
+    }
+    *slot_out = slot;
+    return 0;
+}
+
 // The graph slot holding the captured segment for these settings, captured
 // now if none does.
 int segment_graph(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
@@ -720,35 +754,11 @@ int segment_graph(System& s, int len, int kick, int last, double dt, int use_cun
     for (int q = 0; q < s.nrun_walls; ++q)
         key = mix(key, (uint64_t)(uintptr_t)s.run_wh[q][0]);
     if (s.nrun_walls > 0) key = mix(key, (uint64_t)(uintptr_t)s.run_wnz);
-    int slot = -1;
-    for (int k = 0; k < System::kGraphSlots; ++k)
-        if (s.seg_exec[k] && s.seg_key[k] == key) slot = k;
-    if (slot < 0) {
-        slot = s.seg_next;
-        s.seg_next = (s.seg_next + 1) % System::kGraphSlots;
-        if (s.seg_exec[slot]) {
-            // the slot's graph may still be in flight
-            HIP_TRY(hipStreamSynchronize(s.stream));
-            (void)hipGraphExecDestroy(s.seg_exec[slot]);
-            s.seg_exec[slot] = nullptr;
-        }
-        HIP_TRY(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(s.stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return tmps_fail_hip(e, "hipStreamEndCapture", __LINE__);
-        e = hipGraphInstantiate(&s.seg_exec[slot], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return tmps_fail_hip(e, "hipGraphInstantiate", __LINE__);
-        s.seg_key[slot] = key;
-    }
-    *slot_out = slot;
-    return 0;
+    return graph_slot(
+        s, key, [&] { return enqueue_steps(s, len, kick, last, dt, use_cundall, gl, ga); },
+        slot_out);
 }
+
 
 int launch_segment(System& s, int len, int kick, int last, double dt, int use_cundall, double gl,
                    double ga) {
@@ -919,17 +929,15 @@ int dev_redo_prepare(System& s, DevRedo& R) {
 // record of the build is written now; should the device find the build
 // beyond its buffers, the run's next status says so and dev_redo_recover
 // takes the record back and redoes the build on the host.
-int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall, double gl,
-             double ga) {
+// The kernels of a device-decided redo (redo_dev) on the system's stream,
+// with the host's flags as they stand (which it updates as it goes). A
+// negative step is the step being redone, read on the device: the chain
+// then holds no per-rebuild value and is captured once and replayed.
+static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, double dt,
+                          int use_cundall, double gl, double ga) {
     const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
     int rc = 0;
-    R.stp = stp;
-    R.build_count = s.build_count;
-    R.last_build_step = s.last_build_step;
-    R.has_built = s.has_built;
-    R.seed_pending = s.seed_pending;
-    R.have_old = s.have_old;
     if (par == 1) {
         rc = launch_copy_state(s, 0);
         if (rc) return rc;
@@ -938,7 +946,6 @@ int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall
     if (rc) return rc;
     rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
     if (rc) return rc;
-    s.current_step = stp;
     // history_save: the list (its slot count read on the device) into the
     // copy, which is as large as the list's buffers
     if (s.seed_pending) {
@@ -957,14 +964,14 @@ int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall
     s.x64_valid = true;
     rc = launch_pbc(s);
     if (!rc) rc = launch_build_dev(s, R.max_cutoff, R.max_cells);
-    if (!rc) rc = launch_ctl_built_dev(s, stp);
+    if (!rc) rc = launch_ctl_built_dev(s, step);
     if (!rc && split(s)) rc = launch_x64_to_split_dev(s);
     if (!rc) rc = launch_hist_restore(s);
     if (rc) return rc;
     s.x64_valid = false;
     s.have_old = false;
     s.seed_pending = false;
-    rc = launch_ctl_rebuilt(s, stp);
+    rc = launch_ctl_rebuilt(s, step);
     if (rc) return rc;
     rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
                   : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
@@ -972,7 +979,69 @@ int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall
     if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
     if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
     if (!rc) rc = launch_remap_clear(s, 1);
+    return rc;
+}
+
+// What a captured redo chain depends on: the host flags that choose its
+// kernels, the run's settings, launch shapes, and every buffer that can be
+// reallocated between rebuilds (a changed one gives a new capture).
+static uint64_t redo_graph_key(const System& s, const DevRedo& R, int par, double dt,
+                               int use_cundall, double gl, double ga) {
+    uint64_t key = mix(1469598103934665603ULL, 0x7265646f63686e00ULL);  // not a segment
+    key = mix(key, (uint64_t)par | (uint64_t)(s.seed_pending ? 2 : 0) |
+                       (uint64_t)(s.x64_valid ? 4 : 0) | (uint64_t)(s.have_old ? 8 : 0));
+    for (double d : {dt, gl, ga, R.max_cutoff, s.skin}) key = mix(key, dbits(d));
+    for (long long v : {(long long)use_cundall, (long long)s.pair.style, (long long)s.n,
+                        (long long)s.precision, (long long)(s.have_servo && s.servo.enabled),
+                        (long long)R.max_cells, (long long)s.max_nb, (long long)s.nb_cap,
+                        (long long)s.onb_cap, (long long)s.cell_cap, (long long)s.scan_tmp_cap,
+                        (long long)s.rs_na, (long long)s.rs_nh, (long long)s.nrun_walls})
+        key = mix(key, (uint64_t)v);
+    for (const void* p : {(const void*)s.nb_start, (const void*)s.nb_j, (const void*)s.nb_flag,
+                          (const void*)s.nb_hist, (const void*)s.nb_flag2, (const void*)s.nb_hist2,
+                          (const void*)s.nb_code, (const void*)s.onb_start, (const void*)s.onb_j,
+                          (const void*)s.onb_flag, (const void*)s.onb_hist, (const void*)s.cell_count,
+                          (const void*)s.cell_start, (const void*)s.cell_fill,
+                          (const void*)s.cell_of, (const void*)s.cell_particles,
+                          (const void*)s.bin_x, (const void*)s.row_tmp, (const void*)s.scan_tmp,
+                          (const void*)s.rs_k1, (const void*)s.rs_k2, (const void*)s.rs_v1,
+                          (const void*)s.rs_v2, (const void*)s.rs_hist, (const void*)s.rs_offs,
+                          (const void*)s.run_wh[0][0], (const void*)s.run_wh[1][0],
+                          (const void*)s.run_wnz, (const void*)s.d_run_spec})
+        key = mix(key, (uint64_t)(uintptr_t)p);
+    return key;
+}
+
+int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall, double gl,
+             double ga) {
+    R.stp = stp;
+    R.build_count = s.build_count;
+    R.last_build_step = s.last_build_step;
+    R.has_built = s.has_built;
+    R.seed_pending = s.seed_pending;
+    R.have_old = s.have_old;
+    const bool save = !s.seed_pending;
+    int rc = 0;
+    if (s.launch_mode == TMPS_LAUNCH_DIRECT || s.profiling) {
+        rc = redo_dev_chain(s, R, par, stp, dt, use_cundall, gl, ga);
+    } else {
+        // one graph launch instead of some forty kernel launches, whose host
+        // cost would leave the device idle between the chain's small kernels
+        const uint64_t key = redo_graph_key(s, R, par, dt, use_cundall, gl, ga);
+        int slot = -1;
+        rc = graph_slot(
+            s, key, [&] { return redo_dev_chain(s, R, par, -1, dt, use_cundall, gl, ga); },
+            &slot);
+        if (!rc) HIP_TRY(hipGraphLaunch(s.seg_exec[slot], s.stream));
+    }
     if (rc) return rc;
+    // the host flags as the chain leaves them (a replayed graph ran none of
+    // its host side)
+    if (save) s.onb_n = s.n;
+    s.seed_pending = false;
+    s.x64_valid = false;
+    s.have_old = false;
+    s.current_step = stp;
     // neigh_record's bookkeeping (the slot count and grid come back with the
     // run's last control block)
     s.has_built = 1;

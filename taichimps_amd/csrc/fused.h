@@ -26,13 +26,6 @@
 
 namespace tmps {
 
-// MI355X hands workgroup w to XCD (w mod 8). Blocks of consecutive atoms share
-// neighbours (and so L2 lines), so each XCD is given one contiguous range of
-// blocks: a bijection of [0, nb) onto itself.
-__device__ __forceinline__ int xcd_block(int w, int nb) {
-    const int q = nb >> 3, r = nb & 7, x = w & 7, k = w >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
 
 // A block's share of a fused step pass: the pairwise-tree partial of its 256
 // atoms for the 7 tree leaves (6 virial+kinetic components, touching-slot

@@ -613,9 +613,12 @@ __device__ __forceinline__ void ctl_rebuilt(Ctl* ctl, int step, int split) {
 }
 
 // (a build that recorded an error keeps its abort: the rest of the redo
-// then leaves the step's state for the host)
+// then leaves the step's state for the host). A negative step is the step
+// being redone, read on the device: the aborted step left the timestep one
+// short of it, and nothing in the redo advances it before the step's end.
 __global__ void k_ctl_rebuilt(Ctl* ctl, int step, int split) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->error) ctl_rebuilt(ctl, step, split);
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->error)
+        ctl_rebuilt(ctl, step < 0 ? ctl->timestep + 1 : step, split);
 }
 
 // The host's edited mirror of the control block, stored into the device's
@@ -662,10 +665,11 @@ __global__ void k_ctl_built(Ctl* ctl, int step, int3 fast, int3 gi, int split) {
 // ctl_built for a list build decided on the device, from its grid; the
 // build box is recorded in both precisions (the host derives the reported
 // grid from it after the run).
+// (a negative step: the step being redone, as for k_ctl_rebuilt)
 __global__ void k_ctl_built_dev(Ctl* ctl, int step, int split) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->error) return;
     const GridP& G = ctl->lgrid;
-    ctl_built(ctl, step, make_int3(G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3),
+    ctl_built(ctl, step < 0 ? ctl->timestep + 1 : step, make_int3(G.g[0] >= 3, G.g[1] >= 3, G.g[2] >= 3),
               make_int3(G.g[0], G.g[1], G.g[2]), split);
     if (!split)
         for (int k = 0; k < 3; ++k) {

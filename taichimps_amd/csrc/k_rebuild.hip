@@ -341,6 +341,22 @@ __global__ void k_bin_gather(int n, const int* particles, const double4* xr, dou
     if (p < n) bx[p] = xr[particles[p]];
 }
 
+// Accepted neighbour j of atom i, inserted in ascending order into the
+// thread's LDS row (column of a [LIST_ROW_CAP][TMPS_BLOCK] array); counted
+// past the cap.
+__device__ __forceinline__ void list_insert(int i, int j, int* row, int& nfull, int& nhalf) {
+    if (nfull < LIST_ROW_CAP) {
+        int q2 = nfull - 1;
+        while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
+            row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
+            --q2;
+        }
+        row[(q2 + 1) * TMPS_BLOCK] = j;
+    }
+    ++nfull;
+    if (j > i) ++nhalf;
+}
+
 // Candidate j at pj for atom i at pi: the cutoff test of k_list_fill, each
 // accepted j inserted in ascending order into the thread's LDS row (column
 // of a [LIST_ROW_CAP][TMPS_BLOCK] array). `img` selects the axes that take
@@ -365,18 +381,35 @@ __device__ __forceinline__ void list_test(int i, int j, const double4& pi, const
     if (img & 4) d2 = list_image(d2, prd[2], inv[2], fast[2]);
     const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
     const double rad_sum = pi.w + pj.w + skin;
-    if (rsq < rad_sum * rad_sum) {
-        if (nfull < LIST_ROW_CAP) {
-            int q2 = nfull - 1;
-            while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
-                row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
-                --q2;
-            }
-            row[(q2 + 1) * TMPS_BLOCK] = j;
-        }
-        ++nfull;
-        if (j > i) ++nhalf;
+    if (rsq < rad_sum * rad_sum) list_insert(i, j, row, nfull, nhalf);
+}
+
+// A candidate of list_lean_range: the separation on each axis is the plain
+// difference minus that axis's shift s (0, or +-prd where the candidate's bin
+// was reached across the periodic boundary), which is the nearest image
+// wherever the image decides the test (see list_rows_item).
+__device__ __forceinline__ void list_lean_test(int i, int j, const double4& pi, const double4& pj,
+                                               double s0, double s1, double s2, double skin,
+                                               int* row, int& nfull, int& nhalf) {
+    const double d0 = (pi.x - pj.x) - s0, d1 = (pi.y - pj.y) - s1, d2 = (pi.z - pj.z) - s2;
+    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    const double rad_sum = pi.w + pj.w + skin;
+    if (rsq < rad_sum * rad_sum && j != i) list_insert(i, j, row, nfull, nhalf);
+}
+
+// The candidates [a, b) of one bin piece, two per iteration.
+__device__ __forceinline__ void list_lean_range(int a, int b, int i, const double4& pi, double s0,
+                                                double s1, double s2, const int* particles,
+                                                const double4* bx, double skin, int* row,
+                                                int& nfull, int& nhalf) {
+    int p = a;
+    for (; p + 1 < b; p += 2) {
+        const int j0 = particles[p], j1 = particles[p + 1];
+        const double4 x0 = bx[p], x1 = bx[p + 1];
+        list_lean_test(i, j0, pi, x0, s0, s1, s2, skin, row, nfull, nhalf);
+        list_lean_test(i, j1, pi, x1, s0, s1, s2, skin, row, nfull, nhalf);
     }
+    if (p < b) list_lean_test(i, particles[p], pi, bx[p], s0, s1, s2, skin, row, nfull, nhalf);
 }
 
 // One bin-order range [a, b) of candidates for atom i, read four at a time,
@@ -478,15 +511,42 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
         rbnd[r] = rbase[r] < 0 ? 0 : cell_start[rbase[r] + x1 + 1];
     }
     int nfull = 0, nhalf = 0;
-    const int ximg = per[0] == 1 ? 1 : 0;
+    // With every position inside the box and at least 3 bins on every
+    // periodic axis, a candidate's nearest image on an axis is known from how
+    // its bin was reached. Without crossing the boundary it is the plain
+    // difference (argued at list_test). Across it, |d| < prd and the sign of
+    // d is fixed, so round(d / prd) is k = +-1 (the crossing's direction) or
+    // 0; with k the reference's d - prd * k is exactly d - (+-prd), and where
+    // it is 0 both the image and d - (+-prd) exceed a bin width, so the pair
+    // is out either way. One subtraction per axis then replaces the rounding.
+    const bool lean = wrapped && (per[0] != 1 || gx >= 3) && (per[1] != 1 || gy >= 3) &&
+                      (per[2] != 1 || gz >= 3);
+    if (lean) {
+        const double sw = w == gx - 1 ? -prd[0] : prd[0];  // the x piece across the boundary
 #pragma unroll 1
-    for (int r = 0; r < 9; ++r) {
-        if (rbase[r] < 0) continue;
-        list_range(ra[r], rbnd[r], i, pi, particles, bx, rimg[r], prd, inv, fast, skin, row,
-                   nfull, nhalf);
-        if (w >= 0)
-            list_range(cell_start[rbase[r] + w], cell_start[rbase[r] + w + 1], i, pi, particles,
-                       bx, rimg[r] | ximg, prd, inv, fast, skin, row, nfull, nhalf);
+        for (int r = 0; r < 9; ++r) {
+            if (rbase[r] < 0) continue;
+            const int ny = c[1] + (r % 3 - 1), nz = c[2] + (r / 3 - 1);
+            const double s1 = per[1] != 1 ? 0.0 : (ny < 0 ? -prd[1] : (ny >= gy ? prd[1] : 0.0));
+            const double s2 = per[2] != 1 ? 0.0 : (nz < 0 ? -prd[2] : (nz >= gz ? prd[2] : 0.0));
+            list_lean_range(ra[r], rbnd[r], i, pi, 0.0, s1, s2, particles, bx, skin, row, nfull,
+                            nhalf);
+            if (w >= 0)
+                list_lean_range(cell_start[rbase[r] + w], cell_start[rbase[r] + w + 1], i, pi, sw,
+                                s1, s2, particles, bx, skin, row, nfull, nhalf);
+        }
+    } else {
+        const int ximg = per[0] == 1 ? 1 : 0;
+#pragma unroll 1
+        for (int r = 0; r < 9; ++r) {
+            if (rbase[r] < 0) continue;
+            list_range(ra[r], rbnd[r], i, pi, particles, bx, rimg[r], prd, inv, fast, skin, row,
+                       nfull, nhalf);
+            if (w >= 0)
+                list_range(cell_start[rbase[r] + w], cell_start[rbase[r] + w + 1], i, pi,
+                           particles, bx, rimg[r] | ximg, prd, inv, fast, skin, row, nfull,
+                           nhalf);
+        }
     }
     full_count[i] = nfull;
     half_count[i] = nhalf;
@@ -507,7 +567,9 @@ __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int
                                                           int wrapped) {
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
     if (Gd && ctl->error) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    // consecutive blocks read overlapping candidate ranges: one XCD (and its
+    // L2) takes a contiguous run of them
+    const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p < n)
         list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, Gd ? *Gd : G, cell_start, particles,
                        bx, skin, max_nb, full_count, half_count, flags, rows, wrapped);
@@ -1329,7 +1391,10 @@ int launch_build_dev(System& s, double max_cutoff, int max_cells) {
 }
 
 int launch_list_save(System& s) {
-    const int items = std::max(s.n + 1, s.nslots);
+    // grid-stride over the slot count read on the device: sized for the
+    // list's capacity, so the launch stays valid in a replayed graph
+    const int items = (int)std::min<size_t>(std::max<size_t>((size_t)s.n + 1, s.nb_cap),
+                                            (size_t)INT32_MAX);
     const int blocks = std::min((items + TMPS_BLOCK - 1) / TMPS_BLOCK, 4096);
     const int cap = (int)std::min<size_t>(s.onb_cap, (size_t)INT32_MAX);
     if (s.precision == TMPS_FP64)

@@ -268,7 +268,7 @@ struct System {
     int servo_step_count = 0, servo_nsteps = 0;
 
     // captured step segments, keyed by their launch configuration
-    static const int kGraphSlots = 12;
+    static const int kGraphSlots = 16;
     hipGraphExec_t seg_exec[kGraphSlots] = {};
     uint64_t seg_key[kGraphSlots] = {};
     int seg_next = 0;
@@ -388,6 +388,14 @@ inline hipError_t copy_sync(hipStream_t st, void* dst, const void* src, size_t b
 }
 
 inline int nblocks_for(int n) { return (n + TMPS_BLOCK - 1) / TMPS_BLOCK; }
+
+// MI355X hands workgroup w to XCD (w mod 8). Blocks of consecutive atoms share
+// neighbours (and so L2 lines), so each XCD is given one contiguous range of
+// blocks: a bijection of [0, nb) onto itself.
+__device__ __forceinline__ int xcd_block(int w, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = w & 7, k = w >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
 
 // The fused run's walls as a step that reads buffer set `in` and writes set
 // `out` sees them (in == out: in place).
