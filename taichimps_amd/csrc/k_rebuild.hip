@@ -341,18 +341,29 @@ __global__ void k_bin_gather(int n, const int* particles, const double4* xr, dou
     if (p < n) bx[p] = xr[particles[p]];
 }
 
+// Whether the lean candidate walk applies (see list_rows_item): positions
+// wrapped into the box and at least 3 bins on every periodic axis.
+__device__ __forceinline__ bool list_lean(const DevBox& B, const GridP& G, int wrapped) {
+    return wrapped && (B.per[0] != 1 || G.g[0] >= 3) && (B.per[1] != 1 || G.g[1] >= 3) &&
+           (B.per[2] != 1 || G.g[2] >= 3);
+}
+
+// A row entry v inserted in ascending order into a thread's LDS column of k
+// sorted entries (column stride TMPS_BLOCK).
+__device__ __forceinline__ void column_insert(int* row, int k, int v) {
+    int q2 = k - 1;
+    while (q2 >= 0 && row[q2 * TMPS_BLOCK] > v) {
+        row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
+        --q2;
+    }
+    row[(q2 + 1) * TMPS_BLOCK] = v;
+}
+
 // Accepted neighbour j of atom i, inserted in ascending order into the
 // thread's LDS row (column of a [LIST_ROW_CAP][TMPS_BLOCK] array); counted
 // past the cap.
 __device__ __forceinline__ void list_insert(int i, int j, int* row, int& nfull, int& nhalf) {
-    if (nfull < LIST_ROW_CAP) {
-        int q2 = nfull - 1;
-        while (q2 >= 0 && row[q2 * TMPS_BLOCK] > j) {
-            row[(q2 + 1) * TMPS_BLOCK] = row[q2 * TMPS_BLOCK];
-            --q2;
-        }
-        row[(q2 + 1) * TMPS_BLOCK] = j;
-    }
+    if (nfull < LIST_ROW_CAP) column_insert(row, nfull, j);
     ++nfull;
     if (j > i) ++nhalf;
 }
@@ -519,9 +530,7 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
     // 0; with k the reference's d - prd * k is exactly d - (+-prd), and where
     // it is 0 both the image and d - (+-prd) exceed a bin width, so the pair
     // is out either way. One subtraction per axis then replaces the rounding.
-    const bool lean = wrapped && (per[0] != 1 || gx >= 3) && (per[1] != 1 || gy >= 3) &&
-                      (per[2] != 1 || gz >= 3);
-    if (lean) {
+    if (list_lean(B, G, wrapped)) {
         const double sw = w == gx - 1 ? -prd[0] : prd[0];  // the x piece across the boundary
 #pragma unroll 1
         for (int r = 0; r < 9; ++r) {
@@ -558,21 +567,186 @@ __device__ __forceinline__ void list_rows_item(int p, int* row, const double4* x
     for (int k = 0; k < nfull; ++k) rows[(size_t)k * n + i] = row[k * TMPS_BLOCK];
 }
 
+// lean_skip: launched after k_list_rows_tile, which builds the list itself
+// where the lean walk applies
 __global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows(const double4* xr, int n, const Ctl* ctl,
                                                           GridP G, const GridP* Gd,
                                                           const int* cell_start,
                                                           const int* particles, const double4* bx,
                                                           double skin, int max_nb, int* full_count,
                                                           int* half_count, int* flags, int* rows,
-                                                          int wrapped) {
+                                                          int wrapped, int lean_skip) {
     __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
     if (Gd && ctl->error) return;
+    if (lean_skip && list_lean(ctl->box, Gd ? *Gd : G, wrapped)) return;
     // consecutive blocks read overlapping candidate ranges: one XCD (and its
     // L2) takes a contiguous run of them
     const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p < n)
         list_rows_item(p, lrow + threadIdx.x, xr, n, ctl, Gd ? *Gd : G, cell_start, particles,
                        bx, skin, max_nb, full_count, half_count, flags, rows, wrapped);
+}
+
+// Candidates of k_list_rows_tile: the lean test, an accepted j appended to
+// atom i's scratch row rows[k * n + i] (unsorted; k_list_compact sorts).
+__device__ __forceinline__ void tile_test(int i, int j, const double4& pi, const double4& pj,
+                                          double s0, double s1, double s2, double skin, int* rows,
+                                          int n, int& nfull, int& nhalf) {
+    const double d0 = (pi.x - pj.x) - s0, d1 = (pi.y - pj.y) - s1, d2 = (pi.z - pj.z) - s2;
+    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+    const double rad_sum = pi.w + pj.w + skin;
+    if (rsq < rad_sum * rad_sum && j != i) {
+        if (nfull < LIST_ROW_CAP) rows[(size_t)nfull * n + i] = j;
+        ++nfull;
+        if (j > i) ++nhalf;
+    }
+}
+
+// The candidates [a, b) of a bin piece, from the LDS copy (sj, sx) or from
+// global memory (particles, bx).
+__device__ __forceinline__ void tile_range(int a, int b, int i, const double4& pi, double s0,
+                                           double s1, double s2, const int* js,
+                                           const double4* xs, double skin, int* rows, int n,
+                                           int& nfull, int& nhalf) {
+    int q = a;
+    for (; q + 1 < b; q += 2) {
+        const int j0 = js[q], j1 = js[q + 1];
+        const double4 x0 = xs[q], x1 = xs[q + 1];
+        tile_test(i, j0, pi, x0, s0, s1, s2, skin, rows, n, nfull, nhalf);
+        tile_test(i, j1, pi, x1, s0, s1, s2, skin, rows, n, nfull, nhalf);
+    }
+    if (q < b) tile_test(i, js[q], pi, xs[q], s0, s1, s2, skin, rows, n, nfull, nhalf);
+}
+
+// bin-order candidates of one stencil plane a block stages in LDS
+#define LIST_TILE_CAP 1024
+
+// k_list_rows where the lean walk applies (list_lean; otherwise it returns
+// and k_list_rows, launched after it, builds the list). The block's atoms
+// are consecutive in bin order, so for each plane of the stencil (dz) the
+// candidates of all their rows lie in one contiguous bin-order range: the
+// block copies it into LDS once, with coalesced loads, and its threads test
+// their rows against the copy instead of gathering every candidate from
+// memory themselves. A plane whose range exceeds LIST_TILE_CAP (a block
+// whose rows wrap a periodic boundary in y) is read from memory, as are the
+// single bins reached across the x boundary. Rows go to the scratch
+// unsorted; the accepted set is list_rows_item's.
+__global__ void __launch_bounds__(TMPS_BLOCK) k_list_rows_tile(
+    int n, const Ctl* ctl, GridP G, const GridP* Gd, const int* cell_start, const int* particles,
+    const double4* bx, double skin, int max_nb, int* full_count, int* half_count, int* flags,
+    int* rows, int wrapped) {
+    __shared__ int sj[LIST_TILE_CAP];
+    __shared__ double4 sx[LIST_TILE_CAP];
+    __shared__ int wred[TMPS_BLOCK / 64][6];
+    __shared__ int hull[6];
+    if (Gd && ctl->error) return;
+    const GridP g = Gd ? *Gd : G;
+    const DevBox& B = ctl->box;
+    if (!list_lean(B, g, wrapped)) return;
+    const int p = xcd_block(blockIdx.x, gridDim.x) * TMPS_BLOCK + threadIdx.x;
+    const bool act = p < n;
+    const int i = particles[act ? p : n - 1];
+    const double4 pi = bx[act ? p : n - 1];  // = xr[i]
+    int c[3];
+    cell_coord(B, g, pi, c);
+    const int gx = g.g[0], gy = g.g[1], gz = g.g[2], gxy = gx * gy;
+    const int per[3] = {B.per[0], B.per[1], B.per[2]};
+    const double prd[3] = {B.prd[0], B.prd[1], B.prd[2]};
+    // x-adjacent bins [x0, x1] of every row, plus a wrapped piece [w, w]
+    int x0 = c[0] - 1, x1 = c[0] + 1, w = -1;
+    if (per[0] == 1) {
+        if (x0 < 0) { x0 = 0; w = gx - 1; }
+        if (x1 >= gx) { x1 = gx - 1; w = 0; }
+    } else {
+        x0 = max(x0, 0);
+        x1 = min(x1, gx - 1);
+    }
+    int rbase[9], ra[9], rb[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        int nz = c[2] + r / 3 - 1, ny = c[1] + r % 3 - 1;
+        if (per[2] == 1) nz = nz < 0 ? nz + gz : (nz >= gz ? nz - gz : nz);
+        if (per[1] == 1) ny = ny < 0 ? ny + gy : (ny >= gy ? ny - gy : ny);
+        rbase[r] = (nz < 0 || nz >= gz || ny < 0 || ny >= gy) ? -1 : ny * gx + nz * gxy;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        ra[r] = rbase[r] < 0 ? 0 : cell_start[rbase[r] + x0];
+        rb[r] = rbase[r] < 0 ? 0 : cell_start[rbase[r] + x1 + 1];
+    }
+    // each plane's candidate range over the block: [min ra, max rb)
+    int h[6];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+        h[pl] = INT_MAX;
+        h[3 + pl] = INT_MIN;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const int r = pl * 3 + dy;
+            if (act && rbase[r] >= 0 && rb[r] > ra[r]) {
+                h[pl] = min(h[pl], ra[r]);
+                h[3 + pl] = max(h[3 + pl], rb[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int o = __shfl_xor(h[k], off);
+            h[k] = k < 3 ? min(h[k], o) : max(h[k], o);
+        }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wred[threadIdx.x >> 6][k] = h[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        int v = wred[0][k];
+        for (int wv = 1; wv < TMPS_BLOCK / 64; ++wv) v = k < 3 ? min(v, wred[wv][k]) : max(v, wred[wv][k]);
+        hull[k] = v;
+    }
+    __syncthreads();
+    int nfull = 0, nhalf = 0;
+    const double sw = w == gx - 1 ? -prd[0] : prd[0];  // the x piece across the boundary
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+        const int lo = hull[pl], hi = hull[3 + pl];
+        const bool staged = hi > lo && hi - lo <= LIST_TILE_CAP;
+        if (staged)
+            for (int q = threadIdx.x; q < hi - lo; q += TMPS_BLOCK) {
+                sj[q] = particles[lo + q];
+                sx[q] = bx[lo + q];
+            }
+        __syncthreads();
+        if (act) {
+            const int nz = c[2] + pl - 1;
+            const double s2 = per[2] != 1 ? 0.0 : (nz < 0 ? -prd[2] : (nz >= gz ? prd[2] : 0.0));
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int r = pl * 3 + dy;
+                if (rbase[r] < 0) continue;
+                const int ny = c[1] + dy - 1;
+                const double s1 =
+                    per[1] != 1 ? 0.0 : (ny < 0 ? -prd[1] : (ny >= gy ? prd[1] : 0.0));
+                if (staged)
+                    tile_range(ra[r] - lo, rb[r] - lo, i, pi, 0.0, s1, s2, sj, sx, skin, rows, n,
+                               nfull, nhalf);
+                else
+                    tile_range(ra[r], rb[r], i, pi, 0.0, s1, s2, particles, bx, skin, rows, n,
+                               nfull, nhalf);
+                if (w >= 0)
+                    tile_range(cell_start[rbase[r] + w], cell_start[rbase[r] + w + 1], i, pi, sw,
+                               s1, s2, particles, bx, skin, rows, n, nfull, nhalf);
+            }
+        }
+        __syncthreads();  // the next plane overwrites the copy
+    }
+    if (!act) return;
+    full_count[i] = nfull;
+    half_count[i] = nhalf;
+    if (nhalf > max_nb) flags[0] = 1;
+    if (nfull > LIST_ROW_CAP) flags[1] = 1;
 }
 
 // Rows from the slot-major scratch of k_list_rows into the CSR list.
@@ -582,14 +756,22 @@ __device__ __forceinline__ void list_compact_atom(int i, int n, const int* rows,
     for (int k = 0; k < m; ++k) nb_j[a + k] = rows[(size_t)k * n + i];
 }
 
-// flags: a build decided on the device, which launches this kernel whatever
-// the row lengths: nothing to do where a row exceeded the scratch (k_list_fill
-// writes the list then)
-__global__ void k_list_compact(int n, const int* rows, const int* nb_start, int* nb_j,
-                               const int* flags) {
+// The same, each row sorted ascending in j on the way (k_list_rows_tile
+// appends a row's entries in the order it meets them; k_list_rows' rows are
+// sorted already). flags: a build decided on the device, which launches this
+// kernel whatever the row lengths: nothing to do where a row exceeded the
+// scratch (k_list_fill writes the list then)
+__global__ void __launch_bounds__(TMPS_BLOCK) k_list_compact(int n, const int* rows,
+                                                             const int* nb_start, int* nb_j,
+                                                             const int* flags) {
+    __shared__ int lrow[LIST_ROW_CAP * TMPS_BLOCK];
     if (flags && flags[1]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) list_compact_atom(i, n, rows, nb_start, nb_j);
+    if (i >= n) return;
+    const int a = nb_start[i], m = nb_start[i + 1] - a;  // at most LIST_ROW_CAP here
+    int* row = lrow + threadIdx.x;
+    for (int k = 0; k < m; ++k) column_insert(row, k, rows[(size_t)k * n + i]);
+    for (int k = 0; k < m; ++k) nb_j[a + k] = row[k * TMPS_BLOCK];
 }
 
 // ContactHistory save/restore (contact_history.py:91-139) as a merge of the
@@ -1320,10 +1502,15 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out,
         hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                            s.cell_particles, xr, s.bin_x);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
+        hipLaunchKernelGGL(k_list_rows_tile, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                            s.ctl, s.grid, nullptr, s.cell_start, s.cell_particles, s.bin_x, s.skin,
                            s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp,
                            wrapped ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, xr, s.n,
+                           s.ctl, s.grid, nullptr, s.cell_start, s.cell_particles, s.bin_x, s.skin,
+                           s.max_nb, s.full_count, s.half_count, s.flags, s.row_tmp,
+                           wrapped ? 1 : 0, 1);
         HIP_TRY(hipGetLastError());
     }
     rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
@@ -1370,9 +1557,13 @@ int launch_build_dev(System& s, double max_cutoff, int max_cells) {
     hipLaunchKernelGGL(k_bin_gather, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                        s.cell_particles, s.xr, s.bin_x);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n,
+    hipLaunchKernelGGL(k_list_rows_tile, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.n,
                        s.ctl, none, Gd, s.cell_start, s.cell_particles, s.bin_x, s.skin, s.max_nb,
                        s.full_count, s.half_count, s.flags, s.row_tmp, 1);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_list_rows, grid_for(s.n), dim3(TMPS_BLOCK), 0, s.stream, s.xr, s.n,
+                       s.ctl, none, Gd, s.cell_start, s.cell_particles, s.bin_x, s.skin, s.max_nb,
+                       s.full_count, s.half_count, s.flags, s.row_tmp, 1, 1);
     HIP_TRY(hipGetLastError());
     rc = launch_scan(s, s.full_count, s.nb_start, s.n, s.flags + 2);
     if (rc) return rc;
