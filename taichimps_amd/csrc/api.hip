@@ -933,7 +933,7 @@ int dev_redo_prepare(System& s, DevRedo& R) {
 // with the host's flags as they stand (which it updates as it goes). A
 // negative step is the step being redone, read on the device: the chain
 // then holds no per-rebuild value and is captured once and replayed.
-static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, double dt,
+static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, bool last, double dt,
                           int use_cundall, double gl, double ga) {
     const int nparts = nblocks_for(std::max(s.n, 1));
     const int servo_on = (s.have_servo && s.servo.enabled) ? 1 : 0;
@@ -973,8 +973,12 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, double
     s.seed_pending = false;
     rc = launch_ctl_rebuilt(s, step);
     if (rc) return rc;
-    rc = split(s) ? launch_step_force_split(s, dt, 1, use_cundall, gl, ga, 0, 1)
-                  : launch_step_force_f64(s, dt, 1, use_cundall, gl, ga, 0, 1);
+    // a run's last step stores f, torque and virial; a step inside it also
+    // applies the next step's half-kick (the kick kernel's arithmetic), so
+    // the next segment starts without one
+    const int final_mode = last ? 1 : 2;
+    rc = split(s) ? launch_step_force_split(s, dt, final_mode, use_cundall, gl, ga, 0, 1)
+                  : launch_step_force_f64(s, dt, final_mode, use_cundall, gl, ga, 0, 1);
     if (!rc) rc = launch_step_end(s, dt, nparts, 1, servo_on);
     if (!rc) rc = split(s) ? launch_copy_vw_split(s, 1) : launch_copy_vw(s, 1);
     if (!rc && !split(s)) rc = launch_remap_apply_fused(s);
@@ -985,11 +989,12 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, double
 // What a captured redo chain depends on: the host flags that choose its
 // kernels, the run's settings, launch shapes, and every buffer that can be
 // reallocated between rebuilds (a changed one gives a new capture).
-static uint64_t redo_graph_key(const System& s, const DevRedo& R, int par, double dt,
+static uint64_t redo_graph_key(const System& s, const DevRedo& R, int par, bool last, double dt,
                                int use_cundall, double gl, double ga) {
     uint64_t key = mix(1469598103934665603ULL, 0x7265646f63686e00ULL);  // not a segment
     key = mix(key, (uint64_t)par | (uint64_t)(s.seed_pending ? 2 : 0) |
-                       (uint64_t)(s.x64_valid ? 4 : 0) | (uint64_t)(s.have_old ? 8 : 0));
+                       (uint64_t)(s.x64_valid ? 4 : 0) | (uint64_t)(s.have_old ? 8 : 0) |
+                       (uint64_t)(last ? 16 : 0));
     for (double d : {dt, gl, ga, R.max_cutoff, s.skin}) key = mix(key, dbits(d));
     for (long long v : {(long long)use_cundall, (long long)s.pair.style, (long long)s.n,
                         (long long)s.precision, (long long)(s.have_servo && s.servo.enabled),
@@ -1012,8 +1017,8 @@ static uint64_t redo_graph_key(const System& s, const DevRedo& R, int par, doubl
     return key;
 }
 
-int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall, double gl,
-             double ga) {
+int redo_dev(System& s, DevRedo& R, int par, int stp, bool last, double dt, int use_cundall,
+             double gl, double ga) {
     R.stp = stp;
     R.build_count = s.build_count;
     R.last_build_step = s.last_build_step;
@@ -1023,14 +1028,14 @@ int redo_dev(System& s, DevRedo& R, int par, int stp, double dt, int use_cundall
     const bool save = !s.seed_pending;
     int rc = 0;
     if (s.launch_mode == TMPS_LAUNCH_DIRECT || s.profiling) {
-        rc = redo_dev_chain(s, R, par, stp, dt, use_cundall, gl, ga);
+        rc = redo_dev_chain(s, R, par, stp, last, dt, use_cundall, gl, ga);
     } else {
         // one graph launch instead of some forty kernel launches, whose host
         // cost would leave the device idle between the chain's small kernels
-        const uint64_t key = redo_graph_key(s, R, par, dt, use_cundall, gl, ga);
+        const uint64_t key = redo_graph_key(s, R, par, last, dt, use_cundall, gl, ga);
         int slot = -1;
         rc = graph_slot(
-            s, key, [&] { return redo_dev_chain(s, R, par, -1, dt, use_cundall, gl, ga); },
+            s, key, [&] { return redo_dev_chain(s, R, par, -1, last, dt, use_cundall, gl, ga); },
             &slot);
         if (!rc) HIP_TRY(hipGraphLaunch(s.seg_exec[slot], s.stream));
     }
@@ -1333,11 +1338,12 @@ int run(System& s, int nsteps, double dt, int run_pre, int use_cundall, double g
             s.timestep = done;
             if (s.has_built) s.current_step = done;
             const int stp = done + 1;
-            rc = redo_dev(s, R, ((stp - 1) - sg.start) % 2, stp, dt, use_cundall, gl, ga);
+            const bool last = stp == t_end;
+            rc = redo_dev(s, R, ((stp - 1) - sg.start) % 2, stp, last, dt, use_cundall, gl, ga);
             if (rc) return rc;
             note_rebuild(s, stp);
             t = stp;
-            kick = 1;
+            kick = last ? 1 : 0;  // inside the run the redo applied the next half-kick
             mirror_back = false;
             continue;
         }

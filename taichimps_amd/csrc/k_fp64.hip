@@ -778,7 +778,20 @@ __global__ void k_step_force_f64(F64Ptrs P, Ctl* ctl, PairP pp, double dt, doubl
         if (FINAL) {
             P.f[i] = make_double4(F[0], F[1], F[2], 0.0);
             P.tq[i] = make_double4(T[0], T[1], T[2], 0.0);
-            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+            if (FINAL == 1) {
+                for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+            } else {
+                // FINAL 2, a rebuild step redone inside a run: the next
+                // step's half-kick, k_kick_f64's expressions on the stored
+                // force and torque; no virial (only a run's last step
+                // leaves it for the getters)
+                v.x += dtf * (F[0] / m);
+                v.y += dtf * (F[1] / m);
+                v.z += dtf * (F[2] / m);
+                w.x += dtf * (T[0] / inertia);
+                w.y += dtf * (T[1] / inertia);
+                w.z += dtf * (T[2] / inertia);
+            }
         } else {
             v.x += dtf * am[0];
             v.y += dtf * am[1];
@@ -1464,7 +1477,14 @@ template <int STYLE, int WALL>
 static int launch_force_walls(System& s, const F64Ptrs& P, double dt, int final_mode, int cundall,
                               double gl, double ga) {
     const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
-    if (final_mode) {
+    if (final_mode == 2) {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 2, WALL>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_f64<STYLE, 0, 2, WALL>), g, b, 0, s.stream, P, s.ctl,
+                               s.pair, dt, gl, ga, s.partial);
+    } else if (final_mode) {
         if (cundall)
             hipLaunchKernelGGL((k_step_force_f64<STYLE, 1, 1, WALL>), g, b, 0, s.stream, P, s.ctl,
                                s.pair, dt, gl, ga, s.partial);

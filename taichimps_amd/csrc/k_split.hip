@@ -485,7 +485,17 @@ __device__ __forceinline__ void step_force_split_block(const SplitPtrs& P, Ctl* 
         if (FINAL) {
             P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
             P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
-            for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+            if (FINAL == 1) {
+                for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
+            } else {
+                // FINAL 2, a rebuild step redone inside a run: the next
+                // step's half-kick as k_kick_split computes it from the
+                // stored force and torque (its expressions, so the bits
+                // are the kick kernel's); no virial, which only a run's
+                // last step leaves for the getters
+                v.x += dtf * F[0] * im; v.y += dtf * F[1] * im; v.z += dtf * F[2] * im;
+                w.x += dtf * T[0] * w.w; w.y += dtf * T[1] * w.w; w.z += dtf * T[2] * w.w;
+            }
         } else {
             v.x += dtf * am[0]; v.y += dtf * am[1]; v.z += dtf * am[2];
             w.x += dtf * ai[0]; w.y += dtf * ai[1]; w.z += dtf * ai[2];
@@ -1028,7 +1038,14 @@ static int launch_force_split_walls(System& s, const SplitPtrs& P, double dt, in
                                     int cundall, double gl, double ga) {
     const dim3 g(nblocks_for(s.n)), b(TMPS_BLOCK);
     const float fdt = (float)dt, fgl = (float)gl, fga = (float)ga;
-    if (final_mode) {
+    if (final_mode == 2) {
+        if (cundall)
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 2, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
+        else
+            hipLaunchKernelGGL((k_step_force_split<STYLE, 0, 2, WALL>), g, b, 0, s.stream, P,
+                               s.ctl, s.pair, fdt, fgl, fga, s.partial);
+    } else if (final_mode) {
         if (cundall)
             hipLaunchKernelGGL((k_step_force_split<STYLE, 1, 1, WALL>), g, b, 0, s.stream, P,
                                s.ctl, s.pair, fdt, fgl, fga, s.partial);
