@@ -944,7 +944,14 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, bool l
     }
     rc = launch_ctl_resume(s);
     if (rc) return rc;
-    rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
+    // the step's start, and the pbc of the build: on an FP32-split system
+    // whose FP64 positions are stale, drift, decode and wrap in one pass
+    // (the history save between them touches only the list)
+    const bool fused_start = split(s) && !s.x64_valid;
+    if (fused_start)
+        rc = launch_redo_begin_split(s, dt);
+    else
+        rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
     if (rc) return rc;
     // history_save: the list (its slot count read on the device) into the
     // copy, which is as large as the list's buffers
@@ -956,13 +963,8 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, bool l
         s.onb_n = s.n;
         s.have_old = true;
     }
-    // pbc
-    if (split(s) && !s.x64_valid) {
-        rc = launch_split_to_x64(s);
-        if (rc) return rc;
-    }
     s.x64_valid = true;
-    rc = launch_pbc(s);
+    if (!fused_start) rc = launch_pbc(s);
     if (!rc) rc = launch_build_dev(s, R.max_cutoff, R.max_cells);
     if (!rc) rc = launch_ctl_built_dev(s, step);
     if (!rc && split(s)) rc = launch_x64_to_split_dev(s);

@@ -13,6 +13,7 @@
 
 #include "contact.h"
 #include "fused.h"
+#include "pbc.h"
 #include "tmps_internal.h"
 #include "wall.h"
 
@@ -439,6 +440,18 @@ __global__ void k_step_begin_split(SplitPtrs P, Ctl* ctl, float dt, int kick) {
     if (ctl->abort) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.n) step_begin_split_atom(i, P, ctl, dt, kick);
+}
+
+// The start of a redone step in one pass over the atoms: the drift and skin
+// check of k_step_begin_split (no kick), the FP64 position decoded from the
+// drifted encoding (k_split_to_x64) and wrapped into the box (k_pbc), each
+// as its own kernel computes it.
+__global__ void k_redo_begin_split(SplitPtrs P, Ctl* ctl, float dt, double4* xr, double4* vm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    if (!ctl->abort) step_begin_split_atom(i, P, ctl, dt, 0);
+    split_to_x64_atom(i, P, xr, ctl);
+    pbc_atom(i, xr, vm, ctl);
 }
 
 // Block `blk` of the force pass (its partial: partial[blk]).
@@ -1025,6 +1038,11 @@ int launch_split_consts(System& s, int set_build) {
 
 int launch_step_begin_split(System& s, double dt, int kick, int buf) {
     LAUNCH_S(k_step_begin_split, s.n, sptrs(s, buf, buf), s.ctl, (float)dt, kick);
+    return 0;
+}
+
+int launch_redo_begin_split(System& s, double dt) {
+    LAUNCH_S(k_redo_begin_split, s.n, sptrs(s, 0, 0), s.ctl, (float)dt, s.xr, s.vm);
     return 0;
 }
 
