@@ -965,7 +965,8 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, bool l
     }
     s.x64_valid = true;
     if (!fused_start) rc = launch_pbc(s);
-    if (!rc) rc = launch_build_dev(s, R.max_cutoff, R.max_cells);
+    // (an FP32-split build's x0 is stored by the encoding pass)
+    if (!rc) rc = launch_build_dev(s, R.max_cutoff, R.max_cells, !split(s));
     if (!rc) rc = launch_ctl_built_dev(s, step);
     if (!rc && split(s)) rc = launch_x64_to_split_dev(s);
     if (!rc) rc = launch_hist_restore(s);

@@ -1518,7 +1518,7 @@ int launch_list(System& s, const double4* xr, int* overflow_out, int* total_out,
 // comment): grid, bins, rows, the CSR list and x0, with no host round trip.
 // max_cutoff: twice the largest radius; max_cells: the list grid's cap. The
 // bin arrays must hold max_cells + 1 cells, the row scratch n rows.
-int launch_build_dev(System& s, double max_cutoff, int max_cells) {
+int launch_build_dev(System& s, double max_cutoff, int max_cells, bool store_x0) {
     if (s.n <= 0) return 0;
     const int split = s.precision == TMPS_FP32_SPLIT ? 1 : 0;
     const GridP* Gd = &s.ctl->lgrid;
@@ -1554,7 +1554,7 @@ int launch_build_dev(System& s, double max_cutoff, int max_cells) {
                        s.ctl, none, Gd, (const int*)s.flags, s.cell_start, s.cell_particles,
                        s.skin, (const int*)s.nb_start, s.nb_j);
     HIP_TRY(hipGetLastError());
-    return launch_store_x0(s);
+    return store_x0 ? launch_store_x0(s) : 0;
 }
 
 int launch_list_save(System& s) {

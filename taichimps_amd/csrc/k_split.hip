@@ -242,11 +242,15 @@ __device__ __forceinline__ void x64_to_split_atom(int i, const SplitPtrs& P, con
 }
 
 // Gd: the grid of a build decided on the device (ctl->lgrid), else G
+// x0_store: a build's x0, stored here from xr (the encoding of u0 then reads
+// what it stores) instead of by a copy kernel of its own
 __global__ void k_x64_to_split(SplitPtrs P, const double4* xr, const double4* x0, const Ctl* ctl,
-                               GridP G, const GridP* Gd, uint32_t* cell) {
+                               GridP G, const GridP* Gd, uint32_t* cell, double4* x0_store) {
     if (Gd && ctl->error) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.n) x64_to_split_atom(i, P, xr, x0, ctl, Gd ? *Gd : G, cell);
+    if (i >= P.n) return;
+    if (x0_store) x0_store[i] = xr[i];
+    x64_to_split_atom(i, P, xr, x0, ctl, Gd ? *Gd : G, cell);
 }
 
 // Per slot of the neighbor list: the cell of i minus the cell of j, wrapped
@@ -950,7 +954,8 @@ int launch_split_to_x64(System& s) {
 }
 
 int launch_x64_to_split(System& s) {
-    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, nullptr, s.cell);
+    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, nullptr, s.cell,
+             (double4*)nullptr);
     // the cells may have changed: refresh the list's per-slot cell offsets
     if (s.nslots > 0 && s.nb_code)
         LAUNCH_S(k_slot_codes, s.n, s.n, s.nb_start, s.nb_j, s.cell, s.ctl, s.nb_code);
@@ -958,10 +963,12 @@ int launch_x64_to_split(System& s) {
 }
 
 // The split encoding after a list build decided on the device: its grid is
-// the control block's, and the slot offsets follow the rows the build wrote
-// (an emptied list has none).
+// the control block's, the build's x0 is stored on the way (launch_build_dev
+// leaves it to this pass), and the slot offsets follow the rows the build
+// wrote (an emptied list has none).
 int launch_x64_to_split_dev(System& s) {
-    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, &s.ctl->lgrid, s.cell);
+    LAUNCH_S(k_x64_to_split, s.n, sptrs(s), s.xr, s.x0, s.ctl, s.grid, &s.ctl->lgrid, s.cell,
+             s.x0);
     if (s.nb_code) LAUNCH_S(k_slot_codes, s.n, s.n, s.nb_start, s.nb_j, s.cell, s.ctl, s.nb_code);
     return 0;
 }

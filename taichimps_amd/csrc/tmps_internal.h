@@ -517,7 +517,7 @@ int launch_ctl_built_dev(System& s, int step);
 // the binning's work buffers for the current atom count (k_rebuild.hip)
 int bins_reserve(System& s);
 // a list build decided on the device (k_rebuild.hip)
-int launch_build_dev(System& s, double max_cutoff, int max_cells);
+int launch_build_dev(System& s, double max_cutoff, int max_cells, bool store_x0);
 // the split encoding after it (k_split.hip)
 int launch_x64_to_split_dev(System& s);
 int launch_remap_apply_fused(System& s);
