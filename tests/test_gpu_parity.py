@@ -503,3 +503,33 @@ def test_run_on_a_caller_stream_with_prepared_graphs_is_bitwise_the_plain_run(pr
     assert np.array_equal(sa["boxhi"], sb["boxhi"])
     assert np.array_equal(a.atom._native.last_pressure(), b.atom._native.last_pressure())
     assert a.atom._native.counters() == b.atom._native.counters()
+
+
+def test_device_run_whose_list_outgrows_its_buffers_bitexact():
+    """A lattice of spacing 1.6 (no pair within the cutoff at the start)
+    compressed by erate on every axis to spacing ~1.0: within one
+    device-resident run the list grows from empty to about six entries per
+    atom, past the buffers sized at the run's first build (the need, a
+    quarter more and 1,024 slots). A rebuild decided on the device then finds
+    the list too long for them; the run redoes that step on the host, grows
+    the buffers and goes on on the device, its rebuild chain captured again
+    for the new buffers. A second run continues it. Both equal the oracle bit
+    for bit."""
+    w = workloads.lattice(10, 10, 10, 1.6, 71, "uniform")
+    rng = np.random.default_rng(72)
+    w["v"] = rng.normal(0.0, 0.5, size=w["v"].shape)
+    spec = dict(axes={d: {"style": "erate", "rate": -15.0} for d in "xyz"})
+    sim = H.device_sim(w, style="hertz/history", servo=True, skin=0.05, servo_kw=spec)
+    o = H.oracle_sim(w, style="hertz/history", servo=False, skin=0.05)
+    o.set_servo(O.servo_params(**spec))
+    for steps in (250, 60):
+        sim.run(steps)
+        assert sim.last_run_device_resident
+        assert o.run(steps, 1e-4) == 0
+        dev, orc = H.device_state(sim), H.oracle_state(o)
+        _assert_state_equal(dev, orc)
+        assert np.array_equal(dev["boxlo"], orc["boxlo"])
+        assert np.array_equal(dev["boxhi"], orc["boxhi"])
+        assert sim.neighbor.build_count == o.counters()["build_count"]
+    # the list outgrew the first build's buffers (1,024 slots: nothing touched then)
+    assert 2 * sim.neighbor.npairs[None] > 2048
