@@ -533,3 +533,28 @@ def test_device_run_whose_list_outgrows_its_buffers_bitexact():
         assert sim.neighbor.build_count == o.counters()["build_count"]
     # the list outgrew the first build's buffers (1,024 slots: nothing touched then)
     assert 2 * sim.neighbor.npairs[None] > 2048
+
+
+@pytest.mark.parametrize("check", [True, False])
+@pytest.mark.parametrize("precision", ["f64", "f32split"])
+def test_graph_and_direct_launches_give_the_same_run(precision, check):
+    """tmps_run replays captured graphs (segments, and the rebuild chain with
+    its step read on the device) or launches every kernel itself: over runs
+    with rebuilds the two give the same bits. With `check no` every step is a
+    rebuild, so every run also ends on a redone step (which stores the
+    virial instead of applying the next half-kick)."""
+    w = dense(9, 1.0, seed=81)
+    rng = np.random.default_rng(82)
+    w["v"] = rng.normal(0.0, 1.5, size=w["v"].shape)
+    a = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02, check=check)
+    b = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02, check=check)
+    b.atom._native.set_launch_mode(True)
+    for steps in (40, 17, 1, 23):
+        a.run(steps)
+        b.run(steps)
+        assert a.last_run_device_resident and b.last_run_device_resident
+        sa, sb = H.device_state(a), H.device_state(b)
+        _assert_state_equal(sa, sb, ("x", "v", "omega", "f", "torque", "virial"))
+        assert np.array_equal(sa["boxhi"], sb["boxhi"])
+        assert a.neighbor.build_count == b.neighbor.build_count
+    assert a.neighbor.build_count > 4
