@@ -65,8 +65,10 @@ __device__ __forceinline__ void list_save_items(int first, int stride, int n, co
         if (k <= n) onb_start[k] = nb_start[k];
         if (k < nslots && k < cap) {
             onb_j[k] = nb_j[k];
-            onb_flag[k] = nb_flag[k];
-            onb_hist[k] = nb_hist[k];
+            const uint8_t fl = nb_flag[k];
+            onb_flag[k] = fl;
+            // the restore reads a saved history only where its flag is set
+            if (fl) onb_hist[k] = nb_hist[k];
         }
     }
 }
