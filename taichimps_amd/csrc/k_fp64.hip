@@ -1079,9 +1079,12 @@ __device__ __forceinline__ void copy_state_items(int first, int stride, T4* x0, 
         v0[i] = v1[i];
         w0[i] = w1[i];
     }
+    // a slot's history is read only where its touch flag is set (a step
+    // writes it only there), so only those entries are copied
     for (int k = first; k < nslots; k += stride) {
-        fl0[k] = fl1[k];
-        h0[k] = h1[k];
+        const uint8_t f = fl1[k];
+        fl0[k] = f;
+        if (f) h0[k] = h1[k];
     }
 }
 
