@@ -944,14 +944,15 @@ static int redo_dev_chain(System& s, const DevRedo& R, int par, int step, bool l
     }
     rc = launch_ctl_resume(s);
     if (rc) return rc;
-    // the step's start, and the pbc of the build: on an FP32-split system
-    // whose FP64 positions are stale, drift, decode and wrap in one pass
-    // (the history save between them touches only the list)
-    const bool fused_start = split(s) && !s.x64_valid;
+    // the step's start and the pbc of the build in one pass over the atoms
+    // (on an FP32-split system whose FP64 positions are stale, with the
+    // decode between them); the history save between them touches only the
+    // list
+    const bool fused_start = !split(s) || !s.x64_valid;
     if (fused_start)
-        rc = launch_redo_begin_split(s, dt);
+        rc = split(s) ? launch_redo_begin_split(s, dt) : launch_redo_begin_f64(s, dt);
     else
-        rc = split(s) ? launch_step_begin_split(s, dt, 0, 0) : launch_step_begin_f64(s, dt, 0, 0);
+        rc = launch_step_begin_split(s, dt, 0, 0);
     if (rc) return rc;
     // history_save: the list (its slot count read on the device) into the
     // copy, which is as large as the list's buffers

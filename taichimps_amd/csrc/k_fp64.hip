@@ -9,6 +9,7 @@
 
 #include "contact.h"
 #include "fused.h"
+#include "pbc.h"
 #include "servo.h"
 #include "tmps_internal.h"
 #include "wall.h"
@@ -683,10 +684,8 @@ __global__ void k_ctl_built_dev(Ctl* ctl, int step, int split) {
 // Start of step t: the pending end-of-step remap of step t-1, the
 // initial_integrate half-kick (only for a segment's first step; later steps
 // receive it from the previous force pass), the drift, and the skin check.
-__global__ void k_step_begin_f64(F64Ptrs P, Ctl* ctl, double dt, int kick) {
-    if (ctl->abort) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+__device__ __forceinline__ void step_begin_f64_atom(int i, const F64Ptrs& P, Ctl* ctl, double dt,
+                                                    int kick) {
     const int t = ctl->timestep + 1;
     double4 x = P.xr[i];
     const RemapRec& R = ctl->remap;
@@ -725,6 +724,22 @@ __global__ void k_step_begin_f64(F64Ptrs P, Ctl* ctl, double dt, int kick) {
             if (d0 * d0 + d1 * d1 + d2 * d2 > ctl->trigger_sq) ctl->trigger = t;
         }
     }
+}
+
+__global__ void k_step_begin_f64(F64Ptrs P, Ctl* ctl, double dt, int kick) {
+    if (ctl->abort) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.n) step_begin_f64_atom(i, P, ctl, dt, kick);
+}
+
+// The start of a redone step in one pass over the atoms: k_step_begin_f64
+// (no kick) and the rebuild's wrap (k_pbc), each as its own kernel computes
+// it.
+__global__ void k_redo_begin_f64(F64Ptrs P, Ctl* ctl, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    if (!ctl->abort) step_begin_f64_atom(i, P, ctl, dt, 0);
+    pbc_atom(i, P.xr, P.vm, ctl);
 }
 
 // Step t after the drift: contact forces, damping/cundall, final_integrate,
@@ -1468,6 +1483,11 @@ int launch_ke_f64(System& s, double* out2) {
 
 int launch_step_begin_f64(System& s, double dt, int kick, int buf) {
     LAUNCH_N(k_step_begin_f64, s.n, ptrs(s, buf, buf), s.ctl, dt, kick);
+    return 0;
+}
+
+int launch_redo_begin_f64(System& s, double dt) {
+    LAUNCH_N(k_redo_begin_f64, s.n, ptrs(s, 0, 0), s.ctl, dt);
     return 0;
 }
 

@@ -541,6 +541,8 @@ int launch_step_begin_split(System& s, double dt, int kick, int buf);
 // a redone step's start on an FP32-split system: launch_step_begin_split (no
 // kick), launch_split_to_x64 and launch_pbc in one pass
 int launch_redo_begin_split(System& s, double dt);
+// the same on an FP64 system: launch_step_begin_f64 (no kick) and launch_pbc
+int launch_redo_begin_f64(System& s, double dt);
 int launch_step_force_split(System& s, double dt, int final_mode, int cundall, double gl, double ga,
                             int in_buf, int out_buf);
 int launch_copy_vw_split(System& s, int honor_abort);
