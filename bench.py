@@ -21,8 +21,10 @@ roofline. The `c2_200` block runs both precisions 200 steps from the start
 compares their mean pressure with the CPU oracle's after those 200 steps. The CPU baseline times the oracle's OpenMP
 build on the host's threads on the same workload and on the 8,000-sphere
 hooke/history case; its serial build checks the FP64 device positions bit for
-bit. The adjoint (32k spheres), ensemble (256 packings) and, with
-`--triaxial`, the 4M-sphere triaxial blocks follow in the same JSON line.
+bit. The walls (1M spheres between a sheared floor and a ceiling), adjoint
+(32k spheres), ensemble (256 packings) and 4M-sphere triaxial blocks follow
+in the same JSON line (each can be switched off: --no-walls, --no-adjoint,
+--no-ensemble, --no-triaxial).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10]
 """
@@ -718,7 +720,9 @@ def main(argv=None):
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble block (256 packings, 500 steps)")
     ap.add_argument("--triaxial", action="store_true",
-                    help="also run the 4M-sphere triaxial block (FP32-split and FP64)")
+                    help="run the 4M-sphere triaxial block (the default; kept for old commands)")
+    ap.add_argument("--no-triaxial", action="store_true",
+                    help="skip the 4M-sphere triaxial block (FP32-split and FP64, 1000 steps)")
     ap.add_argument("--triaxial-only", action="store_true",
                     help="only run the triaxial block and print it")
     ap.add_argument("--ensemble-only", action="store_true",
@@ -875,7 +879,7 @@ def main(argv=None):
             cpu_adjoint_check(out["adjoint"])
     if not args.no_ensemble:
         out["ensemble"] = ensemble_c5(n_grad=args.ensemble_grads)
-    if args.triaxial:
+    if not args.no_triaxial:
         out["triaxial"] = triaxial_c3()
     if cpu is not None:
         out["mean_pressure_rel_err_vs_cpu_oracle"] = (
