@@ -108,6 +108,16 @@ int tmps_destroy(tmps_system* s);
 int tmps_set_nlocal(tmps_system* s, int nlocal);          /* AtomSystem.nlocal */
 int tmps_set_field(tmps_system* s, int field, const void* host, int count);  /* Field.from_numpy */
 int tmps_get_field(tmps_system* s, int field, void* host, int count);        /* Field.to_numpy */
+/* The same two through device memory: `dev` is a caller's device buffer
+ * (e.g. a torch tensor's data_ptr()), borrowed for the call, in
+ * tmps_get_field's layout. Both are ordered on the system's stream and return
+ * without waiting: order the caller's work after it (tmps_sync, or events on
+ * a stream given by tmps_set_stream). TMPS_RADIUS and TMPS_RMASS are set
+ * through host memory (they update host-side state), and TMPS_RADIUS is read
+ * from the host's exact copy. No reference counterpart (the reference moves
+ * fields through numpy, Field.from_numpy / to_numpy). */
+int tmps_set_field_device(tmps_system* s, int field, const void* dev, int count);
+int tmps_get_field_device(tmps_system* s, int field, void* dev, int count);
 int tmps_sync(tmps_system* s);                                               /* ti.sync */
 
 /* ---- Domain (domain.py) ---------------------------------------------- */

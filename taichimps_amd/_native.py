@@ -41,7 +41,8 @@ PAIR_GRANULAR = 2
 # the exported entry points (include/tmps.h)
 SYMBOLS = (
     "tmps_version", "tmps_last_error", "tmps_device_name", "tmps_create", "tmps_destroy",
-    "tmps_set_nlocal", "tmps_set_field", "tmps_get_field", "tmps_sync", "tmps_set_box",
+    "tmps_set_nlocal", "tmps_set_field", "tmps_get_field", "tmps_set_field_device",
+    "tmps_get_field_device", "tmps_sync", "tmps_set_box",
     "tmps_get_box", "tmps_pbc", "tmps_clear_forces", "tmps_neigh_config", "tmps_neigh_check",
     "tmps_neigh_decide", "tmps_neigh_build", "tmps_neigh_info", "tmps_neigh_get_pairs",
     "tmps_neigh_get_counts", "tmps_neigh_get_bins", "tmps_history_reset", "tmps_history_save",
@@ -139,6 +140,8 @@ def load(path: str | None = None):
         "tmps_set_nlocal": (C.c_int, [vp, C.c_int]),
         "tmps_set_field": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
         "tmps_get_field": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
+        "tmps_set_field_device": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
+        "tmps_get_field_device": (C.c_int, [vp, C.c_int, C.c_void_p, C.c_int]),
         "tmps_sync": (C.c_int, [vp]),
         "tmps_set_box": (C.c_int, [vp, dp, dp, ip, dp, C.c_int]),
         "tmps_get_box": (C.c_int, [vp, dp, dp, dp, ip]),
@@ -272,6 +275,43 @@ class NativeSystem:
         shape = (count,) if width == 1 else (count, width)
         out = np.zeros(shape, dtype=dtype)
         check(self._L.tmps_get_field(self._h, field, out.ctypes.data_as(C.c_void_p), int(count)))
+        return out
+
+    def set_field_device(self, field, tensor, count, sync=True):
+        """tmps_set_field_device from a contiguous GPU torch tensor (float64;
+        int32 for tags), borrowed for the call. With `sync` the work queued
+        on torch's current stream is finished first and the system's copy
+        after, so the tensor is free on return; without it the caller orders
+        them (e.g. one stream for both, set_stream)."""
+        import torch
+        width = FIELD_WIDTH[field]
+        dtype = torch.int32 if field == F_TAG else torch.float64
+        if not tensor.is_cuda or tensor.dtype != dtype or not tensor.is_contiguous():
+            raise ValueError(f"field data must be a contiguous {dtype} tensor on the GPU")
+        if tensor.numel() < count * width:
+            raise ValueError("field data shorter than count")
+        if sync:
+            torch.cuda.current_stream(tensor.device).synchronize()
+        check(self._L.tmps_set_field_device(self._h, field, C.c_void_p(tensor.data_ptr()),
+                                            int(count)))
+        if sync:
+            self.sync()
+
+    def get_field_device(self, field, count, sync=True):
+        """tmps_get_field_device into a new GPU torch tensor (float64; int32
+        for tags), shaped as get_field's array. With `sync` the system's
+        stream is finished before the tensor is returned."""
+        import torch
+        width = FIELD_WIDTH[field]
+        dtype = torch.int32 if field == F_TAG else torch.float64
+        shape = (count,) if width == 1 else (count, width)
+        out = torch.empty(shape, dtype=dtype, device="cuda")
+        if sync:
+            torch.cuda.current_stream(out.device).synchronize()
+        check(self._L.tmps_get_field_device(self._h, field, C.c_void_p(out.data_ptr()),
+                                            int(count)))
+        if sync:
+            self.sync()
         return out
 
     def sync(self):

@@ -1620,6 +1620,7 @@ void free_all(System& s) {
     dfree(s.rs_k1); dfree(s.rs_k2); dfree(s.rs_v1); dfree(s.rs_v2); dfree(s.rs_hist);
     dfree(s.d_run_spec);
     dfree(s.run_wnz);
+    dfree(s.d_radius);
     dfree(s.rs_offs);
     dfree(s.cell_particles);
     dfree(s.bin_x);
@@ -1815,6 +1816,118 @@ int tmps_set_field(tmps_system* h, int field, const void* host, int count) {
 
 int tmps_get_field(tmps_system* h, int field, void* host, int count) {
     return get_field(*sys(h), field, host, count);
+}
+
+int tmps_get_field_device(tmps_system* h, int field, void* dev, int count) {
+    System& s = *sys_view(h);
+    if (count < 0 || count > s.max_atoms)
+        return tmps_fail(TMPS_ERR_CAPACITY, "field count exceeds max_atoms");
+    const int n = count;
+    if (n == 0) return 0;
+    if (!dev) return tmps_fail(TMPS_ERR_ARG, "null device buffer");
+    double* out = static_cast<double*>(dev);
+    switch (field) {
+        case TMPS_TAG:
+            HIP_TRY(hipMemcpyAsync(dev, s.tag, sizeof(int) * n, hipMemcpyDeviceToDevice, s.stream));
+            return 0;
+        case TMPS_VIRIAL:
+            HIP_TRY(hipMemcpyAsync(dev, s.vir, sizeof(double) * 6 * n, hipMemcpyDeviceToDevice,
+                                   s.stream));
+            return 0;
+        case TMPS_RADIUS:
+            // the exact radii are the host's (an FP32-split system's records
+            // carry them rounded)
+            HIP_TRY(hipMemcpyAsync(dev, s.h_radius.data(), sizeof(double) * n,
+                                   hipMemcpyHostToDevice, s.stream));
+            return 0;
+        case TMPS_X0:
+            return launch_field_pack(s, s.x0, nullptr, n, 0, out);
+        case TMPS_X:
+            if (split(s)) {
+                const int rc = split_positions_to_xr(s);
+                if (rc) return rc;
+            }
+            return launch_field_pack(s, s.xr, nullptr, n, 0, out);
+        default:
+            break;
+    }
+    const bool sp = split(s);
+    switch (field) {
+        case TMPS_V: return launch_field_pack(s, sp ? nullptr : s.vm, s.vmf, n, 0, out);
+        case TMPS_OMEGA: return launch_field_pack(s, sp ? nullptr : s.om, s.omf, n, 0, out);
+        case TMPS_F: return launch_field_pack(s, sp ? nullptr : s.f, s.ff, n, 0, out);
+        case TMPS_TORQUE: return launch_field_pack(s, sp ? nullptr : s.tq, s.tqf, n, 0, out);
+        case TMPS_RMASS: return launch_field_pack(s, sp ? nullptr : s.vm, s.vmf, n, 1, out);
+        default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+    }
+}
+
+int tmps_set_field_device(tmps_system* h, int field, const void* dev, int count) {
+    System& s = *sys(h);
+    if (count < 0 || count > s.max_atoms)
+        return tmps_fail(TMPS_ERR_CAPACITY, "field count exceeds max_atoms");
+    const int n = count;
+    if (n == 0) return 0;
+    if (!dev) return tmps_fail(TMPS_ERR_ARG, "null device buffer");
+    const double* in = static_cast<const double*>(dev);
+    switch (field) {
+        case TMPS_TAG:
+            HIP_TRY(hipMemcpyAsync(s.tag, dev, sizeof(int) * n, hipMemcpyDeviceToDevice, s.stream));
+            return 0;
+        case TMPS_VIRIAL:
+            HIP_TRY(hipMemcpyAsync(s.vir, dev, sizeof(double) * 6 * n, hipMemcpyDeviceToDevice,
+                                   s.stream));
+            return 0;
+        case TMPS_X0:
+            return tmps_fail(TMPS_ERR_ARG, "x0 is read-only");
+        case TMPS_RADIUS:
+        case TMPS_RMASS: {
+            // these also set host state (the radius mirror the neighbor grid
+            // reads, an FP32-split system's inverse inertia): through the
+            // host setter
+            std::vector<double> hv(n);
+            HIP_TRY(copy_sync(s.stream, hv.data(), dev, sizeof(double) * n,
+                              hipMemcpyDeviceToHost));
+            return set_field(s, field, hv.data(), n);
+        }
+        default:
+            break;
+    }
+    if (!split(s)) {
+        switch (field) {
+            case TMPS_X: return launch_field_unpack(s, s.xr, nullptr, n, in, nullptr);
+            case TMPS_V: return launch_field_unpack(s, s.vm, nullptr, n, in, nullptr);
+            case TMPS_OMEGA: return launch_field_unpack(s, s.om, nullptr, n, in, nullptr);
+            case TMPS_F: return launch_field_unpack(s, s.f, nullptr, n, in, nullptr);
+            case TMPS_TORQUE: return launch_field_unpack(s, s.tq, nullptr, n, in, nullptr);
+            default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+        }
+    }
+    if (field == TMPS_X) {
+        // as set_field: the FP64 positions become the authority, each with
+        // its exact radius, atoms beyond `count` decoded where they are
+        if (!s.x64_valid && s.n > n) {
+            const int rc = launch_split_to_x64(s);
+            if (rc) return rc;
+        }
+        if (!s.d_radius) {
+            int rc = dalloc(s.stream, &s.d_radius, (size_t)std::max(s.max_atoms, 1));
+            if (rc) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(s.d_radius, s.h_radius.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, s.stream));
+        const int rc = launch_field_unpack(s, s.xr, nullptr, n, in, s.d_radius);
+        if (rc) return rc;
+        s.x64_valid = true;
+        return 0;
+    }
+    switch (field) {
+        case TMPS_V: return launch_field_unpack(s, nullptr, s.vmf, n, in, nullptr);
+        case TMPS_OMEGA: return launch_field_unpack(s, nullptr, s.omf, n, in, nullptr);
+        case TMPS_F: return launch_field_unpack(s, nullptr, s.ff, n, in, nullptr);
+        case TMPS_TORQUE: return launch_field_unpack(s, nullptr, s.tqf, n, in, nullptr);
+        default: return tmps_fail(TMPS_ERR_ARG, "unknown field");
+    }
 }
 
 int tmps_sync(tmps_system* h) {

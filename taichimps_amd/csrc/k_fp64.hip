@@ -1486,6 +1486,67 @@ int launch_step_begin_f64(System& s, double dt, int kick, int buf) {
     return 0;
 }
 
+// ---------------------------------------------- fields through device memory
+
+// A field's values into a caller's device buffer as tmps_get_field lays them
+// out: x, y, z of each record (count x 3 doubles) or its w (scalar), from
+// FP64 or FP32 records.
+__global__ void k_field_pack(const double4* d4, const float4* f4, int n, int scalar,
+                             double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x, y, z, w;
+    if (d4) {
+        const double4 a = d4[i];
+        x = a.x; y = a.y; z = a.z; w = a.w;
+    } else {
+        const float4 a = f4[i];
+        x = a.x; y = a.y; z = a.z; w = a.w;
+    }
+    if (scalar) {
+        out[i] = w;
+    } else {
+        out[3 * i] = x;
+        out[3 * i + 1] = y;
+        out[3 * i + 2] = z;
+    }
+}
+
+// A caller's device buffer of count x 3 doubles into the x, y, z of FP64 or
+// FP32 records (rounded to FP32 as the host setter does); w kept, or taken
+// from wsrc.
+__global__ void k_field_unpack(double4* d4, float4* f4, int n, const double* in,
+                               const double* wsrc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (d4) {
+        double4 a = d4[i];
+        a.x = in[3 * i];
+        a.y = in[3 * i + 1];
+        a.z = in[3 * i + 2];
+        if (wsrc) a.w = wsrc[i];
+        d4[i] = a;
+    } else {
+        float4 a = f4[i];
+        a.x = (float)in[3 * i];
+        a.y = (float)in[3 * i + 1];
+        a.z = (float)in[3 * i + 2];
+        f4[i] = a;
+    }
+}
+
+int launch_field_pack(System& s, const double4* d4, const float4* f4, int n, int scalar,
+                      double* out) {
+    LAUNCH_N(k_field_pack, n, d4, f4, n, scalar, out);
+    return 0;
+}
+
+int launch_field_unpack(System& s, double4* d4, float4* f4, int n, const double* in,
+                        const double* wsrc) {
+    LAUNCH_N(k_field_unpack, n, d4, f4, n, in, wsrc);
+    return 0;
+}
+
 int launch_redo_begin_f64(System& s, double dt) {
     LAUNCH_N(k_redo_begin_f64, s.n, ptrs(s, 0, 0), s.ctl, dt);
     return 0;

@@ -293,6 +293,7 @@ struct System {
     WallSpec* d_run_spec = nullptr;
     double4* run_wh[TMPS_RUN_WALLS][2] = {};
     uint8_t* run_wnz = nullptr;  // [max_atoms]: WallRefs::nz of the run's walls
+    double* d_radius = nullptr;  // [max_atoms]: h_radius staged for tmps_set_field_device
 
     // the adjoint engine's checkpoints and work buffers (adjoint.hip)
     struct AdjContext* adj = nullptr;
@@ -543,6 +544,11 @@ int launch_step_begin_split(System& s, double dt, int kick, int buf);
 int launch_redo_begin_split(System& s, double dt);
 // the same on an FP64 system: launch_step_begin_f64 (no kick) and launch_pbc
 int launch_redo_begin_f64(System& s, double dt);
+// field records to and from a caller's device buffer (tmps_*_field_device)
+int launch_field_pack(System& s, const double4* d4, const float4* f4, int n, int scalar,
+                      double* out);
+int launch_field_unpack(System& s, double4* d4, float4* f4, int n, const double* in,
+                        const double* wsrc);
 int launch_step_force_split(System& s, double dt, int final_mode, int cundall, double gl, double ga,
                             int in_buf, int out_buf);
 int launch_copy_vw_split(System& s, int honor_abort);

@@ -558,3 +558,40 @@ def test_graph_and_direct_launches_give_the_same_run(precision, check):
         assert np.array_equal(sa["boxhi"], sb["boxhi"])
         assert a.neighbor.build_count == b.neighbor.build_count
     assert a.neighbor.build_count > 4
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32split"])
+def test_fields_through_device_memory_equal_the_host_path(precision):
+    """tmps_get_field_device reads every field as tmps_get_field does, and
+    tmps_set_field_device leaves a system as tmps_set_field does: two
+    systems given the same new positions, velocities and spins (one through
+    host arrays, one through GPU tensors) run on bit for bit."""
+    torch = pytest.importorskip("torch")
+    from taichimps_amd import _native as NT
+    w = dense(8, 1.0, seed=91)
+    rng = np.random.default_rng(92)
+    w["v"] = rng.normal(0.0, 1.0, size=w["v"].shape)
+    a = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02)
+    b = H.device_sim(w, style="hertz/history", precision=precision, skin=0.02)
+    for s in (a, b):
+        s.run(20)
+    n = a.atom.nlocal
+    A, B = a.atom._native, b.atom._native
+    for f in (NT.F_X, NT.F_V, NT.F_OMEGA, NT.F_F, NT.F_TORQUE, NT.F_RADIUS, NT.F_RMASS,
+              NT.F_TAG, NT.F_VIRIAL, NT.F_X0):
+        host = A.get_field(f, n)
+        dev = A.get_field_device(f, n).cpu().numpy()
+        assert dev.dtype == host.dtype and np.array_equal(host, dev), f
+    x = A.get_field(NT.F_X, n) + rng.uniform(-0.01, 0.01, size=(n, 3))
+    v = rng.normal(0.0, 1.0, size=(n, 3))
+    om = rng.normal(0.0, 2.0, size=(n, 3))
+    for f, vals in ((NT.F_X, x), (NT.F_V, v), (NT.F_OMEGA, om)):
+        A.set_field(f, vals, n)
+        B.set_field_device(f, torch.tensor(vals, dtype=torch.float64, device="cuda"), n)
+        assert np.array_equal(A.get_field(f, n), B.get_field(f, n)), f
+    for steps in (30, 11):
+        a.run(steps)
+        b.run(steps)
+        sa, sb = H.device_state(a), H.device_state(b)
+        _assert_state_equal(sa, sb, ("x", "v", "omega", "f", "torque", "virial"))
+    assert a.neighbor.build_count == b.neighbor.build_count
