@@ -90,6 +90,24 @@ __device__ __forceinline__ void split_delta(const SplitConst& sc, const int per[
     }
 }
 
+// The touch test of gather_split for slot partner j (cell cj, offset uj):
+// the contact condition the contact law runs under, by its expressions.
+__device__ __forceinline__ bool split_touch(const SplitConst& sc, const int per[3], int i,
+                                            uint32_t ci, const float4& ui, int j, uint32_t cj,
+                                            const float4& uj) {
+    const bool first = i < j;
+    float d[3];
+    if (first) split_delta(sc, per, ci, ui, cj, uj, d);
+    else split_delta(sc, per, cj, uj, ci, ui, d);
+    const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
+    const float radsum = ra + rb;
+    return rsq < radsum * radsum && rsq > 1e-28f;
+}
+
+// slots whose partner loads one gather_split batch issues together
+#define GATHER_BATCH 4
+
 template <int STYLE>
 __device__ __forceinline__ void gather_split(int i, uint32_t ci, const float4& ui, const float4& vi,
                                              const float4& oi, const SplitPtrs& P,
@@ -100,18 +118,52 @@ __device__ __forceinline__ void gather_split(int i, uint32_t ci, const float4& u
     const int s0 = P.nb_start[i], s1 = P.nb_start[i + 1];
     const bool uses_history = (STYLE != TMPS_PAIR_GRANULAR) || pp.uses_history;
     const float zero3[3] = {0.f, 0.f, 0.f};
-    for (int s = s0; s < s1; ++s) {
-        const int j = P.nb_j[s];
-        const float4 uj = P.up[j];
-        const uint32_t cj = P.cell[j];
-        const bool first = i < j;
-        float d[3];
-        if (first) split_delta(sc, per, ci, ui, cj, uj, d);
-        else split_delta(sc, per, cj, uj, ci, ui, d);
-        const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
-        const float radsum = ra + rb;
-        if (rsq < radsum * radsum && rsq > 1e-28f) {
+    for (int sb = s0; sb < s1; sb += GATHER_BATCH) {
+        // a batch of slots: the partners' offsets, cells and touch flags are
+        // requested together, then the touch test; a slot that does not
+        // touch has its flag cleared here
+        int jb[GATHER_BATCH];
+        // (components in plain arrays: an array of float4 stays in scratch)
+        float ux[GATHER_BATCH], uy[GATHER_BATCH], uz[GATHER_BATCH], uw[GATHER_BATCH];
+        uint32_t cb[GATHER_BATCH];
+        uint8_t fb[GATHER_BATCH];
+#pragma unroll
+        for (int b = 0; b < GATHER_BATCH; ++b) jb[b] = sb + b < s1 ? P.nb_j[sb + b] : i;
+#pragma unroll
+        for (int b = 0; b < GATHER_BATCH; ++b) {
+            const float4 u = P.up[jb[b]];
+            ux[b] = u.x;
+            uy[b] = u.y;
+            uz[b] = u.z;
+            uw[b] = u.w;
+            cb[b] = P.cell[jb[b]];
+            fb[b] = uses_history && sb + b < s1 ? P.nb_flag[sb + b] : 0;
+        }
+        unsigned touch = 0;
+#pragma unroll
+        for (int b = 0; b < GATHER_BATCH; ++b) {
+            const bool in_row = sb + b < s1;
+            if (in_row && split_touch(sc, per, i, ci, ui, jb[b], cb[b],
+                                      make_float4(ux[b], uy[b], uz[b], uw[b])))
+                touch |= 1u << b;
+            else if (fb[b] != 0) P.nb_flag[sb + b] = 0;
+        }
+        // the contact law on the batch's touching slots, in slot order (the
+        // separation recomputed by the same expressions)
+        while (touch) {
+            const int s = sb + __builtin_ctz(touch);
+            touch &= touch - 1;
+            const int j = P.nb_j[s];
+            const float4 uj = P.up[j];
+            const uint32_t cj = P.cell[j];
+            const bool first = i < j;
+            float d[3];
+            if (first) split_delta(sc, per, ci, ui, cj, uj, d);
+            else split_delta(sc, per, cj, uj, ci, ui, d);
+            const float rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const float ra = first ? ui.w : uj.w, rb = first ? uj.w : ui.w;
+            const float radsum = ra + rb;
+            {
             const float4 vj = P.vm[j];
             const float4 oj = P.om[j];
             const float4 va4 = first ? vi : vj;
@@ -163,8 +215,7 @@ __device__ __forceinline__ void gather_split(int i, uint32_t ci, const float4& u
 #pragma unroll
             for (int k = 0; k < 6; ++k) accV[k] += (double)o.vir[k];
             ++ntouch;
-        } else if (uses_history) {
-            if (P.nb_flag[s] != 0) P.nb_flag[s] = 0;
+            }
         }
     }
 }
@@ -349,8 +400,8 @@ __global__ void k_pair_split(SplitPtrs P, const Ctl* ctl, PairP pp, float dt, in
         for (int k = 0; k < 6; ++k) V[k] = P.vir[6 * i + k];
     }
     int nt = 0;
-    gather_split<STYLE>(i, P.cell[i], P.up[i], P.vm[i], P.om[i], P, sc, per, pp, dt, shearupdate, F,
-                        T, V, nt);
+    const float4 ui = P.up[i], vi = P.vm[i], oi = P.om[i];
+    gather_split<STYLE>(i, P.cell[i], ui, vi, oi, P, sc, per, pp, dt, shearupdate, F, T, V, nt);
     P.f[i] = make_float4(F[0], F[1], F[2], 0.f);
     P.tq[i] = make_float4(T[0], T[1], T[2], 0.f);
     for (int k = 0; k < 6; ++k) P.vir[6 * i + k] = V[k];
